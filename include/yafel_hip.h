@@ -1,0 +1,251 @@
+/*
+ * yafel_hip.h -- C ABI of the MI355X (gfx950) implementation of YAFEL's
+ * assembly -> sparse -> Krylov hot path.
+ *
+ * Conventions
+ *  - Every call returns an int status (YAFEL_HIP_OK = 0); yafel_hip_last_error()
+ *    gives the message.  No C++ exception crosses this boundary.
+ *  - Host arrays use int64 indices (the reference's size_t size_type,
+ *    old_handmade_linalg/sparse_matrix.hpp:16); device storage uses int32.
+ *  - Pointers named *_dev are device pointers; everything else is host memory.
+ *  - A context owns one HIP stream; calls on one context are stream-ordered.
+ *    Calls that take or return host data are blocking.  A context is meant for
+ *    one host thread at a time.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the YAFEL source tree).
+ */
+#ifndef YAFEL_HIP_H
+#define YAFEL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define YAFEL_HIP_OK 0
+#define YAFEL_HIP_EINVAL 1     /* bad argument (sizes, null pointers, unsupported element) */
+#define YAFEL_HIP_EDEVICE 2    /* HIP runtime error */
+#define YAFEL_HIP_ENOMEM 3     /* device allocation failed / budget exceeded */
+#define YAFEL_HIP_ESTATE 4     /* object used in the wrong state (e.g. matrix not from this mesh) */
+
+/* precision flags for matrix copies and solver arithmetic */
+#define YAFEL_HIP_FP64 1
+#define YAFEL_HIP_FP32 2
+#define YAFEL_HIP_MIXED 3      /* solver: FP32 inner iteration inside FP64 refinement */
+
+/* matrix formats */
+#define YAFEL_HIP_CSR 1
+#define YAFEL_HIP_BCSR3 3
+
+typedef struct yafel_hip_ctx_s *yafel_hip_ctx;
+typedef struct yafel_hip_matrix_s *yafel_hip_matrix;
+typedef struct yafel_hip_mesh_s *yafel_hip_mesh;
+
+/* ---------------------------------------------------------------- context */
+/* No reference counterpart (the reference has no device state). */
+int yafel_hip_ctx_create(int device, yafel_hip_ctx *out);
+int yafel_hip_ctx_destroy(yafel_hip_ctx ctx);
+/* last error message of ctx (or of the calling thread when ctx is NULL) */
+const char *yafel_hip_last_error(yafel_hip_ctx ctx);
+int yafel_hip_ctx_sync(yafel_hip_ctx ctx);
+/* the context's hipStream_t */
+void *yafel_hip_ctx_stream(yafel_hip_ctx ctx);
+/* device memory helpers (stream-ordered on the context stream) */
+int yafel_hip_malloc(yafel_hip_ctx ctx, size_t bytes, void **dptr);
+int yafel_hip_free(yafel_hip_ctx ctx, void *dptr);
+int yafel_hip_memcpy_h2d(yafel_hip_ctx ctx, void *dst_dev, const void *src, size_t bytes);
+int yafel_hip_memcpy_d2h(yafel_hip_ctx ctx, void *dst, const void *src_dev, size_t bytes);
+int yafel_hip_memcpy_d2d(yafel_hip_ctx ctx, void *dst_dev, const void *src_dev, size_t bytes);
+/* HIP-event timer on the context stream: begin, then end returns elapsed ms */
+int yafel_hip_timer_begin(yafel_hip_ctx ctx);
+int yafel_hip_timer_end(yafel_hip_ctx ctx, double *ms);
+
+/* --------------------------------------------------------------- matrices */
+typedef struct {
+  int format;        /* YAFEL_HIP_CSR or YAFEL_HIP_BCSR3 */
+  int precisions;    /* OR of YAFEL_HIP_FP64 / YAFEL_HIP_FP32 copies held on the device */
+  int64_t rows;      /* scalar rows (BCSR: 3 * block rows) */
+  int64_t cols;      /* scalar cols */
+  int64_t nnz;       /* stored scalars (BCSR: 9 * blocks) */
+  int64_t nblocks;   /* BCSR blocks (CSR: = nnz) */
+  int64_t ptr_len;   /* length of the host row_ptr (rows + 1, or brows + 1) */
+} yafel_hip_matrix_info;
+
+/* sparse_csr<double> from its public arrays row_ptr/col_index/_data
+ * (old_handmade_linalg/sparse_csr.hpp:63-65). */
+int yafel_hip_csr_create(yafel_hip_ctx ctx, int64_t rows, int64_t cols, const int64_t *row_ptr,
+                         const int64_t *col_index, const double *data, int precisions,
+                         yafel_hip_matrix *out);
+/* sparse_bcsr<3,double> from brow_ptr/bcol_index/data, 3x3 row-major blocks
+ * (old_handmade_linalg/sparse_bcsr.hpp:31-33, 227-229). */
+int yafel_hip_bcsr3_create(yafel_hip_ctx ctx, int64_t brows, int64_t bcols, const int64_t *brow_ptr,
+                           const int64_t *bcol_index, const double *data, int precisions,
+                           yafel_hip_matrix *out);
+/* sparse_csr<double>(const std::vector<triplet>&) on the device: sort by
+ * (row, col, value), sum duplicates in that order (sparse_csr.hpp:90-149). */
+int yafel_hip_csr_from_triplets(yafel_hip_ctx ctx, int64_t nt, const int64_t *rows,
+                                const int64_t *cols, const double *vals, int precisions,
+                                yafel_hip_matrix *out);
+/* sparse_bcsr<3,double>(triplets) on the device (sparse_bcsr.hpp:93-176). */
+int yafel_hip_bcsr3_from_triplets(yafel_hip_ctx ctx, int64_t nt, const int64_t *rows,
+                                  const int64_t *cols, const double *vals, int precisions,
+                                  yafel_hip_matrix *out);
+int yafel_hip_matrix_destroy(yafel_hip_matrix A);
+int yafel_hip_matrix_info_get(yafel_hip_matrix A, yafel_hip_matrix_info *info);
+/* copy back in the reference layout (row_ptr/col_index/_data, or BCSR with
+ * 3x3 row-major blocks); any pointer may be NULL */
+int yafel_hip_matrix_download(yafel_hip_matrix A, int64_t *row_ptr, int64_t *col_index,
+                              double *data);
+/* refresh the FP32 copy from the FP64 values (allocating it if needed) */
+int yafel_hip_matrix_sync_fp32(yafel_hip_matrix A);
+/* device pointer to the FP64 diagonal (computed on first use) */
+int yafel_hip_matrix_diagonal(yafel_hip_matrix A, double *diag_dev);
+
+/* ------------------------------------------------------------------- SpMV */
+/* y = A x; precision selects the FP64 or FP32 copy; x_dev/y_dev are double*
+ * or float* accordingly.  Replaces csr_spmv (old_handmade_linalg/csr_spmv.hpp:25-56)
+ * and bcsr_spmv (old_handmade_linalg/bcsr_spmv.hpp:23-62). */
+int yafel_hip_spmv(yafel_hip_matrix A, int precision, const void *x_dev, void *y_dev);
+/* Vector<T> operator*(A, x) (old_handmade_linalg/operators.hpp:117-131): host in, host out. */
+int yafel_hip_spmv_host(yafel_hip_matrix A, const double *x, double *y);
+/* Y = A X for k right-hand sides stored row-major (n x k, the k values of a
+ * row contiguous).  No reference counterpart: multi-RHS form of csr_spmv. */
+int yafel_hip_spmm(yafel_hip_matrix A, int precision, int k, const void *X_dev, void *Y_dev);
+
+/* ------------------------------------------------------------ vector ops */
+/* VectorExpression::dot (old_handmade_linalg/VectorExpression.hpp:31-55), FP64,
+ * deterministic two-stage reduction. */
+int yafel_hip_dot(yafel_hip_ctx ctx, int64_t n, const double *a_dev, const double *b_dev,
+                  double *result);
+
+/* ---------------------------------------------------------------- solvers */
+#define YAFEL_HIP_CG 0              /* cg_solve.hpp:33-81 */
+#define YAFEL_HIP_PCG_JACOBI 1      /* pcg_solve.hpp:126-178 + JacobiPreconditioner.hpp */
+#define YAFEL_HIP_BICGSTAB 2        /* bicgstab_solve.hpp:27-95 */
+#define YAFEL_HIP_BICGSTAB_JACOBI 3 /* BiCGSTAB with right Jacobi preconditioning */
+
+/* stopping rules */
+#define YAFEL_HIP_STOP_REFERENCE 0  /* the reference's own test: squared ratio rr/rr0 (CG),
+                                       rz/rz0 (PCG), ||Ax-b||^2/||b||^2 (BiCGSTAB) < tol */
+#define YAFEL_HIP_STOP_RELRES 1     /* true relative residual ||b - A x|| / ||b|| < tol */
+
+#define YAFEL_HIP_STATUS_CONVERGED 0
+#define YAFEL_HIP_STATUS_MAXITER 1
+#define YAFEL_HIP_STATUS_BREAKDOWN 2
+#define YAFEL_HIP_STATUS_NAN 3
+
+typedef struct {
+  int method;          /* YAFEL_HIP_CG ... */
+  int precision;       /* YAFEL_HIP_FP64 or YAFEL_HIP_MIXED */
+  int stop;            /* YAFEL_HIP_STOP_REFERENCE or YAFEL_HIP_STOP_RELRES */
+  double tol;
+  int64_t max_iter;    /* 0: the reference's limit (n for CG/PCG, 10 n for BiCGSTAB) */
+  int check_every;     /* iterations per host status poll (0: default 16) */
+  double refine_delta; /* MIXED: refine when the inner residual fell by this factor (0: 0.1) */
+} yafel_hip_solve_opts;
+
+typedef struct {
+  int status;           /* YAFEL_HIP_STATUS_* */
+  int64_t iterations;   /* Krylov iterations (inner iterations for MIXED) */
+  int64_t refinements;  /* FP64 residual replacements (MIXED) */
+  double rel_residual;  /* true ||b - A x|| / ||b|| in FP64 at exit */
+  double criterion;     /* last value of the stopping quantity */
+  double ms;            /* device time of the solve */
+} yafel_hip_solve_info;
+
+int yafel_hip_solve_opts_default(yafel_hip_solve_opts *opts);
+/* Solve A x = b; x_dev holds x0 on entry.  b_dev/x_dev are FP64 device vectors. */
+int yafel_hip_solve(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const double *b_dev,
+                    double *x_dev, yafel_hip_solve_info *info);
+/* host-vector form of cg_solve / pcg_solve / bicgstab_solve: x0 may be NULL (zeros) */
+int yafel_hip_solve_host(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const double *b,
+                         const double *x0, double *x, yafel_hip_solve_info *info);
+/* k right-hand sides solved together (B_dev, X_dev row-major n x k, FP64);
+ * info is an array of k.  Method must be BICGSTAB(_JACOBI) or CG/PCG_JACOBI.
+ * No reference counterpart: the reference solves one right-hand side per call. */
+int yafel_hip_solve_multi(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, int k,
+                          const double *B_dev, double *X_dev, yafel_hip_solve_info *info);
+
+/* --------------------------------------------------------- mesh + assembly */
+/* DoFManager(M, CG, p, dof_per_node) (src/utils/DoFManager.cpp:11-36, 143-229,
+ * 296-414) on the device.  corner_nodes is n_corner*3 (the reference's 3-D
+ * coordinate<>), cells is ncell*2^dim corner ids: lexicographic (x fastest)
+ * for p = 1, gmsh counter-clockwise for p >= 2 -- the orders the reference's
+ * p = 1 pass-through and p >= 2 corner interpolation assume.  All cells are
+ * Quad4 (dim 2) or Hex8 (dim 3). */
+int yafel_hip_mesh_create_cg(yafel_hip_ctx ctx, int dim, int p, int dof_per_node,
+                             int64_t n_corner, const double *corner_nodes, int64_t ncell,
+                             const int32_t *cells, yafel_hip_mesh *out);
+int yafel_hip_mesh_destroy(yafel_hip_mesh m);
+/* info[0]=dof nodes, [1]=elements, [2]=nodes per element, [3]=dofs, [4]=dim, [5]=p, [6]=dpn */
+int yafel_hip_mesh_info(yafel_hip_mesh m, int64_t *info);
+/* DoFManager::elements and ::dof_nodes (n_nodes*3) back to the host */
+int yafel_hip_mesh_download(yafel_hip_mesh m, int32_t *elements, double *dof_nodes);
+/* device pointer to dof_nodes (n_nodes*3 doubles) */
+int yafel_hip_mesh_nodes_dev(yafel_hip_mesh m, const double **nodes_dev);
+
+/* The global tangent's sparsity pattern (the closure of element connectivity,
+ * explicit zeros kept, as sparse_csr(triplets) of CGAssembly's triplets yields;
+ * include/assembly/CGAssembly.hpp:149-156 + sparse_csr.hpp:90-149) with zero
+ * values, plus the deterministic gather plan used by yafel_hip_assemble. */
+int yafel_hip_mesh_pattern(yafel_hip_mesh m, int format, int precisions, yafel_hip_matrix *out);
+
+#define YAFEL_HIP_PHYS_POISSON 0     /* apps/BasicPoisson/main.cpp:25-61 */
+#define YAFEL_HIP_PHYS_ELASTICITY 1  /* apps/BasicLinearElasticity/main.cpp:41-113 */
+#define YAFEL_HIP_PHYS_SUPG 2        /* apps/supg_learning/supg_advection.hpp:19-132 */
+
+/* The Physics template parameter of CGAssembly as a descriptor (a GPU cannot
+ * call a host functor). */
+typedef struct {
+  int kind;              /* YAFEL_HIP_PHYS_* */
+  int nsd;               /* must equal the mesh dim */
+  const double *coef;    /* per-element scale of the tangent (k_e, or E_e for elasticity); NULL = 1 */
+  int coef_on_device;    /* coef is a device pointer */
+  double f;              /* Poisson source */
+  double lambda, mu;     /* Lame parameters */
+  double body[3];        /* elasticity body force */
+  double U[3];           /* SUPG velocity */
+  double kdiff, h, alpha, Q; /* SUPG diffusivity, element size, upwinding, source */
+} yafel_hip_physics;
+
+#define YAFEL_HIP_ASSEMBLE_TANGENT 1  /* AssemblyRequirement::Tangent */
+#define YAFEL_HIP_ASSEMBLE_RESIDUAL 2 /* AssemblyRequirement::Residual */
+
+/* CGAssembly<Physics>(FESystem&, reqs) (include/assembly/CGAssembly.hpp:27-186):
+ * overwrites A's values (A from yafel_hip_mesh_pattern of this mesh, FP64
+ * values; the FP32 copy is not touched) and ACCUMULATES into R_dev (FP64,
+ * n_dofs), as the reference accumulates GlobalResidual. */
+int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requirements,
+                       yafel_hip_matrix A, double *R_dev);
+/* element matrices only: Ke_dev holds nel*(nen*dpn)^2 doubles, element-major,
+ * row-major (the local_tangent buffer of CGAssembly.hpp:117-118). */
+int yafel_hip_element_matrices(yafel_hip_mesh m, const yafel_hip_physics *phys, double *Ke_dev);
+
+/* DirichletBC::apply (src/boundary_conditions/DirichletBC.cpp:27-56):
+ * rhs -= A g; entries in BC rows or columns zeroed, diagonal set to 1,
+ * rhs(bc) = g(bc); the pattern is kept.  mask_dev is one byte per dof,
+ * g_dev may be NULL for homogeneous values. */
+int yafel_hip_dirichlet_apply(yafel_hip_matrix A, const uint8_t *mask_dev, const double *g_dev,
+                              double *rhs_dev);
+/* DirichletBC::selectByFunction for axis-aligned planes: marks component comp
+ * (or all components when comp < 0) of every node with |x_axis - value| < tol. */
+int yafel_hip_bc_mask_plane(yafel_hip_mesh m, int axis, double value, double tol, int comp,
+                            uint8_t *mask_dev);
+
+/* ----------------------------------------------------- synthetic inputs */
+/* Structured unit-square/cube corner mesh (cells ordered as
+ * yafel_hip_mesh_create_cg expects for p).  No reference counterpart: the
+ * reference reads gmsh files; this builds the synthetic workloads. */
+int yafel_hip_structured_mesh(int dim, int nx, int ny, int nz, int p, double *corner_nodes,
+                              int32_t *cells);
+/* splitmix64 + Box-Muller normal deviates (host) */
+int yafel_hip_normal_fill(uint64_t seed, int64_t n, double mean, double sd, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,516 @@
+// CGAssembly on the device (include/assembly/CGAssembly.hpp:27-186) and
+// DirichletBC::apply (src/boundary_conditions/DirichletBC.cpp:27-56).
+//
+// Assembly is two passes:
+//  1. element pass: a workgroup takes EPB elements; their node coordinates go
+//     to LDS, one thread per (element, quadrature point) forms J, det J, J^-1
+//     and the physical gradients of every shape function into LDS
+//     (Element::update, include/element/Element.hpp:109-149), then one thread
+//     per (element, local node A) accumulates row block A of K_e over the
+//     quadrature points with the physics formula, and R_e(A).  K_e / R_e go to
+//     a staging buffer, row-major per element (the local_tangent layout).
+//  2. gather pass: one thread per stored node pair sums its contributions in
+//     element order from the plan built with the pattern -- deterministic, no
+//     atomics.  The residual is gathered per node the same way and ADDED to R
+//     (the reference accumulates GlobalResidual).
+#include <algorithm>
+
+#include "internal.hpp"
+#include "mesh.hpp"
+
+using namespace yh;
+
+namespace {
+
+constexpr int NT = 256;
+
+struct Phys {
+  int kind;
+  double f, lambda, mu;
+  double body[3], U[3];
+  double kdiff, h, alpha, Q;
+  // derived SUPG terms
+  double Kt[9];
+  double Uh[3];
+};
+
+template <int DIM, int P>
+struct ElemShape {
+  static constexpr int N1 = P + 1;
+  static constexpr int NEN = DIM == 2 ? N1 * N1 : N1 * N1 * N1;
+  static constexpr int NQP = NEN;
+  static constexpr int LDS_PER = NQP * NEN * DIM * 8 + NQP * 8 + NEN * DIM * 8;
+  static constexpr int EPB0 = NT / NEN;
+  static constexpr int EPB1 = 60000 / LDS_PER;
+  static constexpr int EPB = (EPB0 < EPB1 ? EPB0 : EPB1) > 0 ? (EPB0 < EPB1 ? EPB0 : EPB1) : 1;
+};
+
+template <int DIM>
+__device__ __forceinline__ void jac_inv(const double *J, double &det, double *Ji) {
+  if (DIM == 2) {
+    det = J[0] * J[3] - J[1] * J[2];
+    Ji[0] = J[3] / det;
+    Ji[1] = -J[1] / det;
+    Ji[2] = -J[2] / det;
+    Ji[3] = J[0] / det;
+  } else {
+    const double a = J[0], b = J[1], c = J[2], d = J[3], e = J[4], f = J[5], g = J[6], h = J[7], i = J[8];
+    det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+    Ji[0] = (e * i - f * h) / det;
+    Ji[1] = (c * h - b * i) / det;
+    Ji[2] = (b * f - c * e) / det;
+    Ji[3] = (f * g - d * i) / det;
+    Ji[4] = (a * i - c * g) / det;
+    Ji[5] = (c * d - a * f) / det;
+    Ji[6] = (d * h - e * g) / det;
+    Ji[7] = (b * g - a * h) / det;
+    Ji[8] = (a * e - b * d) / det;
+  }
+}
+
+template <int DIM, int P, int DPN>
+__global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__restrict__ el,
+                                                const double *__restrict__ X,
+                                                const double *__restrict__ tabN,
+                                                const double *__restrict__ tabdN,
+                                                const double *__restrict__ tabw, Phys ph,
+                                                const double *__restrict__ coef, double *__restrict__ Ke,
+                                                double *__restrict__ Re, int want_k, int want_r) {
+  using S = ElemShape<DIM, P>;
+  constexpr int NEN = S::NEN, NQP = S::NQP, EPB = S::EPB, NLD = NEN * DPN;
+  __shared__ double sG[EPB][NQP][NEN][DIM];
+  __shared__ double sW[EPB][NQP];
+  __shared__ double sX[EPB][NEN][DIM];
+  const int64_t e0 = (int64_t)blockIdx.x * EPB;
+  for (int i = threadIdx.x; i < EPB * NEN; i += NT) {
+    const int le = i / NEN, a = i - (i / NEN) * NEN;
+    const int64_t e = e0 + le;
+    if (e < nel) {
+      const int64_t node = el[e * NEN + a];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) sX[le][a][d] = X[3 * node + d];
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < EPB * NQP; i += NT) {
+    const int le = i / NQP, q = i - (i / NQP) * NQP;
+    if (e0 + le >= nel) continue;
+    // J(i,j) = sum_A x_A(i) dN_A/dxi_j  (Element.hpp:116-124)
+    double J[DIM * DIM];
+#pragma unroll
+    for (int k = 0; k < DIM * DIM; ++k) J[k] = 0;
+    for (int A = 0; A < NEN; ++A) {
+      const double *dn = &tabdN[(q * NEN + A) * DIM];
+#pragma unroll
+      for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) J[r * DIM + c] += sX[le][A][r] * dn[c];
+    }
+    double det, Ji[DIM * DIM];
+    jac_inv<DIM>(J, det, Ji);
+    sW[le][q] = det * tabw[q];
+    // grad_x N_A = J^-T grad_xi N_A  (Element.hpp:131-147)
+    for (int A = 0; A < NEN; ++A) {
+      const double *dn = &tabdN[(q * NEN + A) * DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) s += Ji[k * DIM + d] * dn[k];
+        sG[le][q][A][d] = s;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < EPB * NEN; i += NT) {
+    const int le = i / NEN, A = i - (i / NEN) * NEN;
+    const int64_t e = e0 + le;
+    if (e >= nel) continue;
+    const double ke = coef ? coef[e] : 1.0;
+    if (want_k) {
+      double *Krow = Ke + e * (int64_t)NLD * NLD;
+      for (int B = 0; B < NEN; ++B) {
+        double k[DPN][DPN];
+#pragma unroll
+        for (int a = 0; a < DPN; ++a)
+#pragma unroll
+          for (int b = 0; b < DPN; ++b) k[a][b] = 0;
+        for (int q = 0; q < NQP; ++q) {
+          const double jxw = sW[le][q];
+          const double *gA = sG[le][q][A];
+          const double *gB = sG[le][q][B];
+          if (ph.kind == YAFEL_HIP_PHYS_POISSON) {
+            // apps/BasicPoisson/main.cpp:51-58
+            double dt = 0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dt += gA[d] * gB[d];
+            k[0][0] -= ke * dt * jxw;
+          } else if (ph.kind == YAFEL_HIP_PHYS_ELASTICITY) {
+            // apps/BasicLinearElasticity/main.cpp:70-110
+            const double lam = ke * ph.lambda, mu = ke * ph.mu;
+#pragma unroll
+            for (int ii = 0; ii < DPN; ++ii)
+#pragma unroll
+              for (int kk = 0; kk < DPN; ++kk) {
+                double s = 0;
+#pragma unroll
+                for (int j = 0; j < DIM; ++j)
+#pragma unroll
+                  for (int l = 0; l < DIM; ++l) {
+                    const double C = lam * (ii == j) * (kk == l) + mu * ((ii == kk) * (j == l) + (ii == l) * (j == kk));
+                    s += C * (gA[j] * gB[l]);
+                  }
+                k[ii][kk] -= s * jxw;
+              }
+          } else {
+            // apps/supg_learning/supg_advection.hpp:90-103 (advection added once per A component)
+            const double NA = tabN[q * NEN + A];
+            double s = 0;
+#pragma unroll
+            for (int Ac = 0; Ac < DIM; ++Ac)
+#pragma unroll
+              for (int Bc = 0; Bc < DIM; ++Bc)
+                s += (NA * ph.U[Bc] * gB[Bc] + gA[Ac] * gB[Bc] * ph.Kt[Ac * DIM + Bc]) * jxw;
+            k[0][0] += s;
+          }
+        }
+#pragma unroll
+        for (int a = 0; a < DPN; ++a)
+#pragma unroll
+          for (int b = 0; b < DPN; ++b) Krow[(A * DPN + a) * NLD + B * DPN + b] = k[a][b];
+      }
+    }
+    if (want_r) {
+      double r[DPN];
+#pragma unroll
+      for (int a = 0; a < DPN; ++a) r[a] = 0;
+      for (int q = 0; q < NQP; ++q) {
+        const double jxw = sW[le][q];
+        const double NA = tabN[q * NEN + A];
+        if (ph.kind == YAFEL_HIP_PHYS_POISSON) {
+          r[0] += (ph.f * jxw) * NA;
+        } else if (ph.kind == YAFEL_HIP_PHYS_ELASTICITY) {
+#pragma unroll
+          for (int a = 0; a < DPN; ++a) r[a] += ph.body[a] * NA * jxw;
+        } else {
+          const double *gA = sG[le][q][A];
+#pragma unroll
+          for (int Ac = 0; Ac < DIM; ++Ac) r[0] -= (NA * ph.Q + ph.Uh[Ac] * gA[Ac]) * jxw;
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < DPN; ++a) Re[(e * NEN + A) * DPN + a] = r[a];
+    }
+  }
+}
+
+// one thread per stored node pair: sum the element contributions in order
+template <int DPN, int FMT>
+__global__ __launch_bounds__(NT) void k_gather(int64_t nruns, int nen, const int32_t *__restrict__ cptr,
+                                               const uint32_t *__restrict__ contrib,
+                                               const double *__restrict__ Ke,
+                                               const int32_t *__restrict__ run_row,
+                                               const int32_t *__restrict__ node_ptr,
+                                               const int32_t *__restrict__ spos, double *__restrict__ val) {
+  const int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (j >= nruns) return;
+  const int nld = nen * DPN;
+  const int64_t nen2 = (int64_t)nen * nen;
+  double acc[DPN][DPN];
+#pragma unroll
+  for (int a = 0; a < DPN; ++a)
+#pragma unroll
+    for (int b = 0; b < DPN; ++b) acc[a][b] = 0;
+  const int c0 = cptr[j], c1 = cptr[j + 1];
+  for (int c = c0; c < c1; ++c) {
+    const int64_t t = contrib[c];
+    const int64_t e = t / nen2;
+    const int r = (int)(t - e * nen2);
+    const int a = r / nen, b = r - (r / nen) * nen;
+    const double *k = Ke + e * (int64_t)nld * nld + (int64_t)(a * DPN) * nld + b * DPN;
+#pragma unroll
+    for (int i = 0; i < DPN; ++i)
+#pragma unroll
+      for (int jj = 0; jj < DPN; ++jj) acc[i][jj] += k[i * nld + jj];
+  }
+  if (FMT == YAFEL_HIP_BCSR3) {
+#pragma unroll
+    for (int i = 0; i < DPN; ++i)
+#pragma unroll
+      for (int jj = 0; jj < DPN; ++jj) val[(int64_t)(i * 3 + jj) * nruns + j] = acc[i][jj];
+  } else if (DPN == 1) {
+    val[j] = acc[0][0];
+  } else {
+    const int A = run_row[j];
+    const int64_t len = node_ptr[A + 1] - node_ptr[A];
+    const int64_t p0 = spos[j];
+#pragma unroll
+    for (int i = 0; i < DPN; ++i)
+#pragma unroll
+      for (int jj = 0; jj < DPN; ++jj) val[p0 + i * DPN * len + jj] = acc[i][jj];
+  }
+}
+
+__global__ void k_gather_res(int64_t nnodes, int dpn, const int32_t *__restrict__ ptr,
+                             const uint32_t *__restrict__ inc, const double *__restrict__ Re,
+                             double *__restrict__ R) {
+  const int64_t A = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (A >= nnodes) return;
+  for (int c = 0; c < dpn; ++c) {
+    double acc = 0;
+    for (int k = ptr[A]; k < ptr[A + 1]; ++k) acc += Re[(int64_t)inc[k] * dpn + c];
+    R[A * dpn + c] += acc;
+  }
+}
+
+// ------------------------------------------------------------ Dirichlet
+__global__ void k_masked_copy(int64_t n, const uint8_t *mask, const double *g, double *gm) {
+  int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (i < n) gm[i] = mask[i] ? g[i] : 0.0;
+}
+__global__ void k_sub(int64_t n, const double *a, double *rhs) {
+  int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (i < n) rhs[i] -= a[i];
+}
+
+__global__ void k_dirichlet_csr(int64_t rows, const int32_t *ptr, const int32_t *col, double *val,
+                                const uint8_t *mask, const double *g, double *rhs) {
+  int64_t r = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (r >= rows) return;
+  const bool mr = mask[r] != 0;
+  for (int k = ptr[r]; k < ptr[r + 1]; ++k) {
+    const int c = col[k];
+    if (mr || mask[c]) {
+      val[k] = 0;
+      if (r == c) {
+        rhs[r] = g ? g[c] : 0.0;
+        val[k] = 1;
+      }
+    }
+  }
+}
+
+__global__ void k_dirichlet_bcsr(int64_t brows, int64_t nb, const int32_t *ptr, const int32_t *col,
+                                 double *val, const uint8_t *mask, const double *g, double *rhs) {
+  int64_t br = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (br >= brows) return;
+  for (int k = ptr[br]; k < ptr[br + 1]; ++k) {
+    const int64_t bc = col[k];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        const int64_t r = 3 * br + i, c = 3 * bc + j;
+        if (mask[r] || mask[c]) {
+          val[(int64_t)(i * 3 + j) * nb + k] = 0;
+          if (r == c) {
+            rhs[r] = g ? g[c] : 0.0;
+            val[(int64_t)(i * 3 + j) * nb + k] = 1;
+          }
+        }
+      }
+  }
+}
+
+Phys make_phys(const yafel_hip_physics *p) {
+  Phys ph{};
+  ph.kind = p->kind;
+  ph.f = p->f;
+  ph.lambda = p->lambda;
+  ph.mu = p->mu;
+  for (int d = 0; d < 3; ++d) {
+    ph.body[d] = p->body[d];
+    ph.U[d] = p->U[d];
+  }
+  ph.kdiff = p->kdiff;
+  ph.h = p->h;
+  ph.alpha = p->alpha;
+  ph.Q = p->Q;
+  const int nsd = p->nsd;
+  double nU = 0;
+  for (int d = 0; d < nsd; ++d) nU += p->U[d] * p->U[d];
+  nU = std::sqrt(nU);
+  for (int a = 0; a < nsd; ++a)
+    for (int b = 0; b < nsd; ++b) {
+      double v = (a == b) ? p->kdiff : 0.0;
+      if (nU > 0) v += (p->alpha * p->h / (nU * 2)) * p->U[a] * p->U[b];
+      ph.Kt[a * nsd + b] = v;
+    }
+  for (int d = 0; d < 3; ++d) ph.Uh[d] = (nU > 0 && d < nsd) ? (p->alpha * p->h / 2) * p->U[d] / nU : 0.0;
+  return ph;
+}
+
+template <int DIM, int P, int DPN>
+void launch_elem_dpn(yafel_hip_mesh_s *m, const Phys &ph, const double *coef, double *Ke, double *Re, int wk,
+                     int wr) {
+  using S = ElemShape<DIM, P>;
+  const int grid = ceil_div(m->nel, S::EPB);
+  k_element<DIM, P, DPN><<<grid, NT, 0, m->ctx->stream>>>(m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(),
+                                                          m->tabN.as<double>(), m->tabdN.as<double>(),
+                                                          m->tabw.as<double>(), ph, coef, Ke, Re, wk, wr);
+  HIPCHK(hipGetLastError());
+}
+
+template <int DIM, int P>
+void launch_elem_p(yafel_hip_mesh_s *m, const Phys &ph, const double *coef, double *Ke, double *Re, int wk, int wr) {
+  switch (m->dpn) {
+    case 1: launch_elem_dpn<DIM, P, 1>(m, ph, coef, Ke, Re, wk, wr); break;
+    case 2: launch_elem_dpn<DIM, P, 2>(m, ph, coef, Ke, Re, wk, wr); break;
+    default: launch_elem_dpn<DIM, P, 3>(m, ph, coef, Ke, Re, wk, wr); break;
+  }
+}
+
+void launch_element(yafel_hip_mesh_s *m, const Phys &ph, const double *coef, double *Ke, double *Re, int wk, int wr) {
+  if (m->dim == 2) {
+    switch (m->p) {
+      case 1: launch_elem_p<2, 1>(m, ph, coef, Ke, Re, wk, wr); break;
+      case 2: launch_elem_p<2, 2>(m, ph, coef, Ke, Re, wk, wr); break;
+      case 3: launch_elem_p<2, 3>(m, ph, coef, Ke, Re, wk, wr); break;
+      case 4: launch_elem_p<2, 4>(m, ph, coef, Ke, Re, wk, wr); break;
+      default: fail(YAFEL_HIP_EINVAL, "assemble: 2-D elements are supported for p = 1..4");
+    }
+  } else {
+    switch (m->p) {
+      case 1: launch_elem_p<3, 1>(m, ph, coef, Ke, Re, wk, wr); break;
+      case 2: launch_elem_p<3, 2>(m, ph, coef, Ke, Re, wk, wr); break;
+      default: fail(YAFEL_HIP_EINVAL, "assemble: 3-D elements are supported for p = 1..2");
+    }
+  }
+}
+
+void check_phys(yafel_hip_mesh_s *m, const yafel_hip_physics *p) {
+  REQUIRE(p, "assemble: null physics");
+  REQUIRE(p->nsd == m->dim, "assemble: physics nsd must equal the mesh dimension");
+  REQUIRE(p->kind >= 0 && p->kind <= 2, "assemble: unknown physics kind");
+  if (p->kind == YAFEL_HIP_PHYS_ELASTICITY) REQUIRE(m->dpn == m->dim, "assemble: elasticity needs dim dofs per node");
+  else REQUIRE(m->dpn == 1, "assemble: scalar physics needs 1 dof per node");
+}
+
+}  // namespace
+
+extern "C" {
+
+int yafel_hip_element_matrices(yafel_hip_mesh m, const yafel_hip_physics *phys, double *Ke_dev) {
+  YH_TRY_BEGIN
+  REQUIRE(m && Ke_dev, "element_matrices: null argument");
+  check_phys(m, phys);
+  HIPCHK(hipSetDevice(m->ctx->device));
+  DBuf coefd;
+  const double *coef = nullptr;
+  if (phys->coef) {
+    if (phys->coef_on_device) coef = phys->coef;
+    else {
+      coefd.alloc(m->nel * 8);
+      HIPCHK(hipMemcpyAsync(coefd.p, phys->coef, m->nel * 8, hipMemcpyHostToDevice, m->ctx->stream));
+      coef = coefd.as<double>();
+    }
+  }
+  Phys ph = make_phys(phys);
+  launch_element(m, ph, coef, Ke_dev, nullptr, 1, 0);
+  HIPCHK(hipStreamSynchronize(m->ctx->stream));
+  return YAFEL_HIP_OK;
+  YH_TRY_END(m ? m->ctx : nullptr)
+}
+
+int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requirements,
+                       yafel_hip_matrix A, double *R_dev) {
+  YH_TRY_BEGIN
+  REQUIRE(m, "assemble: null mesh");
+  check_phys(m, phys);
+  const bool wk = (requirements & YAFEL_HIP_ASSEMBLE_TANGENT) != 0;
+  const bool wr = (requirements & YAFEL_HIP_ASSEMBLE_RESIDUAL) != 0;
+  REQUIRE(!wk || A, "assemble: tangent requested without a matrix");
+  REQUIRE(!wr || R_dev, "assemble: residual requested without R");
+  REQUIRE(!wk || (A->mesh == m && A->plan), "assemble: matrix was not created from this mesh's pattern");
+  yafel_hip_ctx_s *ctx = m->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int nld = m->nen * m->dpn;
+  DBuf coefd;
+  const double *coef = nullptr;
+  if (phys->coef) {
+    if (phys->coef_on_device) coef = phys->coef;
+    else {
+      coefd.alloc(m->nel * 8);
+      HIPCHK(hipMemcpyAsync(coefd.p, phys->coef, m->nel * 8, hipMemcpyHostToDevice, s));
+      coef = coefd.as<double>();
+    }
+  }
+  if (wk && m->stageK.bytes < (size_t)m->nel * nld * nld * 8) m->stageK.alloc((size_t)m->nel * nld * nld * 8);
+  if (wr && m->stageR.bytes < (size_t)m->nel * nld * 8) m->stageR.alloc((size_t)m->nel * nld * 8);
+  Phys ph = make_phys(phys);
+  launch_element(m, ph, coef, wk ? m->stageK.as<double>() : nullptr, wr ? m->stageR.as<double>() : nullptr, wk, wr);
+  if (wk) {
+    YhPlan *pl = A->plan;
+    const int g = ceil_div(pl->nruns, NT);
+    const int32_t *sp = pl->scal_pos.as<int32_t>();
+    double *val = A->val64.as<double>();
+    if (pl->nruns > 0) {
+      if (A->format == YAFEL_HIP_BCSR3)
+        k_gather<3, YAFEL_HIP_BCSR3><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
+                                                      m->stageK.as<double>(), pl->run_row.as<int32_t>(),
+                                                      pl->node_ptr.as<int32_t>(), sp, val);
+      else if (m->dpn == 1)
+        k_gather<1, YAFEL_HIP_CSR><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
+                                                    m->stageK.as<double>(), pl->run_row.as<int32_t>(),
+                                                    pl->node_ptr.as<int32_t>(), sp, val);
+      else if (m->dpn == 2)
+        k_gather<2, YAFEL_HIP_CSR><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
+                                                    m->stageK.as<double>(), pl->run_row.as<int32_t>(),
+                                                    pl->node_ptr.as<int32_t>(), sp, val);
+      else
+        k_gather<3, YAFEL_HIP_CSR><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
+                                                    m->stageK.as<double>(), pl->run_row.as<int32_t>(),
+                                                    pl->node_ptr.as<int32_t>(), sp, val);
+      HIPCHK(hipGetLastError());
+    }
+    A->has32 = false;
+    A->diag_ok = false;
+  }
+  if (wr) {
+    mesh_build_incidence(m);
+    k_gather_res<<<ceil_div(m->n_nodes, NT), NT, 0, s>>>(m->n_nodes, m->dpn, m->inc_ptr.as<int32_t>(),
+                                                         m->inc.as<uint32_t>(), m->stageR.as<double>(), R_dev);
+    HIPCHK(hipGetLastError());
+  }
+  if (coefd.p) HIPCHK(hipStreamSynchronize(s));
+  return YAFEL_HIP_OK;
+  YH_TRY_END(m ? m->ctx : nullptr)
+}
+
+int yafel_hip_dirichlet_apply(yafel_hip_matrix A, const uint8_t *mask_dev, const double *g_dev,
+                              double *rhs_dev) {
+  YH_TRY_BEGIN
+  REQUIRE(A && mask_dev && rhs_dev, "dirichlet_apply: null argument");
+  REQUIRE(A->has64, "dirichlet_apply: matrix has no FP64 values");
+  REQUIRE(A->rows == A->cols, "dirichlet_apply: matrix must be square");
+  yafel_hip_ctx_s *ctx = A->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t n = A->rows;
+  if (n == 0) return YAFEL_HIP_OK;
+  DBuf gm, ag;
+  if (g_dev) {
+    // rhs -= A * bc_values, bc_values zero outside the BC dofs (DirichletBC.cpp:34-41)
+    gm.alloc(n * 8);
+    ag.alloc(n * 8);
+    k_masked_copy<<<ceil_div(n, NT), NT, 0, s>>>(n, mask_dev, g_dev, gm.as<double>());
+    HIPCHK(hipGetLastError());
+    spmv_launch(A, YAFEL_HIP_FP64, gm.p, ag.p, s);
+    k_sub<<<ceil_div(n, NT), NT, 0, s>>>(n, ag.as<double>(), rhs_dev);
+    HIPCHK(hipGetLastError());
+  }
+  const double *gv = g_dev ? gm.as<double>() : nullptr;
+  if (A->format == YAFEL_HIP_CSR)
+    k_dirichlet_csr<<<ceil_div(A->brows, NT), NT, 0, s>>>(A->brows, A->ptr.as<int32_t>(), A->col.as<int32_t>(),
+                                                          A->val64.as<double>(), mask_dev, gv, rhs_dev);
+  else
+    k_dirichlet_bcsr<<<ceil_div(A->brows, NT), NT, 0, s>>>(A->brows, A->nblk, A->ptr.as<int32_t>(), A->col.as<int32_t>(),
+                                                           A->val64.as<double>(), mask_dev, gv, rhs_dev);
+  HIPCHK(hipGetLastError());
+  A->has32 = false;
+  A->diag_ok = false;
+  if (g_dev) HIPCHK(hipStreamSynchronize(s));
+  return YAFEL_HIP_OK;
+  YH_TRY_END(A ? A->ctx : nullptr)
+}
+
+}  // extern "C"

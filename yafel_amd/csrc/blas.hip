@@ -1,0 +1,51 @@
+// FP64 dot product with a deterministic two-stage reduction
+// (replaces VectorExpression::dot, old_handmade_linalg/VectorExpression.hpp:31-55).
+#include "internal.hpp"
+
+using namespace yh;
+
+namespace {
+
+constexpr int NT = 256;
+
+__global__ __launch_bounds__(NT) void k_dot_partial(int64_t n, const double *__restrict__ a,
+                                                    const double *__restrict__ b, double *part) {
+  __shared__ double lds[NT / 64];
+  const int64_t chunk = ((n + gridDim.x - 1) / gridDim.x + 63) & ~(int64_t)63;
+  const int64_t i0 = (int64_t)dev::xcd_block() * chunk;
+  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
+  double s = 0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += NT) s = fma(a[i], b[i], s);
+  s = dev::block_sum<NT>(s, lds);
+  if (threadIdx.x == 0) part[dev::xcd_block()] = s;
+}
+
+__global__ __launch_bounds__(NT) void k_dot_final(int npart, const double *part, double *out) {
+  __shared__ double lds[NT / 64];
+  double s = dev::reduce_partials<NT>(part, npart, 1, lds);
+  if (threadIdx.x == 0) *out = s;
+}
+
+}  // namespace
+
+extern "C" int yafel_hip_dot(yafel_hip_ctx ctx, int64_t n, const double *a, const double *b,
+                             double *result) {
+  YH_TRY_BEGIN
+  REQUIRE(ctx && result && (n == 0 || (a && b)), "dot: null argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (n == 0) {
+    *result = 0;
+    return YAFEL_HIP_OK;
+  }
+  const int grid = grid_for(ctx, n, NT);
+  DBuf part;
+  part.alloc((grid + 1) * sizeof(double));
+  k_dot_partial<<<grid, NT, 0, ctx->stream>>>(n, a, b, part.as<double>());
+  HIPCHK(hipGetLastError());
+  k_dot_final<<<1, NT, 0, ctx->stream>>>(grid, part.as<double>(), part.as<double>() + grid);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(result, part.as<double>() + grid, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return YAFEL_HIP_OK;
+  YH_TRY_END(ctx)
+}

@@ -1,0 +1,196 @@
+// Internal declarations shared by the HIP translation units of libyafel_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "yafel_hip.h"
+
+namespace yh {
+
+// ---------------------------------------------------------------- errors
+struct Error {
+  int code;
+  std::string msg;
+};
+
+void set_thread_error(const std::string &msg);
+const char *thread_error();
+
+#define YH_TRY_BEGIN try {
+#define YH_TRY_END(ctx)                                   \
+  }                                                       \
+  catch (const yh::Error &e) {                            \
+    yh::record_error(ctx, e.msg);                         \
+    return e.code;                                        \
+  }                                                       \
+  catch (const std::exception &e) {                       \
+    yh::record_error(ctx, e.what());                      \
+    return YAFEL_HIP_EINVAL;                              \
+  }
+
+[[noreturn]] void fail(int code, const std::string &msg);
+void hip_check(hipError_t e, const char *what, const char *file, int line);
+#define HIPCHK(x) ::yh::hip_check((x), #x, __FILE__, __LINE__)
+#define REQUIRE(cond, msg)                                     \
+  do {                                                         \
+    if (!(cond)) ::yh::fail(YAFEL_HIP_EINVAL, std::string(msg)); \
+  } while (0)
+
+// -------------------------------------------------------- device buffers
+// RAII device allocation (stream-ordered free on the owning stream).
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void alloc(size_t b);
+  void release();
+  template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+}  // namespace yh
+
+struct yafel_hip_ctx_s;
+
+namespace yh {
+void record_error(yafel_hip_ctx_s *ctx, const std::string &msg);
+}  // namespace yh
+
+// Solver workspace: grown on demand, owned by the context.
+struct YhWorkspace {
+  yh::DBuf vec64[10];   // FP64 n-vectors
+  yh::DBuf vec32[10];   // FP32 n-vectors
+  yh::DBuf partials;    // per-block partial sums
+  yh::DBuf state;       // device solver state
+  size_t n64 = 0, n32 = 0;
+};
+
+struct yafel_hip_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  std::string err;
+  int num_cus = 256;
+  YhWorkspace ws;
+  void *pinned = nullptr;  // host-pinned status words
+  size_t pinned_bytes = 0;
+};
+
+struct YhPlan;  // assembly gather plan (per mesh & format)
+
+struct yafel_hip_matrix_s {
+  yafel_hip_ctx_s *ctx = nullptr;
+  int format = YAFEL_HIP_CSR;
+  int64_t rows = 0, cols = 0;   // scalar dims
+  int64_t brows = 0, bcols = 0; // rows of the stored structure (CSR: = rows)
+  int64_t nblk = 0;             // stored blocks (CSR: scalars)
+  int64_t ptr_len = 0;          // reference host row_ptr length
+  yh::DBuf ptr, col;            // int32 row pointer (brows+1) and block columns
+  yh::DBuf val64, val32;        // CSR: nnz; BCSR3: 9 arrays of nblk (entry-major, SoA)
+  yh::DBuf diag64, dinv64, dinv32;
+  bool has64 = false, has32 = false, diag_ok = false;
+  void *mesh = nullptr;         // mesh this pattern came from (assembly)
+  YhPlan *plan = nullptr;
+  int lanes = 8;                // lanes per row chosen for the CSR kernel
+  // reference-layout extras: the host row_ptr may be longer than brows+1
+  std::vector<int64_t> ptr_tail;
+};
+
+namespace yh {
+
+// ------------------------------------------------------- launch helpers
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+// grid size for grid-stride kernels: a multiple of 8 (one share per XCD)
+int grid_for(yafel_hip_ctx_s *ctx, int64_t work_items, int block);
+
+// Kernels and host launchers implemented across the .hip files.
+void spmv_launch(yafel_hip_matrix_s *A, int precision, const void *x, void *y, hipStream_t s);
+void spmm_launch(yafel_hip_matrix_s *A, int precision, int k, const void *X, void *Y,
+                 hipStream_t s);
+void matrix_build_fp32(yafel_hip_matrix_s *A);
+void matrix_build_diag(yafel_hip_matrix_s *A);
+void matrix_choose_lanes(yafel_hip_matrix_s *A);
+
+}  // namespace yh
+
+// ----------------------------------------------------------------- device
+namespace yh {
+namespace dev {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <int L, class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum (blockDim.x a multiple of 64, at most 1024); result valid in
+// all threads.  Fixed combination order -> deterministic.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *lds /* >= NT/64 */) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  double t = 0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) t += lds[i];
+  return t;
+}
+
+// XCD-aware logical block id: consecutive logical ids land on one XCD
+// (hardware dispatches workgroups round-robin over the 8 XCDs).
+__device__ __forceinline__ int xcd_block() {
+  const int G = gridDim.x, b = blockIdx.x;
+  if ((G & 7) != 0) return b;
+  return (b & 7) * (G >> 3) + (b >> 3);
+}
+
+// Signals completion of this block's partial; returns true in the last block
+// to finish (all partials are then visible).  counter must start at 0 and is
+// reset by the last block.
+__device__ __forceinline__ bool last_block_done(unsigned int *counter) {
+  __shared__ bool is_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int prev = atomicAdd(counter, 1u);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (is_last) __threadfence();
+  return is_last;
+}
+
+// Deterministic sum of npart partials (fixed order), executed by one block.
+template <int NT>
+__device__ __forceinline__ double reduce_partials(const double *part, int npart, int stride,
+                                                  double *lds) {
+  double s = 0;
+  for (int i = threadIdx.x; i < npart; i += NT) s += part[(size_t)i * stride];
+  return block_sum<NT>(s, lds);
+}
+
+}  // namespace dev
+}  // namespace yh
