@@ -273,6 +273,10 @@ class Reference:
         L.ref_tensor_rule.restype = C.c_int
         L.ref_tensor_rule.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P]
         L.ref_rand_mod.argtypes = [C.c_uint, C.c_int, C.c_int, _P]
+        L.ref_matrix_from_arrays.restype = _P
+        L.ref_matrix_from_arrays.argtypes = [C.c_int, _I64, _P, _P, _P]
+        L.ref_cg_iterations.restype = _D
+        L.ref_cg_iterations.argtypes = [_P, _P, C.c_int]
 
     @staticmethod
     def available(fast=False):
@@ -282,6 +286,12 @@ class Reference:
         fn = self.lib.ref_csr_from_triplets if block == 1 else self.lib.ref_bcsr3_from_triplets
         h = fn(len(rows), _ptr(np.ascontiguousarray(rows, np.int64)),
                _ptr(np.ascontiguousarray(cols, np.int64)), _ptr(np.ascontiguousarray(vals, np.float64)))
+        return RMatrix(self, h, block)
+
+    def matrix_from_arrays(self, block, nrows, ptr, col, val):
+        h = self.lib.ref_matrix_from_arrays(block, nrows, _ptr(np.ascontiguousarray(ptr, np.int64)),
+                                            _ptr(np.ascontiguousarray(col, np.int64)),
+                                            _ptr(np.ascontiguousarray(val, np.float64)))
         return RMatrix(self, h, block)
 
     def quadrature_1d(self, qtype, n):
@@ -349,3 +359,6 @@ class RMatrix:
         x = np.zeros_like(b)
         self.r.lib.ref_bicgstab_solve(self.h, _ptr(b), _ptr(x0), _ptr(x))
         return x
+
+    def cg_iterations(self, b, k):
+        return self.r.lib.ref_cg_iterations(self.h, _ptr(np.ascontiguousarray(b, np.float64)), k)

@@ -146,6 +146,55 @@ void ref_bicgstab_solve(void *hp, const double *b, const double *x0, double *x) 
   else out(bicgstab_solve(*h->bcsr, vec(n, b), vec(n, x0)), x);
 }
 
+/* A sparse_bcsr<3> / sparse_csr whose public arrays are filled from caller
+ * arrays (the dimension bookkeeping comes from two zero triplets at the
+ * corners), for timing the reference kernels on a matrix assembled elsewhere. */
+void *ref_matrix_from_arrays(int block, int64_t nrows, const int64_t *ptr, const int64_t *col,
+                             const double *val) {
+  std::vector<std::tuple<std::size_t, std::size_t, double>> ts = {
+      std::make_tuple((std::size_t)0, (std::size_t)0, 0.0),
+      std::make_tuple((std::size_t)(nrows - 1), (std::size_t)(nrows - 1), 0.0)};
+  auto *h = new Handle{block};
+  const int64_t nr = block == 1 ? nrows : nrows / 3;
+  const int64_t nnz = ptr[nr];
+  if (block == 1) {
+    h->csr = new CSR(ts);
+    h->csr->row_ptr.assign(ptr, ptr + nr + 1);
+    h->csr->col_index.assign(col, col + nnz);
+    h->csr->_data.assign(val, val + nnz);
+  } else {
+    h->bcsr = new BCSR(ts);
+    h->bcsr->brow_ptr.assign(ptr, ptr + nr + 1);
+    h->bcsr->bcol_index.assign(col, col + nnz);
+    h->bcsr->data.assign(val, val + nnz * 9);
+  }
+  return h;
+}
+
+/* k iterations of cg_solve's loop body (cg_solve.hpp:56-77) with the
+ * reference's own Vector / operator* / dot / expression templates, for a
+ * bounded CPU timing sample (cg_solve itself only stops on its tolerance). */
+double ref_cg_iterations(void *hp, const double *b, int k) {
+  auto *h = (Handle *)hp;
+  int64_t n = nrows(h);
+  Vector<double> x(n, 0.0);
+  Vector<double> r = vec(n, b);
+  Vector<double> p(r);
+  double rr_old = r.dot(r);
+  for (int it = 0; it < k; ++it) {
+    Vector<double> Ap = h->block == 1 ? (*h->csr) * p : (*h->bcsr) * p;
+    double alpha = rr_old / p.dot(Ap);
+    x += p * alpha;
+    r += -alpha * Ap;
+    double rr_new = r.dot(r);
+    double beta = rr_new / rr_old;
+    rr_old = rr_new;
+    p *= beta;
+    p += r;
+  }
+  return rr_old;
+}
+
 /* glibc rand() stream after srand(seed), as the reference's tests draw it
  * (test/test_cg_solve.cpp:16-27: rhs(i) = rand()%100 after srand(0)). */
 void ref_rand_mod(unsigned seed, int n, int mod, double *out_v) {

@@ -1,0 +1,349 @@
+"""GPU parity: the HIP library through its C ABI against the oracle and the
+golden fixtures written from the reference's own sources.
+
+Tolerances: integer/index work (patterns, numbering, triplet sums of integer
+values) is bit-exact; assembled FP64 values within 1e-12 relative; SpMV within
+1e-13 relative (summation order differs); solutions at the stated relative
+residual."""
+import numpy as np
+import pytest
+
+from oracle.oracle import physics as ophysics
+
+pytestmark = pytest.mark.gpu
+
+LAM, MU = 0.3 / (1.3 * 0.4), 1 / 2.6
+
+
+@pytest.fixture(scope="module")
+def H():
+    from yafel_amd import hip
+    return hip
+
+
+def edge_mask(dn, dim):
+    m = np.zeros(len(dn), bool)
+    for d in range(dim):
+        m |= (np.abs(dn[:, d]) < 1e-6) | (np.abs(dn[:, d] - 1) < 1e-6)
+    return m.astype(np.uint8)
+
+
+# ------------------------------------------------------------------ KATs
+def test_kat_identity_cg_exact(H, ctx, golden):
+    k = golden("kat_solvers.npz")
+    A = H.Matrix.from_triplets(ctx, np.arange(10), np.arange(10), np.ones(10))
+    x, info = A.solve_host(k["id_rhs"], method=H.CG)
+    assert np.array_equal(x, k["id_rhs"])       # test/test_cg_solve.cpp:29: res == rhs
+    x, info = A.solve_host(k["id_rhs"], method=H.BICGSTAB)
+    assert np.array_equal(x, k["id_bicgstab_x"])
+
+
+@pytest.mark.parametrize("method", ["CG", "BICGSTAB", "PCG_JACOBI"])
+def test_kat_tridiag(H, ctx, golden, method):
+    k = golden("kat_solvers.npz")
+    A = H.Matrix.from_triplets(ctx, k["tri_rows"], k["tri_cols"], k["tri_vals"])
+    x, info = A.solve_host(k["tri_b"], method=getattr(H, method))
+    assert ((x - np.arange(10)) ** 2).sum() < 1e-10    # test/test_cg_solve.cpp:58-61
+    ref = k["tri_bicgstab_x"] if method == "BICGSTAB" else k["tri_cg_x"]
+    assert np.abs(x - ref).max() < 1e-12
+
+
+def test_kat_spmv_identity(H, ctx, golden):
+    k = golden("kat_solvers.npz")
+    u = np.arange(1, 31, dtype=float)
+    B = H.Matrix.from_triplets(ctx, np.arange(30), np.arange(30), np.ones(30), block=3)
+    assert np.array_equal(B.spmv_host(u), k["bcsr_id_y"])
+    A = H.Matrix.from_triplets(ctx, np.arange(10), np.arange(10), np.ones(10))
+    assert np.array_equal(A.spmv_host(u[:10]), k["csr_id_y"])
+
+
+# --------------------------------------------------------- sparse construction
+@pytest.mark.parametrize("tag", ["q1", "q2", "q4", "hex", "rnd"])
+def test_csr_from_triplets_matches_reference(H, ctx, golden, tag):
+    s = golden("sparse_build.npz")
+    A = H.Matrix.from_triplets(ctx, s[f"{tag}_rows"], s[f"{tag}_cols"], s[f"{tag}_vals"])
+    ptr, col, val = A.arrays()
+    assert np.array_equal(ptr, s[f"{tag}_ptr"]) and np.array_equal(col, s[f"{tag}_col"])
+    assert np.array_equal(val, s[f"{tag}_val"])       # (row, col, value) order: bit-exact
+
+
+def test_bcsr_from_triplets_matches_reference(H, ctx, golden):
+    s = golden("sparse_build.npz")
+    A = H.Matrix.from_triplets(ctx, s["hex_rows"], s["hex_cols"], s["hex_vals"], block=3)
+    ptr, col, val = A.arrays()
+    assert np.array_equal(ptr, s["hex_bptr"]) and np.array_equal(col, s["hex_bcol"])
+    assert np.abs(val - s["hex_bval"]).max() <= 1e-15 * np.abs(val).max()
+
+
+@pytest.mark.parametrize("blk,tag", [(1, "dup_csr"), (3, "dup_bcsr")])
+def test_duplicate_accumulation(H, ctx, golden, blk, tag):
+    s = golden("sparse_build.npz")
+    N = 60
+    A = H.Matrix.from_triplets(ctx, np.array([N - 1, 0] * N), np.array([N - 1, 0] * N), np.ones(2 * N),
+                               block=blk)
+    ptr, col, val = A.arrays()
+    assert np.array_equal(ptr, s[f"{tag}_ptr"]) and np.array_equal(val, s[f"{tag}_val"])
+    assert A.info().rows == s[f"{tag}_dims"][0] and A.info().cols == s[f"{tag}_dims"][1]
+
+
+# -------------------------------------------------------------------- SpMV
+@pytest.mark.parametrize("tag,blk", [("q1", 1), ("q2", 1), ("q4", 1), ("hex", 1), ("hex", 3)])
+def test_spmv_matches_oracle(H, ctx, golden, orc, tag, blk):
+    s = golden("sparse_build.npz")
+    r, c, v = s[f"{tag}_rows"], s[f"{tag}_cols"], s[f"{tag}_vals"]
+    A = H.Matrix.from_triplets(ctx, r, c, v, block=blk, precisions=H.FP64 | H.FP32)
+    O = orc.csr(r, c, v, block=blk)
+    x = np.random.default_rng(3).standard_normal(O.rows)
+    y = A.spmv_host(x)
+    yo = O.spmv(x)
+    assert np.abs(y - yo).max() <= 1e-13 * np.abs(yo).max()
+    xd, yd = ctx.upload(x.astype(np.float32)), ctx.alloc(O.rows, np.float32)
+    A.spmv(xd, yd, precision=H.FP32)
+    assert np.abs(yd.download() - yo).max() <= 2e-6 * np.abs(yo).max()
+
+
+def test_spmm_matches_columnwise_spmv(H, ctx, golden):
+    s = golden("sparse_build.npz")
+    for blk in (1, 3):
+        A = H.Matrix.from_triplets(ctx, s["hex_rows"], s["hex_cols"], s["hex_vals"], block=blk)
+        n = A.info().rows
+        k = 5
+        X = np.random.default_rng(5).standard_normal((n, k))
+        Xd, Yd = ctx.upload(X.ravel()), ctx.alloc(n * k)
+        A.spmm(Xd, Yd, k)
+        Y = Yd.download().reshape(n, k)
+        for j in range(k):
+            yj = A.spmv_host(X[:, j])
+            assert np.abs(Y[:, j] - yj).max() <= 1e-13 * np.abs(yj).max()
+
+
+def test_dot_matches_oracle(H, ctx, orc):
+    rng = np.random.default_rng(9)
+    a, b = rng.standard_normal(100003), rng.standard_normal(100003)
+    assert abs(ctx.dot(ctx.upload(a), ctx.upload(b)) - orc.dot(a, b)) < 1e-12
+
+
+# -------------------------------------------------------- DoF numbering
+@pytest.mark.parametrize("dim,p,n", [(2, 1, 8), (2, 2, 5), (2, 3, 3), (2, 4, 3), (3, 1, 4), (3, 2, 2)])
+def test_dofmanager_matches_oracle(H, ctx, orc, dim, p, n):
+    nodes, cells = H.structured_mesh(dim, n, n, n if dim == 3 else 0, p)
+    el_o, dn_o = orc.dofmanager(dim, p, nodes, cells)
+    m = H.Mesh(ctx, dim, p, 1, nodes, cells)
+    el, dn = m.download()
+    assert np.array_equal(el, el_o)
+    assert np.abs(dn - dn_o).max() <= 1e-15
+
+
+# ------------------------------------------------------------ assembly
+CASES = [
+    # tag, dim, p, dpn, n, fmt
+    ("poisson_q1", 2, 1, 1, 8, "CSR"),
+    ("poisson_q2", 2, 2, 1, 5, "CSR"),
+    ("poisson_q3", 2, 3, 1, 3, "CSR"),
+    ("poisson_q4", 2, 4, 1, 4, "CSR"),
+    ("elast_hex", 3, 1, 3, 4, "BCSR3"),
+    ("elast_hex_csr", 3, 1, 3, 3, "CSR"),
+    ("elast_hex2", 3, 2, 3, 2, "BCSR3"),
+    ("supg_q2", 2, 2, 1, 4, "CSR"),
+]
+
+
+def case_physics(H, tag, dim, nel):
+    coef = np.linspace(0.5, 2.0, nel)
+    if tag.startswith("poisson"):
+        return H.physics(H.POISSON, dim, coef=coef, f=2.0), ophysics(0, dim, coef=coef, f=2.0)
+    if tag.startswith("elast"):
+        kw = dict(lam=LAM, mu=MU, body=(0.1, 0.2, -1.0))
+        return H.physics(H.ELASTICITY, dim, coef=coef, **kw), ophysics(1, dim, coef=coef, **kw)
+    kw = dict(U=(0.866, 0.5, 0.0), kdiff=1e-3, h=0.25, alpha=0.9)
+    return H.physics(H.SUPG, dim, **kw), ophysics(2, dim, **kw)
+
+
+@pytest.mark.parametrize("tag,dim,p,dpn,n,fmt", CASES)
+def test_assembly_pattern_and_values(H, ctx, orc, tag, dim, p, dpn, n, fmt):
+    m = H.Mesh.structured(ctx, dim, p, dpn, n)
+    A = m.pattern(getattr(H, fmt))
+    ph, oph = case_physics(H, tag, dim, m.nel)
+    R = ctx.upload(np.zeros(m.ndofs))
+    m.assemble(ph, A, R)
+    el, dn = orc.mesh(dim, p, n)
+    rows, cols, vals, Ro = orc.assemble(dim, p, dpn, dn, el, oph)
+    O = orc.csr(rows, cols, vals, block=3 if fmt == "BCSR3" else 1)
+    ptr, col, val = A.arrays()
+    optr, ocol, oval = O.arrays()
+    assert np.array_equal(ptr, optr) and np.array_equal(col, ocol)      # pattern: bit-exact
+    assert np.abs(val - oval).max() <= 1e-12 * np.abs(oval).max()
+    assert np.abs(R.download() - Ro).max() <= 1e-12 * max(np.abs(Ro).max(), 1e-300)
+    # element matrices against the oracle
+    Ke = m.element_matrices(ph)
+    for e in (0, m.nel // 2, m.nel - 1):
+        Ko, _ = orc.element_matrix(dim, p, dpn, dn, el, e, oph)
+        assert np.abs(Ke[e] - Ko).max() <= 1e-12 * np.abs(Ko).max()
+
+
+@pytest.mark.parametrize("fmt", ["CSR", "BCSR3"])
+def test_dirichlet_matches_oracle(H, ctx, orc, fmt):
+    dim, p, dpn, n = 3, 1, 3, 3
+    m = H.Mesh.structured(ctx, dim, p, dpn, n)
+    A = m.pattern(getattr(H, fmt))
+    ph, oph = case_physics(H, "elast", dim, m.nel)
+    R = ctx.upload(np.zeros(m.ndofs))
+    m.assemble(ph, A, R)
+    mask = ctx.upload(np.zeros(m.ndofs, np.uint8))
+    m.bc_mask_plane(mask, 0, 0.0)
+    m.bc_mask_plane(mask, 2, 1.0, comp=1)
+    g = np.random.default_rng(2).standard_normal(m.ndofs)
+    A.dirichlet(mask, R, ctx.upload(g))
+    el, dn = orc.mesh(dim, p, n)
+    rows, cols, vals, Ro = orc.assemble(dim, p, dpn, dn, el, oph)
+    O = orc.csr(rows, cols, vals, block=3 if fmt == "BCSR3" else 1)
+    hmask = mask.download()
+    om = np.zeros(m.ndofs, np.uint8)
+    om[np.repeat(np.abs(dn[:, 0]) < 1e-6, 3)] = 1
+    om[1::3][np.abs(dn[:, 2] - 1) < 1e-6] = 1
+    assert np.array_equal(hmask, om)
+    bo = O.dirichlet(Ro, om, np.where(om == 1, g, 0.0))
+    _, _, val = A.arrays()
+    _, _, oval = O.arrays()
+    assert np.abs(val - oval).max() <= 1e-12 * np.abs(oval).max()
+    assert np.abs(R.download() - bo).max() <= 1e-12 * np.abs(bo).max()
+
+
+# --------------------------------------------------------------- solves
+def assembled_system(H, ctx, orc, dim, p, dpn, n, oph, hph, fmt, mask_fn):
+    m = H.Mesh.structured(ctx, dim, p, dpn, n)
+    A = m.pattern(fmt, H.FP64 | H.FP32)
+    R = ctx.upload(np.zeros(m.ndofs))
+    m.assemble(hph, A, R)
+    el, dn = orc.mesh(dim, p, n)
+    mask = mask_fn(dn)
+    A.dirichlet(ctx.upload(mask), R)
+    rows, cols, vals, Ro = orc.assemble(dim, p, dpn, dn, el, oph)
+    O = orc.csr(rows, cols, vals, block=3 if fmt == H.BCSR3 else 1)
+    bo = O.dirichlet(Ro, mask)
+    return m, A, R, O, bo
+
+
+def test_poisson_c1_pcg_reference_criterion(H, ctx, orc):
+    """c1 at full size: Q1 128x128, f = 1, Dirichlet on all edges, Jacobi-PCG."""
+    ph, oph = H.physics(H.POISSON, 2, f=1.0), ophysics(0, 2, f=1.0)
+    m, A, R, O, bo = assembled_system(H, ctx, orc, 2, 1, 1, 128, oph, ph, H.CSR,
+                                      lambda dn: edge_mask(dn, 2))
+    b = R.download()
+    assert np.abs(b - bo).max() <= 1e-12 * np.abs(bo).max()
+    ptr, col, _ = A.arrays()
+    optr, ocol, _ = O.arrays()
+    assert np.array_equal(ptr, optr) and np.array_equal(col, ocol)
+    assert len(col) == 148225                      # nnz of the 128^2 Q1 pattern
+    x, info = A.solve_host(b, method=H.PCG_JACOBI, tol=1e-20)
+    xo, ito = O.pcg_jacobi(bo, tol=1e-20)
+    assert abs(info.iterations - ito) <= 2
+    assert info.rel_residual < 1e-10
+    assert np.abs(x - xo).max() <= 1e-8 * np.abs(xo).max()
+    x, info = A.solve_host(b, method=H.PCG_JACOBI, stop=H.STOP_RELRES, tol=1e-10)
+    assert info.status == 0 and info.rel_residual < 1e-10
+
+
+@pytest.mark.parametrize("fmt", ["BCSR3", "CSR"])
+@pytest.mark.parametrize("prec", ["FP64", "MIXED"])
+@pytest.mark.parametrize("method", ["CG", "PCG_JACOBI"])
+def test_elasticity_solve_matches_oracle(H, ctx, orc, fmt, prec, method):
+    kw = dict(lam=LAM, mu=MU, body=(0.0, 0.0, -1.0))
+    ph, oph = H.physics(H.ELASTICITY, 3, **kw), ophysics(1, 3, **kw)
+    m, A, R, O, bo = assembled_system(H, ctx, orc, 3, 1, 3, 10, oph, ph, getattr(H, fmt),
+                                      lambda dn: np.repeat(np.abs(dn[:, 0]) < 1e-6, 3).astype(np.uint8))
+    x, info = A.solve_host(R.download(), method=getattr(H, method), precision=getattr(H, prec),
+                           stop=H.STOP_RELRES, tol=1e-8)
+    assert info.status == 0 and info.rel_residual < 1e-8
+    xo, _ = O.cg(bo, tol=1e-24)
+    assert np.abs(x - xo).max() <= 1e-6 * np.abs(xo).max()
+    if prec == "MIXED":
+        assert info.refinements >= 1
+
+
+def test_variable_coefficient_q4(H, ctx, orc):
+    """c3 shape at reduced size: Q4, k = exp(g) per element, N(0,1) rhs."""
+    n, p = 24, 4
+    coef = np.exp(H.normal(11, n * n, 0.0, 0.5))
+    ph, oph = H.physics(H.POISSON, 2, coef=coef), ophysics(0, 2, coef=coef)
+    m = H.Mesh.structured(ctx, 2, p, 1, n)
+    A = m.pattern(H.CSR, H.FP64 | H.FP32)
+    m.assemble(ph, A, None, H.TANGENT)
+    el, dn = orc.mesh(2, p, n)
+    rows, cols, vals, _ = orc.assemble(2, p, 1, dn, el, oph)
+    O = orc.csr(rows, cols, vals)
+    rhs = H.normal(12, m.ndofs)
+    mask = edge_mask(dn, 2)
+    Rd = ctx.upload(rhs)
+    A.dirichlet(ctx.upload(mask), Rd)
+    bo = O.dirichlet(rhs, mask)
+    _, _, val = A.arrays()
+    _, _, oval = O.arrays()
+    assert np.abs(val - oval).max() <= 1e-12 * np.abs(oval).max()
+    for prec in ("FP64", "MIXED"):
+        x, info = A.solve_host(Rd.download(), method=H.CG, precision=getattr(H, prec), stop=H.STOP_RELRES,
+                               tol=1e-8)
+        assert info.status == 0 and info.rel_residual < 1e-8
+        r = bo - O.spmv(x)
+        assert np.linalg.norm(r) / np.linalg.norm(bo) < 1e-8
+
+
+@pytest.mark.parametrize("prec", ["FP64", "MIXED"])
+def test_supg_bicgstab_matches_oracle(H, ctx, orc, prec):
+    n = 16
+    kw = dict(U=(0.8660254037844387, 0.5, 0.0), kdiff=1.0 / n / 200, h=1.0 / n, alpha=0.99)
+    ph, oph = H.physics(H.SUPG, 2, **kw), ophysics(2, 2, **kw)
+    m = H.Mesh.structured(ctx, 2, 2, 1, n)
+    A = m.pattern(H.CSR, H.FP64 | H.FP32)
+    m.assemble(ph, A, None, H.TANGENT)
+    el, dn = orc.mesh(2, 2, n)
+    rows, cols, vals, _ = orc.assemble(2, 2, 1, dn, el, oph)
+    O = orc.csr(rows, cols, vals)
+    rhs = H.normal(77, m.ndofs)
+    mask = edge_mask(dn, 2)
+    Rd = ctx.upload(rhs)
+    A.dirichlet(ctx.upload(mask), Rd)
+    bo = O.dirichlet(rhs, mask)
+    x, info = A.solve_host(Rd.download(), method=H.BICGSTAB_JACOBI, precision=getattr(H, prec),
+                           stop=H.STOP_RELRES, tol=1e-8)
+    assert info.status == 0 and info.rel_residual < 1e-8
+    xo, _ = O.bicgstab_jacobi(bo, 1e-12, 100000)
+    assert np.abs(x - xo).max() <= 1e-5 * np.abs(xo).max()
+
+
+def test_solver_edge_cases(H, ctx):
+    A = H.Matrix.from_triplets(ctx, np.arange(5), np.arange(5), np.full(5, 2.0))
+    # zero rhs: the reference returns x0 (cg) -- here x0 = 0
+    x, info = A.solve_host(np.zeros(5), method=H.CG)
+    assert np.array_equal(x, np.zeros(5))
+    # bicgstab returns rhs when ||b||^2 < TOL (bicgstab_solve.hpp:43-45)
+    b = np.full(5, 1e-9)
+    x, info = A.solve_host(b, method=H.BICGSTAB)
+    assert np.array_equal(x, b)
+    # maxiter exhaustion is reported, not silent
+    T = H.Matrix.from_triplets(ctx, *tridiag_arrays(50))
+    x, info = T.solve_host(np.ones(50), method=H.CG, max_iter=3, tol=1e-30)
+    assert info.status == 1 and info.iterations == 3
+    # zero diagonal: Jacobi stores 1/0 (the reference only warns) -> NaN reported
+    Z = H.Matrix.from_triplets(ctx, np.array([0, 0, 1, 1]), np.array([0, 1, 0, 1]),
+                               np.array([0.0, 1.0, 1.0, 0.0]))
+    x, info = Z.solve_host(np.ones(2), method=H.PCG_JACOBI)
+    assert info.status in (2, 3)
+
+
+def tridiag_arrays(n):
+    r = list(range(n)) + list(range(1, n)) + list(range(n - 1))
+    c = list(range(n)) + list(range(n - 1)) + list(range(1, n))
+    v = [-2.0] * n + [1.0] * (2 * n - 2)
+    return np.array(r), np.array(c), np.array(v)
+
+
+def test_error_paths(H, ctx):
+    with pytest.raises(H.YafelHipError):
+        H.Matrix.csr(ctx, 2, 2, np.array([0, 1, 3]), np.array([0, 5, 1]), np.ones(3))   # col out of range
+    m = H.Mesh.structured(ctx, 2, 1, 1, 4)
+    with pytest.raises(H.YafelHipError):
+        m.pattern(H.BCSR3)                                    # needs 3 dofs per node
+    A = H.Matrix.from_triplets(ctx, np.arange(3), np.arange(3), np.ones(3))
+    with pytest.raises(H.YafelHipError):
+        m.assemble(H.physics(H.POISSON, 2, f=1.0), A, None, H.TANGENT)   # not this mesh's pattern

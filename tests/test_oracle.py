@@ -1,0 +1,364 @@
+"""CPU checks that pin the oracle: golden vectors from the reference's own
+sources, closed forms, invariants and brute force on tiny inputs."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import physics
+
+LAM, MU = 0.3 / (1.3 * 0.4), 1 / 2.6
+
+
+# ---------------------------------------------------------------- quadrature
+@pytest.mark.parametrize("npts", [2, 3, 4, 5, 6])
+def test_quadrature_1d_matches_reference(orc, golden, npts):
+    q = golden("quadrature.npz")
+    x, w = orc.gauss_legendre(npts)
+    assert np.array_equal(x, q[f"gl{npts}_x"]) and np.array_equal(w, q[f"gl{npts}_w"])
+    x, w = orc.gauss_lobatto(npts)
+    assert np.array_equal(x, q[f"gll{npts}_x"]) and np.array_equal(w, q[f"gll{npts}_w"])
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+def test_tensor_rule_matches_reference(orc, golden, dim, order):
+    q = golden("quadrature.npz")
+    xi, w = orc.tensor_rule(0, dim, order)
+    assert np.array_equal(xi, q[f"tp{dim}_{order}_xi"]) and np.array_equal(w, q[f"tp{dim}_{order}_w"])
+
+
+@pytest.mark.parametrize("npts", [2, 3, 4, 5])
+def test_quadrature_invariants(orc, npts):
+    # weights sum to 2 (test/test_GaussLegQuadrature.cpp:10-17) and exactness
+    x, w = orc.gauss_legendre(npts)
+    assert abs(w.sum() - 2) < 1e-14
+    xr, wr = np.polynomial.legendre.leggauss(npts)
+    assert np.allclose(x, xr, atol=1e-14) and np.allclose(w, wr, atol=1e-14)
+    for k in range(2 * npts):
+        exact = (1 - (-1) ** (k + 1)) / (k + 1)
+        assert abs((w * x ** k).sum() - exact) < 1e-13
+    xl, wl = orc.gauss_lobatto(npts)
+    assert xl[0] == -1 and xl[-1] == 1 and abs(wl.sum() - 2) < 1e-14
+    for k in range(2 * npts - 2):
+        exact = (1 - (-1) ** (k + 1)) / (k + 1)
+        assert abs((wl * xl ** k).sum() - exact) < 1e-13
+
+
+# ------------------------------------------------------------ shape functions
+def lagrange_1d(nodes, t):
+    """textbook Lagrange basis values/derivatives at points t (independent of the oracle)"""
+    n = len(nodes)
+    V = np.ones((len(t), n))
+    D = np.zeros((len(t), n))
+    for i in range(n):
+        others = [m for m in range(n) if m != i]
+        den = np.prod([nodes[i] - nodes[m] for m in others])
+        V[:, i] = np.prod([t - nodes[m] for m in others], axis=0) / den
+        for k in others:
+            rest = [m for m in others if m != k]
+            D[:, i] += np.prod([t - nodes[m] for m in rest], axis=0) / den if rest else 1 / den
+    return V, D
+
+
+@pytest.mark.parametrize("dim,p", [(2, 1), (2, 2), (2, 3), (2, 4), (3, 1), (3, 2)])
+def test_shape_tables_closed_form(orc, dim, p):
+    N, dN, w, xi = orc.shape_tables(dim, p)
+    gll, _ = orc.gauss_lobatto(p + 1)
+    n1 = p + 1
+    nen = n1 ** dim
+    idx = np.array([[(a // n1 ** d) % n1 for d in range(dim)] for a in range(nen)])
+    Nc = np.ones_like(N)
+    dNc = np.ones_like(dN)
+    for d in range(dim):
+        V, D = lagrange_1d(gll, xi[:, d])
+        Nc *= V[:, idx[:, d]]
+        for g in range(dim):
+            dNc[:, :, g] *= (D if g == d else V)[:, idx[:, d]]
+    assert np.abs(N - Nc).max() < 1e-13
+    assert np.abs(dN - dNc).max() < 1e-12
+    assert np.abs(N.sum(axis=1) - 1).max() < 1e-13          # partition of unity
+    assert np.abs(dN.sum(axis=1)).max() < 1e-12            # gradients sum to zero
+    assert abs(w.sum() - 2 ** dim) < 1e-13
+
+
+# ------------------------------------------------------------ DoF numbering
+@pytest.mark.parametrize("dim,p,n", [(2, 2, 3), (2, 4, 2), (3, 2, 2), (2, 3, 4)])
+def test_dof_numbering_high_order_is_row_major_gll_grid(orc, dim, p, n):
+    # p >= 2: the (z, y, x) coordinate sort numbers the (p n + 1)^d grid x fastest
+    el, dn = orc.mesh(dim, p, n)
+    m = p * n + 1
+    assert dn.shape[0] == m ** dim
+    gll, _ = orc.gauss_lobatto(p + 1)
+    pts1 = np.unique(np.round(np.concatenate([(c + (gll + 1) / 2) / n for c in range(n)]), 12))
+    assert len(pts1) == m
+    grid = np.stack(np.meshgrid(*([pts1] * dim), indexing="ij")[::-1], axis=-1).reshape(-1, dim)
+    assert np.abs(dn[:, :dim] - grid).max() < 1e-12   # grid was rounded to 12 digits
+    # brute force: every element's local node (x fastest) sits at its GLL position
+    xl = orc.local_nodes(dim, p)
+    for e in range(el.shape[0]):
+        cell = np.unravel_index(e, [n] * dim, order="F")
+        for a in range(el.shape[1]):
+            pos = [(cell[d] + (xl[a, d] + 1) / 2) / n for d in range(dim)]
+            assert np.abs(dn[el[e, a], :dim] - pos).max() < 1e-14
+
+
+def test_dof_numbering_p1_passthrough(orc):
+    nodes, cells = orc.structured_mesh(2, 4, 3, 0, 0)
+    el, dn = orc.dofmanager(2, 1, nodes, cells)
+    assert np.array_equal(el, cells) and np.array_equal(dn, nodes)
+
+
+# ------------------------------------------------------------ element matrices
+def gll_1d_mats(orc, p, h):
+    """1-D matrices on [0,h] of the GLL Lagrange basis, by high-order numpy Gauss."""
+    gll, _ = orc.gauss_lobatto(p + 1)
+    t, wt = np.polynomial.legendre.leggauss(20)
+    V, D = lagrange_1d(gll, t)
+    M = (V.T * wt) @ V * (h / 2)
+    A = (D.T * wt) @ D * (2 / h)
+    C = (D.T * wt) @ V          # int l_a' l_b  (scale (2/h)(h/2) = 1)
+    return M, A, C
+
+
+def kron_all(mats):
+    out = np.ones((1, 1))
+    for m in mats[::-1]:        # x fastest: last factor is x
+        out = np.kron(out, m)
+    return out
+
+
+def grad_integrals(orc, dim, p, h):
+    """I[j][l][A,B] = int d_j N_A d_l N_B over an axis-aligned cube of side h"""
+    M, A, C = gll_1d_mats(orc, p, h)
+    I = [[None] * dim for _ in range(dim)]
+    for j in range(dim):
+        for l in range(dim):
+            fac = []
+            for d in range(dim):
+                if d == j and d == l:
+                    fac.append(A)
+                elif d == j:
+                    fac.append(C)
+                elif d == l:
+                    fac.append(C.T)
+                else:
+                    fac.append(M)
+            I[j][l] = kron_all(fac)
+    return I
+
+
+@pytest.mark.parametrize("p", [1, 2, 4])
+def test_poisson_element_kronecker(orc, p):
+    el, dn = orc.mesh(2, p, 3)
+    coef = np.linspace(0.5, 2.0, el.shape[0])
+    ph = physics(0, 2, coef=coef, f=3.0)
+    I = grad_integrals(orc, 2, p, 1 / 3)
+    M, _, _ = gll_1d_mats(orc, p, 1 / 3)
+    for e in [0, 4, 8]:
+        K, R = orc.element_matrix(2, p, 1, dn, el, e, ph)
+        Kc = -coef[e] * (I[0][0] + I[1][1])
+        assert np.abs(K - Kc).max() < 1e-12 * np.abs(Kc).max()
+        Rc = 3.0 * kron_all([M, M]).sum(axis=1)
+        assert np.abs(R - Rc).max() < 1e-13
+
+
+def test_q1_poisson_stencil(orc):
+    # interior row of the Q1 Laplacian under the -K sign: -8/3 centre, +1/3 neighbours
+    el, dn = orc.mesh(2, 1, 4)
+    rows, cols, vals, R = orc.assemble(2, 1, 1, dn, el, physics(0, 2, f=1.0))
+    A = orc.csr(rows, cols, vals)
+    ptr, col, val = A.arrays()
+    r = 12  # node (2,2)
+    row = dict(zip(col[ptr[r]:ptr[r + 1]], val[ptr[r]:ptr[r + 1]]))
+    assert abs(row[r] + 8 / 3) < 1e-14
+    assert all(abs(v - 1 / 3) < 1e-14 for c, v in row.items() if c != r)
+    assert abs(R[r] - (1 / 4) ** 2) < 1e-15      # load f h^2 per interior node
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_elasticity_element_closed_form(orc, p):
+    el, dn = orc.mesh(3, p, 2)
+    coef = np.array([1.0, 2.0, 0.5, 1.5, 1.0, 1.0, 3.0, 1.0])
+    ph = physics(1, 3, coef=coef, lam=LAM, mu=MU, body=(0.1, -0.2, -1.0))
+    I = grad_integrals(orc, 3, p, 0.5)
+    M, _, _ = gll_1d_mats(orc, p, 0.5)
+    nen = el.shape[1]
+    for e in [0, 6]:
+        K, R = orc.element_matrix(3, p, 3, dn, el, e, ph)
+        lam, mu = coef[e] * LAM, coef[e] * MU
+        tr = sum(I[j][j] for j in range(3))
+        Kc = np.zeros_like(K)
+        for i in range(3):
+            for k in range(3):
+                blk = lam * I[i][k] + mu * I[k][i] + (mu * tr if i == k else 0)
+                Kc[i::3, k::3] = -blk
+        assert np.abs(K - Kc).max() < 1e-12 * np.abs(Kc).max()
+        assert np.abs(K - K.T).max() < 1e-14
+        mass = kron_all([M, M, M]).sum(axis=1)
+        for i, bi in enumerate((0.1, -0.2, -1.0)):
+            assert np.abs(R[i::3] - bi * mass).max() < 1e-14
+        assert nen == (p + 1) ** 3
+
+
+def test_supg_element_closed_form(orc):
+    p, n = 2, 4
+    el, dn = orc.mesh(2, p, n)
+    U = (math.cos(math.pi / 6), math.sin(math.pi / 6), 0)
+    h, kd, al = 1 / n, 0.01, 0.7
+    ph = physics(2, 2, U=U, kdiff=kd, h=h, alpha=al)
+    I = grad_integrals(orc, 2, p, h)
+    M, A1, C = gll_1d_mats(orc, p, h)
+    # int N_A d_c N_B : derivative on N_B only -> C^T factor in direction c
+    NdN = [kron_all([C.T, M]), kron_all([M, C.T])]
+    nU = math.hypot(U[0], U[1])
+    Kt = kd * np.eye(2) + (al * h / (2 * nU)) * np.outer(U[:2], U[:2])
+    K, R = orc.element_matrix(2, p, 1, dn, el, 5, ph)
+    Kc = 2 * (U[0] * NdN[0] + U[1] * NdN[1])      # advection added once per A component
+    for a in range(2):
+        for b in range(2):
+            Kc = Kc + Kt[a, b] * I[a][b]
+    assert np.abs(K - Kc).max() < 1e-12 * np.abs(Kc).max()
+
+
+# ------------------------------------------------------------ sparse build
+@pytest.mark.parametrize("tag", ["q1", "q2", "q4", "hex"])
+def test_csr_from_triplets_matches_reference(orc, golden, tag):
+    s = golden("sparse_build.npz")
+    A = orc.csr(s[f"{tag}_rows"], s[f"{tag}_cols"], s[f"{tag}_vals"])
+    ptr, col, val = A.arrays()
+    assert np.array_equal(ptr, s[f"{tag}_ptr"]) and np.array_equal(col, s[f"{tag}_col"])
+    assert np.array_equal(val, s[f"{tag}_val"])        # same summation order: bit-exact
+
+
+def test_bcsr_from_triplets_matches_reference(orc, golden):
+    s = golden("sparse_build.npz")
+    A = orc.csr(s["hex_rows"], s["hex_cols"], s["hex_vals"], block=3)
+    ptr, col, val = A.arrays()
+    assert np.array_equal(ptr, s["hex_bptr"]) and np.array_equal(col, s["hex_bcol"])
+    assert np.abs(val - s["hex_bval"]).max() <= 1e-15 * np.abs(val).max()
+
+
+@pytest.mark.parametrize("blk,tag", [(1, "dup_csr"), (3, "dup_bcsr")])
+def test_duplicate_accumulation(orc, golden, blk, tag):
+    s = golden("sparse_build.npz")
+    N = 60
+    A = orc.csr(np.array([N - 1, 0] * N), np.array([N - 1, 0] * N), np.ones(2 * N), block=blk)
+    ptr, col, val = A.arrays()
+    assert np.array_equal(ptr, s[f"{tag}_ptr"]) and np.array_equal(val, s[f"{tag}_val"])
+
+
+def test_random_integer_sum(orc, golden):
+    s = golden("sparse_build.npz")
+    A = orc.csr(s["rnd_rows"], s["rnd_cols"], s["rnd_vals"])
+    ptr, col, val = A.arrays()
+    assert np.array_equal(val, s["rnd_val"]) and val.sum() == s["rnd_vals"].sum()
+
+
+def test_pattern_brute_force(orc):
+    el, dn = orc.mesh(2, 2, 3)
+    rows, cols, vals, _ = orc.assemble(2, 2, 1, dn, el, physics(0, 2, f=1.0))
+    A = orc.csr(rows, cols, vals)
+    ptr, col, _ = A.arrays()
+    pairs = sorted({(int(a), int(b)) for e in el for a in e for b in e})
+    got = [(r, int(c)) for r in range(len(ptr) - 1) for c in col[ptr[r]:ptr[r + 1]]]
+    assert got == pairs
+    # closed-form row count: (cells*(p+1)^2 - (cells-1))^d pairs
+    assert len(col) == (3 * 9 - 2) ** 2
+
+
+# ------------------------------------------------------------ SpMV / dot
+def test_spmv_dense_brute_force(orc):
+    rng = np.random.default_rng(0)
+    D = rng.standard_normal((30, 30)) * (rng.random((30, 30)) < 0.2)
+    r, c = np.nonzero(D)
+    A = orc.csr(r, c, D[r, c])
+    x = rng.standard_normal(30)
+    assert np.abs(A.spmv(x)[:A.rows] - (D @ x)[:A.rows]).max() < 1e-13
+    B = orc.csr(r, c, D[r, c], block=3)
+    assert np.abs(B.spmv(x)[:B.rows] - (D @ x)[:B.rows]).max() < 1e-13
+
+
+def test_dot_matches_reference_order(orc):
+    from oracle.oracle import Reference
+    if not Reference.available():
+        pytest.skip("reference harness not built here")
+    rng = np.random.default_rng(1)
+    a, b = rng.standard_normal(1001), rng.standard_normal(1001)
+    assert orc.dot(a, b) == Reference().dot(a, b)
+
+
+# ------------------------------------------------------------ Dirichlet
+def test_dirichlet_closed_form(orc):
+    el, dn = orc.mesh(2, 1, 4)
+    rows, cols, vals, R = orc.assemble(2, 1, 1, dn, el, physics(0, 2, f=1.0))
+    A = orc.csr(rows, cols, vals)
+    ptr0, col0, val0 = A.arrays()
+    mask = (np.abs(dn[:, 0]) < 1e-6).astype(np.uint8)
+    g = np.where(mask, 2.0, 0.0)
+    Ad = np.zeros((25, 25))
+    for r in range(25):
+        Ad[r, col0[ptr0[r]:ptr0[r + 1]]] = val0[ptr0[r]:ptr0[r + 1]]
+    b = A.dirichlet(R, mask, g)
+    ptr, col, val = A.arrays()
+    assert np.array_equal(ptr, ptr0) and np.array_equal(col, col0)   # pattern kept
+    exp_b = R - Ad @ g
+    exp_b[mask == 1] = 2.0
+    assert np.abs(b - exp_b).max() < 1e-14
+    for r in range(25):
+        for k in range(ptr[r], ptr[r + 1]):
+            c = col[k]
+            if mask[r] or mask[c]:
+                assert val[k] == (1.0 if r == c else 0.0)
+            else:
+                assert val[k] == val0[k]
+
+
+# ------------------------------------------------------------ solvers
+def test_solver_kats(orc, golden):
+    k = golden("kat_solvers.npz")
+    I = orc.csr(np.arange(10), np.arange(10), np.ones(10))
+    x, _ = I.cg(k["id_rhs"])
+    assert np.array_equal(x, k["id_rhs"]) and np.array_equal(x, k["id_cg_x"])
+    T = orc.csr(k["tri_rows"], k["tri_cols"], k["tri_vals"])
+    x, _ = T.cg(k["tri_b"])
+    assert np.array_equal(x, k["tri_cg_x"])
+    assert ((x - np.arange(10)) ** 2).sum() < 1e-10
+    x, _ = T.bicgstab(k["tri_b"])
+    assert np.array_equal(x, k["tri_bicgstab_x"])
+
+
+def test_solves_match_reference(orc, golden):
+    v = golden("solves.npz")
+    A = orc.wrap(1, len(v["p_ptr"]) - 1, len(v["p_ptr"]) - 1, v["p_ptr"], v["p_col"], v["p_val"])
+    x, _ = A.cg(v["p_b"], tol=1e-20)
+    assert np.array_equal(x, v["p_cg_x"])
+    x, _ = A.pcg_jacobi(v["p_b"], tol=1e-20)
+    assert np.array_equal(x, v["p_pcg_x"])
+
+
+def test_cg_true_residual_and_monotone_energy(orc):
+    el, dn = orc.mesh(2, 1, 16)
+    rows, cols, vals, R = orc.assemble(2, 1, 1, dn, el, physics(0, 2, f=1.0))
+    A = orc.csr(rows, cols, vals)
+    mask = ((dn[:, 0] < 1e-6) | (dn[:, 0] > 1 - 1e-6) | (dn[:, 1] < 1e-6) | (dn[:, 1] > 1 - 1e-6)).astype(np.uint8)
+    b = A.dirichlet(R, mask)
+    x, it = A.cg(b, tol=1e-20)
+    assert np.linalg.norm(b - A.spmv(x)) / np.linalg.norm(b) < 1e-9
+    assert 0 < it < len(b)
+
+
+def test_bicgstab_jacobi_nonsymmetric(orc):
+    el, dn = orc.mesh(2, 2, 8)
+    ph = physics(2, 2, U=(0.866, 0.5, 0), kdiff=1e-3, h=1 / 8, alpha=0.9)
+    rows, cols, vals, _ = orc.assemble(2, 2, 1, dn, el, ph)
+    A = orc.csr(rows, cols, vals)
+    mask = ((dn[:, 0] < 1e-6) | (dn[:, 0] > 1 - 1e-6) | (dn[:, 1] < 1e-6) | (dn[:, 1] > 1 - 1e-6)).astype(np.uint8)
+    b = A.dirichlet(orc.normal(7, len(dn)), mask)
+    x, it = A.bicgstab_jacobi(b, 1e-10, 10000)
+    assert np.linalg.norm(b - A.spmv(x)) / np.linalg.norm(b) < 1e-10
+
+
+def test_rng_box_muller_moments(orc):
+    z = orc.normal(42, 200001)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01

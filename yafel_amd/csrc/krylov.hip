@@ -18,6 +18,7 @@
 // inner loop pauses, x_hi += x_lo and r = b - A x_hi are formed in FP64 (a
 // residual replacement that keeps the search direction), and the FP64 true
 // residual decides convergence.
+#include <chrono>
 #include <cmath>
 
 #include "internal.hpp"
@@ -767,6 +768,7 @@ struct Solver {
     chunk(1);
     int polled = 0;
     int64_t guard = 0;
+    const auto t_start = std::chrono::steady_clock::now();
     while (true) {
       HIPCHK(hipEventSynchronize(ev[polled & 1]));
       KState hs;
@@ -774,7 +776,11 @@ struct Solver {
       if (hs.done) break;
       chunk(polled & 1);
       ++polled;
-      if (++guard > (o.max_iter / K + 16) * 4 + 1000000) fail(YAFEL_HIP_EDEVICE, "solver: no progress");
+      // every chunk advances at least one iteration or one refinement
+      if (++guard > 2 * o.max_iter + 1000) fail(YAFEL_HIP_EDEVICE, "solver: no progress");
+      if ((guard & 63) == 0 &&
+          std::chrono::steady_clock::now() - t_start > std::chrono::seconds(900))
+        fail(YAFEL_HIP_EDEVICE, "solver: exceeded 900 s");
     }
     HIPCHK(hipStreamSynchronize(s));
     (void)hipEventDestroy(ev[0]);
@@ -788,7 +794,7 @@ struct Solver {
     rep = base(rep, 7);
     rep.xh = x;
     rep.b = b;
-    rep.out = part + 4 * (size_t)ctx->num_cus * 8 * 4;
+    rep.out = part + 4 * (size_t)ctx->num_cus * 8;  // past the NP x grid partials
     launch_spmv_op<double>(A, rep, s);
     double rb[2];
     HIPCHK(hipMemcpyAsync(rb, rep.out, 16, hipMemcpyDeviceToHost, s));

@@ -216,7 +216,9 @@ int yafel_hip_structured_mesh(int dim, int nx, int ny, int nz, int p, double *no
   YH_TRY_END(nullptr)
 }
 
-// splitmix64 (Steele, Lea & Flood, 2014) feeding Box-Muller pairs.
+// splitmix64 (Steele, Lea & Flood, 2014) feeding Box-Muller pairs.  No FMA
+// contraction, so every host produces the same deviates for a seed.
+#pragma clang fp contract(off)
 static inline uint64_t splitmix64(uint64_t &s) {
   uint64_t z = (s += 0x9E3779B97F4A7C15ull);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
