@@ -1,0 +1,238 @@
+// YAFEL's FE layer over the HIP C ABI: Mesh, DoFManager, FESystem,
+// CGAssembly<Physics> and DirichletBC with the reference's names.
+//
+//   DoFManager(M, CG, p, dof_per_node)   src/utils/DoFManager.cpp:11-36 (numbering on the device)
+//   FESystem(dofm)                       include/fe_system/FESystem.hpp:18-56
+//   CGAssembly<Physics>(fesys, reqs)     include/assembly/CGAssembly.hpp:27-186
+//   DirichletBC(dofm, value, comp)       include/boundary_conditions/DirichletBC.hpp:20-89
+//
+// A GPU cannot call a host Physics functor, so a Physics type provides
+//     static yafel_hip_physics device_descriptor();
+// (kind + coefficients) instead of LocalTangent / LocalResidual.  A Physics
+// without it does not compile: there is no CPU fallback.
+#pragma once
+
+#include <functional>
+#include <vector>
+
+#include "yafel_amd/linalg.hpp"
+
+namespace yafel {
+
+struct coordinate {
+  double x[3] = {0, 0, 0};
+  double operator()(int i) const { return x[i]; }
+  double &operator()(int i) { return x[i]; }
+};
+
+// Explicit mesh (geometry nodes + uniform Quad4 / Hex8 cells), the input the
+// reference's Mesh(nodes, cells, offsets, types) constructor takes
+// (src/mesh/Mesh.cpp:11-29).  Corner order: lexicographic for p = 1,
+// gmsh counter-clockwise for p >= 2 (see yafel_hip_mesh_create_cg).
+class Mesh {
+ public:
+  Mesh(int dim, std::vector<double> nodes_xyz, std::vector<int32_t> cells)
+      : dim_(dim), nodes_(std::move(nodes_xyz)), cells_(std::move(cells)) {}
+  // unit square / cube with nx*ny(*nz) cells, corners ordered for polynomial order p
+  static Mesh structured(int dim, int nx, int ny, int nz, int p) {
+    const std::size_t nn = (std::size_t)(nx + 1) * (ny + 1) * (dim == 3 ? nz + 1 : 1);
+    const std::size_t nc = (std::size_t)nx * ny * (dim == 3 ? nz : 1);
+    std::vector<double> nodes(3 * nn);
+    std::vector<int32_t> cells(nc * (dim == 3 ? 8 : 4));
+    hipdetail::check(yafel_hip_structured_mesh(dim, nx, ny, nz, p, nodes.data(), cells.data()));
+    return Mesh(dim, std::move(nodes), std::move(cells));
+  }
+  int dim() const { return dim_; }
+  std::size_t nNodes() const { return nodes_.size() / 3; }
+  std::size_t nCells() const { return cells_.size() / (dim_ == 3 ? 8 : 4); }
+  const std::vector<double> &nodes() const { return nodes_; }
+  const std::vector<int32_t> &cells() const { return cells_; }
+
+ private:
+  int dim_;
+  std::vector<double> nodes_;
+  std::vector<int32_t> cells_;
+};
+
+class DoFManager {
+ public:
+  enum class ManagerType { CG, DG };
+
+  DoFManager(const Mesh &M, ManagerType type, int polyOrder, int dof_per_node)
+      : dof_per_node(dof_per_node), polyOrder(polyOrder) {
+    if (type != ManagerType::CG) throw std::invalid_argument("DoFManager: only CG numbering runs on the device");
+    auto ctx = hipdetail::context();
+    yafel_hip_mesh m = nullptr;
+    hipdetail::check(yafel_hip_mesh_create_cg(ctx, M.dim(), polyOrder, dof_per_node, (int64_t)M.nNodes(),
+                                              M.nodes().data(), (int64_t)M.nCells(), M.cells().data(), &m),
+                     ctx);
+    m_ = std::shared_ptr<yafel_hip_mesh_s>(m, [](yafel_hip_mesh_s *p) { yafel_hip_mesh_destroy(p); });
+    int64_t info[8];
+    hipdetail::check(yafel_hip_mesh_info(m, info));
+    nen_ = (int)info[2];
+    dim_ = (int)info[4];
+    elements.resize((std::size_t)info[1] * nen_);
+    std::vector<double> xyz((std::size_t)info[0] * 3);
+    hipdetail::check(yafel_hip_mesh_download(m, elements.data(), xyz.data()));
+    dof_nodes.resize((std::size_t)info[0]);
+    for (std::size_t i = 0; i < dof_nodes.size(); ++i)
+      for (int d = 0; d < 3; ++d) dof_nodes[i].x[d] = xyz[3 * i + d];
+    element_offsets.resize((std::size_t)info[1] + 1);
+    for (std::size_t e = 0; e < element_offsets.size(); ++e) element_offsets[e] = (int)(e * nen_);
+  }
+
+  int nCells() const { return (int)(element_offsets.size() - 1); }
+  int nsd() const { return dim_; }
+  // DoFManager::getGlobalDofs (DoFManager.cpp:39-52): dof = node * dpn + comp
+  void getGlobalDofs(int elnum, std::vector<int> &c) const {
+    c.clear();
+    for (int i = element_offsets[elnum]; i < element_offsets[elnum + 1]; ++i)
+      for (int d = 0; d < dof_per_node; ++d) c.push_back(elements[i] * dof_per_node + d);
+  }
+  void getGlobalNodes(int elnum, std::vector<int> &c) const {
+    c.assign(elements.begin() + element_offsets[elnum], elements.begin() + element_offsets[elnum + 1]);
+  }
+  yafel_hip_mesh handle() const { return m_.get(); }
+
+  int dof_per_node;
+  int polyOrder;
+  std::vector<int> elements;
+  std::vector<int> element_offsets;
+  std::vector<coordinate> dof_nodes;
+
+ private:
+  std::shared_ptr<yafel_hip_mesh_s> m_;
+  int nen_ = 0, dim_ = 0;
+};
+
+enum class AssemblyRequirement : int { Residual, Tangent, DtMass, LumpedDtMassInverse, DtDtMass };
+
+// Owns the device tangent (the mesh's sparsity pattern), residual and the host
+// solution.  format: YAFEL_HIP_CSR or YAFEL_HIP_BCSR3 (3 dofs per node).
+class FESystem {
+ public:
+  explicit FESystem(DoFManager &dofm, int format = YAFEL_HIP_CSR) : dofm_(dofm) {
+    auto ctx = hipdetail::context();
+    yafel_hip_matrix A = nullptr;
+    hipdetail::check(yafel_hip_mesh_pattern(dofm.handle(), format, YAFEL_HIP_FP64, &A), ctx);
+    A_ = hipdetail::MatrixPtr(A, hipdetail::MatrixDeleter());
+    n_ = dofm.dof_nodes.size() * dofm.dof_per_node;
+    void *r = nullptr;
+    hipdetail::check(yafel_hip_malloc(ctx, sizeof(double) * n_, &r), ctx);
+    R_ = std::shared_ptr<double>((double *)r, [ctx](double *p) { yafel_hip_free(ctx, p); });
+    std::vector<double> z(n_, 0.0);
+    hipdetail::check(yafel_hip_memcpy_h2d(ctx, r, z.data(), sizeof(double) * n_), ctx);
+    solution_ = Vector<double>(n_, 0.0);
+  }
+
+  yafel_hip_matrix tangent_handle() const { return A_.get(); }
+  double *residual_device() const { return R_.get(); }
+  std::size_t ndofs() const { return n_; }
+  DoFManager &getDoFManager() { return dofm_; }
+  double &currentTime() { return time_; }
+
+  // host copies of the device tangent / residual
+  sparse_csr<double> getGlobalTangent() const { return sparse_csr<double>(A_.get()); }
+  sparse_bcsr<3, double> getGlobalTangentBlocks() const { return sparse_bcsr<3, double>(A_.get()); }
+  Vector<double> getGlobalResidual() const {
+    Vector<double> r(n_, 0.0);
+    auto ctx = hipdetail::context();
+    hipdetail::check(yafel_hip_memcpy_d2h(ctx, r.data(), R_.get(), sizeof(double) * n_), ctx);
+    return r;
+  }
+  Vector<double> &getSolution() { return solution_; }
+
+  // solve tangent * U = residual on the device (b stays resident)
+  yafel_hip_solve_info solve(int method = YAFEL_HIP_PCG_JACOBI, int precision = YAFEL_HIP_FP64,
+                             double tol = 1e-8) {
+    auto ctx = hipdetail::context();
+    yafel_hip_solve_opts o;
+    yafel_hip_solve_opts_default(&o);
+    o.method = method;
+    o.precision = precision;
+    o.stop = YAFEL_HIP_STOP_RELRES;
+    o.tol = tol;
+    void *x = nullptr;
+    hipdetail::check(yafel_hip_malloc(ctx, sizeof(double) * n_, &x), ctx);
+    yafel_hip_solve_info info{};
+    int rc = yafel_hip_memcpy_h2d(ctx, x, solution_.data(), sizeof(double) * n_);
+    if (rc == YAFEL_HIP_OK) rc = yafel_hip_solve(A_.get(), &o, R_.get(), (double *)x, &info);
+    if (rc == YAFEL_HIP_OK) rc = yafel_hip_memcpy_d2h(ctx, solution_.data(), x, sizeof(double) * n_);
+    yafel_hip_free(ctx, x);
+    hipdetail::check(rc, ctx);
+    return info;
+  }
+
+ private:
+  DoFManager &dofm_;
+  hipdetail::MatrixPtr A_;
+  std::shared_ptr<double> R_;
+  std::size_t n_ = 0;
+  Vector<double> solution_;
+  double time_ = 0;
+};
+
+// CGAssembly<Physics> (include/assembly/CGAssembly.hpp:27-186): tangent values
+// into the fixed pattern (overwritten), residual ACCUMULATED.
+template <typename Physics>
+void CGAssembly(FESystem &fes, std::vector<AssemblyRequirement> requirements = {AssemblyRequirement::Residual,
+                                                                                AssemblyRequirement::Tangent}) {
+  int req = 0;
+  for (auto r : requirements) {
+    if (r == AssemblyRequirement::Residual) req |= YAFEL_HIP_ASSEMBLE_RESIDUAL;
+    else if (r == AssemblyRequirement::Tangent) req |= YAFEL_HIP_ASSEMBLE_TANGENT;
+    else throw std::invalid_argument("CGAssembly: mass requirements are not part of the device path");
+  }
+  const yafel_hip_physics ph = Physics::device_descriptor();
+  hipdetail::check(yafel_hip_assemble(fes.getDoFManager().handle(), &ph, req, fes.tangent_handle(),
+                                      fes.residual_device()),
+                   hipdetail::context());
+}
+
+// DirichletBC (DirichletBC.hpp:20-89, DirichletBC.cpp:27-56)
+class DirichletBC {
+ public:
+  DirichletBC(const DoFManager &dofm, double bcval, int comp = 0)
+      : dofm_(dofm), comp_(comp), value_([bcval](const coordinate &, double) { return bcval; }) {}
+  DirichletBC(const DoFManager &dofm, std::function<double(const coordinate &, double)> f, int comp = 0)
+      : dofm_(dofm), comp_(comp), value_(std::move(f)) {}
+
+  template <typename Lambda>
+  void selectByFunction(Lambda &&func) {
+    for (std::size_t i = 0; i < dofm_.dof_nodes.size(); ++i)
+      if (func(dofm_.dof_nodes[i])) nodes_.push_back((int)i);
+  }
+  std::vector<int> &getNodes() { return nodes_; }
+
+  void apply(FESystem &fes, double time = 0.0) {
+    const std::size_t n = fes.ndofs();
+    std::vector<uint8_t> mask(n, 0);
+    std::vector<double> g(n, 0.0);
+    for (int node : nodes_) {
+      const std::size_t d = (std::size_t)node * dofm_.dof_per_node + comp_;
+      mask[d] = 1;
+      g[d] = value_(dofm_.dof_nodes[node], time);
+    }
+    auto ctx = hipdetail::context();
+    void *dm = nullptr, *dg = nullptr;
+    hipdetail::check(yafel_hip_malloc(ctx, n, &dm), ctx);
+    hipdetail::check(yafel_hip_malloc(ctx, sizeof(double) * n, &dg), ctx);
+    int rc = yafel_hip_memcpy_h2d(ctx, dm, mask.data(), n);
+    if (rc == YAFEL_HIP_OK) rc = yafel_hip_memcpy_h2d(ctx, dg, g.data(), sizeof(double) * n);
+    if (rc == YAFEL_HIP_OK)
+      rc = yafel_hip_dirichlet_apply(fes.tangent_handle(), (const uint8_t *)dm, (const double *)dg,
+                                     fes.residual_device());
+    if (rc == YAFEL_HIP_OK) rc = yafel_hip_ctx_sync(ctx);
+    yafel_hip_free(ctx, dm);
+    yafel_hip_free(ctx, dg);
+    hipdetail::check(rc, ctx);
+  }
+
+ private:
+  const DoFManager &dofm_;
+  int comp_;
+  std::function<double(const coordinate &, double)> value_;
+  std::vector<int> nodes_;
+};
+
+}  // namespace yafel

@@ -236,6 +236,61 @@ int yafel_hip_dirichlet_apply(yafel_hip_matrix A, const uint8_t *mask_dev, const
 int yafel_hip_bc_mask_plane(yafel_hip_mesh m, int axis, double value, double tol, int comp,
                             uint8_t *mask_dev);
 
+/* ---------------------------------------------------------------- serving */
+/* No reference counterpart: a device-resident request loop.  Requests are
+ * admitted under an HBM budget and a slot limit; every admitted request gets
+ * its own DoF numbering, pattern, assembled FP64/FP32 matrix and Krylov state
+ * on the device, and all admitted requests advance together in batched
+ * Jacobi-PCG launches (FP32 inner iterations, FP64 refinement), each on its
+ * own block range with its own scalars, so a result does not depend on what
+ * else is in the batch.  Requests that arrive while others run join at the
+ * next chunk boundary; finished ones retire and free their memory. */
+typedef struct yafel_hip_server_s *yafel_hip_server;
+
+#define YAFEL_HIP_BC_BOX 0      /* homogeneous Dirichlet on every face of the mesh's bounding box */
+#define YAFEL_HIP_BC_XMIN 1     /* clamp every component on the x = min face */
+
+typedef struct {
+  int dim, p, dof_per_node;
+  int64_t n_corner;
+  const double *corner_nodes;   /* n_corner*3, host */
+  int64_t ncell;
+  const int32_t *cells;         /* ncell*2^dim, host, order as yafel_hip_mesh_create_cg */
+  yafel_hip_physics phys;       /* coef (per element) must be a host pointer */
+  const double *load;           /* right-hand side per dof (host); NULL: the assembled residual */
+  int64_t load_len;             /* length of load (must equal the request's dof count) */
+  int bc;                       /* YAFEL_HIP_BC_* */
+  double tol;                   /* ||b - A x|| / ||b|| target */
+  int64_t max_iter;             /* 0: number of dofs */
+} yafel_hip_request;
+
+#define YAFEL_HIP_REQ_QUEUED 0
+#define YAFEL_HIP_REQ_RUNNING 1
+#define YAFEL_HIP_REQ_DONE 2
+#define YAFEL_HIP_REQ_FAILED 3
+#define YAFEL_HIP_REQ_CANCELLED 4
+
+typedef struct {
+  int64_t submitted, admitted, completed, failed, cancelled, queued, active;
+  int64_t chunks, batched_launches;
+  size_t hbm_in_use, hbm_peak, hbm_budget;
+} yafel_hip_server_stats;
+
+int yafel_hip_server_create(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots, yafel_hip_server *out);
+int yafel_hip_server_destroy(yafel_hip_server s);
+/* copies the request's inputs; YAFEL_HIP_ENOMEM when it could never fit the budget */
+int yafel_hip_server_submit(yafel_hip_server s, const yafel_hip_request *req, int64_t *id);
+/* admits what fits, runs one chunk of batched iterations for every active
+ * request, retires finished ones; ids of requests that finished are written
+ * to done_ids (up to max_ids), their count to n_done */
+int yafel_hip_server_progress(yafel_hip_server s, int64_t *done_ids, int max_ids, int *n_done);
+/* state (YAFEL_HIP_REQ_*) and, for finished requests, the dof count */
+int yafel_hip_server_query(yafel_hip_server s, int64_t id, int *state, int64_t *ndofs);
+/* copies a finished request's solution (DoFManager numbering) and releases it */
+int yafel_hip_server_result(yafel_hip_server s, int64_t id, double *x, int64_t n, yafel_hip_solve_info *info);
+int yafel_hip_server_cancel(yafel_hip_server s, int64_t id);
+int yafel_hip_server_stats_get(yafel_hip_server s, yafel_hip_server_stats *st);
+
 /* ----------------------------------------------------- synthetic inputs */
 /* Structured unit-square/cube corner mesh (cells ordered as
  * yafel_hip_mesh_create_cg expects for p).  No reference counterpart: the

@@ -120,7 +120,8 @@ def test_spmm_matches_columnwise_spmv(H, ctx, golden):
 def test_dot_matches_oracle(H, ctx, orc):
     rng = np.random.default_rng(9)
     a, b = rng.standard_normal(100003), rng.standard_normal(100003)
-    assert abs(ctx.dot(ctx.upload(a), ctx.upload(b)) - orc.dot(a, b)) < 1e-12
+    # both sum in FP64 in different orders: bound by n eps sum|a_i b_i| scaled down
+    assert abs(ctx.dot(ctx.upload(a), ctx.upload(b)) - orc.dot(a, b)) <= 1e-14 * np.abs(a * b).sum()
 
 
 # -------------------------------------------------------- DoF numbering
@@ -347,3 +348,38 @@ def test_error_paths(H, ctx):
     A = H.Matrix.from_triplets(ctx, np.arange(3), np.arange(3), np.ones(3))
     with pytest.raises(H.YafelHipError):
         m.assemble(H.physics(H.POISSON, 2, f=1.0), A, None, H.TANGENT)   # not this mesh's pattern
+
+
+# ------------------------------------------------------------ multi-RHS
+@pytest.mark.parametrize("method", ["BICGSTAB_JACOBI", "PCG_JACOBI"])
+@pytest.mark.parametrize("prec", ["FP64", "MIXED"])
+def test_multi_rhs_alone_vs_together(H, ctx, orc, method, prec):
+    """k right-hand sides together reach the same solutions as one at a time."""
+    n = 12
+    if method == "BICGSTAB_JACOBI":
+        kw = dict(U=(0.8660254037844387, 0.5, 0.0), kdiff=1.0 / n / 200, h=1.0 / n, alpha=0.99)
+        ph = H.physics(H.SUPG, 2, **kw)
+    else:
+        ph = H.physics(H.POISSON, 2, coef=np.linspace(0.5, 2, n * n))
+    m = H.Mesh.structured(ctx, 2, 2, 1, n)
+    A = m.pattern(H.CSR, H.FP64 | H.FP32)
+    m.assemble(ph, A, None, H.TANGENT)
+    _, dn = m.download()
+    mask = edge_mask(dn, 2)
+    A.dirichlet(ctx.upload(mask), ctx.upload(np.zeros(m.ndofs)))
+    k = 70  # two batches (64 + 6)
+    B = np.stack([H.normal(100 + j, m.ndofs) for j in range(k)], axis=1)
+    B[mask == 1, :] = 0.0
+    Bd, Xd = ctx.upload(B.ravel()), ctx.upload(np.zeros(m.ndofs * k))
+    infos = A.solve_multi(Bd, Xd, k, method=getattr(H, method), precision=getattr(H, prec),
+                          stop=H.STOP_RELRES, tol=1e-8)
+    X = Xd.download().reshape(m.ndofs, k)
+    ptr, col, val = A.arrays()
+    O = orc.wrap(1, m.ndofs, m.ndofs, ptr, col, val)
+    for j in (0, 1, 37, 63, 64, 69):
+        assert infos[j].status == 0 and infos[j].rel_residual < 1e-8
+        r = B[:, j] - O.spmv(X[:, j])
+        assert np.linalg.norm(r) / np.linalg.norm(B[:, j]) < 1e-8
+        x1, i1 = A.solve_host(B[:, j], method=getattr(H, method), precision=getattr(H, prec),
+                              stop=H.STOP_RELRES, tol=1e-8)
+        assert np.abs(X[:, j] - x1).max() <= 1e-6 * np.abs(x1).max()

@@ -1,0 +1,116 @@
+"""Serving: continuous batching, admission under an HBM budget, and the
+same result for a request alone and in a mix."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MIX = [dict(dim=2, p=1, dpn=1, n=20), dict(dim=2, p=2, dpn=1, n=16), dict(dim=3, p=1, dpn=3, n=8),
+       dict(dim=2, p=1, dpn=1, n=40), dict(dim=2, p=2, dpn=1, n=24), dict(dim=3, p=1, dpn=3, n=6)]
+
+
+def materialised(i, r):
+    from yafel_amd import workloads
+    r = dict(r)
+    r["nel"] = r["n"] ** r["dim"]
+    r["ndofs"] = (r["n"] + 1) ** 3 * 3 if r["dim"] == 3 else (r["p"] * r["n"] + 1) ** 2
+    r["seed"] = 900 + i
+    return workloads.c4_materialise(r)
+
+
+def run_all(H, ctx, reqs, budget=8 << 30, slots=16, stagger=False):
+    srv = H.Server(ctx, budget, slots)
+    ids, out = [], {}
+    for i, m in enumerate(reqs):
+        ids.append(srv.submit(m["dim"], m["p"], m["dpn"], m["nodes"], m["cells"], m["ph"], load=m["load"],
+                              bc=m["bc"], tol=1e-8))
+        if stagger:
+            srv.progress()       # later requests arrive while earlier ones run
+    pending = set(ids)
+    guard = 0
+    while pending:
+        for rid in srv.progress():
+            pending.discard(rid)
+        guard += 1
+        assert guard < 200000
+    for rid in ids:
+        out[rid] = srv.result(rid)
+    st = srv.stats()
+    srv.close()
+    return [out[r] for r in ids], st
+
+
+def test_alone_and_in_the_mix_are_identical(ctx):
+    from yafel_amd import hip as H
+    reqs = [materialised(i, r) for i, r in enumerate(MIX)]
+    mix, st = run_all(H, ctx, reqs, stagger=True)
+    assert st["completed"] == len(MIX) and st["failed"] == 0
+    for i, m in enumerate(reqs):
+        (x_alone, info_alone), _ = run_all(H, ctx, [m])
+        x_mix, info_mix = mix[i]
+        assert info_mix.status == 0 and info_mix.rel_residual < 1e-8
+        assert np.array_equal(x_alone, x_mix)          # bit-identical alone vs in the mix
+        assert info_alone.iterations == info_mix.iterations
+
+
+def test_served_result_matches_direct_solve(ctx, orc):
+    from yafel_amd import hip as H
+    reqs = [materialised(i, r) for i, r in enumerate(MIX[:3])]
+    served, _ = run_all(H, ctx, reqs)
+    for m, (x, info) in zip(reqs, served):
+        mesh = H.Mesh(ctx, m["dim"], m["p"], m["dpn"], m["nodes"], m["cells"])
+        A = mesh.pattern(H.BCSR3 if m["dpn"] == 3 else H.CSR, H.FP64 | H.FP32)
+        mesh.assemble(m["ph"], A, None, H.TANGENT)
+        b = ctx.upload(m["load"])
+        mask = ctx.upload(np.zeros(mesh.ndofs, np.uint8))
+        if m["bc"] == H.BC_BOX:
+            for d in range(m["dim"]):
+                mesh.bc_mask_plane(mask, d, 0.0, 1e-9)
+                mesh.bc_mask_plane(mask, d, 1.0, 1e-9)
+        else:
+            mesh.bc_mask_plane(mask, 0, 0.0, 1e-9)
+        A.dirichlet(mask, b)
+        bh = b.download()
+        ptr, col, val = A.arrays()
+        O = orc.wrap(3 if m["dpn"] == 3 else 1, mesh.ndofs // m["dpn"] if m["dpn"] == 3 else mesh.ndofs,
+                     mesh.ndofs // m["dpn"] if m["dpn"] == 3 else mesh.ndofs, ptr, col, val)
+        r = bh - O.spmv(x)
+        assert np.linalg.norm(r) / np.linalg.norm(bh) < 1e-8
+        x2, i2 = A.solve_host(bh, method=H.PCG_JACOBI, precision=H.MIXED, stop=H.STOP_RELRES, tol=1e-8)
+        assert np.abs(x - x2).max() <= 1e-6 * np.abs(x2).max()
+
+
+def test_admission_under_budget_and_rejection(ctx):
+    from yafel_amd import hip as H
+    reqs = [materialised(i, r) for i, r in enumerate(MIX)]
+    # a budget that holds about two requests at a time
+    sizes = []
+    srv = H.Server(ctx, 1 << 40, 4)
+    budget = 60 << 20
+    srv.close()
+    res, st = run_all(H, ctx, reqs, budget=budget, slots=2)
+    assert st["completed"] == len(reqs) and st["hbm_peak"] <= budget
+    for x, info in res:
+        assert info.status == 0 and info.rel_residual < 1e-8
+    srv = H.Server(ctx, 1 << 20, 4)  # 1 MiB: nothing fits
+    m = reqs[0]
+    with pytest.raises(H.YafelHipError) as e:
+        srv.submit(m["dim"], m["p"], m["dpn"], m["nodes"], m["cells"], m["ph"], load=m["load"], bc=m["bc"])
+    assert e.value.code == H.ENOMEM
+    srv.close()
+
+
+def test_cancel_queued_request(ctx):
+    from yafel_amd import hip as H
+    reqs = [materialised(i, r) for i, r in enumerate(MIX[:3])]
+    srv = H.Server(ctx, 8 << 30, 1)
+    ids = [srv.submit(m["dim"], m["p"], m["dpn"], m["nodes"], m["cells"], m["ph"], load=m["load"], bc=m["bc"])
+           for m in reqs]
+    srv.cancel(ids[2])
+    pending = set(ids[:2])
+    while pending:
+        for rid in srv.progress():
+            pending.discard(rid)
+    assert srv.query(ids[2])[0] == H.REQ_CANCELLED
+    assert srv.stats()["cancelled"] == 1
+    srv.close()

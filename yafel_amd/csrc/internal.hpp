@@ -170,13 +170,13 @@ __device__ __forceinline__ int xcd_block() {
 // Signals completion of this block's partial; returns true in the last block
 // to finish (all partials are then visible).  counter must start at 0 and is
 // reset by the last block.
-__device__ __forceinline__ bool last_block_done(unsigned int *counter) {
+__device__ __forceinline__ bool last_block_done(unsigned int *counter, unsigned int nblk) {
   __shared__ bool is_last;
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned int prev = atomicAdd(counter, 1u);
-    is_last = (prev == gridDim.x - 1);
+    is_last = (prev == nblk - 1);
   }
   __syncthreads();
   if (is_last) __threadfence();

@@ -26,509 +26,11 @@
 
 using namespace yh;
 
+#include "krylov_ops.hpp"
+
+using namespace yh::kr;
+
 namespace {
-
-constexpr int NT = 256;
-
-struct KState {
-  double rho;      // CG: r.z (r.r unpreconditioned); BiCGSTAB: rhat.r
-  double rho0;     // initial rho (reference stopping ratio)
-  double rho_prev; // BiCGSTAB: rho of the previous iteration
-  double rr;       // recurrence r.r
-  double rr_ref;   // r.r right after the last refinement
-  double bb;       // b.b
-  double alpha, beta, omega;
-  double crit;     // last stopping quantity
-  double true_rr;  // last FP64 true residual squared
-  double tol;
-  double delta2;
-  long long iter, max_iter, refinements;
-  int done, status, paused, first;
-  int method, stop, mixed, special;
-  int pidx, pad0;
-  unsigned int counter[8];
-};
-
-enum { M_CG = 0, M_PCG = 1, M_BICG = 2, M_BICGJ = 3 };
-
-__device__ __forceinline__ bool bad(double v) { return !(fabs(v) <= 1.7976931348623157e308); }
-
-// ------------------------------------------------------------------ drivers
-template <class T> struct MatView {
-  const int *ptr;
-  const int *col;
-  const T *val;
-  int64_t brows;
-  int64_t nb;
-};
-
-// SpMV with a fused per-row epilogue and up to 4 block-reduced sums; the last
-// block to finish folds the partials (fixed order) and runs op.last().
-template <int FMT, int L, class T, class Op>
-__global__ __launch_bounds__(NT) void k_spmv_fused(MatView<T> A, Op op0) {
-  Op op = op0;
-  if (op.skip()) return;
-  op.begin();
-  const int lane = threadIdx.x & (L - 1);
-  constexpr int NP = Op::NP;
-  double acc[NP > 0 ? NP : 1];
-#pragma unroll
-  for (int j = 0; j < (NP > 0 ? NP : 1); ++j) acc[j] = 0;
-  const int64_t rpb = NT / L;
-  const int64_t chunk = (A.brows + gridDim.x - 1) / gridDim.x;
-  const int64_t r0 = (int64_t)dev::xcd_block() * chunk;
-  const int64_t r1 = r0 + chunk < A.brows ? r0 + chunk : A.brows;
-  for (int64_t r = r0 + threadIdx.x / L; r < r1; r += rpb) {
-    if (FMT == YAFEL_HIP_CSR) {
-      T y = dev::csr_row<L, T>(A.ptr, A.col, A.val, r, lane, [&](int c) { return op.x(c); });
-      if (lane == 0) op.row(r, y, acc);
-    } else {
-      T a[3];
-      dev::bcsr_row<L, T>(A.ptr, A.col, A.val, A.nb, r, lane, a, [&](int64_t c) { return op.x(c); });
-      if (lane == 0) {
-        op.row(3 * r, a[0], acc);
-        op.row(3 * r + 1, a[1], acc);
-        op.row(3 * r + 2, a[2], acc);
-      }
-    }
-  }
-  if constexpr (NP > 0) {
-    __shared__ double lds[NT / 64];
-    __shared__ double sums[NP];
-    const int blk = dev::xcd_block();
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      double s = dev::block_sum<NT>(acc[j], lds);
-      if (threadIdx.x == 0) op.part[(size_t)j * gridDim.x + blk] = s;
-    }
-    if (dev::last_block_done(op.counter)) {
-#pragma unroll
-      for (int j = 0; j < NP; ++j) {
-        double s = dev::reduce_partials<NT>(op.part + (size_t)j * gridDim.x, gridDim.x, 1, lds);
-        if (threadIdx.x == 0) sums[j] = s;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        *op.counter = 0;
-        op.last(sums);
-      }
-    }
-  }
-}
-
-// Elementwise pass with up to 4 block-reduced sums.
-template <class Op>
-__global__ __launch_bounds__(NT) void k_elem(int64_t n, Op op0) {
-  Op op = op0;
-  if (op.skip()) return;
-  op.begin();
-  constexpr int NP = Op::NP;
-  double acc[NP > 0 ? NP : 1];
-#pragma unroll
-  for (int j = 0; j < (NP > 0 ? NP : 1); ++j) acc[j] = 0;
-  const int64_t chunk = ((n + gridDim.x - 1) / gridDim.x + 63) & ~(int64_t)63;
-  const int64_t i0 = (int64_t)dev::xcd_block() * chunk;
-  const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += NT) op.elem(i, acc);
-  if constexpr (NP > 0) {
-    __shared__ double lds[NT / 64];
-    __shared__ double sums[NP];
-    const int blk = dev::xcd_block();
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      double s = dev::block_sum<NT>(acc[j], lds);
-      if (threadIdx.x == 0) op.part[(size_t)j * gridDim.x + blk] = s;
-    }
-    if (dev::last_block_done(op.counter)) {
-#pragma unroll
-      for (int j = 0; j < NP; ++j) {
-        double s = dev::reduce_partials<NT>(op.part + (size_t)j * gridDim.x, gridDim.x, 1, lds);
-        if (threadIdx.x == 0) sums[j] = s;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        *op.counter = 0;
-        op.last(sums);
-      }
-    }
-  }
-}
-
-// --------------------------------------------------------------------- ops
-struct OpBase {
-  KState *st;
-  double *part;
-  unsigned int *counter;
-  __device__ bool running() const { return !st->done && !st->paused; }
-};
-
-// Stopping test after an iteration (FP64 path) or pause request (MIXED).
-__device__ void after_iteration(KState *st, double rho_new, double rr) {
-  st->iter += 1;
-  st->rr = rr;
-  const bool cgfam = st->method == M_CG || st->method == M_PCG;
-  double crit;
-  if (st->stop == YAFEL_HIP_STOP_RELRES) crit = st->bb > 0 ? sqrt(rr / st->bb) : 0.0;
-  else crit = cgfam ? rho_new / st->rho0 : rr / st->bb;
-  st->crit = crit;
-  if (bad(rho_new) || bad(rr)) {
-    st->status = YAFEL_HIP_STATUS_NAN;
-    st->done = 1;
-    return;
-  }
-  const bool conv = crit < st->tol;
-  if (st->mixed || !cgfam) {
-    // verify / refine against the FP64 true residual
-    if (conv || rr < st->delta2 * st->rr_ref || st->iter >= st->max_iter) st->paused = 1;
-  } else {
-    if (conv) {
-      st->status = YAFEL_HIP_STATUS_CONVERGED;
-      st->done = 1;
-      return;
-    }
-    if (st->iter >= st->max_iter) {
-      st->status = YAFEL_HIP_STATUS_MAXITER;
-      st->done = 1;
-      return;
-    }
-  }
-  if (!st->paused) {
-    if (cgfam) {
-      st->beta = rho_new / st->rho;
-      st->rho = rho_new;
-    } else {
-      st->beta = (rho_new / st->rho) * (st->alpha / st->omega);
-      st->rho_prev = st->rho;
-      st->rho = rho_new;
-    }
-  } else if (!cgfam) {
-    st->rho_prev = st->rho;
-  }
-}
-
-// CG kernel A: q = A p_new with p_new = z + beta p_old.  The two p buffers
-// alternate; st->pidx names the one holding the current p, so iterations
-// skipped after a pause or convergence never desynchronise them.
-template <class T>
-struct OpCgA : OpBase {
-  static constexpr int NP = 1;
-  const T *z;
-  T *pbuf0, *pbuf1;
-  T *q;
-  const T *pold;
-  T *pnew;
-  int cur;
-  T beta;
-  __device__ bool skip() const { return !running(); }
-  __device__ void begin() {
-    beta = st->first ? T(0) : (T)st->beta;
-    cur = st->pidx;
-    pold = cur ? pbuf1 : pbuf0;
-    pnew = cur ? pbuf0 : pbuf1;
-  }
-  __device__ T x(int64_t c) const { return fma(beta, pold[c], z[c]); }
-  __device__ void row(int64_t i, T y, double *acc) const {
-    const T pn = fma(beta, pold[i], z[i]);
-    pnew[i] = pn;
-    q[i] = y;
-    acc[0] += (double)pn * (double)y;
-  }
-  __device__ void last(const double *s) const {
-    const double a = st->rho / s[0];
-    st->first = 0;
-    st->alpha = a;
-    st->pidx = cur ^ 1;
-    if (s[0] == 0.0 || bad(a)) {
-      st->status = YAFEL_HIP_STATUS_BREAKDOWN;
-      st->done = 1;
-    }
-  }
-};
-
-// CG kernel B: x += alpha p, r -= alpha q, z = D^-1 r.
-template <class T, bool PREC>
-struct OpCgB : OpBase {
-  static constexpr int NP = 2;
-  T *x, *r, *z;
-  const T *pbuf0, *pbuf1, *q, *dinv;
-  const T *p;
-  T alpha;
-  __device__ bool skip() const { return !running(); }
-  __device__ void begin() {
-    alpha = (T)st->alpha;
-    p = st->pidx ? pbuf1 : pbuf0;
-  }
-  __device__ void elem(int64_t i, double *acc) const {
-    x[i] = fma(alpha, p[i], x[i]);
-    const T ri = fma(-alpha, q[i], r[i]);
-    r[i] = ri;
-    if (PREC) {
-      const T zi = dinv[i] * ri;
-      z[i] = zi;
-      acc[0] += (double)ri * (double)zi;
-    } else {
-      acc[0] += (double)ri * (double)ri;
-    }
-    acc[1] += (double)ri * (double)ri;
-  }
-  __device__ void last(const double *s) const { after_iteration(st, s[0], s[1]); }
-};
-
-// FP64 residual r = b - A x_hi; stores r (in the inner precision TL) and z,
-// and folds r.w (w = z, r or rhat), r.r, the FP64 true r.r and b.b.
-// MODE 0 = initialisation, 1 = refinement / verification.
-template <class TL, int MODE>
-struct OpResidual : OpBase {
-  static constexpr int NP = 4;
-  const double *xh, *b;
-  TL *r, *z;
-  const TL *dinv, *rhat;
-  int wkind;  // 0: r.r, 1: r.(D^-1 r) stored in z, 2: r.rhat
-  __device__ bool skip() const {
-    if (MODE == 0) return false;
-    return st->done || !st->paused;
-  }
-  __device__ void begin() {}
-  __device__ double x(int64_t c) const { return xh[c]; }
-  __device__ void row(int64_t i, double y, double *acc) const {
-    const double rh = b[i] - y;
-    const TL rl = (TL)rh;
-    r[i] = rl;
-    double w;
-    if (wkind == 1) {
-      const TL zi = dinv[i] * rl;
-      z[i] = zi;
-      w = (double)zi;
-    } else if (wkind == 2) {
-      w = MODE == 0 ? (double)rl : (double)rhat[i];  // rhat = r0 at the start
-    } else {
-      w = (double)rl;
-    }
-    acc[0] += (double)rl * w;
-    acc[1] += (double)rl * (double)rl;
-    acc[2] += rh * rh;
-    acc[3] += b[i] * b[i];
-  }
-  __device__ void last(const double *s) const {
-    const bool cgfam = st->method == M_CG || st->method == M_PCG;
-    st->true_rr = s[2];
-    if (MODE == 0) {
-      st->bb = s[3];
-      st->rho = s[0];
-      st->rho0 = s[0];
-      st->rho_prev = 1.0;
-      st->rr = s[1];
-      st->rr_ref = s[1];
-      st->first = 1;
-      st->iter = 0;
-      st->alpha = 1.0;
-      st->omega = 1.0;
-      st->beta = 0.0;
-      st->paused = 0;
-      st->pidx = 0;
-      // early exits of the reference
-      if (cgfam && st->stop == YAFEL_HIP_STOP_REFERENCE && s[0] == 0.0) {
-        st->done = 1;  // cg_solve.hpp:52-54 / pcg_solve.hpp:148-150: x0 solves it
-        st->crit = 0;
-      } else if (!cgfam && st->stop == YAFEL_HIP_STOP_REFERENCE && s[3] < st->tol) {
-        st->done = 1;  // bicgstab_solve.hpp:43-45 returns rhs
-        st->special = 1;
-        st->crit = 0;
-      } else if (s[3] == 0.0) {
-        st->done = 1;  // b = 0: x = 0 is exact
-        st->special = 2;
-        st->crit = 0;
-      } else if (st->stop == YAFEL_HIP_STOP_RELRES && sqrt(s[2] / s[3]) < st->tol) {
-        st->done = 1;
-        st->crit = sqrt(s[2] / s[3]);
-      }
-      return;
-    }
-    // refinement / verification against the FP64 residual
-    st->refinements += 1;
-    double crit;
-    if (st->stop == YAFEL_HIP_STOP_RELRES) crit = sqrt(s[2] / st->bb);
-    else if (cgfam) crit = (st->method == M_PCG ? s[0] : s[2]) / st->rho0;
-    else crit = s[2] / st->bb;
-    st->crit = crit;
-    if (bad(s[2])) {
-      st->status = YAFEL_HIP_STATUS_NAN;
-      st->done = 1;
-      return;
-    }
-    if (crit < st->tol) {
-      st->status = YAFEL_HIP_STATUS_CONVERGED;
-      st->done = 1;
-      return;
-    }
-    if (st->iter >= st->max_iter) {
-      st->status = YAFEL_HIP_STATUS_MAXITER;
-      st->done = 1;
-      return;
-    }
-    if (cgfam) {
-      st->beta = s[0] / st->rho;
-      st->rho = s[0];
-    } else {
-      // rho_prev holds rho of the finished iteration's predecessor
-      st->beta = (s[0] / st->rho_prev) * (st->alpha / st->omega);
-      st->rho = s[0];
-    }
-    st->rr = s[1];
-    st->rr_ref = s[1];
-    st->paused = 0;
-  }
-};
-
-// MIXED: x_hi += x_lo, x_lo = 0 (runs only while paused).
-template <class TL>
-struct OpFold : OpBase {
-  static constexpr int NP = 0;
-  double *xh;
-  TL *xl;
-  __device__ bool skip() const { return st->done || !st->paused; }
-  __device__ void begin() {}
-  __device__ void elem(int64_t i, double *) const {
-    xh[i] += (double)xl[i];
-    xl[i] = TL(0);
-  }
-  __device__ void last(const double *) const {}
-};
-
-// BiCGSTAB (bicgstab_solve.hpp:55-91): p = r + beta (p - omega v), ph = D^-1 p.
-template <class T, bool PREC>
-struct OpBiP : OpBase {
-  static constexpr int NP = 0;
-  const T *r, *v, *dinv;
-  T *p, *ph;
-  T beta, omega;
-  __device__ bool skip() const { return !running(); }
-  __device__ void begin() {
-    beta = st->first ? T(0) : (T)st->beta;
-    omega = (T)st->omega;
-  }
-  __device__ void elem(int64_t i, double *) const {
-    const T pi = r[i] + beta * (p[i] - omega * v[i]);
-    p[i] = pi;
-    if (PREC) ph[i] = dinv[i] * pi;
-  }
-  __device__ void last(const double *) const {}
-};
-
-// v = A ph, rhat.v -> alpha
-template <class T>
-struct OpBiV : OpBase {
-  static constexpr int NP = 1;
-  const T *ph, *rhat;
-  T *v;
-  __device__ bool skip() const { return !running(); }
-  __device__ void begin() {}
-  __device__ T x(int64_t c) const { return ph[c]; }
-  __device__ void row(int64_t i, T y, double *acc) const {
-    v[i] = y;
-    acc[0] += (double)rhat[i] * (double)y;
-  }
-  __device__ void last(const double *s) const {
-    const double a = st->rho / s[0];
-    st->first = 0;
-    st->alpha = a;
-    if (s[0] == 0.0 || bad(a)) {
-      st->status = YAFEL_HIP_STATUS_BREAKDOWN;
-      st->done = 1;
-    }
-  }
-};
-
-// s = r - alpha v, sh = D^-1 s
-template <class T, bool PREC>
-struct OpBiS : OpBase {
-  static constexpr int NP = 0;
-  const T *r, *v, *dinv;
-  T *s, *sh;
-  T alpha;
-  __device__ bool skip() const { return !running(); }
-  __device__ void begin() { alpha = (T)st->alpha; }
-  __device__ void elem(int64_t i, double *) const {
-    const T si = r[i] - v[i] * alpha;
-    s[i] = si;
-    if (PREC) sh[i] = dinv[i] * si;
-  }
-  __device__ void last(const double *) const {}
-};
-
-// t = A sh, t.t and t.s -> omega
-template <class T>
-struct OpBiT : OpBase {
-  static constexpr int NP = 2;
-  const T *sh, *s;
-  T *t;
-  __device__ bool skip() const { return !running(); }
-  __device__ void begin() {}
-  __device__ T x(int64_t c) const { return sh[c]; }
-  __device__ void row(int64_t i, T y, double *acc) const {
-    t[i] = y;
-    acc[0] += (double)y * (double)y;
-    acc[1] += (double)y * (double)s[i];
-  }
-  __device__ void last(const double *sm) const {
-    const double w = sm[0] == 0.0 ? 0.0 : sm[1] / sm[0];
-    st->omega = w;
-    if (bad(w)) {
-      st->status = YAFEL_HIP_STATUS_NAN;
-      st->done = 1;
-    }
-  }
-};
-
-// x += alpha ph + omega sh, r = s - omega t, rhat.r, r.r
-template <class T>
-struct OpBiX : OpBase {
-  static constexpr int NP = 2;
-  T *x, *r;
-  const T *ph, *sh, *s, *t, *rhat;
-  T alpha, omega;
-  __device__ bool skip() const { return !running(); }
-  __device__ void begin() {
-    alpha = (T)st->alpha;
-    omega = (T)st->omega;
-  }
-  __device__ void elem(int64_t i, double *acc) const {
-    x[i] += ph[i] * alpha + sh[i] * omega;
-    const T ri = s[i] - t[i] * omega;
-    r[i] = ri;
-    acc[0] += (double)rhat[i] * (double)ri;
-    acc[1] += (double)ri * (double)ri;
-  }
-  __device__ void last(const double *s) const {
-    if (st->omega == 0.0) {
-      // t = 0: the reference keeps going with w = 0, which makes beta infinite
-      st->status = YAFEL_HIP_STATUS_BREAKDOWN;
-      st->done = 1;
-      st->iter += 1;
-      return;
-    }
-    after_iteration(st, s[0], s[1]);
-  }
-};
-
-// final report: r = b - A x (FP64), r.r and b.b into out[0..1]
-struct OpReport : OpBase {
-  static constexpr int NP = 2;
-  const double *xh, *b;
-  double *out;
-  __device__ bool skip() const { return false; }
-  __device__ void begin() {}
-  __device__ double x(int64_t c) const { return xh[c]; }
-  __device__ void row(int64_t i, double y, double *acc) const {
-    const double r = b[i] - y;
-    acc[0] += r * r;
-    acc[1] += b[i] * b[i];
-  }
-  __device__ void last(const double *s) const {
-    out[0] = s[0];
-    out[1] = s[1];
-  }
-};
 
 // ---------------------------------------------------------------- launchers
 template <class T>

@@ -1,12 +1,733 @@
-// Multi-RHS Krylov solves (k right-hand sides advanced together).
+// k right-hand sides advanced together (k <= 64 per batch; larger k runs in
+// batches of 64).  Vectors are row-major n x k, so one wavefront owns a matrix
+// row and its 64 lanes are the right-hand sides: the row's values and column
+// indices are wave-uniform (scalar loads, read once for all k), the gathered
+// rows of X are 64-wide coalesced reads, and every per-RHS dot product is a
+// per-lane accumulation.  Each right-hand side keeps its own FP64 scalars and
+// its own stopping decision; a right-hand side that has converged is frozen.
+//
+// Methods: BiCGSTAB with right Jacobi preconditioning (ph = D^-1 p and
+// sh = D^-1 s are formed on the fly, never stored) and CG / Jacobi-PCG, in
+// FP64 or as FP32 inner iterations with FP64 residual replacement.
+#include <chrono>
+#include <cmath>
+
 #include "internal.hpp"
 
 using namespace yh;
 
+namespace {
+
+constexpr int NT = 256;
+constexpr int KMAX = 64;
+
+struct MState {
+  int done, paused, first, k;
+  int method, stop, mixed, pidx;
+  long long iter, max_iter, refinements;
+  double tol, delta2;
+  unsigned int counter[8];
+  double rho[KMAX], rho_prev[KMAX], rho0[KMAX], alpha[KMAX], omega[KMAX], beta[KMAX];
+  double rr[KMAX], rr_ref[KMAX], bb[KMAX], true_rr[KMAX], crit[KMAX];
+  int active[KMAX], status[KMAX];
+  long long it[KMAX];
+};
+
+__device__ __forceinline__ bool bad(double v) { return !(fabs(v) <= 1.7976931348623157e308); }
+
+template <class T> struct CsrView {
+  const int *ptr;
+  const int *col;
+  const T *val;
+  int64_t rows;
+};
+
+// Per-lane block reduction of NP sums into part[(j*grid + blk)*64 + lane];
+// the last block folds them (fixed order) and calls op.last_lane per RHS,
+// then op.last_all once.
+template <int NP, class Op>
+__device__ __forceinline__ void finish(Op &op, double (&acc)[NP > 0 ? NP : 1]) {
+  if constexpr (NP > 0) {
+    __shared__ double lds[NP][NT / 64][64];
+    __shared__ double sums[NP][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int blk = dev::xcd_block();
+#pragma unroll
+    for (int j = 0; j < NP; ++j) lds[j][w][lane] = acc[j];
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        double s = 0;
+#pragma unroll
+        for (int q = 0; q < NT / 64; ++q) s += lds[j][q][lane];
+        op.part[((size_t)j * gridDim.x + blk) * 64 + lane] = s;
+      }
+    }
+    if (dev::last_block_done(op.counter, gridDim.x)) {
+      // thread (w, lane) sums blocks w, w+4, ... ; then the 4 quarters in order
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        double s = 0;
+        for (int b = w; b < (int)gridDim.x; b += NT / 64) s += op.part[((size_t)j * gridDim.x + b) * 64 + lane];
+        lds[j][w][lane] = s;
+      }
+      __syncthreads();
+      if (w == 0) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+          double s = 0;
+#pragma unroll
+          for (int q = 0; q < NT / 64; ++q) s += lds[j][q][lane];
+          sums[j][lane] = s;
+        }
+      }
+      __syncthreads();
+      if (w == 0 && lane < op.st->k) {
+        double v[NP];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) v[j] = sums[j][lane];
+        op.last_lane(lane, v);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        *op.counter = 0;
+        op.last_all();
+      }
+    }
+  }
+}
+
+// Y-producing kernel: wave per row, y[lane] = sum_q val[q] x(col[q], lane).
+template <class T, class Op>
+__global__ __launch_bounds__(NT) void k_mspmm(CsrView<T> A, Op op0) {
+  Op op = op0;
+  if (op.skip()) return;
+  op.begin();
+  constexpr int NP = Op::NP;
+  double acc[NP > 0 ? NP : 1];
+#pragma unroll
+  for (int j = 0; j < (NP > 0 ? NP : 1); ++j) acc[j] = 0;
+  const int lane = threadIdx.x & 63;
+  const int k = op.st->k;
+  const int64_t chunk = (A.rows + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = (int64_t)dev::xcd_block() * chunk;
+  const int64_t r1 = r0 + chunk < A.rows ? r0 + chunk : A.rows;
+  for (int64_t rr = r0 + (threadIdx.x >> 6); rr < r1; rr += NT / 64) {
+    const int r = __builtin_amdgcn_readfirstlane((int)rr);
+    const int b = A.ptr[r], e = A.ptr[r + 1];
+    if (lane < k) {
+      T y = 0;
+      for (int q = b; q < e; ++q) y = fma(A.val[q], op.x(A.col[q], lane), y);
+      op.row(r, lane, y, acc);
+    }
+  }
+  finish<NP>(op, acc);
+}
+
+template <class Op>
+__global__ __launch_bounds__(NT) void k_melem(int64_t n, Op op0) {
+  Op op = op0;
+  if (op.skip()) return;
+  op.begin();
+  constexpr int NP = Op::NP;
+  double acc[NP > 0 ? NP : 1];
+#pragma unroll
+  for (int j = 0; j < (NP > 0 ? NP : 1); ++j) acc[j] = 0;
+  const int lane = threadIdx.x & 63;
+  const int k = op.st->k;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = (int64_t)dev::xcd_block() * chunk;
+  const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
+  for (int64_t r = r0 + (threadIdx.x >> 6); r < r1; r += NT / 64)
+    if (lane < k) op.elem(r, lane, acc);
+  finish<NP>(op, acc);
+}
+
+struct MBase {
+  MState *st;
+  double *part;
+  unsigned int *counter;
+  int k;
+  __device__ bool running() const { return !st->done && !st->paused; }
+  __device__ void last_all() const {}
+};
+
+__device__ void request_pause_or_advance(MState *st, int j, double rho_new, double rr, bool cgfam) {
+  st->it[j] += 1;
+  const double crit = st->bb[j] > 0 ? sqrt(rr / st->bb[j]) : 0.0;
+  st->crit[j] = crit;
+  st->rr[j] = rr;
+  if (bad(rho_new) || bad(rr)) {
+    st->status[j] = YAFEL_HIP_STATUS_NAN;
+    st->active[j] = 0;
+    return;
+  }
+  if (crit < st->tol || rr < st->delta2 * st->rr_ref[j] || st->it[j] >= st->max_iter) {
+    st->paused = 1;  // plain store from several lanes, same value
+    st->rho_prev[j] = st->rho[j];
+    return;
+  }
+  if (cgfam) {
+    st->beta[j] = rho_new / st->rho[j];
+  } else {
+    st->beta[j] = (rho_new / st->rho[j]) * (st->alpha[j] / st->omega[j]);
+  }
+  st->rho_prev[j] = st->rho[j];
+  st->rho[j] = rho_new;
+}
+
+__device__ void all_inactive_done(MState *st) {
+  int any = 0;
+  for (int j = 0; j < st->k; ++j) any |= st->active[j];
+  if (!any) st->done = 1;
+}
+
+// ---- BiCGSTAB (right Jacobi): p = r + beta (p - omega v)
+template <class T>
+struct MP : MBase {
+  static constexpr int NP = 0;
+  const T *r, *v;
+  T *p;
+  __device__ bool skip() const { return !running(); }
+  __device__ void begin() {}
+  __device__ void elem(int64_t i, int j, double *) const {
+    if (!st->active[j]) return;
+    const int64_t o = i * k + j;
+    const T beta = st->first ? T(0) : (T)st->beta[j], om = (T)st->omega[j];
+    p[o] = r[o] + beta * (p[o] - om * v[o]);
+  }
+  __device__ void last_lane(int, const double *) const {}
+};
+
+// v = A (D^-1 p), rhat.v -> alpha
+template <class T>
+struct MV : MBase {
+  static constexpr int NP = 1;
+  const T *p, *rhat, *dinv;
+  T *v;
+  __device__ bool skip() const { return !running(); }
+  __device__ void begin() {}
+  __device__ T x(int c, int j) const { return dinv[c] * p[(int64_t)c * k + j]; }
+  __device__ void row(int64_t i, int j, T y, double *acc) const {
+    if (!st->active[j]) return;
+    v[i * k + j] = y;
+    acc[0] += (double)rhat[i * k + j] * (double)y;
+  }
+  __device__ void last_lane(int j, const double *s) const {
+    if (!st->active[j]) return;
+    const double a = st->rho[j] / s[0];
+    st->alpha[j] = a;
+    if (s[0] == 0.0 || bad(a)) {
+      st->status[j] = YAFEL_HIP_STATUS_BREAKDOWN;
+      st->active[j] = 0;
+    }
+  }
+  __device__ void last_all() const {
+    st->first = 0;
+    all_inactive_done(st);
+  }
+};
+
+// s = r - alpha v
+template <class T>
+struct MS : MBase {
+  static constexpr int NP = 0;
+  const T *r, *v;
+  T *s;
+  __device__ bool skip() const { return !running(); }
+  __device__ void begin() {}
+  __device__ void elem(int64_t i, int j, double *) const {
+    if (!st->active[j]) return;
+    const int64_t o = i * k + j;
+    s[o] = r[o] - v[o] * (T)st->alpha[j];
+  }
+  __device__ void last_lane(int, const double *) const {}
+};
+
+// t = A (D^-1 s), t.t, t.s -> omega
+template <class T>
+struct MT : MBase {
+  static constexpr int NP = 2;
+  const T *s, *dinv;
+  T *t;
+  __device__ bool skip() const { return !running(); }
+  __device__ void begin() {}
+  __device__ T x(int c, int j) const { return dinv[c] * s[(int64_t)c * k + j]; }
+  __device__ void row(int64_t i, int j, T y, double *acc) const {
+    if (!st->active[j]) return;
+    t[i * k + j] = y;
+    acc[0] += (double)y * (double)y;
+    acc[1] += (double)y * (double)s[i * k + j];
+  }
+  __device__ void last_lane(int j, const double *sm) const {
+    if (!st->active[j]) return;
+    const double w = sm[0] == 0.0 ? 0.0 : sm[1] / sm[0];
+    st->omega[j] = w;
+    if (w == 0.0 || bad(w)) {
+      st->status[j] = YAFEL_HIP_STATUS_BREAKDOWN;
+      st->active[j] = 0;
+    }
+  }
+  __device__ void last_all() const { all_inactive_done(st); }
+};
+
+// x += alpha D^-1 p + omega D^-1 s, r = s - omega t; rhat.r, r.r
+template <class T>
+struct MX : MBase {
+  static constexpr int NP = 2;
+  T *x, *r;
+  const T *p, *s, *t, *rhat, *dinv;
+  __device__ bool skip() const { return !running(); }
+  __device__ void begin() {}
+  __device__ void elem(int64_t i, int j, double *acc) const {
+    if (!st->active[j]) return;
+    const int64_t o = i * k + j;
+    const T a = (T)st->alpha[j], w = (T)st->omega[j], d = dinv[i];
+    x[o] += (d * p[o]) * a + (d * s[o]) * w;
+    const T ri = s[o] - t[o] * w;
+    r[o] = ri;
+    acc[0] += (double)rhat[o] * (double)ri;
+    acc[1] += (double)ri * (double)ri;
+  }
+  __device__ void last_lane(int j, const double *sm) const {
+    if (!st->active[j]) return;
+    request_pause_or_advance(st, j, sm[0], sm[1], false);
+  }
+  __device__ void last_all() const {
+    st->iter += 1;
+    all_inactive_done(st);
+  }
+};
+
+// ---- CG / Jacobi-PCG: A kernel q = A p_new, p_new = z + beta p_old
+template <class T>
+struct MCgA : MBase {
+  static constexpr int NP = 1;
+  const T *z;
+  T *p0, *p1, *q;
+  const T *pold;
+  T *pnew;
+  int cur;
+  __device__ bool skip() const { return !running(); }
+  __device__ void begin() {
+    cur = st->pidx;
+    pold = cur ? p1 : p0;
+    pnew = cur ? p0 : p1;
+  }
+  __device__ T bet(int j) const { return st->first ? T(0) : (T)st->beta[j]; }
+  __device__ T x(int c, int j) const {
+    const int64_t o = (int64_t)c * k + j;
+    return fma(bet(j), pold[o], z[o]);
+  }
+  __device__ void row(int64_t i, int j, T y, double *acc) const {
+    if (!st->active[j]) return;
+    const int64_t o = i * k + j;
+    const T pn = fma(bet(j), pold[o], z[o]);
+    pnew[o] = pn;
+    q[o] = y;
+    acc[0] += (double)pn * (double)y;
+  }
+  __device__ void last_lane(int j, const double *s) const {
+    if (!st->active[j]) return;
+    const double a = st->rho[j] / s[0];
+    st->alpha[j] = a;
+    if (s[0] == 0.0 || bad(a)) {
+      st->status[j] = YAFEL_HIP_STATUS_BREAKDOWN;
+      st->active[j] = 0;
+    }
+  }
+  __device__ void last_all() const {
+    st->first = 0;
+    st->pidx = cur ^ 1;
+    all_inactive_done(st);
+  }
+};
+
+// x += alpha p, r -= alpha q, z = D^-1 r; r.z, r.r
+template <class T>
+struct MCgB : MBase {
+  static constexpr int NP = 2;
+  T *x, *r, *z;
+  const T *p0, *p1, *q, *dinv;
+  int prec;
+  const T *p;
+  __device__ bool skip() const { return !running(); }
+  __device__ void begin() { p = st->pidx ? p1 : p0; }
+  __device__ void elem(int64_t i, int j, double *acc) const {
+    if (!st->active[j]) return;
+    const int64_t o = i * k + j;
+    const T a = (T)st->alpha[j];
+    x[o] = fma(a, p[o], x[o]);
+    const T ri = fma(-a, q[o], r[o]);
+    r[o] = ri;
+    T zi = ri;
+    if (prec) {
+      zi = dinv[i] * ri;
+      z[o] = zi;
+    }
+    acc[0] += (double)ri * (double)zi;
+    acc[1] += (double)ri * (double)ri;
+  }
+  __device__ void last_lane(int j, const double *sm) const {
+    if (!st->active[j]) return;
+    request_pause_or_advance(st, j, sm[0], sm[1], true);
+  }
+  __device__ void last_all() const {
+    st->iter += 1;
+    all_inactive_done(st);
+  }
+};
+
+// FP64 residual R = B - A X_hi -> r (inner precision), z (CG), rhat (init,
+// BiCGSTAB); per RHS: r.w, r.r, true r.r, b.b.  MODE 0 init, 1 refinement.
+template <class TL, int MODE>
+struct MRes : MBase {
+  static constexpr int NP = 4;
+  const double *X, *B;
+  TL *r, *z, *rhat;
+  const TL *dinvl;
+  int cgfam, prec;
+  __device__ bool skip() const { return MODE == 0 ? false : (st->done || !st->paused); }
+  __device__ void begin() {}
+  __device__ double x(int c, int j) const { return X[(int64_t)c * k + j]; }
+  __device__ void row(int64_t i, int j, double y, double *acc) const {
+    const int64_t o = i * k + j;
+    const bool act = MODE == 0 || st->active[j];
+    const double rh = B[o] - y;
+    const TL rl = (TL)rh;
+    double w = (double)rl;
+    if (act) {
+      r[o] = rl;
+      if (cgfam && prec) {
+        const TL zi = dinvl[i] * rl;
+        z[o] = zi;
+        w = (double)zi;
+      } else if (!cgfam) {
+        if (MODE == 0) rhat[o] = rl;
+        else w = (double)rhat[o];
+      }
+    }
+    acc[0] += (double)rl * w;
+    acc[1] += (double)rl * (double)rl;
+    acc[2] += rh * rh;
+    acc[3] += B[o] * B[o];
+  }
+  __device__ void last_lane(int j, const double *s) const {
+    st->true_rr[j] = s[2];
+    if (MODE == 0) {
+      st->bb[j] = s[3];
+      st->rho[j] = s[0];
+      st->rho0[j] = s[0];
+      st->rho_prev[j] = 1.0;
+      st->rr[j] = s[1];
+      st->rr_ref[j] = s[1];
+      st->alpha[j] = 1.0;
+      st->omega[j] = 1.0;
+      st->beta[j] = 0.0;
+      st->it[j] = 0;
+      st->status[j] = YAFEL_HIP_STATUS_MAXITER;
+      st->active[j] = 1;
+      if (s[3] == 0.0 || sqrt(s[2] / s[3]) < st->tol) {
+        st->status[j] = YAFEL_HIP_STATUS_CONVERGED;
+        st->active[j] = 0;
+        st->crit[j] = s[3] == 0.0 ? 0.0 : sqrt(s[2] / s[3]);
+      }
+      return;
+    }
+    if (!st->active[j]) return;
+    const double crit = sqrt(s[2] / st->bb[j]);
+    st->crit[j] = crit;
+    if (bad(s[2])) {
+      st->status[j] = YAFEL_HIP_STATUS_NAN;
+      st->active[j] = 0;
+      return;
+    }
+    if (crit < st->tol) {
+      st->status[j] = YAFEL_HIP_STATUS_CONVERGED;
+      st->active[j] = 0;
+      return;
+    }
+    if (st->it[j] >= st->max_iter) {
+      st->status[j] = YAFEL_HIP_STATUS_MAXITER;
+      st->active[j] = 0;
+      return;
+    }
+    // rho_prev[j] is rho before the last iteration for every lane -- whether the
+    // lane asked for this refinement (rho not advanced) or advanced normally
+    if (cgfam) st->beta[j] = s[0] / st->rho_prev[j];
+    else st->beta[j] = (s[0] / st->rho_prev[j]) * (st->alpha[j] / st->omega[j]);
+    st->rho[j] = s[0];
+    st->rr[j] = s[1];
+    st->rr_ref[j] = s[1];
+  }
+  __device__ void last_all() const {
+    if (MODE == 0) {
+      st->first = 1;
+      st->paused = 0;
+      st->pidx = 0;
+    } else {
+      st->refinements += 1;
+      st->paused = 0;
+    }
+    all_inactive_done(st);
+  }
+};
+
+// x_hi += x_lo, x_lo = 0 (MIXED, while paused)
+template <class TL>
+struct MFold : MBase {
+  static constexpr int NP = 0;
+  double *X;
+  TL *xl;
+  __device__ bool skip() const { return st->done || !st->paused; }
+  __device__ void begin() {}
+  __device__ void elem(int64_t i, int j, double *) const {
+    const int64_t o = i * k + j;
+    X[o] += (double)xl[o];
+    xl[o] = TL(0);
+  }
+  __device__ void last_lane(int, const double *) const {}
+};
+
+// final per-RHS true residual
+struct MReport : MBase {
+  static constexpr int NP = 2;
+  const double *X, *B;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() {}
+  __device__ double x(int c, int j) const { return X[(int64_t)c * k + j]; }
+  __device__ void row(int64_t i, int j, double y, double *acc) const {
+    const double r = B[i * k + j] - y;
+    acc[0] += r * r;
+    acc[1] += B[i * k + j] * B[i * k + j];
+  }
+  __device__ void last_lane(int j, const double *s) const {
+    st->true_rr[j] = s[0];
+    st->bb[j] = s[1];
+  }
+};
+
+template <class T>
+CsrView<T> cview(yafel_hip_matrix_s *A) {
+  return CsrView<T>{A->ptr.as<int>(), A->col.as<int>(),
+                    (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->rows};
+}
+
+struct Multi {
+  yafel_hip_ctx_s *ctx;
+  yafel_hip_matrix_s *A;
+  yafel_hip_solve_opts o;
+  int64_t n;
+  int k;
+  bool mixed, cgfam, prec;
+  MState *st;
+  double *part;
+  hipStream_t s;
+  int grid;
+  DBuf vecs[9];
+  DBuf partbuf, stbuf;
+
+  template <class O> O base(O op, int c) const {
+    op.st = st;
+    op.part = part;
+    op.counter = st->counter + c;
+    op.k = k;
+    return op;
+  }
+  template <class T, class O> void spmm(const O &op) {
+    k_mspmm<T, O><<<grid, NT, 0, s>>>(cview<T>(A), op);
+    HIPCHK(hipGetLastError());
+  }
+  template <class O> void elem(const O &op) {
+    k_melem<O><<<grid, NT, 0, s>>>(n, op);
+    HIPCHK(hipGetLastError());
+  }
+
+  template <class T>
+  void run(const double *B, double *X, yafel_hip_solve_info *info) {
+    const size_t nk = (size_t)n * k;
+    for (auto &v : vecs) v.alloc(nk * sizeof(T));
+    T *r = vecs[0].as<T>(), *rhat = vecs[1].as<T>(), *p0 = vecs[2].as<T>(), *p1 = vecs[3].as<T>();
+    T *v = vecs[4].as<T>(), *sv = vecs[5].as<T>(), *t = vecs[6].as<T>(), *xl = vecs[7].as<T>();
+    T *z = vecs[8].as<T>();
+    const T *dinv = (const T *)(sizeof(T) == 8 ? A->dinv64.p : A->dinv32.p);
+    HIPCHK(hipMemsetAsync(p0, 0, nk * sizeof(T), s));
+    HIPCHK(hipMemsetAsync(v, 0, nk * sizeof(T), s));
+    if (mixed) HIPCHK(hipMemsetAsync(xl, 0, nk * sizeof(T), s));
+    MState h{};
+    h.k = k;
+    h.method = o.method;
+    h.stop = o.stop;
+    h.mixed = mixed;
+    h.tol = o.tol;
+    h.delta2 = o.refine_delta * o.refine_delta;
+    h.max_iter = o.max_iter;
+    static_assert(sizeof(MState) + 3 * 1024 <= 65536, "pinned staging too small");
+    std::memcpy(ctx->pinned, &h, sizeof(MState));
+    HIPCHK(hipMemcpyAsync(st, ctx->pinned, sizeof(MState), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const T *zz = prec ? z : r;
+    auto residual = [&](int mode) {
+      auto fill = [&](auto op) {
+        op.X = X; op.B = B; op.r = r; op.z = z; op.rhat = rhat; op.dinvl = dinv;
+        op.cgfam = cgfam; op.prec = prec;
+        return op;
+      };
+      if (mode == 0) spmm<double>(fill(base(MRes<T, 0>{}, 0)));
+      else spmm<double>(fill(base(MRes<T, 1>{}, 0)));
+    };
+    residual(0);
+    const int K = o.check_every > 0 ? o.check_every : 16;
+    auto iteration = [&]() {
+      if (cgfam) {
+        MCgA<T> a = base(MCgA<T>{}, 1);
+        a.z = zz; a.p0 = p0; a.p1 = p1; a.q = v;
+        spmm<T>(a);
+        MCgB<T> bb = base(MCgB<T>{}, 2);
+        bb.x = mixed ? xl : (T *)X; bb.r = r; bb.z = z; bb.p0 = p0; bb.p1 = p1; bb.q = v; bb.dinv = dinv;
+        bb.prec = prec;
+        elem(bb);
+      } else {
+        MP<T> pp = base(MP<T>{}, 1);
+        pp.r = r; pp.v = v; pp.p = p0;
+        elem(pp);
+        MV<T> vv = base(MV<T>{}, 2);
+        vv.p = p0; vv.rhat = rhat; vv.dinv = dinv; vv.v = v;
+        spmm<T>(vv);
+        MS<T> ss = base(MS<T>{}, 3);
+        ss.r = r; ss.v = v; ss.s = sv;
+        elem(ss);
+        MT<T> tt = base(MT<T>{}, 4);
+        tt.s = sv; tt.dinv = dinv; tt.t = t;
+        spmm<T>(tt);
+        MX<T> xx = base(MX<T>{}, 5);
+        xx.x = mixed ? xl : (T *)X; xx.r = r; xx.p = p0; xx.s = sv; xx.t = t; xx.rhat = rhat; xx.dinv = dinv;
+        elem(xx);
+      }
+    };
+    auto refine = [&]() {
+      if (mixed) {
+        MFold<T> f = base(MFold<T>{}, 6);
+        f.X = X;
+        f.xl = xl;
+        elem(f);
+      }
+      residual(1);
+    };
+    char *pin = (char *)ctx->pinned;
+    hipEvent_t ev[2];
+    HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    const size_t slot = 1024;  // status headers after the state image
+    auto chunk = [&](int si) {
+      for (int i = 0; i < K; ++i) iteration();
+      refine();
+      HIPCHK(hipMemcpyAsync(pin + 16384 + slot * si, st, 64, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipEventRecord(ev[si], s));
+    };
+    chunk(0);
+    chunk(1);
+    int polled = 0;
+    int64_t guard = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (true) {
+      HIPCHK(hipEventSynchronize(ev[polled & 1]));
+      const int *hs_done = (const int *)(pin + 16384 + slot * (polled & 1));
+      if (hs_done[0]) break;
+      chunk(polled & 1);
+      ++polled;
+      if (++guard > 2 * o.max_iter + 1000) fail(YAFEL_HIP_EDEVICE, "solve_multi: no progress");
+      if ((guard & 63) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(900))
+        fail(YAFEL_HIP_EDEVICE, "solve_multi: exceeded 900 s");
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    MReport rep = base(MReport{}, 7);
+    rep.X = X;
+    rep.B = B;
+    spmm<double>(rep);
+    MState fs;
+    HIPCHK(hipMemcpyAsync(&fs, st, sizeof(MState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int j = 0; j < k; ++j) {
+      info[j].status = fs.status[j];
+      info[j].iterations = fs.it[j];
+      info[j].refinements = fs.refinements;
+      info[j].criterion = fs.crit[j];
+      info[j].rel_residual = fs.bb[j] > 0 ? std::sqrt(fs.true_rr[j] / fs.bb[j]) : 0.0;
+    }
+  }
+};
+
+}  // namespace
+
 extern "C" int yafel_hip_solve_multi(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, int k,
                                      const double *B_dev, double *X_dev, yafel_hip_solve_info *info) {
   YH_TRY_BEGIN
-  (void)opts; (void)k; (void)B_dev; (void)X_dev; (void)info;
-  fail(YAFEL_HIP_EINVAL, "solve_multi: not available in this build");
+  REQUIRE(A && opts && info && k > 0, "solve_multi: bad argument");
+  REQUIRE(A->format == YAFEL_HIP_CSR, "solve_multi: CSR matrices only");
+  REQUIRE(A->rows == A->cols, "solve_multi: matrix must be square");
+  REQUIRE(A->rows == 0 || (B_dev && X_dev), "solve_multi: null vector");
+  REQUIRE(opts->method >= 0 && opts->method <= 3, "solve_multi: unknown method");
+  REQUIRE(opts->precision == YAFEL_HIP_FP64 || opts->precision == YAFEL_HIP_MIXED,
+          "solve_multi: precision must be FP64 or MIXED");
+  REQUIRE(opts->stop == YAFEL_HIP_STOP_RELRES, "solve_multi: the multi-RHS solver stops on the relative residual");
+  yafel_hip_ctx_s *ctx = A->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  const int64_t n = A->rows;
+  std::memset(info, 0, sizeof(*info) * k);
+  if (n == 0) return YAFEL_HIP_OK;
+  REQUIRE(A->has64, "solve_multi: matrix has no FP64 values");
+  if (!A->diag_ok) matrix_build_diag(A);
+  const bool mixed = opts->precision == YAFEL_HIP_MIXED;
+  if (mixed && !A->has32) matrix_build_fp32(A);
+  HIPCHK(hipEventRecord(ctx->t0, ctx->stream));
+  // batches of up to 64 right-hand sides, each a contiguous column block;
+  // the batch is copied into a packed n x kb buffer and back
+  for (int j0 = 0; j0 < k; j0 += KMAX) {
+    const int kb = k - j0 < KMAX ? k - j0 : KMAX;
+    Multi M{};
+    M.ctx = ctx;
+    M.A = A;
+    M.o = *opts;
+    M.n = n;
+    M.k = kb;
+    M.mixed = mixed;
+    M.cgfam = opts->method == YAFEL_HIP_CG || opts->method == YAFEL_HIP_PCG_JACOBI;
+    M.prec = opts->method == YAFEL_HIP_PCG_JACOBI || opts->method == YAFEL_HIP_BICGSTAB_JACOBI;
+    if (!M.prec && !M.cgfam) fail(YAFEL_HIP_EINVAL, "solve_multi: BiCGSTAB runs with right Jacobi preconditioning only");
+    if (M.o.max_iter <= 0) M.o.max_iter = M.cgfam ? n : 10 * n;
+    if (M.o.refine_delta <= 0) M.o.refine_delta = 0.1;
+    M.s = ctx->stream;
+    M.grid = grid_for(ctx, n * 64, NT);
+    M.partbuf.alloc((size_t)4 * M.grid * 64 * sizeof(double) + 4096);
+    M.stbuf.alloc(sizeof(MState) + 256);
+    M.part = M.partbuf.as<double>();
+    M.st = M.stbuf.as<MState>();
+    DBuf Bb, Xb;
+    const double *B = B_dev;
+    double *X = X_dev;
+    if (kb != k) {
+      Bb.alloc((size_t)n * kb * 8);
+      Xb.alloc((size_t)n * kb * 8);
+      HIPCHK(hipMemcpy2DAsync(Bb.p, kb * 8, B_dev + j0, (size_t)k * 8, kb * 8, n, hipMemcpyDeviceToDevice, ctx->stream));
+      HIPCHK(hipMemcpy2DAsync(Xb.p, kb * 8, X_dev + j0, (size_t)k * 8, kb * 8, n, hipMemcpyDeviceToDevice, ctx->stream));
+      B = Bb.as<double>();
+      X = Xb.as<double>();
+    }
+    if (mixed) M.run<float>(B, X, info + j0);
+    else M.run<double>(B, X, info + j0);
+    if (kb != k)
+      HIPCHK(hipMemcpy2DAsync(X_dev + j0, (size_t)k * 8, Xb.p, kb * 8, kb * 8, n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  HIPCHK(hipEventRecord(ctx->t1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->t1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+  for (int j = 0; j < k; ++j) info[j].ms = ms;
+  return YAFEL_HIP_OK;
   YH_TRY_END(A ? A->ctx : nullptr)
 }

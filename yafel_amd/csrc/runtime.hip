@@ -69,7 +69,7 @@ int yafel_hip_ctx_create(int device, yafel_hip_ctx *out) {
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&c->t0));
   HIPCHK(hipEventCreate(&c->t1));
-  c->pinned_bytes = 4096;
+  c->pinned_bytes = 65536;
   HIPCHK(hipHostMalloc(&c->pinned, c->pinned_bytes, hipHostMallocDefault));
   *out = c;
   return YAFEL_HIP_OK;

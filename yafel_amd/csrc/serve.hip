@@ -1,0 +1,663 @@
+// Device-resident request loop with continuous batching.
+//
+// Every admitted request owns a slot: its DoF numbering, pattern, assembled
+// FP64 + FP32 matrix, Jacobi diagonal and FP32 Krylov vectors stay on the
+// device.  A table of slot descriptors drives batched launches: one launch
+// covers every active slot, each slot on its own contiguous block range
+// (block counts fixed by the slot's size alone) with its own FP64 scalars and
+// its own last-block reductions -- so a request's result is bit-identical
+// whether it runs alone or in a mix.  The host loop (progress) admits queued
+// requests under the HBM budget on a separate setup stream while up to two
+// chunks of batched iterations run on the solve stream, and retires slots
+// whose solve finished.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <map>
+
+#include "krylov_ops.hpp"
+#include "mesh.hpp"
+
+using namespace yh;
+using namespace yh::kr;
+
+namespace {
+
+constexpr int GCAP = 1024;  // block cap per slot and phase
+
+struct SlotDesc {
+  int fmt, lanes;
+  int g_spmv, g_elem;
+  int off_spmv, off_elem;
+  int64_t n, brows, nb;
+  const int *ptr, *col;
+  const float *v32;
+  const double *v64;
+  const float *dinv;
+  float *r, *z, *p0, *p1, *q, *xl;
+  double *x;
+  const double *b;
+  KState *st;
+  double *part;
+  double tol;
+  long long max_iter;
+};
+
+__device__ __forceinline__ int find_slot(const SlotDesc *d, int ns, int b, bool spmv) {
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int off = spmv ? d[mid].off_spmv : d[mid].off_elem;
+    if (off <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+enum { PH_INIT = 0, PH_A = 1, PH_B = 2, PH_FOLD = 3, PH_REF = 4 };
+
+template <class O>
+__device__ __forceinline__ void bind(O &op, const SlotDesc &d, int counter) {
+  op.st = d.st;
+  op.part = d.part;
+  op.counter = d.st->counter + counter;
+}
+
+template <class T, class O>
+__device__ __forceinline__ void run_spmv(const SlotDesc &d, const T *val, O &op, int blk) {
+  const MatView<T> A{d.ptr, d.col, val, d.brows, d.nb};
+  if (d.fmt == YAFEL_HIP_BCSR3) spmv_fused_body<YAFEL_HIP_BCSR3, 32, T, O>(A, op, blk, d.g_spmv);
+  else if (d.lanes >= 16) spmv_fused_body<YAFEL_HIP_CSR, 16, T, O>(A, op, blk, d.g_spmv);
+  else spmv_fused_body<YAFEL_HIP_CSR, 8, T, O>(A, op, blk, d.g_spmv);
+}
+
+template <int PH>
+__global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slots, int ns) {
+  constexpr bool SP = PH == PH_INIT || PH == PH_A || PH == PH_REF;
+  const int s = find_slot(slots, ns, blockIdx.x, SP);
+  const SlotDesc d = slots[s];
+  const int blk = blockIdx.x - (SP ? d.off_spmv : d.off_elem);
+  if (PH == PH_INIT || PH == PH_REF) {
+    OpResidual<float, PH == PH_INIT ? 0 : 1> op{};
+    bind(op, d, 0);
+    op.xh = d.x;
+    op.b = d.b;
+    op.r = d.r;
+    op.z = d.z;
+    op.dinv = d.dinv;
+    op.rhat = nullptr;
+    op.wkind = 1;
+    run_spmv<double>(d, d.v64, op, blk);
+  } else if (PH == PH_A) {
+    OpCgA<float> op{};
+    bind(op, d, 1);
+    op.z = d.z;
+    op.pbuf0 = d.p0;
+    op.pbuf1 = d.p1;
+    op.q = d.q;
+    run_spmv<float>(d, d.v32, op, blk);
+  } else if (PH == PH_B) {
+    OpCgB<float, true> op{};
+    bind(op, d, 2);
+    op.x = d.xl;
+    op.r = d.r;
+    op.z = d.z;
+    op.pbuf0 = d.p0;
+    op.pbuf1 = d.p1;
+    op.q = d.q;
+    op.dinv = d.dinv;
+    elem_body(d.n, op, blk, d.g_elem);
+  } else {
+    OpFold<float> op{};
+    bind(op, d, 6);
+    op.xh = d.x;
+    op.xl = d.xl;
+    elem_body(d.n, op, blk, d.g_elem);
+  }
+}
+
+// zero a joining slot's solver state and set its configuration
+__global__ void kb_init_state(const SlotDesc *slots, int ns) {
+  const int j = blockIdx.x;
+  if (j >= ns) return;
+  KState *st = slots[j].st;
+  unsigned int *w = (unsigned int *)st;
+  for (int i = threadIdx.x; i < (int)(sizeof(KState) / 4); i += blockDim.x) w[i] = 0u;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st->tol = slots[j].tol;
+    st->stop = YAFEL_HIP_STOP_RELRES;
+    st->delta2 = 0.01;
+    st->max_iter = slots[j].max_iter;
+    st->method = 1;  // Jacobi-PCG
+    st->mixed = 1;
+  }
+}
+
+struct SlotStatus {
+  int done, status;
+  long long iter, refinements;
+  double crit;
+};
+
+__global__ void kb_status(const SlotDesc *slots, int ns, SlotStatus *out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  const KState *st = slots[j].st;
+  out[j].done = st->done;
+  out[j].status = st->status;
+  out[j].iter = st->iter;
+  out[j].refinements = st->refinements;
+  out[j].crit = st->crit;
+}
+
+struct Req {
+  int64_t id;
+  int state = YAFEL_HIP_REQ_QUEUED;
+  int dim, p, dpn, bc;
+  double tol;
+  int64_t max_iter;
+  std::vector<double> nodes, coef, load;
+  std::vector<int32_t> cells;
+  yafel_hip_physics phys;
+  size_t est_bytes = 0;
+  int64_t ndofs = 0;
+  std::vector<double> x;
+  yafel_hip_solve_info info{};
+  std::string error;
+};
+
+struct Slot {
+  Req *req = nullptr;
+  yafel_hip_matrix A = nullptr;
+  DBuf b, mask;
+  void *buf = nullptr;  // one stream-ordered allocation for the solve vectors and state
+  size_t buf_bytes = 0;
+  SlotDesc d{};
+  bool joined = false, retired = false;
+};
+
+template <class T>
+T *carve(char *&p, size_t count) {
+  T *r = (T *)p;
+  p += ((count * sizeof(T)) + 255) & ~(size_t)255;
+  return r;
+}
+
+}  // namespace
+
+struct yafel_hip_server_s {
+  yafel_hip_ctx_s *ctx = nullptr;   // solve stream
+  yafel_hip_ctx_s *sctx = nullptr;  // setup stream (same device)
+  size_t budget = 0, in_use = 0, peak = 0;
+  int max_slots = 0;
+  int64_t next_id = 1;
+  std::map<int64_t, std::unique_ptr<Req>> reqs;
+  std::deque<Req *> queue;
+  std::vector<Slot *> active;
+  std::vector<Slot *> grave;  // retired, buffers freed stream-ordered
+  yafel_hip_server_stats stats{};
+  // descriptor / status rings: each progress call uses at most two entries and
+  // at most two calls' work is in flight, so an entry is never rewritten while
+  // a launch that reads it can still run
+  static constexpr int RING = 8;
+  static constexpr size_t STAGE = 32768;  // pinned bytes per ring entry (table / statuses)
+  DBuf desc[RING], status[RING];
+  int ring = 0;
+  struct Inflight {
+    hipEvent_t ev;
+    int ring;
+    std::vector<Slot *> slots;
+    char *pin;
+  };
+  std::deque<Inflight> inflight;
+  hipEvent_t evs[RING];
+  char *pin = nullptr;  // RING status areas, then RING table staging areas
+  int K = 8;
+  std::vector<Slot *> joining;
+};
+
+namespace {
+
+size_t estimate_bytes(const Req &r) {
+  const int64_t nen = (int64_t)std::pow(r.p + 1, r.dim);
+  const int64_t ncell = (int64_t)r.cells.size() / (r.dim == 3 ? 8 : 4);
+  const int64_t T = ncell * nen * nen;                     // element node pairs
+  const int64_t nnodes_up = ncell * nen;
+  const int64_t nld = nen * r.dpn;
+  const int64_t nnz_up = T * r.dpn * r.dpn;
+  size_t b = 0;
+  b += (size_t)T * (8 + 4) * 2 + (size_t)T * 4 * 3;          // pattern sort (keys, values, temp) and plan
+  b += (size_t)ncell * nld * nld * 8;                         // element staging
+  b += (size_t)nnz_up * (8 + 4 + 4);                          // FP64 + FP32 values, column indices
+  b += (size_t)nnodes_up * r.dpn * (8 * 6 + 4 * 8);           // vectors
+  return b + (1 << 20);
+}
+
+void plane_bounds(const Req &r, double lo[3], double hi[3]) {
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = 1e300;
+    hi[d] = -1e300;
+  }
+  const size_t nn = r.nodes.size() / 3;
+  for (size_t i = 0; i < nn; ++i)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], r.nodes[3 * i + d]);
+      hi[d] = std::max(hi[d], r.nodes[3 * i + d]);
+    }
+}
+
+void check_rc(int rc, yafel_hip_ctx_s *c) {
+  if (rc != YAFEL_HIP_OK) fail(rc, yafel_hip_last_error(c));
+}
+
+// Build the request's device system on the setup stream.
+Slot *setup_slot(yafel_hip_server_s *S, Req *r) {
+  auto *sl = new Slot();
+  sl->req = r;
+  yafel_hip_ctx_s *c = S->sctx;
+  yafel_hip_mesh m = nullptr;
+  try {
+    check_rc(yafel_hip_mesh_create_cg(c, r->dim, r->p, r->dpn, (int64_t)r->nodes.size() / 3, r->nodes.data(),
+                                      (int64_t)r->cells.size() / (r->dim == 3 ? 8 : 4), r->cells.data(), &m),
+             c);
+    const int fmt = r->dpn == 3 ? YAFEL_HIP_BCSR3 : YAFEL_HIP_CSR;
+    check_rc(yafel_hip_mesh_pattern(m, fmt, YAFEL_HIP_FP64, &sl->A), c);
+    const int64_t n = m->n_nodes * r->dpn;
+    r->ndofs = n;
+    sl->b.alloc(n * 8);
+    HIPCHK(hipMemsetAsync(sl->b.p, 0, n * 8, c->stream));
+    yafel_hip_physics ph = r->phys;
+    ph.coef = r->coef.empty() ? nullptr : r->coef.data();
+    ph.coef_on_device = 0;
+    int req = YAFEL_HIP_ASSEMBLE_TANGENT;
+    if (r->load.empty()) {
+      req |= YAFEL_HIP_ASSEMBLE_RESIDUAL;
+    } else {
+      REQUIRE((int64_t)r->load.size() == n, "server: load length differs from the number of dofs");
+      HIPCHK(hipMemcpyAsync(sl->b.p, r->load.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    check_rc(yafel_hip_assemble(m, &ph, req, sl->A, sl->b.as<double>()), c);
+    sl->mask.alloc(n);
+    HIPCHK(hipMemsetAsync(sl->mask.p, 0, n, c->stream));
+    double lo[3], hi[3];
+    plane_bounds(*r, lo, hi);
+    const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    const double tol = 1e-9 * (ext > 0 ? ext : 1.0);
+    if (r->bc == YAFEL_HIP_BC_BOX) {
+      for (int d = 0; d < r->dim; ++d) {
+        check_rc(yafel_hip_bc_mask_plane(m, d, lo[d], tol, -1, sl->mask.as<uint8_t>()), c);
+        check_rc(yafel_hip_bc_mask_plane(m, d, hi[d], tol, -1, sl->mask.as<uint8_t>()), c);
+      }
+    } else {
+      check_rc(yafel_hip_bc_mask_plane(m, 0, lo[0], tol, -1, sl->mask.as<uint8_t>()), c);
+    }
+    check_rc(yafel_hip_dirichlet_apply(sl->A, sl->mask.as<uint8_t>(), nullptr, sl->b.as<double>()), c);
+    matrix_build_fp32(sl->A);
+    matrix_build_diag(sl->A);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // the plan and staging are not needed once assembled
+    sl->A->mesh = nullptr;
+    sl->A->plan = nullptr;
+    yafel_hip_mesh_destroy(m);
+    m = nullptr;
+  } catch (...) {
+    if (m) yafel_hip_mesh_destroy(m);
+    if (sl->A) yafel_hip_matrix_destroy(sl->A);
+    delete sl;
+    throw;
+  }
+  return sl;
+}
+
+// Allocate the slot's solve buffers on the solve stream and fill its descriptor.
+void join_slot(yafel_hip_server_s *S, Slot *sl) {
+  yafel_hip_matrix_s *A = sl->A;
+  const int64_t n = A->rows;
+  Req *r = sl->req;
+  const int L = A->format == YAFEL_HIP_BCSR3 ? 32 : (A->lanes >= 16 ? 16 : 8);
+  const int g_spmv = std::max(1, std::min<int>(GCAP, ceil_div(A->brows * (int64_t)L, NT)));
+  const int g_elem = std::max(1, std::min<int>(GCAP, ceil_div(n, NT)));
+  const size_t nbytes = 6 * (((size_t)n * 4 + 255) & ~(size_t)255) + (((size_t)n * 8 + 255) & ~(size_t)255) +
+                        ((sizeof(KState) + 255) & ~(size_t)255) + (size_t)4 * std::max(g_spmv, g_elem) * 8 + 256;
+  HIPCHK(hipMallocAsync(&sl->buf, nbytes, S->ctx->stream));
+  sl->buf_bytes = nbytes;
+  char *p = (char *)sl->buf;
+  SlotDesc &d = sl->d;
+  d.fmt = A->format;
+  d.lanes = L;
+  d.g_spmv = g_spmv;
+  d.g_elem = g_elem;
+  d.n = n;
+  d.brows = A->brows;
+  d.nb = A->nblk;
+  d.ptr = A->ptr.as<int>();
+  d.col = A->col.as<int>();
+  d.v32 = A->val32.as<float>();
+  d.v64 = A->val64.as<double>();
+  d.dinv = A->dinv32.as<float>();
+  d.r = carve<float>(p, n);
+  d.z = carve<float>(p, n);
+  d.p0 = carve<float>(p, n);
+  d.p1 = carve<float>(p, n);
+  d.q = carve<float>(p, n);
+  d.xl = carve<float>(p, n);
+  d.x = carve<double>(p, n);
+  d.st = carve<KState>(p, 1);
+  d.part = carve<double>(p, (size_t)4 * std::max(g_spmv, g_elem));
+  d.b = sl->b.as<double>();
+  hipStream_t s = S->ctx->stream;
+  d.tol = r->tol;
+  d.max_iter = r->max_iter > 0 ? r->max_iter : n;
+  HIPCHK(hipMemsetAsync(d.p0, 0, n * 4, s));
+  HIPCHK(hipMemsetAsync(d.xl, 0, n * 4, s));
+  HIPCHK(hipMemsetAsync(d.x, 0, n * 8, s));
+}
+
+void release_slot(yafel_hip_server_s *S, Slot *sl) {
+  if (sl->buf) (void)hipFreeAsync(sl->buf, S->ctx->stream);
+  sl->buf = nullptr;
+  if (sl->A) yafel_hip_matrix_destroy(sl->A);
+  sl->A = nullptr;
+  sl->b.release();
+  sl->mask.release();
+  S->in_use -= std::min(S->in_use, sl->req ? sl->req->est_bytes : 0);
+}
+
+// upload a descriptor table for `slots` (offsets recomputed) into ring entry
+int upload_table(yafel_hip_server_s *S, const std::vector<Slot *> &slots, int *tot_spmv, int *tot_elem) {
+  std::vector<SlotDesc> h(slots.size());
+  int os = 0, oe = 0;
+  for (size_t i = 0; i < slots.size(); ++i) {
+    h[i] = slots[i]->d;
+    h[i].off_spmv = os;
+    h[i].off_elem = oe;
+    os += h[i].g_spmv;
+    oe += h[i].g_elem;
+  }
+  *tot_spmv = os;
+  *tot_elem = oe;
+  const int e = S->ring;
+  S->ring = (S->ring + 1) % yafel_hip_server_s::RING;
+  const size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(SlotDesc);
+  REQUIRE(bytes <= yafel_hip_server_s::STAGE, "server: too many slots for the descriptor table");
+  if (S->desc[e].bytes < yafel_hip_server_s::STAGE) S->desc[e].alloc(yafel_hip_server_s::STAGE);
+  if (S->status[e].bytes < yafel_hip_server_s::STAGE) S->status[e].alloc(yafel_hip_server_s::STAGE);
+  // stream-ordered copy from this entry's own pinned staging area
+  char *stage = S->pin + (yafel_hip_server_s::RING + e) * yafel_hip_server_s::STAGE;
+  std::memcpy(stage, h.data(), bytes);
+  HIPCHK(hipMemcpyAsync(S->desc[e].p, stage, bytes, hipMemcpyHostToDevice, S->ctx->stream));
+  return e;
+}
+
+template <int PH>
+void launch_phase(yafel_hip_server_s *S, int e, int ns, int blocks) {
+  if (ns == 0 || blocks == 0) return;
+  kb_phase<PH><<<blocks, NT, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns);
+  HIPCHK(hipGetLastError());
+  S->stats.batched_launches++;
+}
+
+void drain_one(yafel_hip_server_s *S, std::vector<int64_t> &done_ids) {
+  auto f = S->inflight.front();
+  S->inflight.pop_front();
+  HIPCHK(hipEventSynchronize(f.ev));
+  const SlotStatus *st = (const SlotStatus *)f.pin;
+  for (size_t i = 0; i < f.slots.size(); ++i) {
+    Slot *sl = f.slots[i];
+    if (sl->retired || !st[i].done) continue;
+    Req *r = sl->req;
+    r->info.status = st[i].status;
+    r->info.iterations = st[i].iter;
+    r->info.refinements = st[i].refinements;
+    r->info.criterion = st[i].crit;
+    r->info.rel_residual = st[i].crit;  // the FP64 true residual at the last refinement
+    r->x.resize(sl->d.n);
+    // x is final: its chunk completed; copy on the setup stream
+    HIPCHK(hipMemcpyAsync(r->x.data(), sl->d.x, sl->d.n * 8, hipMemcpyDeviceToHost, S->sctx->stream));
+    HIPCHK(hipStreamSynchronize(S->sctx->stream));
+    r->state = YAFEL_HIP_REQ_DONE;
+    S->stats.completed++;
+    done_ids.push_back(r->id);
+    sl->retired = true;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int yafel_hip_server_create(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots, yafel_hip_server *out) {
+  YH_TRY_BEGIN
+  REQUIRE(ctx && out && max_slots > 0 && hbm_budget > 0, "server_create: bad argument");
+  REQUIRE((size_t)max_slots * sizeof(SlotDesc) <= yafel_hip_server_s::STAGE, "server_create: max_slots too large");
+  HIPCHK(hipSetDevice(ctx->device));
+  auto *S = new yafel_hip_server_s();
+  S->ctx = ctx;
+  int rc = yafel_hip_ctx_create(ctx->device, &S->sctx);
+  if (rc != YAFEL_HIP_OK) {
+    delete S;
+    fail(rc, yafel_hip_last_error(nullptr));
+  }
+  S->budget = hbm_budget;
+  S->max_slots = max_slots;
+  S->stats.hbm_budget = hbm_budget;
+  for (auto &e : S->evs) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIPCHK(hipHostMalloc((void **)&S->pin, 2 * yafel_hip_server_s::RING * yafel_hip_server_s::STAGE,
+                       hipHostMallocDefault));
+  *out = S;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(ctx)
+}
+
+int yafel_hip_server_destroy(yafel_hip_server S) {
+  if (!S) return YAFEL_HIP_OK;
+  (void)hipStreamSynchronize(S->ctx->stream);
+  for (Slot *sl : S->active) {
+    release_slot(S, sl);
+    delete sl;
+  }
+  for (Slot *sl : S->joining) {
+    release_slot(S, sl);
+    delete sl;
+  }
+  for (Slot *sl : S->grave) delete sl;
+  (void)hipStreamSynchronize(S->ctx->stream);
+  for (auto &e : S->evs) (void)hipEventDestroy(e);
+  if (S->pin) (void)hipHostFree(S->pin);
+  yafel_hip_ctx_destroy(S->sctx);
+  delete S;
+  return YAFEL_HIP_OK;
+}
+
+int yafel_hip_server_submit(yafel_hip_server S, const yafel_hip_request *q, int64_t *id) {
+  YH_TRY_BEGIN
+  REQUIRE(S && q && id, "server_submit: null argument");
+  REQUIRE(q->dim == 2 || q->dim == 3, "server_submit: dim must be 2 or 3");
+  REQUIRE(q->n_corner > 0 && q->ncell > 0 && q->corner_nodes && q->cells, "server_submit: empty mesh");
+  REQUIRE(q->tol > 0, "server_submit: tol must be positive");
+  auto r = std::make_unique<Req>();
+  r->id = S->next_id++;
+  r->dim = q->dim;
+  r->p = q->p;
+  r->dpn = q->dof_per_node;
+  r->bc = q->bc;
+  r->tol = q->tol;
+  r->max_iter = q->max_iter;
+  r->nodes.assign(q->corner_nodes, q->corner_nodes + q->n_corner * 3);
+  r->cells.assign(q->cells, q->cells + q->ncell * (q->dim == 3 ? 8 : 4));
+  r->phys = q->phys;
+  if (q->phys.coef) {
+    REQUIRE(!q->phys.coef_on_device, "server_submit: coefficients must be host data");
+    r->coef.assign(q->phys.coef, q->phys.coef + q->ncell);
+  }
+  r->phys.coef = nullptr;
+  r->est_bytes = estimate_bytes(*r);
+  if (r->est_bytes > S->budget) {
+    S->stats.failed++;
+    fail(YAFEL_HIP_ENOMEM, "server_submit: the request alone needs " + std::to_string(r->est_bytes) +
+                                " bytes, more than the HBM budget");
+  }
+  if (q->load) {
+    REQUIRE(q->load_len > 0, "server_submit: load given without its length");
+    r->load.assign(q->load, q->load + q->load_len);
+  }
+  *id = r->id;
+  S->queue.push_back(r.get());
+  S->reqs[r->id] = std::move(r);
+  S->stats.submitted++;
+  S->stats.queued = (int64_t)S->queue.size();
+  return YAFEL_HIP_OK;
+  YH_TRY_END(S ? S->ctx : nullptr)
+}
+
+int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids, int *n_done) {
+  YH_TRY_BEGIN
+  REQUIRE(S && n_done, "server_progress: null argument");
+  HIPCHK(hipSetDevice(S->ctx->device));
+  std::vector<int64_t> done;
+  // 1. admission (setup stream; overlaps the chunks in flight)
+  while (!S->queue.empty() && (int)(S->active.size() + S->joining.size()) < S->max_slots) {
+    Req *r = S->queue.front();
+    if (r->state == YAFEL_HIP_REQ_CANCELLED) {
+      S->queue.pop_front();
+      continue;
+    }
+    if (S->in_use + r->est_bytes > S->budget) break;  // FIFO: wait for memory
+    S->queue.pop_front();
+    try {
+      Slot *sl = setup_slot(S, r);
+      S->in_use += r->est_bytes;
+      S->peak = std::max(S->peak, S->in_use);
+      S->joining.push_back(sl);
+      r->state = YAFEL_HIP_REQ_RUNNING;
+      S->stats.admitted++;
+    } catch (const Error &e) {
+      r->state = YAFEL_HIP_REQ_FAILED;
+      r->error = e.msg;
+      S->stats.failed++;
+      done.push_back(r->id);
+    }
+  }
+  // 2. drain the oldest chunk when two are in flight
+  while (S->inflight.size() >= 2) drain_one(S, done);
+  // 3. retire finished slots (their x is on the host); buffers freed stream-ordered
+  std::vector<Slot *> keep;
+  for (Slot *sl : S->active) {
+    if (sl->retired) {
+      release_slot(S, sl);
+      S->grave.push_back(sl);
+    } else {
+      keep.push_back(sl);
+    }
+  }
+  S->active.swap(keep);
+  // 4. joining slots: allocate, initialise (FP64 residual) and add to the batch
+  if (!S->joining.empty()) {
+    for (Slot *sl : S->joining) join_slot(S, sl);
+    int ts, te;
+    const int e = upload_table(S, S->joining, &ts, &te);
+    kb_init_state<<<(int)S->joining.size(), 64, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(),
+                                                                     (int)S->joining.size());
+    HIPCHK(hipGetLastError());
+    launch_phase<PH_INIT>(S, e, (int)S->joining.size(), ts);
+    for (Slot *sl : S->joining) S->active.push_back(sl);
+    S->joining.clear();
+  }
+  // 5. one chunk for every active slot
+  if (!S->active.empty()) {
+    int ts, te;
+    const int e = upload_table(S, S->active, &ts, &te);
+    const int ns = (int)S->active.size();
+    for (int k = 0; k < S->K; ++k) {
+      launch_phase<PH_A>(S, e, ns, ts);
+      launch_phase<PH_B>(S, e, ns, te);
+    }
+    launch_phase<PH_FOLD>(S, e, ns, te);
+    launch_phase<PH_REF>(S, e, ns, ts);
+    kb_status<<<ceil_div(ns, 64), 64, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns,
+                                                            S->status[e].as<SlotStatus>());
+    HIPCHK(hipGetLastError());
+    yafel_hip_server_s::Inflight f;
+    f.ev = S->evs[e];
+    f.ring = e;
+    f.slots = S->active;
+    f.pin = S->pin + e * yafel_hip_server_s::STAGE;
+    HIPCHK(hipMemcpyAsync(f.pin, S->status[e].p, ns * sizeof(SlotStatus), hipMemcpyDeviceToHost, S->ctx->stream));
+    HIPCHK(hipEventRecord(f.ev, S->ctx->stream));
+    S->inflight.push_back(f);
+    S->stats.chunks++;
+  } else {
+    while (!S->inflight.empty()) drain_one(S, done);
+  }
+  // graveyard entries are safe to delete once nothing in flight references them
+  if (S->inflight.empty()) {
+    for (Slot *sl : S->grave) delete sl;
+    S->grave.clear();
+  }
+  S->stats.queued = (int64_t)S->queue.size();
+  S->stats.active = (int64_t)S->active.size();
+  S->stats.hbm_in_use = S->in_use;
+  S->stats.hbm_peak = S->peak;
+  *n_done = 0;
+  for (int64_t id : done) {
+    if (done_ids && *n_done < max_ids) done_ids[*n_done] = id;
+    ++*n_done;
+  }
+  return YAFEL_HIP_OK;
+  YH_TRY_END(S ? S->ctx : nullptr)
+}
+
+int yafel_hip_server_query(yafel_hip_server S, int64_t id, int *state, int64_t *ndofs) {
+  YH_TRY_BEGIN
+  REQUIRE(S, "server_query: null server");
+  auto it = S->reqs.find(id);
+  REQUIRE(it != S->reqs.end(), "server_query: unknown request id");
+  if (state) *state = it->second->state;
+  if (ndofs) *ndofs = it->second->ndofs;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(S ? S->ctx : nullptr)
+}
+
+int yafel_hip_server_result(yafel_hip_server S, int64_t id, double *x, int64_t n, yafel_hip_solve_info *info) {
+  YH_TRY_BEGIN
+  REQUIRE(S, "server_result: null server");
+  auto it = S->reqs.find(id);
+  REQUIRE(it != S->reqs.end(), "server_result: unknown request id");
+  Req *r = it->second.get();
+  if (r->state == YAFEL_HIP_REQ_FAILED) fail(YAFEL_HIP_ESTATE, "server_result: request failed: " + r->error);
+  REQUIRE(r->state == YAFEL_HIP_REQ_DONE, "server_result: request not finished");
+  REQUIRE(n >= (int64_t)r->x.size(), "server_result: output too short");
+  if (x) std::memcpy(x, r->x.data(), r->x.size() * 8);
+  if (info) *info = r->info;
+  S->reqs.erase(it);
+  return YAFEL_HIP_OK;
+  YH_TRY_END(S ? S->ctx : nullptr)
+}
+
+int yafel_hip_server_cancel(yafel_hip_server S, int64_t id) {
+  YH_TRY_BEGIN
+  REQUIRE(S, "server_cancel: null server");
+  auto it = S->reqs.find(id);
+  REQUIRE(it != S->reqs.end(), "server_cancel: unknown request id");
+  Req *r = it->second.get();
+  if (r->state == YAFEL_HIP_REQ_QUEUED) {
+    r->state = YAFEL_HIP_REQ_CANCELLED;
+    S->stats.cancelled++;
+  } else {
+    fail(YAFEL_HIP_ESTATE, "server_cancel: only queued requests can be cancelled");
+  }
+  return YAFEL_HIP_OK;
+  YH_TRY_END(S ? S->ctx : nullptr)
+}
+
+int yafel_hip_server_stats_get(yafel_hip_server S, yafel_hip_server_stats *st) {
+  YH_TRY_BEGIN
+  REQUIRE(S && st, "server_stats: null argument");
+  *st = S->stats;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(S ? S->ctx : nullptr)
+}
+
+}  // extern "C"

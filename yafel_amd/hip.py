@@ -55,6 +55,22 @@ class Physics(C.Structure):
                 ("h", _D), ("alpha", _D), ("Q", _D)]
 
 
+class Request(C.Structure):
+    _fields_ = [("dim", _I), ("p", _I), ("dof_per_node", _I), ("n_corner", _I64), ("corner_nodes", _P),
+                ("ncell", _I64), ("cells", _P), ("phys", Physics), ("load", _P), ("load_len", _I64),
+                ("bc", _I), ("tol", _D), ("max_iter", _I64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", _I64), ("admitted", _I64), ("completed", _I64), ("failed", _I64),
+                ("cancelled", _I64), ("queued", _I64), ("active", _I64), ("chunks", _I64),
+                ("batched_launches", _I64), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t),
+                ("hbm_budget", C.c_size_t)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def _sig(name, res, args):
     fn = getattr(lib, name)
     fn.restype = res
@@ -91,6 +107,14 @@ for _n, _a in [
     ("yafel_hip_bc_mask_plane", [_P, _I, _D, _D, _I, _P]),
     ("yafel_hip_structured_mesh", [_I, _I, _I, _I, _I, _P, _P]),
     ("yafel_hip_normal_fill", [C.c_uint64, _I64, _D, _D, _P]),
+    ("yafel_hip_server_create", [_P, C.c_size_t, _I, C.POINTER(_P)]),
+    ("yafel_hip_server_destroy", [_P]),
+    ("yafel_hip_server_submit", [_P, C.POINTER(Request), C.POINTER(_I64)]),
+    ("yafel_hip_server_progress", [_P, _P, _I, C.POINTER(_I)]),
+    ("yafel_hip_server_query", [_P, _I64, C.POINTER(_I), C.POINTER(_I64)]),
+    ("yafel_hip_server_result", [_P, _I64, _P, _I64, C.POINTER(SolveInfo)]),
+    ("yafel_hip_server_cancel", [_P, _I64]),
+    ("yafel_hip_server_stats_get", [_P, C.POINTER(ServerStats)]),
 ]:
     _sig(_n, _I, _a)
 _sig("yafel_hip_last_error", C.c_char_p, [_P])
@@ -390,3 +414,72 @@ class Mesh:
 
     def bc_mask_plane(self, mask_dev, axis, value, tol=1e-6, comp=-1):
         check(lib.yafel_hip_bc_mask_plane(self.h, axis, value, tol, comp, mask_dev.ptr), self.ctx.h)
+
+
+BC_BOX, BC_XMIN = 0, 1
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
+
+
+class Server:
+    """Device-resident request loop with continuous batching (yafel_hip_server_*)."""
+
+    def __init__(self, ctx, hbm_budget, max_slots=64):
+        self.ctx = ctx
+        h = _P()
+        check(lib.yafel_hip_server_create(ctx.h, int(hbm_budget), max_slots, C.byref(h)), ctx.h)
+        self.h = h
+        self._ids = (_I64 * 4096)()
+
+    def close(self):
+        if self.h:
+            lib.yafel_hip_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, dim, p, dpn, nodes, cells, ph, load=None, bc=BC_BOX, tol=1e-8, max_iter=0):
+        nodes = np.ascontiguousarray(nodes, np.float64)
+        cells = np.ascontiguousarray(cells, np.int32)
+        q = Request()
+        q.dim, q.p, q.dof_per_node = dim, p, dpn
+        q.n_corner, q.corner_nodes = nodes.shape[0], nodes.ctypes.data
+        q.ncell, q.cells = cells.shape[0], cells.ctypes.data
+        q.phys = ph
+        if load is not None:
+            load = np.ascontiguousarray(load, np.float64)
+            q.load, q.load_len = load.ctypes.data, load.size
+        else:
+            q.load, q.load_len = None, 0
+        q.bc, q.tol, q.max_iter = bc, tol, max_iter
+        rid = _I64()
+        check(lib.yafel_hip_server_submit(self.h, C.byref(q), C.byref(rid)), self.ctx.h)
+        return rid.value
+
+    def progress(self):
+        n = _I()
+        check(lib.yafel_hip_server_progress(self.h, self._ids, len(self._ids), C.byref(n)), self.ctx.h)
+        return [self._ids[i] for i in range(min(n.value, len(self._ids)))]
+
+    def query(self, rid):
+        st, nd = _I(), _I64()
+        check(lib.yafel_hip_server_query(self.h, rid, C.byref(st), C.byref(nd)), self.ctx.h)
+        return st.value, nd.value
+
+    def result(self, rid):
+        st, nd = self.query(rid)
+        x = np.zeros(max(nd, 1))
+        info = SolveInfo()
+        check(lib.yafel_hip_server_result(self.h, rid, _ptr(x), x.size, C.byref(info)), self.ctx.h)
+        return x[:nd], info
+
+    def cancel(self, rid):
+        check(lib.yafel_hip_server_cancel(self.h, rid), self.ctx.h)
+
+    def stats(self):
+        s = ServerStats()
+        check(lib.yafel_hip_server_stats_get(self.h, C.byref(s)), self.ctx.h)
+        return s.as_dict()

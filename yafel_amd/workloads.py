@@ -105,3 +105,46 @@ def c5(ctx, n=512, nrhs=64):
     B = np.stack([hip.normal(SEED_C5_RHS + j, w.n) for j in range(nrhs)], axis=1)
     w.B_host = np.ascontiguousarray(B)
     return w
+
+
+# ---------------------------------------------------------------- serving
+def c4_requests(n_req, seed=4000, lo2=16, hi2=256, lo3=8, hi3=48):
+    """The served stream's requests: 2-D Poisson (Q1 or Q2, equal odds) or, one
+    in four, 3-D Q1 elasticity; cells per axis log-uniform in [16,256] (2-D) or
+    [8,48] (3-D); lognormal(0, 0.5) per-element coefficients; N(0,1) loads per
+    dof; homogeneous Dirichlet on every face (Poisson) or a clamped x = 0 face
+    (elasticity).  Deterministic in `seed`."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n_req):
+        if rng.random() < 0.25:
+            dim, p, dpn = 3, 1, 3
+            n = int(round(math.exp(rng.uniform(math.log(lo3), math.log(hi3)))))
+            nel, ndofs = n ** 3, (n + 1) ** 3 * 3
+        else:
+            dim, dpn = 2, 1
+            p = 1 if rng.random() < 0.5 else 2
+            n = int(round(math.exp(rng.uniform(math.log(lo2), math.log(hi2)))))
+            nel, ndofs = n * n, (p * n + 1) ** 2
+        reqs.append(dict(dim=dim, p=p, dpn=dpn, n=n, seed=seed * 7919 + i, nel=nel, ndofs=ndofs))
+    return reqs
+
+
+def c4_materialise(r):
+    """Host arrays of one request (mesh, coefficients, load, physics)."""
+    dim, p, n = r["dim"], r["p"], r["n"]
+    nodes, cells = hip.structured_mesh(dim, n, n, n if dim == 3 else 0, p)
+    coef = np.exp(hip.normal(r["seed"], r["nel"], 0.0, 0.5))
+    load = hip.normal(r["seed"] + 1, r["ndofs"])
+    if dim == 3:
+        ph = hip.physics(hip.ELASTICITY, 3, coef=coef, lam=LAMBDA, mu=MU)
+        bc = hip.BC_XMIN
+    else:
+        ph = hip.physics(hip.POISSON, 2, coef=coef)
+        bc = hip.BC_BOX
+    return dict(dim=dim, p=p, dpn=r["dpn"], nodes=nodes, cells=cells, ph=ph, load=load, bc=bc)
+
+
+def poisson_arrivals(n, rate, seed):
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
