@@ -167,22 +167,6 @@ __device__ __forceinline__ int xcd_block() {
   return (b & 7) * (G >> 3) + (b >> 3);
 }
 
-// Signals completion of this block's partial; returns true in the last block
-// to finish (all partials are then visible).  counter must start at 0 and is
-// reset by the last block.
-__device__ __forceinline__ bool last_block_done(unsigned int *counter, unsigned int nblk) {
-  __shared__ bool is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int prev = atomicAdd(counter, 1u);
-    is_last = (prev == nblk - 1);
-  }
-  __syncthreads();
-  if (is_last) __threadfence();
-  return is_last;
-}
-
 // Deterministic sum of npart partials (fixed order), executed by one block.
 template <int NT>
 __device__ __forceinline__ double reduce_partials(const double *part, int npart, int stride,

@@ -62,6 +62,10 @@ void launch_spmv_op(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
     }
   }
   HIPCHK(hipGetLastError());
+  if constexpr (Op::NP > 0) {
+    k_finish<Op><<<1, NT, 0, s>>>(op, grid);
+    HIPCHK(hipGetLastError());
+  }
 }
 
 template <class Op>
@@ -69,6 +73,10 @@ void launch_elem(yafel_hip_ctx_s *ctx, int64_t n, const Op &op, hipStream_t s) {
   const int grid = grid_for(ctx, n, NT);
   k_elem<Op><<<grid, NT, 0, s>>>(n, op);
   HIPCHK(hipGetLastError());
+  if constexpr (Op::NP > 0) {
+    k_finish<Op><<<1, NT, 0, s>>>(op, grid);
+    HIPCHK(hipGetLastError());
+  }
 }
 
 // ------------------------------------------------------------- solve driver

@@ -44,31 +44,47 @@ template <class T> struct MatView {
   int64_t nb;
 };
 
-// Block-reduce NP sums into op.part[j * nblk + blk]; the last of the nblk
-// blocks folds them in block order and runs op.last().
+// Block-reduce NP sums into op.part[j * nblk + blk].  The fold of the
+// partials and the scalar recurrence run in k_finish, a one-workgroup kernel
+// launched right after on the same stream: kernel boundaries make the
+// partials visible across XCDs without a device-scope fence in every
+// workgroup (on gfx950 such a fence writes back and invalidates the whole L2,
+// which would throw away the gathered vector's cache reuse).
 template <int NP, class Op>
 __device__ __forceinline__ void finish_partials(Op &op, double (&acc)[NP > 0 ? NP : 1], int blk, int nblk) {
   if constexpr (NP > 0) {
     __shared__ double lds[NT / 64];
-    __shared__ double sums[NP];
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
       double s = dev::block_sum<NT>(acc[j], lds);
       if (threadIdx.x == 0) op.part[(size_t)j * nblk + blk] = s;
     }
-    if (dev::last_block_done(op.counter, nblk)) {
-#pragma unroll
-      for (int j = 0; j < NP; ++j) {
-        double s = dev::reduce_partials<NT>(op.part + (size_t)j * nblk, nblk, 1, lds);
-        if (threadIdx.x == 0) sums[j] = s;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        *op.counter = 0;
-        op.last(sums);
-      }
-    }
   }
+}
+
+// Fold the nblk partials of each sum in block order (deterministic) and run
+// op.last().  Evaluates op.skip() exactly as the producing kernel did: only
+// op.last() changes the state, so both see the same flags.
+template <class Op>
+__device__ __forceinline__ void finish_body(Op &op, int nblk) {
+  if (op.skip()) return;
+  op.begin();
+  constexpr int NP = Op::NP;
+  __shared__ double lds[NT / 64];
+  __shared__ double sums[NP > 0 ? NP : 1];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    double s = dev::reduce_partials<NT>(op.part + (size_t)j * nblk, nblk, 1, lds);
+    if (threadIdx.x == 0) sums[j] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) op.last(sums);
+}
+
+template <class Op>
+__global__ __launch_bounds__(NT) void k_finish(Op op0, int nblk) {
+  Op op = op0;
+  finish_body(op, nblk);
 }
 
 // SpMV with a fused per-row epilogue and up to 4 block-reduced sums; the last

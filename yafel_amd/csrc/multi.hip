@@ -43,13 +43,12 @@ template <class T> struct CsrView {
 };
 
 // Per-lane block reduction of NP sums into part[(j*grid + blk)*64 + lane];
-// the last block folds them (fixed order) and calls op.last_lane per RHS,
-// then op.last_all once.
+// k_mfinish folds them (fixed order) in a one-workgroup kernel launched next
+// on the stream and calls op.last_lane per RHS, then op.last_all once.
 template <int NP, class Op>
 __device__ __forceinline__ void finish(Op &op, double (&acc)[NP > 0 ? NP : 1]) {
   if constexpr (NP > 0) {
     __shared__ double lds[NP][NT / 64][64];
-    __shared__ double sums[NP][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int blk = dev::xcd_block();
 #pragma unroll
@@ -64,38 +63,44 @@ __device__ __forceinline__ void finish(Op &op, double (&acc)[NP > 0 ? NP : 1]) {
         op.part[((size_t)j * gridDim.x + blk) * 64 + lane] = s;
       }
     }
-    if (dev::last_block_done(op.counter, gridDim.x)) {
-      // thread (w, lane) sums blocks w, w+4, ... ; then the 4 quarters in order
+  }
+}
+
+template <class Op>
+__global__ __launch_bounds__(NT) void k_mfinish(Op op0, int nblk) {
+  Op op = op0;
+  if (op.skip()) return;
+  op.begin();
+  constexpr int NP = Op::NP;
+  __shared__ double lds[NP][NT / 64][64];
+  __shared__ double sums[NP][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // thread (w, lane) sums blocks w, w+4, ... ; then the 4 quarters in order
 #pragma unroll
-      for (int j = 0; j < NP; ++j) {
-        double s = 0;
-        for (int b = w; b < (int)gridDim.x; b += NT / 64) s += op.part[((size_t)j * gridDim.x + b) * 64 + lane];
-        lds[j][w][lane] = s;
-      }
-      __syncthreads();
-      if (w == 0) {
+  for (int j = 0; j < NP; ++j) {
+    double s = 0;
+    for (int b = w; b < nblk; b += NT / 64) s += op.part[((size_t)j * nblk + b) * 64 + lane];
+    lds[j][w][lane] = s;
+  }
+  __syncthreads();
+  if (w == 0) {
 #pragma unroll
-        for (int j = 0; j < NP; ++j) {
-          double s = 0;
+    for (int j = 0; j < NP; ++j) {
+      double s = 0;
 #pragma unroll
-          for (int q = 0; q < NT / 64; ++q) s += lds[j][q][lane];
-          sums[j][lane] = s;
-        }
-      }
-      __syncthreads();
-      if (w == 0 && lane < op.st->k) {
-        double v[NP];
-#pragma unroll
-        for (int j = 0; j < NP; ++j) v[j] = sums[j][lane];
-        op.last_lane(lane, v);
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        *op.counter = 0;
-        op.last_all();
-      }
+      for (int q = 0; q < NT / 64; ++q) s += lds[j][q][lane];
+      sums[j][lane] = s;
     }
   }
+  __syncthreads();
+  if (w == 0 && lane < op.st->k) {
+    double v[NP];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) v[j] = sums[j][lane];
+    op.last_lane(lane, v);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) op.last_all();
 }
 
 // Y-producing kernel: wave per row, y[lane] = sum_q val[q] x(col[q], lane).
@@ -538,10 +543,18 @@ struct Multi {
   template <class T, class O> void spmm(const O &op) {
     k_mspmm<T, O><<<grid, NT, 0, s>>>(cview<T>(A), op);
     HIPCHK(hipGetLastError());
+    if constexpr (O::NP > 0) {
+      k_mfinish<O><<<1, NT, 0, s>>>(op, grid);
+      HIPCHK(hipGetLastError());
+    }
   }
   template <class O> void elem(const O &op) {
     k_melem<O><<<grid, NT, 0, s>>>(n, op);
     HIPCHK(hipGetLastError());
+    if constexpr (O::NP > 0) {
+      k_mfinish<O><<<1, NT, 0, s>>>(op, grid);
+      HIPCHK(hipGetLastError());
+    }
   }
 
   template <class T>

@@ -72,12 +72,12 @@ __device__ __forceinline__ void run_spmv(const SlotDesc &d, const T *val, O &op,
   else spmv_fused_body<YAFEL_HIP_CSR, 8, T, O>(A, op, blk, d.g_spmv);
 }
 
-template <int PH>
-__global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slots, int ns) {
+// Builds the phase's operation for slot d and either runs the phase body on
+// block blk (FIN = false) or folds the slot's partials (FIN = true).
+template <int PH, bool FIN>
+__device__ __forceinline__ void slot_phase(const SlotDesc &d, int blk) {
   constexpr bool SP = PH == PH_INIT || PH == PH_A || PH == PH_REF;
-  const int s = find_slot(slots, ns, blockIdx.x, SP);
-  const SlotDesc d = slots[s];
-  const int blk = blockIdx.x - (SP ? d.off_spmv : d.off_elem);
+  const int nblk = SP ? d.g_spmv : d.g_elem;
   if (PH == PH_INIT || PH == PH_REF) {
     OpResidual<float, PH == PH_INIT ? 0 : 1> op{};
     bind(op, d, 0);
@@ -88,7 +88,8 @@ __global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slot
     op.dinv = d.dinv;
     op.rhat = nullptr;
     op.wkind = 1;
-    run_spmv<double>(d, d.v64, op, blk);
+    if (FIN) finish_body(op, nblk);
+    else run_spmv<double>(d, d.v64, op, blk);
   } else if (PH == PH_A) {
     OpCgA<float> op{};
     bind(op, d, 1);
@@ -96,7 +97,8 @@ __global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slot
     op.pbuf0 = d.p0;
     op.pbuf1 = d.p1;
     op.q = d.q;
-    run_spmv<float>(d, d.v32, op, blk);
+    if (FIN) finish_body(op, nblk);
+    else run_spmv<float>(d, d.v32, op, blk);
   } else if (PH == PH_B) {
     OpCgB<float, true> op{};
     bind(op, d, 2);
@@ -107,14 +109,30 @@ __global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slot
     op.pbuf1 = d.p1;
     op.q = d.q;
     op.dinv = d.dinv;
-    elem_body(d.n, op, blk, d.g_elem);
-  } else {
+    if (FIN) finish_body(op, nblk);
+    else elem_body(d.n, op, blk, nblk);
+  } else if (!FIN) {
     OpFold<float> op{};
     bind(op, d, 6);
     op.xh = d.x;
     op.xl = d.xl;
-    elem_body(d.n, op, blk, d.g_elem);
+    elem_body(d.n, op, blk, nblk);
   }
+}
+
+template <int PH>
+__global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slots, int ns) {
+  constexpr bool SP = PH == PH_INIT || PH == PH_A || PH == PH_REF;
+  const int s = find_slot(slots, ns, blockIdx.x, SP);
+  const SlotDesc d = slots[s];
+  slot_phase<PH, false>(d, blockIdx.x - (SP ? d.off_spmv : d.off_elem));
+}
+
+// one workgroup per slot folds that slot's partials
+template <int PH>
+__global__ __launch_bounds__(NT) void kb_finish(const SlotDesc *__restrict__ slots, int ns) {
+  const SlotDesc d = slots[blockIdx.x];
+  slot_phase<PH, true>(d, 0);
 }
 
 // zero a joining slot's solver state and set its configuration
@@ -396,6 +414,10 @@ void launch_phase(yafel_hip_server_s *S, int e, int ns, int blocks) {
   if (ns == 0 || blocks == 0) return;
   kb_phase<PH><<<blocks, NT, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns);
   HIPCHK(hipGetLastError());
+  if (PH != PH_FOLD) {
+    kb_finish<PH><<<ns, NT, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns);
+    HIPCHK(hipGetLastError());
+  }
   S->stats.batched_launches++;
 }
 
