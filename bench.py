@@ -272,7 +272,8 @@ def main():
     bytes64 = bcsr_bytes(inf.nblocks, nbr, w.n, 8)
     gbs32 = bytes32 / (ms32 * 1e-3) / 1e9
     gbs64 = bytes64 / (ms64 * 1e-3) / 1e9
-    log(f"spmv fp32 {ms32 * 1e3:.1f}us {gbs32:.0f} GB/s; fp64 {ms64 * 1e3:.1f}us {gbs64:.0f} GB/s")
+    copy_gbs = ctx.copy_bandwidth(1 << 30, 20)
+    log(f"spmv fp32 {ms32 * 1e3:.1f}us {gbs32:.0f} GB/s; fp64 {ms64 * 1e3:.1f}us {gbs64:.0f} GB/s; copy {copy_gbs:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -296,7 +297,7 @@ def main():
         "rel_residual": infos[-1].rel_residual, "setup_s": setup_s,
         "fp64_gpu": fp64,
         "speedup_vs_fp64_cg": (value / fp64["cg"]["solves_per_s"]) if "cg" in fp64 else None,
-        "spmv_fp64_gbs": gbs64,
+        "spmv_fp64_gbs": gbs64, "copy_gbs": copy_gbs,
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": gbs32 / PEAK_HBM_GBS, "traffic": None,
                      "kernel": "k_bcsr_spmv<32,float>", "bytes_per_launch": bytes32,

@@ -48,6 +48,9 @@ typedef struct yafel_hip_mesh_s *yafel_hip_mesh;
 /* ---------------------------------------------------------------- context */
 /* No reference counterpart (the reference has no device state). */
 int yafel_hip_ctx_create(int device, yafel_hip_ctx *out);
+/* Drops the caller's reference.  Matrices, meshes and servers created on ctx
+ * hold references of their own, so they stay usable and may be destroyed
+ * after the context; its stream and memory go with the last reference. */
 int yafel_hip_ctx_destroy(yafel_hip_ctx ctx);
 /* last error message of ctx (or of the calling thread when ctx is NULL) */
 const char *yafel_hip_last_error(yafel_hip_ctx ctx);
@@ -102,6 +105,9 @@ int yafel_hip_matrix_download(yafel_hip_matrix A, int64_t *row_ptr, int64_t *col
                               double *data);
 /* refresh the FP32 copy from the FP64 values (allocating it if needed) */
 int yafel_hip_matrix_sync_fp32(yafel_hip_matrix A);
+/* lanes per row (CSR: 2..64) or per block row (BCSR3: 8..64) of the SpMV
+ * kernels; chosen from the mean row length at creation, settable for tuning */
+int yafel_hip_matrix_set_lanes(yafel_hip_matrix A, int lanes);
 /* device pointer to the FP64 diagonal (computed on first use) */
 int yafel_hip_matrix_diagonal(yafel_hip_matrix A, double *diag_dev);
 
@@ -121,6 +127,11 @@ int yafel_hip_spmm(yafel_hip_matrix A, int precision, int k, const void *X_dev, 
  * deterministic two-stage reduction. */
 int yafel_hip_dot(yafel_hip_ctx ctx, int64_t n, const double *a_dev, const double *b_dev,
                   double *result);
+
+/* ------------------------------------------------------------ measurement */
+/* No reference counterpart: achievable HBM bandwidth of a plain copy kernel
+ * (read + write bytes / time) over `bytes` per buffer, `reps` launches. */
+int yafel_hip_copy_bandwidth(yafel_hip_ctx ctx, size_t bytes, int reps, double *gbs);
 
 /* ---------------------------------------------------------------- solvers */
 #define YAFEL_HIP_CG 0              /* cg_solve.hpp:33-81 */

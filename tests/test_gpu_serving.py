@@ -46,7 +46,8 @@ def test_alone_and_in_the_mix_are_identical(ctx):
     mix, st = run_all(H, ctx, reqs, stagger=True)
     assert st["completed"] == len(MIX) and st["failed"] == 0
     for i, m in enumerate(reqs):
-        (x_alone, info_alone), _ = run_all(H, ctx, [m])
+        alone, _ = run_all(H, ctx, [m])
+        x_alone, info_alone = alone[0]
         x_mix, info_mix = mix[i]
         assert info_mix.status == 0 and info_mix.rel_residual < 1e-8
         assert np.array_equal(x_alone, x_mix)          # bit-identical alone vs in the mix

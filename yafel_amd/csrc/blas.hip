@@ -26,7 +26,38 @@ __global__ __launch_bounds__(NT) void k_dot_final(int npart, const double *part,
   if (threadIdx.x == 0) *out = s;
 }
 
+// STREAM-style copy, 16-byte vector loads/stores, grid-stride
+typedef int v4i __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(NT) void k_copy(int64_t n16, const v4i *__restrict__ a, v4i *__restrict__ b) {
+  int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (; i < n16; i += stride) b[i] = __builtin_nontemporal_load(&a[i]);
+}
+
 }  // namespace
+
+extern "C" int yafel_hip_copy_bandwidth(yafel_hip_ctx ctx, size_t bytes, int reps, double *gbs) {
+  YH_TRY_BEGIN
+  REQUIRE(ctx && gbs && reps > 0 && bytes >= 16, "copy_bandwidth: bad argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  const int64_t n16 = (int64_t)(bytes / 16);
+  DBuf a, b;
+  a.alloc(n16 * 16);
+  b.alloc(n16 * 16);
+  HIPCHK(hipMemsetAsync(a.p, 1, n16 * 16, ctx->stream));
+  const int grid = ctx->num_cus * 16;
+  k_copy<<<grid, NT, 0, ctx->stream>>>(n16, a.as<v4i>(), b.as<v4i>());
+  HIPCHK(hipEventRecord(ctx->t0, ctx->stream));
+  for (int r = 0; r < reps; ++r) k_copy<<<grid, NT, 0, ctx->stream>>>(n16, a.as<v4i>(), b.as<v4i>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->t1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->t1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+  *gbs = 2.0 * (double)n16 * 16 * reps / (ms * 1e-3) / 1e9;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(ctx)
+}
 
 extern "C" int yafel_hip_dot(yafel_hip_ctx ctx, int64_t n, const double *a, const double *b,
                              double *result) {

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -64,6 +65,12 @@ struct yafel_hip_ctx_s;
 
 namespace yh {
 void record_error(yafel_hip_ctx_s *ctx, const std::string &msg);
+// Contexts are reference counted: every matrix, mesh and server holds one
+// reference, and yafel_hip_ctx_destroy drops the caller's.  The stream,
+// workspace and pinned memory are freed when the last reference goes, so
+// handles may be destroyed in any order.
+void ctx_retain(yafel_hip_ctx_s *ctx);
+void ctx_release(yafel_hip_ctx_s *ctx);
 }  // namespace yh
 
 // Solver workspace: grown on demand, owned by the context.
@@ -84,6 +91,7 @@ struct yafel_hip_ctx_s {
   YhWorkspace ws;
   void *pinned = nullptr;  // host-pinned status words
   size_t pinned_bytes = 0;
+  std::atomic<int> refs{1};
 };
 
 struct YhPlan;  // assembly gather plan (per mesh & format)
@@ -104,6 +112,7 @@ struct yafel_hip_matrix_s {
   int lanes = 8;                // lanes per row chosen for the CSR kernel
   // reference-layout extras: the host row_ptr may be longer than brows+1
   std::vector<int64_t> ptr_tail;
+  ~yafel_hip_matrix_s() { yh::ctx_release(ctx); }
 };
 
 namespace yh {

@@ -91,8 +91,10 @@ __global__ __launch_bounds__(NT) void k_finish(Op op0, int nblk) {
 // block to finish folds the partials (fixed order) and runs op.last().  The
 // body takes its block id and block count explicitly so that a batched launch
 // can run several independent solves, each on its own block range.
+// T is the stored matrix type; Op::value_type the vector / accumulation type.
 template <int FMT, int L, class T, class Op>
 __device__ __forceinline__ void spmv_fused_body(const MatView<T> &A, Op &op, int blk, int nblk) {
+  using TV = typename Op::value_type;
   if (op.skip()) return;
   op.begin();
   const int lane = threadIdx.x & (L - 1);
@@ -106,11 +108,11 @@ __device__ __forceinline__ void spmv_fused_body(const MatView<T> &A, Op &op, int
   const int64_t r1 = r0 + chunk < A.brows ? r0 + chunk : A.brows;
   for (int64_t r = r0 + threadIdx.x / L; r < r1; r += rpb) {
     if (FMT == YAFEL_HIP_CSR) {
-      T y = dev::csr_row<L, T>(A.ptr, A.col, A.val, r, lane, [&](int c) { return op.x(c); });
+      TV y = dev::csr_row<L, T, TV>(A.ptr, A.col, A.val, r, lane, [&](int c) { return op.x(c); });
       if (lane == 0) op.row(r, y, acc);
     } else {
-      T a[3];
-      dev::bcsr_row<L, T>(A.ptr, A.col, A.val, A.nb, r, lane, a, [&](int64_t c) { return op.x(c); });
+      TV a[3];
+      dev::bcsr_row<L, T, TV>(A.ptr, A.col, A.val, A.nb, r, lane, a, [&](int64_t c) { return op.x(c); });
       if (lane == 0) {
         op.row(3 * r, a[0], acc);
         op.row(3 * r + 1, a[1], acc);
@@ -206,6 +208,7 @@ __device__ void after_iteration(KState *st, double rho_new, double rr) {
 // skipped after a pause or convergence never desynchronise them.
 template <class T>
 struct OpCgA : OpBase {
+  using value_type = T;
   static constexpr int NP = 1;
   const T *z;
   T *pbuf0, *pbuf1;
@@ -274,6 +277,7 @@ struct OpCgB : OpBase {
 // MODE 0 = initialisation, 1 = refinement / verification.
 template <class TL, int MODE>
 struct OpResidual : OpBase {
+  using value_type = double;
   static constexpr int NP = 4;
   const double *xh, *b;
   TL *r, *z;
@@ -413,6 +417,7 @@ struct OpBiP : OpBase {
 // v = A ph, rhat.v -> alpha
 template <class T>
 struct OpBiV : OpBase {
+  using value_type = T;
   static constexpr int NP = 1;
   const T *ph, *rhat;
   T *v;
@@ -454,6 +459,7 @@ struct OpBiS : OpBase {
 // t = A sh, t.t and t.s -> omega
 template <class T>
 struct OpBiT : OpBase {
+  using value_type = T;
   static constexpr int NP = 2;
   const T *sh, *s;
   T *t;
@@ -508,6 +514,7 @@ struct OpBiX : OpBase {
 
 // final report: r = b - A x (FP64), r.r and b.b into out[0..1]
 struct OpReport : OpBase {
+  using value_type = double;
   static constexpr int NP = 2;
   const double *xh, *b;
   double *out;

@@ -387,6 +387,7 @@ int yafel_hip_mesh_create_cg(yafel_hip_ctx ctx, int dim, int p, int dpn, int64_t
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   std::unique_ptr<yafel_hip_mesh_s> m(new yafel_hip_mesh_s());
+  ctx_retain(ctx);
   m->ctx = ctx;
   m->dim = dim;
   m->p = p;

@@ -50,6 +50,32 @@ int grid_for(yafel_hip_ctx_s *ctx, int64_t work_items, int block) {
 
 }  // namespace yh
 
+namespace yh {
+
+static void ctx_free(yafel_hip_ctx_s *ctx) {
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &b : ctx->ws.vec64) b.release();
+  for (auto &b : ctx->ws.vec32) b.release();
+  ctx->ws.partials.release();
+  ctx->ws.state.release();
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  (void)hipEventDestroy(ctx->t0);
+  (void)hipEventDestroy(ctx->t1);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+void ctx_retain(yafel_hip_ctx_s *ctx) {
+  if (ctx) ctx->refs.fetch_add(1, std::memory_order_relaxed);
+}
+
+void ctx_release(yafel_hip_ctx_s *ctx) {
+  if (ctx && ctx->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) ctx_free(ctx);
+}
+
+}  // namespace yh
+
 using namespace yh;
 
 extern "C" {
@@ -77,18 +103,7 @@ int yafel_hip_ctx_create(int device, yafel_hip_ctx *out) {
 }
 
 int yafel_hip_ctx_destroy(yafel_hip_ctx ctx) {
-  if (!ctx) return YAFEL_HIP_OK;
-  (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  for (auto &b : ctx->ws.vec64) b.release();
-  for (auto &b : ctx->ws.vec32) b.release();
-  ctx->ws.partials.release();
-  ctx->ws.state.release();
-  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-  (void)hipEventDestroy(ctx->t0);
-  (void)hipEventDestroy(ctx->t1);
-  (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
+  yh::ctx_release(ctx);
   return YAFEL_HIP_OK;
 }
 

@@ -456,10 +456,12 @@ int yafel_hip_server_create(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots,
   REQUIRE((size_t)max_slots * sizeof(SlotDesc) <= yafel_hip_server_s::STAGE, "server_create: max_slots too large");
   HIPCHK(hipSetDevice(ctx->device));
   auto *S = new yafel_hip_server_s();
+  ctx_retain(ctx);
   S->ctx = ctx;
   int rc = yafel_hip_ctx_create(ctx->device, &S->sctx);
   if (rc != YAFEL_HIP_OK) {
     delete S;
+    ctx_release(ctx);
     fail(rc, yafel_hip_last_error(nullptr));
   }
   S->budget = hbm_budget;
@@ -489,7 +491,9 @@ int yafel_hip_server_destroy(yafel_hip_server S) {
   for (auto &e : S->evs) (void)hipEventDestroy(e);
   if (S->pin) (void)hipHostFree(S->pin);
   yafel_hip_ctx_destroy(S->sctx);
+  yafel_hip_ctx_s *ctx = S->ctx;
   delete S;
+  ctx_release(ctx);
   return YAFEL_HIP_OK;
 }
 

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void k_csr_spmv(int64_t rows, const int *__res
   const int64_t r0 = (int64_t)dev::xcd_block() * chunk;
   const int64_t r1 = r0 + chunk < rows ? r0 + chunk : rows;
   for (int64_t r = r0 + threadIdx.x / L; r < r1 + 0; r += rpb) {
-    T acc = dev::csr_row<L, T>(ptr, col, val, r, lane, [&](int c) { return x[c]; });
+    T acc = dev::csr_row<L, T, T>(ptr, col, val, r, lane, [&](int c) { return x[c]; });
     if (lane == 0) y[r] = acc;
   }
 }
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_bcsr_spmv(int64_t brows, int64_t nb,
   const int64_t r1 = r0 + chunk < brows ? r0 + chunk : brows;
   for (int64_t br = r0 + threadIdx.x / G; br < r1; br += rpb) {
     T a[3];
-    dev::bcsr_row<G, T>(ptr, col, val, nb, br, lane, a, [&](int64_t c) { return x[c]; });
+    dev::bcsr_row<G, T, T>(ptr, col, val, nb, br, lane, a, [&](int64_t c) { return x[c]; });
     if (lane < 3) y[3 * br + lane] = lane == 0 ? a[0] : (lane == 1 ? a[1] : a[2]);
   }
 }
@@ -211,6 +211,7 @@ void launch_bcsr(yafel_hip_matrix_s *A, const T *x, T *y, hipStream_t s) {
 
 yafel_hip_matrix_s *new_matrix(yafel_hip_ctx_s *ctx, int format) {
   auto *A = new yafel_hip_matrix_s();
+  ctx_retain(ctx);
   A->ctx = ctx;
   A->format = format;
   return A;
@@ -462,6 +463,17 @@ int yafel_hip_matrix_sync_fp32(yafel_hip_matrix A) {
   REQUIRE(A && A->has64, "matrix_sync_fp32: matrix has no FP64 values");
   HIPCHK(hipSetDevice(A->ctx->device));
   matrix_build_fp32(A);
+  return YAFEL_HIP_OK;
+  YH_TRY_END(A ? A->ctx : nullptr)
+}
+
+int yafel_hip_matrix_set_lanes(yafel_hip_matrix A, int lanes) {
+  YH_TRY_BEGIN
+  REQUIRE(A, "matrix_set_lanes: null matrix");
+  const bool pow2 = lanes > 0 && (lanes & (lanes - 1)) == 0;
+  if (A->format == YAFEL_HIP_CSR) REQUIRE(pow2 && lanes >= 2 && lanes <= 64, "matrix_set_lanes: CSR lanes must be 2..64, a power of two");
+  else REQUIRE(pow2 && lanes >= 8 && lanes <= 64, "matrix_set_lanes: BCSR3 lanes must be 8..64, a power of two");
+  A->lanes = lanes;
   return YAFEL_HIP_OK;
   YH_TRY_END(A ? A->ctx : nullptr)
 }

@@ -90,9 +90,10 @@ for _n, _a in [
     ("yafel_hip_bcsr3_from_triplets", [_P, _I64, _P, _P, _P, _I, C.POINTER(_P)]),
     ("yafel_hip_matrix_destroy", [_P]), ("yafel_hip_matrix_info_get", [_P, C.POINTER(MatrixInfo)]),
     ("yafel_hip_matrix_download", [_P, _P, _P, _P]), ("yafel_hip_matrix_sync_fp32", [_P]),
-    ("yafel_hip_matrix_diagonal", [_P, _P]),
+    ("yafel_hip_matrix_diagonal", [_P, _P]), ("yafel_hip_matrix_set_lanes", [_P, _I]),
     ("yafel_hip_spmv", [_P, _I, _P, _P]), ("yafel_hip_spmv_host", [_P, _P, _P]),
     ("yafel_hip_spmm", [_P, _I, _I, _P, _P]), ("yafel_hip_dot", [_P, _I64, _P, _P, C.POINTER(_D)]),
+    ("yafel_hip_copy_bandwidth", [_P, C.c_size_t, _I, C.POINTER(_D)]),
     ("yafel_hip_solve_opts_default", [C.POINTER(SolveOpts)]),
     ("yafel_hip_solve", [_P, C.POINTER(SolveOpts), _P, _P, C.POINTER(SolveInfo)]),
     ("yafel_hip_solve_host", [_P, C.POINTER(SolveOpts), _P, _P, _P, C.POINTER(SolveInfo)]),
@@ -173,6 +174,11 @@ class Context:
         d = DeviceArray(self, a.size, a.dtype)
         d.upload(a)
         return d
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=20):
+        g = _D()
+        check(lib.yafel_hip_copy_bandwidth(self.h, nbytes, reps, C.byref(g)), self.h)
+        return g.value
 
     def dot(self, a, b):
         r = _D()
@@ -275,6 +281,9 @@ class Matrix:
         val = np.zeros(i.nnz)
         check(lib.yafel_hip_matrix_download(self.h, _ptr(ptr), _ptr(col), _ptr(val)), self.ctx.h)
         return ptr, col, val
+
+    def set_lanes(self, lanes):
+        check(lib.yafel_hip_matrix_set_lanes(self.h, lanes), self.ctx.h)
 
     def sync_fp32(self):
         check(lib.yafel_hip_matrix_sync_fp32(self.h), self.ctx.h)
