@@ -36,6 +36,7 @@ extern "C" {
 #define YAFEL_HIP_FP64 1
 #define YAFEL_HIP_FP32 2
 #define YAFEL_HIP_MIXED 3      /* solver: FP32 inner iteration inside FP64 refinement */
+#define YAFEL_HIP_MIXED_MATRIX 4 /* solver: FP32 matrix streams, FP64 vectors/scalars, FP64 refinement */
 
 /* matrix formats */
 #define YAFEL_HIP_CSR 1
@@ -105,9 +106,9 @@ int yafel_hip_matrix_download(yafel_hip_matrix A, int64_t *row_ptr, int64_t *col
                               double *data);
 /* refresh the FP32 copy from the FP64 values (allocating it if needed) */
 int yafel_hip_matrix_sync_fp32(yafel_hip_matrix A);
-/* lanes per row (CSR: 2..64) or per block row (BCSR3: 8..64) of the SpMV
- * kernels; chosen from the mean row length at creation, settable for tuning */
-int yafel_hip_matrix_set_lanes(yafel_hip_matrix A, int lanes);
+/* slots per step (1, 2, 4 or 8; default 4) of the sliced SpMV kernel behind
+ * yafel_hip_spmv; a tuning knob, results are identical for every value */
+int yafel_hip_matrix_set_unroll(yafel_hip_matrix A, int unroll);
 /* device pointer to the FP64 diagonal (computed on first use) */
 int yafel_hip_matrix_diagonal(yafel_hip_matrix A, double *diag_dev);
 
@@ -151,18 +152,18 @@ int yafel_hip_copy_bandwidth(yafel_hip_ctx ctx, size_t bytes, int reps, double *
 
 typedef struct {
   int method;          /* YAFEL_HIP_CG ... */
-  int precision;       /* YAFEL_HIP_FP64 or YAFEL_HIP_MIXED */
+  int precision;       /* YAFEL_HIP_FP64, YAFEL_HIP_MIXED or YAFEL_HIP_MIXED_MATRIX */
   int stop;            /* YAFEL_HIP_STOP_REFERENCE or YAFEL_HIP_STOP_RELRES */
   double tol;
   int64_t max_iter;    /* 0: the reference's limit (n for CG/PCG, 10 n for BiCGSTAB) */
   int check_every;     /* iterations per host status poll (0: default 16) */
-  double refine_delta; /* MIXED: refine when the inner residual fell by this factor (0: 0.1) */
+  double refine_delta; /* MIXED*: refine when the inner residual fell by this factor (0: 0.1) */
 } yafel_hip_solve_opts;
 
 typedef struct {
   int status;           /* YAFEL_HIP_STATUS_* */
-  int64_t iterations;   /* Krylov iterations (inner iterations for MIXED) */
-  int64_t refinements;  /* FP64 residual replacements (MIXED) */
+  int64_t iterations;   /* Krylov iterations (inner iterations for MIXED*) */
+  int64_t refinements;  /* FP64 residual replacements (MIXED*) */
   double rel_residual;  /* true ||b - A x|| / ||b|| in FP64 at exit */
   double criterion;     /* last value of the stopping quantity */
   double ms;            /* device time of the solve */

@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "internal.hpp"
+#include "kernels.hpp"
 #include "mesh.hpp"
 
 using namespace yh;
@@ -211,7 +212,9 @@ __global__ __launch_bounds__(NT) void k_gather(int64_t nruns, int nen, const int
                                                const double *__restrict__ Ke,
                                                const int32_t *__restrict__ run_row,
                                                const int32_t *__restrict__ node_ptr,
-                                               const int32_t *__restrict__ spos, double *__restrict__ val) {
+                                               const int32_t *__restrict__ sptr,
+                                               const int32_t *__restrict__ iperm, int64_t ntot,
+                                               double *__restrict__ val) {
   const int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x;
   if (j >= nruns) return;
   const int nld = nen * DPN;
@@ -233,21 +236,22 @@ __global__ __launch_bounds__(NT) void k_gather(int64_t nruns, int nen, const int
 #pragma unroll
       for (int jj = 0; jj < DPN; ++jj) acc[i][jj] += k[i * nld + jj];
   }
+  // sliced position of the run's entries: run j is entry t of node row A
+  const int A = run_row[j];
+  const int64_t t = j - node_ptr[A];
   if (FMT == YAFEL_HIP_BCSR3) {
+    const int64_t k = dev::sell_pos(sptr, iperm, A, t);
 #pragma unroll
     for (int i = 0; i < DPN; ++i)
 #pragma unroll
-      for (int jj = 0; jj < DPN; ++jj) val[(int64_t)(i * 3 + jj) * nruns + j] = acc[i][jj];
-  } else if (DPN == 1) {
-    val[j] = acc[0][0];
+      for (int jj = 0; jj < DPN; ++jj) val[(int64_t)(i * 3 + jj) * ntot + k] = acc[i][jj];
   } else {
-    const int A = run_row[j];
-    const int64_t len = node_ptr[A + 1] - node_ptr[A];
-    const int64_t p0 = spos[j];
+    // scalar row A * DPN + i holds component jj of run j at entry t * DPN + jj
 #pragma unroll
     for (int i = 0; i < DPN; ++i)
 #pragma unroll
-      for (int jj = 0; jj < DPN; ++jj) val[p0 + i * DPN * len + jj] = acc[i][jj];
+      for (int jj = 0; jj < DPN; ++jj)
+        val[dev::sell_pos(sptr, iperm, (int64_t)A * DPN + i, t * DPN + jj)] = acc[i][jj];
   }
 }
 
@@ -273,12 +277,14 @@ __global__ void k_sub(int64_t n, const double *a, double *rhs) {
   if (i < n) rhs[i] -= a[i];
 }
 
-__global__ void k_dirichlet_csr(int64_t rows, const int32_t *ptr, const int32_t *col, double *val,
-                                const uint8_t *mask, const double *g, double *rhs) {
+__global__ void k_dirichlet_csr(int64_t rows, const int32_t *ptr, const int32_t *sptr, const int32_t *iperm,
+                                const int32_t *col, double *val, const uint8_t *mask, const double *g, double *rhs) {
   int64_t r = blockIdx.x * (int64_t)NT + threadIdx.x;
   if (r >= rows) return;
   const bool mr = mask[r] != 0;
-  for (int k = ptr[r]; k < ptr[r + 1]; ++k) {
+  const int len = ptr[r + 1] - ptr[r];
+  for (int t = 0; t < len; ++t) {
+    const int64_t k = dev::sell_pos(sptr, iperm, r, t);
     const int c = col[k];
     if (mr || mask[c]) {
       val[k] = 0;
@@ -289,21 +295,23 @@ __global__ void k_dirichlet_csr(int64_t rows, const int32_t *ptr, const int32_t 
     }
   }
 }
-
-__global__ void k_dirichlet_bcsr(int64_t brows, int64_t nb, const int32_t *ptr, const int32_t *col,
-                                 double *val, const uint8_t *mask, const double *g, double *rhs) {
+__global__ void k_dirichlet_bcsr(int64_t brows, int64_t ntot, const int32_t *ptr, const int32_t *sptr,
+                                 const int32_t *iperm, const int32_t *col, double *val, const uint8_t *mask,
+                                 const double *g, double *rhs) {
   int64_t br = blockIdx.x * (int64_t)NT + threadIdx.x;
   if (br >= brows) return;
-  for (int k = ptr[br]; k < ptr[br + 1]; ++k) {
+  const int len = ptr[br + 1] - ptr[br];
+  for (int t = 0; t < len; ++t) {
+    const int64_t k = dev::sell_pos(sptr, iperm, br, t);
     const int64_t bc = col[k];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
         const int64_t r = 3 * br + i, c = 3 * bc + j;
         if (mask[r] || mask[c]) {
-          val[(int64_t)(i * 3 + j) * nb + k] = 0;
+          val[(int64_t)(i * 3 + j) * ntot + k] = 0;
           if (r == c) {
             rhs[r] = g ? g[c] : 0.0;
-            val[(int64_t)(i * 3 + j) * nb + k] = 1;
+            val[(int64_t)(i * 3 + j) * ntot + k] = 1;
           }
         }
       }
@@ -441,25 +449,26 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
   if (wk) {
     YhPlan *pl = A->plan;
     const int g = ceil_div(pl->nruns, NT);
-    const int32_t *sp = pl->scal_pos.as<int32_t>();
+    const int32_t *sp = A->sptr.as<int32_t>();
+    const int32_t *ip = A->iperm.p ? A->iperm.as<int32_t>() : nullptr;
     double *val = A->val64.as<double>();
     if (pl->nruns > 0) {
       if (A->format == YAFEL_HIP_BCSR3)
         k_gather<3, YAFEL_HIP_BCSR3><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
                                                       m->stageK.as<double>(), pl->run_row.as<int32_t>(),
-                                                      pl->node_ptr.as<int32_t>(), sp, val);
+                                                      pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val);
       else if (m->dpn == 1)
         k_gather<1, YAFEL_HIP_CSR><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
                                                     m->stageK.as<double>(), pl->run_row.as<int32_t>(),
-                                                    pl->node_ptr.as<int32_t>(), sp, val);
+                                                    pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val);
       else if (m->dpn == 2)
         k_gather<2, YAFEL_HIP_CSR><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
                                                     m->stageK.as<double>(), pl->run_row.as<int32_t>(),
-                                                    pl->node_ptr.as<int32_t>(), sp, val);
+                                                    pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val);
       else
         k_gather<3, YAFEL_HIP_CSR><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
                                                     m->stageK.as<double>(), pl->run_row.as<int32_t>(),
-                                                    pl->node_ptr.as<int32_t>(), sp, val);
+                                                    pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val);
       HIPCHK(hipGetLastError());
     }
     A->has32 = false;
@@ -499,11 +508,14 @@ int yafel_hip_dirichlet_apply(yafel_hip_matrix A, const uint8_t *mask_dev, const
     HIPCHK(hipGetLastError());
   }
   const double *gv = g_dev ? gm.as<double>() : nullptr;
+  const int32_t *ip = A->iperm.p ? A->iperm.as<int32_t>() : nullptr;
   if (A->format == YAFEL_HIP_CSR)
-    k_dirichlet_csr<<<ceil_div(A->brows, NT), NT, 0, s>>>(A->brows, A->ptr.as<int32_t>(), A->col.as<int32_t>(),
-                                                          A->val64.as<double>(), mask_dev, gv, rhs_dev);
+    k_dirichlet_csr<<<ceil_div(A->brows, NT), NT, 0, s>>>(A->brows, A->ptr.as<int32_t>(), A->sptr.as<int32_t>(), ip,
+                                                          A->col.as<int32_t>(), A->val64.as<double>(), mask_dev, gv,
+                                                          rhs_dev);
   else
-    k_dirichlet_bcsr<<<ceil_div(A->brows, NT), NT, 0, s>>>(A->brows, A->nblk, A->ptr.as<int32_t>(), A->col.as<int32_t>(),
+    k_dirichlet_bcsr<<<ceil_div(A->brows, NT), NT, 0, s>>>(A->brows, A->ntot, A->ptr.as<int32_t>(),
+                                                           A->sptr.as<int32_t>(), ip, A->col.as<int32_t>(),
                                                            A->val64.as<double>(), mask_dev, gv, rhs_dev);
   HIPCHK(hipGetLastError());
   A->has32 = false;

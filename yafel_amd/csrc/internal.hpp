@@ -96,20 +96,34 @@ struct yafel_hip_ctx_s {
 
 struct YhPlan;  // assembly gather plan (per mesh & format)
 
+// Device storage of both formats is sliced (SELL-64): the stored rows (block
+// rows for BCSR3) are cut into slices of 64, one per wavefront, and a slice
+// holds its rows' entries slot-major -- entry t of the row at stored position
+// i (slice s = i / 64, lane l = i % 64) lives at (sptr[s] + t) * 64 + l.  Rows
+// shorter than their slice's longest are padded with zero values whose column
+// repeats the row's last column.  Stored positions are the rows themselves, or,
+// when sorting rows by length inside windows of 256 saves padding, perm[i] is
+// the row at position i and iperm its inverse.  The CSR row pointer is kept
+// for the row lengths and the reference layout.
 struct yafel_hip_matrix_s {
   yafel_hip_ctx_s *ctx = nullptr;
   int format = YAFEL_HIP_CSR;
   int64_t rows = 0, cols = 0;   // scalar dims
   int64_t brows = 0, bcols = 0; // rows of the stored structure (CSR: = rows)
-  int64_t nblk = 0;             // stored blocks (CSR: scalars)
+  int64_t nblk = 0;             // stored blocks (CSR: scalars), padding excluded
   int64_t ptr_len = 0;          // reference host row_ptr length
-  yh::DBuf ptr, col;            // int32 row pointer (brows+1) and block columns
-  yh::DBuf val64, val32;        // CSR: nnz; BCSR3: 9 arrays of nblk (entry-major, SoA)
+  int64_t nsl = 0;              // slices of 64 rows
+  int64_t ntot = 0;             // sliced entries per value array (64 * slots)
+  yh::DBuf ptr;                 // int32 CSR row pointer (brows+1)
+  yh::DBuf sptr;                // int32 slot offset of each slice (nsl+1)
+  yh::DBuf perm, iperm;         // int32 position->row / row->position (empty: identity)
+  yh::DBuf col;                 // int32 sliced block columns (ntot)
+  yh::DBuf val64, val32;        // CSR: ntot; BCSR3: 9 arrays of ntot (v[e * ntot + k], e = 3i+j)
   yh::DBuf diag64, dinv64, dinv32;
   bool has64 = false, has32 = false, diag_ok = false;
   void *mesh = nullptr;         // mesh this pattern came from (assembly)
   YhPlan *plan = nullptr;
-  int lanes = 8;                // lanes per row chosen for the CSR kernel
+  int unroll = 4;               // slots per step of the plain SpMV kernel
   // reference-layout extras: the host row_ptr may be longer than brows+1
   std::vector<int64_t> ptr_tail;
   ~yafel_hip_matrix_s() { yh::ctx_release(ctx); }
@@ -123,12 +137,22 @@ inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 int grid_for(yafel_hip_ctx_s *ctx, int64_t work_items, int block);
 
 // Kernels and host launchers implemented across the .hip files.
+// Grid of a sliced kernel with 256-thread blocks: one slice per wavefront
+// (every block has work, so the XCD-aware block order keeps all XCDs busy),
+// at most `cap` blocks, beyond which wavefronts loop over slices.
+inline int sell_grid(int64_t nsl, int cap) {
+  const int64_t g = (nsl + 3) / 4;
+  return (int)(g < 1 ? 1 : (g < cap ? g : cap));
+}
 void spmv_launch(yafel_hip_matrix_s *A, int precision, const void *x, void *y, hipStream_t s);
 void spmm_launch(yafel_hip_matrix_s *A, int precision, int k, const void *X, void *Y,
                  hipStream_t s);
 void matrix_build_fp32(yafel_hip_matrix_s *A);
 void matrix_build_diag(yafel_hip_matrix_s *A);
-void matrix_choose_lanes(yafel_hip_matrix_s *A);
+// Converts CSR-ordered device arrays (int32 ptr[brows+1], col[nblk], FP64
+// values: CSR nblk, BCSR3 9 entry-major arrays of nblk) into A's sliced
+// storage; ptr is kept, col and values are replaced.
+void matrix_make_sliced(yafel_hip_matrix_s *A, DBuf &ptr, DBuf &col, DBuf &val64);
 
 }  // namespace yh
 

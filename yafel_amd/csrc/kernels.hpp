@@ -1,58 +1,112 @@
-// Device building blocks shared by the SpMV and Krylov kernels.
+// Device building blocks shared by the SpMV and Krylov kernels: the sliced
+// (SELL-64) matrix view and its per-row products.
 #pragma once
 #include "internal.hpp"
 
 namespace yh {
+
+// Slots per step of the sliced row loops inside the fused Krylov kernels.
+constexpr int kSellUnroll = 4;
+
+template <class T>
+struct SellView {
+  const int *sptr;   // slot offset of each slice (nsl + 1)
+  const int *perm;   // stored position -> row, nullptr for the identity
+  const int *col;    // sliced block columns
+  const T *val;      // sliced values (BCSR3: 9 arrays of ntot)
+  int64_t brows;     // stored rows (block rows for BCSR3)
+  int64_t ntot;      // sliced entries per value array
+  __device__ __forceinline__ int64_t nsl() const { return (brows + 63) >> 6; }
+  __device__ __forceinline__ int64_t row_of(int64_t i) const { return perm ? (int64_t)perm[i] : i; }
+};
+
 namespace dev {
 
-// One CSR row with L lanes: lane l sums entries l, l+L, ... of the row, then
-// the L partial sums are combined with xor-shuffles.  Matrix values and
-// column indices are streamed once, so they are loaded non-temporally; the
-// gathered vector stays in cache.  xf(c) returns x[c].
-// T is the stored value type, TV the vector / accumulation type.
-template <int L, class T, class TV, class XF>
-__device__ __forceinline__ TV csr_row(const int *__restrict__ ptr, const int *__restrict__ col,
-                                      const T *__restrict__ val, int64_t r, int lane, XF xf) {
-  const int b = ptr[r], e = ptr[r + 1];
-  TV acc = 0, acc2 = 0;
-  int k = b + lane;
-  for (; k + L < e; k += 2 * L) {
-    const int c = __builtin_nontemporal_load(&col[k]);
-    const int c2 = __builtin_nontemporal_load(&col[k + L]);
-    const T v = __builtin_nontemporal_load(&val[k]);
-    const T v2 = __builtin_nontemporal_load(&val[k + L]);
-    acc = fma((TV)v, xf(c), acc);
-    acc2 = fma((TV)v2, xf(c2), acc2);
-  }
-  if (k < e) acc = fma((TV)__builtin_nontemporal_load(&val[k]), xf(__builtin_nontemporal_load(&col[k])), acc);
-  return group_sum<L>(acc + acc2);
+// Sliced position of entry t of row r (iperm may be nullptr: identity).
+__device__ __forceinline__ int64_t sell_pos(const int *__restrict__ sptr, const int *__restrict__ iperm,
+                                            int64_t r, int64_t t) {
+  const int64_t i = iperm ? (int64_t)iperm[r] : r;
+  return ((int64_t)sptr[i >> 6] + t) * 64 + (i & 63);
 }
 
-// One 3x3-block row with G lanes, lane per block; values are entry-major
-// (val[e * nb + k]).  a[0..2] receive the three row sums (valid in every lane
-// of the group).  xf(i) returns x[i] for a scalar index.
-template <int G, class T, class TV, class XF>
-__device__ __forceinline__ void bcsr_row(const int *__restrict__ ptr, const int *__restrict__ col,
-                                         const T *__restrict__ val, int64_t nb, int64_t br,
-                                         int lane, TV a[3], XF xf) {
-  const int b = ptr[br], e = ptr[br + 1];
-  TV a0 = 0, a1 = 0, a2 = 0;
-  for (int k = b + lane; k < e; k += G) {
-    const int64_t c = 3 * (int64_t)__builtin_nontemporal_load(&col[k]);
-    const TV x0 = xf(c), x1 = xf(c + 1), x2 = xf(c + 2);
-    a0 = fma((TV)__builtin_nontemporal_load(&val[0 * nb + k]), x0, a0);
-    a0 = fma((TV)__builtin_nontemporal_load(&val[1 * nb + k]), x1, a0);
-    a0 = fma((TV)__builtin_nontemporal_load(&val[2 * nb + k]), x2, a0);
-    a1 = fma((TV)__builtin_nontemporal_load(&val[3 * nb + k]), x0, a1);
-    a1 = fma((TV)__builtin_nontemporal_load(&val[4 * nb + k]), x1, a1);
-    a1 = fma((TV)__builtin_nontemporal_load(&val[5 * nb + k]), x2, a1);
-    a2 = fma((TV)__builtin_nontemporal_load(&val[6 * nb + k]), x0, a2);
-    a2 = fma((TV)__builtin_nontemporal_load(&val[7 * nb + k]), x1, a2);
-    a2 = fma((TV)__builtin_nontemporal_load(&val[8 * nb + k]), x2, a2);
+// One CSR row per lane: the lane at `lane` of slice s sums its row's slots in
+// order (padding contributes exact zeros), U slots per step with every load
+// of the step issued before the first FMA.  Matrix values and columns are
+// streamed once, so they are loaded non-temporally; the gathered vector stays
+// in cache.  xf(c) returns x[c].  T is the stored value type, TV the
+// accumulation type.
+template <int U, class T, class TV, class XF>
+__device__ __forceinline__ TV sell_row_csr(const SellView<T> &A, int64_t s, int lane, XF xf) {
+  const int p0 = A.sptr[s], p1 = A.sptr[s + 1];
+  const int64_t base = (int64_t)p0 * 64 + lane;
+  const int len = p1 - p0;
+  TV acc = 0;
+  int t = 0;
+  for (; t + U <= len; t += U) {
+    int c[U];
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = base + (int64_t)(t + u) * 64;
+      c[u] = __builtin_nontemporal_load(&A.col[k]);
+      v[u] = __builtin_nontemporal_load(&A.val[k]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc = fma((TV)v[u], xf(c[u]), acc);
   }
-  a[0] = group_sum<G>(a0);
-  a[1] = group_sum<G>(a1);
-  a[2] = group_sum<G>(a2);
+  for (; t < len; ++t) {
+    const int64_t k = base + (int64_t)t * 64;
+    acc = fma((TV)__builtin_nontemporal_load(&A.val[k]), xf(__builtin_nontemporal_load(&A.col[k])), acc);
+  }
+  return acc;
+}
+
+// One 3x3-block row per lane; a[0..2] receive the three row sums.  Blocks are
+// summed in order, entries of a block row-wise (j = 0, 1, 2).  xf(i) returns
+// x[i] for a scalar index.
+template <int U, class T, class TV, class XF>
+__device__ __forceinline__ void sell_row_bcsr(const SellView<T> &A, int64_t s, int lane, TV a[3], XF xf) {
+  const int p0 = A.sptr[s], p1 = A.sptr[s + 1];
+  const int64_t base = (int64_t)p0 * 64 + lane;
+  const int len = p1 - p0;
+  const int64_t nt = A.ntot;
+  TV a0 = 0, a1 = 0, a2 = 0;
+  int t = 0;
+  for (; t + U <= len; t += U) {
+    int64_t c[U];
+    T v[U][9];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = base + (int64_t)(t + u) * 64;
+      c[u] = 3 * (int64_t)__builtin_nontemporal_load(&A.col[k]);
+#pragma unroll
+      for (int e = 0; e < 9; ++e) v[u][e] = __builtin_nontemporal_load(&A.val[e * nt + k]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const TV x0 = xf(c[u]), x1 = xf(c[u] + 1), x2 = xf(c[u] + 2);
+      a0 = fma((TV)v[u][0], x0, a0); a0 = fma((TV)v[u][1], x1, a0); a0 = fma((TV)v[u][2], x2, a0);
+      a1 = fma((TV)v[u][3], x0, a1); a1 = fma((TV)v[u][4], x1, a1); a1 = fma((TV)v[u][5], x2, a1);
+      a2 = fma((TV)v[u][6], x0, a2); a2 = fma((TV)v[u][7], x1, a2); a2 = fma((TV)v[u][8], x2, a2);
+    }
+  }
+  for (; t < len; ++t) {
+    const int64_t k = base + (int64_t)t * 64;
+    const int64_t c = 3 * (int64_t)__builtin_nontemporal_load(&A.col[k]);
+    const TV x0 = xf(c), x1 = xf(c + 1), x2 = xf(c + 2);
+    a0 = fma((TV)__builtin_nontemporal_load(&A.val[0 * nt + k]), x0, a0);
+    a0 = fma((TV)__builtin_nontemporal_load(&A.val[1 * nt + k]), x1, a0);
+    a0 = fma((TV)__builtin_nontemporal_load(&A.val[2 * nt + k]), x2, a0);
+    a1 = fma((TV)__builtin_nontemporal_load(&A.val[3 * nt + k]), x0, a1);
+    a1 = fma((TV)__builtin_nontemporal_load(&A.val[4 * nt + k]), x1, a1);
+    a1 = fma((TV)__builtin_nontemporal_load(&A.val[5 * nt + k]), x2, a1);
+    a2 = fma((TV)__builtin_nontemporal_load(&A.val[6 * nt + k]), x0, a2);
+    a2 = fma((TV)__builtin_nontemporal_load(&A.val[7 * nt + k]), x1, a2);
+    a2 = fma((TV)__builtin_nontemporal_load(&A.val[8 * nt + k]), x2, a2);
+  }
+  a[0] = a0;
+  a[1] = a1;
+  a[2] = a2;
 }
 
 }  // namespace dev

@@ -1,23 +1,24 @@
 // Device-resident Krylov solvers: CG / Jacobi-PCG / BiCGSTAB (+ right Jacobi),
-// in FP64 or as FP32 inner iterations inside FP64 refinement.
+// in FP64 or with FP32 matrix streams inside FP64 refinement.
 //
 // Every scalar recurrence (alpha, beta, omega, rho, the stopping test) runs
-// on the device in FP64 inside the last workgroup of the kernel that produced
-// the reduction (deterministic: partials are summed in block order), so a
-// solve needs no host round trip per iteration.  The host launches chunks of
+// on the device in FP64, in a one-workgroup kernel that folds the producing
+// kernel's per-workgroup partials in a fixed order (deterministic), so a solve
+// needs no host round trip per iteration.  The host launches chunks of
 // iterations and polls a status word once per chunk; kernels after
 // convergence return immediately.
 //
-// One CG iteration is two kernels:
+// One CG iteration is two kernels (each followed by its fold):
 //   A: p_new = z + beta p_old (reconstructed in the SpMV gather, written by the
-//      row owner), q = A p_new, partial p_new.q  -> last block: alpha
+//      row owner), q = A p_new, partial p_new.q  -> alpha
 //   B: x += alpha p_new, r -= alpha q, z = D^-1 r, partials r.z, r.r
-//      -> last block: beta, stopping test
-// MIXED: vectors and matrix are FP32 (dots accumulate in FP64); when the
-// recurrence residual has dropped by refine_delta (or claims convergence) the
-// inner loop pauses, x_hi += x_lo and r = b - A x_hi are formed in FP64 (a
-// residual replacement that keeps the search direction), and the FP64 true
-// residual decides convergence.
+//      -> beta, stopping test
+// MIXED: vectors and matrix are FP32 (row sums and dots accumulate in FP64);
+// MIXED_MATRIX: only the matrix is streamed in FP32, vectors stay FP64.  In
+// both, when the recurrence residual has dropped by refine_delta (or claims
+// convergence) the inner loop pauses, x_hi += x_lo and r = b - A x_hi are
+// formed with the FP64 matrix (a residual replacement that keeps the search
+// direction), and the FP64 true residual decides convergence.
 #include <chrono>
 #include <cmath>
 
@@ -34,33 +35,19 @@ namespace {
 
 // ---------------------------------------------------------------- launchers
 template <class T>
-MatView<T> view(yafel_hip_matrix_s *A) {
-  return MatView<T>{A->ptr.as<int>(), A->col.as<int>(),
-                    (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->brows, A->nblk};
+SellView<T> view(yafel_hip_matrix_s *A) {
+  return SellView<T>{A->sptr.as<int>(), A->perm.p ? A->perm.as<int>() : nullptr, A->col.as<int>(),
+                     (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->brows, A->ntot};
 }
 
+// Fused SpMV grids stay within num_cus * 8 workgroups so that NP <= 4
+// partial sums per workgroup fit the partials workspace.
 template <class T, class Op>
 void launch_spmv_op(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
-  const int L = A->lanes;
-  const int grid = grid_for(A->ctx, A->brows * (int64_t)L, NT);
-  const MatView<T> v = view<T>(A);
-  if (A->format == YAFEL_HIP_CSR) {
-    switch (L) {
-      case 2: k_spmv_fused<YAFEL_HIP_CSR, 2, T, Op><<<grid, NT, 0, s>>>(v, op); break;
-      case 4: k_spmv_fused<YAFEL_HIP_CSR, 4, T, Op><<<grid, NT, 0, s>>>(v, op); break;
-      case 8: k_spmv_fused<YAFEL_HIP_CSR, 8, T, Op><<<grid, NT, 0, s>>>(v, op); break;
-      case 16: k_spmv_fused<YAFEL_HIP_CSR, 16, T, Op><<<grid, NT, 0, s>>>(v, op); break;
-      case 32: k_spmv_fused<YAFEL_HIP_CSR, 32, T, Op><<<grid, NT, 0, s>>>(v, op); break;
-      default: k_spmv_fused<YAFEL_HIP_CSR, 64, T, Op><<<grid, NT, 0, s>>>(v, op); break;
-    }
-  } else {
-    switch (L) {
-      case 8: k_spmv_fused<YAFEL_HIP_BCSR3, 8, T, Op><<<grid, NT, 0, s>>>(v, op); break;
-      case 16: k_spmv_fused<YAFEL_HIP_BCSR3, 16, T, Op><<<grid, NT, 0, s>>>(v, op); break;
-      case 32: k_spmv_fused<YAFEL_HIP_BCSR3, 32, T, Op><<<grid, NT, 0, s>>>(v, op); break;
-      default: k_spmv_fused<YAFEL_HIP_BCSR3, 64, T, Op><<<grid, NT, 0, s>>>(v, op); break;
-    }
-  }
+  const int grid = sell_grid(A->nsl, A->ctx->num_cus * 8);
+  const SellView<T> v = view<T>(A);
+  if (A->format == YAFEL_HIP_CSR) k_spmv_fused<YAFEL_HIP_CSR, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
+  else k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
   HIPCHK(hipGetLastError());
   if constexpr (Op::NP > 0) {
     k_finish<Op><<<1, NT, 0, s>>>(op, grid);
@@ -143,7 +130,8 @@ struct Solver {
     }
   }
 
-  template <class T>
+  // TM: stored matrix type of the inner SpMV; T: inner vector type
+  template <class TM, class T>
   void cg_iteration(const Vecs &v) {
     OpCgA<T> a{};
     a = base(a, 1);
@@ -151,7 +139,7 @@ struct Solver {
     a.pbuf0 = (T *)v.p0;
     a.pbuf1 = (T *)v.p1;
     a.q = (T *)v.q;
-    launch_spmv_op<T>(A, a, s);
+    launch_spmv_op<TM>(A, a, s);
     const T *dinv = (const T *)(sizeof(T) == 8 ? A->dinv64.p : A->dinv32.p);
     T *xx = mixed ? (T *)v.xl : (T *)x;
     if (prec) {
@@ -169,7 +157,7 @@ struct Solver {
     }
   }
 
-  template <class T>
+  template <class TM, class T>
   void bicg_iteration(const Vecs &v) {
     const T *dinv = (const T *)(sizeof(T) == 8 ? A->dinv64.p : A->dinv32.p);
     if (prec) {
@@ -187,7 +175,7 @@ struct Solver {
     OpBiV<T> vv{};
     vv = base(vv, 2);
     vv.ph = ph; vv.rhat = (const T *)v.rhat; vv.v = (T *)v.v;
-    launch_spmv_op<T>(A, vv, s);
+    launch_spmv_op<TM>(A, vv, s);
     if (prec) {
       OpBiS<T, true> so{};
       so = base(so, 3);
@@ -203,7 +191,7 @@ struct Solver {
     OpBiT<T> to{};
     to = base(to, 4);
     to.sh = sh; to.s = (const T *)v.s; to.t = (T *)v.t;
-    launch_spmv_op<T>(A, to, s);
+    launch_spmv_op<TM>(A, to, s);
     OpBiX<T> xo{};
     xo = base(xo, 5);
     xo.x = mixed ? (T *)v.xl : (T *)x; xo.r = (T *)v.r; xo.ph = ph; xo.sh = sh; xo.s = (const T *)v.s;
@@ -223,7 +211,7 @@ struct Solver {
     residual<T>(1, v);
   }
 
-  template <class T>
+  template <class TM, class T>
   void run(yafel_hip_solve_info *info) {
     YhWorkspace &w = ctx->ws;
     const size_t es = sizeof(T);
@@ -265,9 +253,9 @@ struct Solver {
     auto chunk = [&](int slot_i) {
       for (int k = 0; k < K; ++k) {
         if (cgfam) {
-          cg_iteration<T>(v);
+          cg_iteration<TM, T>(v);
         } else {
-          bicg_iteration<T>(v);
+          bicg_iteration<TM, T>(v);
         }
       }
       if (with_refine) refine<T>(v);
@@ -340,7 +328,9 @@ int yafel_hip_solve(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const 
   REQUIRE(A->rows == A->cols, "solve: matrix must be square");
   REQUIRE(A->rows == 0 || (b_dev && x_dev), "solve: null vector");
   REQUIRE(opts->method >= 0 && opts->method <= 3, "solve: unknown method");
-  REQUIRE(opts->precision == YAFEL_HIP_FP64 || opts->precision == YAFEL_HIP_MIXED, "solve: precision must be FP64 or MIXED");
+  REQUIRE(opts->precision == YAFEL_HIP_FP64 || opts->precision == YAFEL_HIP_MIXED ||
+              opts->precision == YAFEL_HIP_MIXED_MATRIX,
+          "solve: precision must be FP64, MIXED or MIXED_MATRIX");
   REQUIRE(opts->stop == 0 || opts->stop == 1, "solve: unknown stopping rule");
   yafel_hip_ctx_s *ctx = A->ctx;
   HIPCHK(hipSetDevice(ctx->device));
@@ -355,7 +345,7 @@ int yafel_hip_solve(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const 
   S.n = n;
   S.cgfam = opts->method == YAFEL_HIP_CG || opts->method == YAFEL_HIP_PCG_JACOBI;
   S.prec = opts->method == YAFEL_HIP_PCG_JACOBI || opts->method == YAFEL_HIP_BICGSTAB_JACOBI;
-  S.mixed = opts->precision == YAFEL_HIP_MIXED;
+  S.mixed = opts->precision == YAFEL_HIP_MIXED || opts->precision == YAFEL_HIP_MIXED_MATRIX;
   if (S.o.max_iter <= 0) S.o.max_iter = S.cgfam ? n : 10 * n;
   if (S.o.refine_delta <= 0) S.o.refine_delta = 0.1;
   if (S.o.check_every <= 0) S.o.check_every = 16;
@@ -371,8 +361,9 @@ int yafel_hip_solve(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const 
   S.b = b_dev;
   S.x = x_dev;
   HIPCHK(hipEventRecord(ctx->t0, S.s));
-  if (S.mixed) S.run<float>(info);
-  else S.run<double>(info);
+  if (opts->precision == YAFEL_HIP_MIXED) S.run<float, float>(info);
+  else if (opts->precision == YAFEL_HIP_MIXED_MATRIX) S.run<float, double>(info);
+  else S.run<double, double>(info);
   HIPCHK(hipEventRecord(ctx->t1, S.s));
   HIPCHK(hipEventSynchronize(ctx->t1));
   float ms = 0;

@@ -36,14 +36,6 @@ enum { M_CG = 0, M_PCG = 1, M_BICG = 2, M_BICGJ = 3 };
 __device__ __forceinline__ bool bad(double v) { return !(fabs(v) <= 1.7976931348623157e308); }
 
 // ------------------------------------------------------------------ drivers
-template <class T> struct MatView {
-  const int *ptr;
-  const int *col;
-  const T *val;
-  int64_t brows;
-  int64_t nb;
-};
-
 // Block-reduce NP sums into op.part[j * nblk + blk].  The fold of the
 // partials and the scalar recurrence run in k_finish, a one-workgroup kernel
 // launched right after on the same stream: kernel boundaries make the
@@ -87,46 +79,48 @@ __global__ __launch_bounds__(NT) void k_finish(Op op0, int nblk) {
   finish_body(op, nblk);
 }
 
-// SpMV with a fused per-row epilogue and up to 4 block-reduced sums; the last
-// block to finish folds the partials (fixed order) and runs op.last().  The
-// body takes its block id and block count explicitly so that a batched launch
-// can run several independent solves, each on its own block range.
-// T is the stored matrix type; Op::value_type the vector / accumulation type.
-template <int FMT, int L, class T, class Op>
-__device__ __forceinline__ void spmv_fused_body(const MatView<T> &A, Op &op, int blk, int nblk) {
+// Sliced SpMV (one slice of 64 rows per wavefront, a row per lane) with a
+// fused per-row epilogue and up to 4 block-reduced sums folded afterwards by
+// k_finish.  The body takes its block id and block count explicitly so that a
+// batched launch can run several independent solves, each on its own block
+// range; the row -> block mapping depends on nothing else, so a solve's
+// results do not depend on what runs beside it.
+// T is the stored matrix type, Op::value_type the vector type.
+template <int FMT, int U, class T, class Op>
+__device__ __forceinline__ void spmv_fused_body(const SellView<T> &A, Op &op, int blk, int nblk) {
   using TV = typename Op::value_type;
   if (op.skip()) return;
   op.begin();
-  const int lane = threadIdx.x & (L - 1);
+  const int lane = threadIdx.x & 63;
   constexpr int NP = Op::NP;
   double acc[NP > 0 ? NP : 1];
 #pragma unroll
   for (int j = 0; j < (NP > 0 ? NP : 1); ++j) acc[j] = 0;
-  const int64_t rpb = NT / L;
-  const int64_t chunk = (A.brows + nblk - 1) / nblk;
-  const int64_t r0 = (int64_t)blk * chunk;
-  const int64_t r1 = r0 + chunk < A.brows ? r0 + chunk : A.brows;
-  for (int64_t r = r0 + threadIdx.x / L; r < r1; r += rpb) {
+  const int64_t nsl = A.nsl(), nw = (int64_t)nblk * (NT / 64);
+  for (int64_t s = (int64_t)blk * (NT / 64) + (threadIdx.x >> 6); s < nsl; s += nw) {
+    const int64_t i = s * 64 + lane;
+    // row sums accumulate in FP64 whatever the storage and vector precision
     if (FMT == YAFEL_HIP_CSR) {
-      TV y = dev::csr_row<L, T, TV>(A.ptr, A.col, A.val, r, lane, [&](int c) { return op.x(c); });
-      if (lane == 0) op.row(r, y, acc);
+      const double y = dev::sell_row_csr<U, T, double>(A, s, lane, [&](int c) { return (double)op.x(c); });
+      if (i < A.brows) op.row(A.row_of(i), (TV)y, acc);
     } else {
-      TV a[3];
-      dev::bcsr_row<L, T, TV>(A.ptr, A.col, A.val, A.nb, r, lane, a, [&](int64_t c) { return op.x(c); });
-      if (lane == 0) {
-        op.row(3 * r, a[0], acc);
-        op.row(3 * r + 1, a[1], acc);
-        op.row(3 * r + 2, a[2], acc);
+      double a[3];
+      dev::sell_row_bcsr<U, T, double>(A, s, lane, a, [&](int64_t c) { return (double)op.x(c); });
+      if (i < A.brows) {
+        const int64_t r = A.row_of(i);
+        op.row(3 * r, (TV)a[0], acc);
+        op.row(3 * r + 1, (TV)a[1], acc);
+        op.row(3 * r + 2, (TV)a[2], acc);
       }
     }
   }
   finish_partials<NP>(op, acc, blk, nblk);
 }
 
-template <int FMT, int L, class T, class Op>
-__global__ __launch_bounds__(NT) void k_spmv_fused(MatView<T> A, Op op0) {
+template <int FMT, int U, class T, class Op>
+__global__ __launch_bounds__(NT) void k_spmv_fused(SellView<T> A, Op op0) {
   Op op = op0;
-  spmv_fused_body<FMT, L, T, Op>(A, op, dev::xcd_block(), gridDim.x);
+  spmv_fused_body<FMT, U, T, Op>(A, op, dev::xcd_block(), gridDim.x);
 }
 
 // Elementwise pass with up to 4 block-reduced sums.

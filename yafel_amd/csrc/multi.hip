@@ -13,6 +13,7 @@
 #include <cmath>
 
 #include "internal.hpp"
+#include "kernels.hpp"
 
 using namespace yh;
 
@@ -35,8 +36,12 @@ struct MState {
 
 __device__ __forceinline__ bool bad(double v) { return !(fabs(v) <= 1.7976931348623157e308); }
 
+// Sliced CSR storage addressed by row (kernels.hpp): entry t of row r is at
+// sell_pos(sptr, iperm, r, t).
 template <class T> struct CsrView {
   const int *ptr;
+  const int *sptr;
+  const int *iperm;
   const int *col;
   const T *val;
   int64_t rows;
@@ -120,10 +125,14 @@ __global__ __launch_bounds__(NT) void k_mspmm(CsrView<T> A, Op op0) {
   const int64_t r1 = r0 + chunk < A.rows ? r0 + chunk : A.rows;
   for (int64_t rr = r0 + (threadIdx.x >> 6); rr < r1; rr += NT / 64) {
     const int r = __builtin_amdgcn_readfirstlane((int)rr);
-    const int b = A.ptr[r], e = A.ptr[r + 1];
+    const int len = A.ptr[r + 1] - A.ptr[r];
+    const int64_t base = dev::sell_pos(A.sptr, A.iperm, r, 0);
     if (lane < k) {
       T y = 0;
-      for (int q = b; q < e; ++q) y = fma(A.val[q], op.x(A.col[q], lane), y);
+      for (int t = 0; t < len; ++t) {
+        const int64_t q = base + (int64_t)t * 64;
+        y = fma(A.val[q], op.x(A.col[q], lane), y);
+      }
       op.row(r, lane, y, acc);
     }
   }
@@ -515,8 +524,8 @@ struct MReport : MBase {
 
 template <class T>
 CsrView<T> cview(yafel_hip_matrix_s *A) {
-  return CsrView<T>{A->ptr.as<int>(), A->col.as<int>(),
-                    (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->rows};
+  return CsrView<T>{A->ptr.as<int>(), A->sptr.as<int>(), A->iperm.p ? A->iperm.as<int>() : nullptr,
+                    A->col.as<int>(), (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->rows};
 }
 
 struct Multi {

@@ -27,11 +27,11 @@ namespace {
 constexpr int GCAP = 1024;  // block cap per slot and phase
 
 struct SlotDesc {
-  int fmt, lanes;
+  int fmt, pad0;
   int g_spmv, g_elem;
   int off_spmv, off_elem;
-  int64_t n, brows, nb;
-  const int *ptr, *col;
+  int64_t n, brows, ntot;
+  const int *sptr, *perm, *col;
   const float *v32;
   const double *v64;
   const float *dinv;
@@ -66,10 +66,9 @@ __device__ __forceinline__ void bind(O &op, const SlotDesc &d, int counter) {
 
 template <class T, class O>
 __device__ __forceinline__ void run_spmv(const SlotDesc &d, const T *val, O &op, int blk) {
-  const MatView<T> A{d.ptr, d.col, val, d.brows, d.nb};
-  if (d.fmt == YAFEL_HIP_BCSR3) spmv_fused_body<YAFEL_HIP_BCSR3, 32, T, O>(A, op, blk, d.g_spmv);
-  else if (d.lanes >= 16) spmv_fused_body<YAFEL_HIP_CSR, 16, T, O>(A, op, blk, d.g_spmv);
-  else spmv_fused_body<YAFEL_HIP_CSR, 8, T, O>(A, op, blk, d.g_spmv);
+  const SellView<T> A{d.sptr, d.perm, d.col, val, d.brows, d.ntot};
+  if (d.fmt == YAFEL_HIP_BCSR3) spmv_fused_body<YAFEL_HIP_BCSR3, kSellUnroll, T, O>(A, op, blk, d.g_spmv);
+  else spmv_fused_body<YAFEL_HIP_CSR, kSellUnroll, T, O>(A, op, blk, d.g_spmv);
 }
 
 // Builds the phase's operation for slot d and either runs the phase body on
@@ -334,8 +333,7 @@ void join_slot(yafel_hip_server_s *S, Slot *sl) {
   yafel_hip_matrix_s *A = sl->A;
   const int64_t n = A->rows;
   Req *r = sl->req;
-  const int L = A->format == YAFEL_HIP_BCSR3 ? 32 : (A->lanes >= 16 ? 16 : 8);
-  const int g_spmv = std::max(1, std::min<int>(GCAP, ceil_div(A->brows * (int64_t)L, NT)));
+  const int g_spmv = sell_grid(A->nsl, GCAP);
   const int g_elem = std::max(1, std::min<int>(GCAP, ceil_div(n, NT)));
   const size_t nbytes = 6 * (((size_t)n * 4 + 255) & ~(size_t)255) + (((size_t)n * 8 + 255) & ~(size_t)255) +
                         ((sizeof(KState) + 255) & ~(size_t)255) + (size_t)4 * std::max(g_spmv, g_elem) * 8 + 256;
@@ -344,13 +342,14 @@ void join_slot(yafel_hip_server_s *S, Slot *sl) {
   char *p = (char *)sl->buf;
   SlotDesc &d = sl->d;
   d.fmt = A->format;
-  d.lanes = L;
+  d.pad0 = 0;
   d.g_spmv = g_spmv;
   d.g_elem = g_elem;
   d.n = n;
   d.brows = A->brows;
-  d.nb = A->nblk;
-  d.ptr = A->ptr.as<int>();
+  d.ntot = A->ntot;
+  d.sptr = A->sptr.as<int>();
+  d.perm = A->perm.p ? A->perm.as<int>() : nullptr;
   d.col = A->col.as<int>();
   d.v32 = A->val32.as<float>();
   d.v64 = A->val64.as<double>();

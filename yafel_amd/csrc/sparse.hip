@@ -1,21 +1,25 @@
 // Device sparse matrices (CSR and 3x3-block BCSR), SpMV and multi-RHS SpMM.
 //
-// Device layout
-//  CSR   : int32 row_ptr[rows+1], int32 col[nnz], values[nnz] (FP64 and/or FP32).
-//  BCSR3 : int32 brow_ptr[brows+1], int32 bcol[nb], values entry-major
-//          v[e * nb + b] for e = 3 i + j (structure-of-arrays), so that a lane
-//          per block reads every entry stream with unit stride across lanes.
-//          The reference keeps data[b * 9 + i * 3 + j]; conversion happens on
-//          upload/download.
+// Device layout: both formats are stored sliced (SELL-64, kernels.hpp): the
+// stored rows (block rows for BCSR3) are cut into slices of 64, one per
+// wavefront, each lane owns a row and walks its entries slot by slot, so the
+// matrix streams are read with unit stride across the lanes and the gathered
+// vector entries of neighbouring rows are neighbours too.  BCSR3 values are
+// entry-major, v[e * ntot + k] for e = 3 i + j.  The reference's host layout
+// (row_ptr, col_index, data[b * 9 + i * 3 + j]) is converted on upload and
+// download.
 #include <algorithm>
 #include <cmath>
 
 #include "internal.hpp"
 #include "kernels.hpp"
+#include "sortprim.hpp"
 
 using namespace yh;
 
 namespace {
+
+constexpr int kSortWindow = 256;  // rows sorted by length within windows of this size
 
 __global__ void k_aos_to_soa(const double *__restrict__ aos, double *__restrict__ soa, int64_t nb) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -27,48 +31,137 @@ __global__ void k_aos_to_soa(const double *__restrict__ aos, double *__restrict_
   }
 }
 
-__global__ void k_soa_to_aos(const double *__restrict__ soa, double *__restrict__ aos, int64_t nb) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < nb * 9; i += stride) {
-    int64_t b = i / 9;
-    int e = (int)(i - b * 9);
-    aos[i] = soa[(int64_t)e * nb + b];
-  }
-}
-
 __global__ void k_to_fp32(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) b[i] = (float)a[i];
 }
 
+// ------------------------------------------------------------ slicing
+// Longest row of each slice with rows in their natural order.
+__global__ void k_slice_len(int64_t brows, int64_t nsl, const int *__restrict__ ptr, int64_t *slen) {
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s > nsl) return;
+  int64_t m = 0;
+  if (s < nsl) {
+    const int64_t r1 = (s + 1) * 64 < brows ? (s + 1) * 64 : brows;
+    for (int64_t r = s * 64; r < r1; ++r) m = max(m, (int64_t)(ptr[r + 1] - ptr[r]));
+  }
+  slen[s] = m;  // slen[nsl] = 0 closes the scan
+}
+
+// Stable sort of the rows of each window of kSortWindow rows by decreasing
+// length (rank counting, one workgroup per window); writes perm and the
+// longest row of each slice in sorted order.
+__global__ __launch_bounds__(kSortWindow) void k_window_sort(int64_t brows, int64_t nsl, const int *__restrict__ ptr,
+                                                             int *__restrict__ perm, int64_t *__restrict__ slen) {
+  __shared__ int len[kSortWindow];
+  __shared__ int slen_l[kSortWindow];
+  const int64_t w0 = (int64_t)blockIdx.x * kSortWindow;
+  const int n = (int)min((int64_t)kSortWindow, brows - w0);
+  const int t = threadIdx.x;
+  const int mine = t < n ? ptr[w0 + t + 1] - ptr[w0 + t] : -1;
+  len[t] = mine;
+  __syncthreads();
+  if (t < n) {
+    int rank = 0;
+    for (int j = 0; j < n; ++j) rank += (len[j] > mine) || (len[j] == mine && j < t);
+    perm[w0 + rank] = (int)(w0 + t);
+    slen_l[rank] = mine;
+  }
+  __syncthreads();
+  if (t < kSortWindow / 64) {
+    const int64_t s = w0 / 64 + t;
+    if (s < nsl) slen[s] = t * 64 < n ? slen_l[t * 64] : 0;
+  }
+  if (blockIdx.x == 0 && t == 0) slen[nsl] = 0;
+}
+
+__global__ void k_slots_to_i32(int64_t n, const int64_t *__restrict__ a, int *__restrict__ b) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (int)a[i];
+}
+
+__global__ void k_inverse(int64_t n, const int *__restrict__ perm, int *__restrict__ iperm) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) iperm[perm[i]] = (int)i;
+}
+
+// Scatter CSR-ordered columns/values into the sliced arrays; one thread per
+// stored position writes every slot of its lane (real entries, then padding).
+__global__ void k_sell_fill(int64_t brows, int64_t nsl, int64_t nblk, int64_t ntot, int nval,
+                            const int *__restrict__ ptr, const int *__restrict__ col,
+                            const double *__restrict__ val, const int *__restrict__ perm,
+                            const int *__restrict__ sptr, int *__restrict__ scol, double *__restrict__ sval) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nsl * 64) return;
+  const int64_t s = i >> 6;
+  const int lane = (int)(i & 63);
+  const int p0 = sptr[s], ns = sptr[s + 1] - p0;
+  int b = 0, len = 0, padc = 0;
+  if (i < brows) {
+    const int64_t r = perm ? perm[i] : i;
+    b = ptr[r];
+    len = ptr[r + 1] - b;
+    padc = len > 0 ? col[b + len - 1] : 0;
+  }
+  for (int t = 0; t < ns; ++t) {
+    const int64_t k = ((int64_t)p0 + t) * 64 + lane;
+    const bool real = t < len;
+    scol[k] = real ? col[b + t] : padc;
+    for (int e = 0; e < nval; ++e) sval[e * ntot + k] = real ? val[e * nblk + b + t] : 0.0;
+  }
+}
+
+// Sliced storage back to the reference's host layout (CSR order; BCSR3
+// blocks as data[b * 9 + e]).
+__global__ void k_sell_to_csr(int64_t brows, int64_t ntot, int nval, const int *__restrict__ ptr,
+                              const int *__restrict__ sptr, const int *__restrict__ iperm,
+                              const int *__restrict__ scol, const double *__restrict__ sval,
+                              int *__restrict__ col, double *__restrict__ val) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= brows) return;
+  const int b = ptr[r], len = ptr[r + 1] - b;
+  for (int t = 0; t < len; ++t) {
+    const int64_t k = dev::sell_pos(sptr, iperm, r, t);
+    if (col) col[b + t] = scol[k];
+    if (val)
+      for (int e = 0; e < nval; ++e) val[(int64_t)(b + t) * nval + e] = sval[e * ntot + k];
+  }
+}
+
 // Diagonal and its inverse (JacobiPreconditioner.hpp:124-137: 1/a_ii, kept
 // even when a_ii == 0 -- the reference only warns).
-__global__ void k_diag_csr(const int *__restrict__ ptr, const int *__restrict__ col,
-                           const double *__restrict__ val, int64_t rows, double *diag,
-                           double *dinv, float *dinv32) {
+__global__ void k_diag_csr(int64_t rows, const int *__restrict__ ptr, const int *__restrict__ sptr,
+                           const int *__restrict__ iperm, const int *__restrict__ col,
+                           const double *__restrict__ val, double *diag, double *dinv, float *dinv32) {
   int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (r >= rows) return;
   double d = 0;
-  for (int k = ptr[r]; k < ptr[r + 1]; ++k)
+  const int len = ptr[r + 1] - ptr[r];
+  for (int t = 0; t < len; ++t) {
+    const int64_t k = dev::sell_pos(sptr, iperm, r, t);
     if (col[k] == r) { d = val[k]; break; }
+  }
   diag[r] = d;
   dinv[r] = 1.0 / d;
   dinv32[r] = (float)(1.0 / d);
 }
 
-__global__ void k_diag_bcsr(const int *__restrict__ ptr, const int *__restrict__ col,
-                            const double *__restrict__ val, int64_t brows, int64_t nb, double *diag,
-                            double *dinv, float *dinv32) {
+__global__ void k_diag_bcsr(int64_t brows, int64_t ntot, const int *__restrict__ ptr, const int *__restrict__ sptr,
+                            const int *__restrict__ iperm, const int *__restrict__ col,
+                            const double *__restrict__ val, double *diag, double *dinv, float *dinv32) {
   int64_t br = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (br >= brows) return;
   double d[3] = {0, 0, 0};
-  for (int k = ptr[br]; k < ptr[br + 1]; ++k)
+  const int len = ptr[br + 1] - ptr[br];
+  for (int t = 0; t < len; ++t) {
+    const int64_t k = dev::sell_pos(sptr, iperm, br, t);
     if (col[k] == br) {
-      for (int i = 0; i < 3; ++i) d[i] = val[(int64_t)(4 * i) * nb + k];
+      for (int i = 0; i < 3; ++i) d[i] = val[(int64_t)(4 * i) * ntot + k];
       break;
     }
+  }
   for (int i = 0; i < 3; ++i) {
     diag[3 * br + i] = d[i];
     dinv[3 * br + i] = 1.0 / d[i];
@@ -76,92 +169,97 @@ __global__ void k_diag_bcsr(const int *__restrict__ ptr, const int *__restrict__
   }
 }
 
-template <int L, class T>
-__global__ __launch_bounds__(256) void k_csr_spmv(int64_t rows, const int *__restrict__ ptr,
-                                                  const int *__restrict__ col,
-                                                  const T *__restrict__ val,
-                                                  const T *__restrict__ x, T *__restrict__ y) {
-  const int lane = threadIdx.x & (L - 1);
-  const int64_t rpb = 256 / L;
-  const int64_t chunk = (rows + gridDim.x - 1) / gridDim.x;
-  const int64_t r0 = (int64_t)dev::xcd_block() * chunk;
-  const int64_t r1 = r0 + chunk < rows ? r0 + chunk : rows;
-  for (int64_t r = r0 + threadIdx.x / L; r < r1 + 0; r += rpb) {
-    T acc = dev::csr_row<L, T, T>(ptr, col, val, r, lane, [&](int c) { return x[c]; });
-    if (lane == 0) y[r] = acc;
+// ---------------------------------------------------------------- SpMV
+// One slice per wavefront (grid-stride over slices when the grid is capped).
+// Row sums accumulate in FP64 for either storage precision; the FP32 product
+// is rounded once, at the store.
+template <int U, class T>
+__global__ __launch_bounds__(256) void k_sell_spmv_csr(SellView<T> A, const T *__restrict__ x, T *__restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nsl = A.nsl(), nw = (int64_t)gridDim.x * 4;
+  for (int64_t s = (int64_t)dev::xcd_block() * 4 + (threadIdx.x >> 6); s < nsl; s += nw) {
+    const double acc = dev::sell_row_csr<U, T, double>(A, s, lane, [&](int c) { return (double)x[c]; });
+    const int64_t i = s * 64 + lane;
+    if (i < A.brows) y[A.row_of(i)] = (T)acc;
   }
 }
 
-template <int G, class T>
-__global__ __launch_bounds__(256) void k_bcsr_spmv(int64_t brows, int64_t nb,
-                                                   const int *__restrict__ ptr,
-                                                   const int *__restrict__ col,
-                                                   const T *__restrict__ val,
-                                                   const T *__restrict__ x, T *__restrict__ y) {
-  const int lane = threadIdx.x & (G - 1);
-  const int64_t rpb = 256 / G;
-  const int64_t chunk = (brows + gridDim.x - 1) / gridDim.x;
-  const int64_t r0 = (int64_t)dev::xcd_block() * chunk;
-  const int64_t r1 = r0 + chunk < brows ? r0 + chunk : brows;
-  for (int64_t br = r0 + threadIdx.x / G; br < r1; br += rpb) {
-    T a[3];
-    dev::bcsr_row<G, T, T>(ptr, col, val, nb, br, lane, a, [&](int64_t c) { return x[c]; });
-    if (lane < 3) y[3 * br + lane] = lane == 0 ? a[0] : (lane == 1 ? a[1] : a[2]);
+template <int U, class T>
+__global__ __launch_bounds__(256) void k_sell_spmv_bcsr(SellView<T> A, const T *__restrict__ x, T *__restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nsl = A.nsl(), nw = (int64_t)gridDim.x * 4;
+  for (int64_t s = (int64_t)dev::xcd_block() * 4 + (threadIdx.x >> 6); s < nsl; s += nw) {
+    double a[3];
+    dev::sell_row_bcsr<U, T, double>(A, s, lane, a, [&](int64_t c) { return (double)x[c]; });
+    const int64_t i = s * 64 + lane;
+    if (i < A.brows) {
+      const int64_t r = A.row_of(i);
+      y[3 * r] = (T)a[0];
+      y[3 * r + 1] = (T)a[1];
+      y[3 * r + 2] = (T)a[2];
+    }
   }
 }
 
 // Y = A X for k right-hand sides, row-major X (n x k).  One wavefront per row;
 // lanes run over right-hand sides, so the X/Y rows are read/written coalesced
-// and each matrix entry is one broadcast load.
+// and each matrix entry is one wave-uniform load.
 template <class T>
 __global__ __launch_bounds__(256) void k_csr_spmm(int64_t rows, int k, const int *__restrict__ ptr,
-                                                  const int *__restrict__ col,
-                                                  const T *__restrict__ val,
+                                                  const int *__restrict__ sptr, const int *__restrict__ iperm,
+                                                  const int *__restrict__ col, const T *__restrict__ val,
                                                   const T *__restrict__ X, T *__restrict__ Y) {
   const int lane = threadIdx.x & 63;
   const int64_t wpb = 4;
   const int64_t chunk = (rows + gridDim.x - 1) / gridDim.x;
   const int64_t r0 = (int64_t)dev::xcd_block() * chunk;
   const int64_t r1 = r0 + chunk < rows ? r0 + chunk : rows;
-  for (int64_t r = r0 + threadIdx.x / 64; r < r1; r += wpb) {
-    const int b = ptr[r], e = ptr[r + 1];
+  for (int64_t rr = r0 + threadIdx.x / 64; rr < r1; rr += wpb) {
+    const int r = __builtin_amdgcn_readfirstlane((int)rr);
+    const int len = ptr[r + 1] - ptr[r];
+    const int64_t base = dev::sell_pos(sptr, iperm, r, 0);
     for (int j0 = 0; j0 < k; j0 += 64) {
       const int j = j0 + lane;
       T acc = 0;
       if (j < k)
-        for (int q = b; q < e; ++q) acc = fma(val[q], X[(int64_t)col[q] * k + j], acc);
+        for (int t = 0; t < len; ++t) {
+          const int64_t q = base + (int64_t)t * 64;
+          acc = fma(val[q], X[(int64_t)col[q] * k + j], acc);
+        }
       if (j < k) Y[r * (int64_t)k + j] = acc;
     }
   }
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_bcsr_spmm(int64_t brows, int64_t nb, int k,
-                                                   const int *__restrict__ ptr,
-                                                   const int *__restrict__ col,
-                                                   const T *__restrict__ val,
+__global__ __launch_bounds__(256) void k_bcsr_spmm(int64_t brows, int64_t ntot, int k, const int *__restrict__ ptr,
+                                                   const int *__restrict__ sptr, const int *__restrict__ iperm,
+                                                   const int *__restrict__ col, const T *__restrict__ val,
                                                    const T *__restrict__ X, T *__restrict__ Y) {
   const int lane = threadIdx.x & 63;
   const int64_t chunk = (brows + gridDim.x - 1) / gridDim.x;
   const int64_t r0 = (int64_t)dev::xcd_block() * chunk;
   const int64_t r1 = r0 + chunk < brows ? r0 + chunk : brows;
-  for (int64_t br = r0 + threadIdx.x / 64; br < r1; br += 4) {
-    const int b = ptr[br], e = ptr[br + 1];
+  for (int64_t rr = r0 + threadIdx.x / 64; rr < r1; rr += 4) {
+    const int br = __builtin_amdgcn_readfirstlane((int)rr);
+    const int len = ptr[br + 1] - ptr[br];
+    const int64_t base = dev::sell_pos(sptr, iperm, br, 0);
     for (int j0 = 0; j0 < k; j0 += 64) {
       const int j = j0 + lane;
       T a0 = 0, a1 = 0, a2 = 0;
       if (j < k)
-        for (int q = b; q < e; ++q) {
+        for (int t = 0; t < len; ++t) {
+          const int64_t q = base + (int64_t)t * 64;
           const int64_t c = col[q];
           const T x0 = X[(3 * c + 0) * k + j], x1 = X[(3 * c + 1) * k + j], x2 = X[(3 * c + 2) * k + j];
-          a0 = fma(val[0 * nb + q], x0, fma(val[1 * nb + q], x1, fma(val[2 * nb + q], x2, a0)));
-          a1 = fma(val[3 * nb + q], x0, fma(val[4 * nb + q], x1, fma(val[5 * nb + q], x2, a1)));
-          a2 = fma(val[6 * nb + q], x0, fma(val[7 * nb + q], x1, fma(val[8 * nb + q], x2, a2)));
+          a0 = fma(val[0 * ntot + q], x0, a0); a0 = fma(val[1 * ntot + q], x1, a0); a0 = fma(val[2 * ntot + q], x2, a0);
+          a1 = fma(val[3 * ntot + q], x0, a1); a1 = fma(val[4 * ntot + q], x1, a1); a1 = fma(val[5 * ntot + q], x2, a1);
+          a2 = fma(val[6 * ntot + q], x0, a2); a2 = fma(val[7 * ntot + q], x1, a2); a2 = fma(val[8 * ntot + q], x2, a2);
         }
       if (j < k) {
-        Y[(3 * br + 0) * k + j] = a0;
-        Y[(3 * br + 1) * k + j] = a1;
-        Y[(3 * br + 2) * k + j] = a2;
+        Y[(3 * (int64_t)br + 0) * k + j] = a0;
+        Y[(3 * (int64_t)br + 1) * k + j] = a1;
+        Y[(3 * (int64_t)br + 2) * k + j] = a2;
       }
     }
   }
@@ -178,33 +276,36 @@ std::vector<int32_t> to_i32(const int64_t *a, int64_t n, int64_t limit, const ch
 }
 
 template <class T>
+SellView<T> sell_view(yafel_hip_matrix_s *A) {
+  return SellView<T>{A->sptr.as<int>(), A->perm.p ? A->perm.as<int>() : nullptr, A->col.as<int>(),
+                     (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->brows, A->ntot};
+}
+
+// Cap on the plain SpMV grid: past one slice per wavefront the waves loop.
+inline int spmv_cap(yafel_hip_ctx_s *ctx) { return ctx->num_cus * 16; }
+
+template <class T>
 void launch_csr(yafel_hip_matrix_s *A, const T *x, T *y, hipStream_t s) {
-  const int L = A->lanes;
-  const int grid = grid_for(A->ctx, A->rows * (int64_t)L, 256);
-  const int *p = A->ptr.as<int>(), *c = A->col.as<int>();
-  const T *v = (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p);
-  switch (L) {
-    case 2: k_csr_spmv<2, T><<<grid, 256, 0, s>>>(A->rows, p, c, v, x, y); break;
-    case 4: k_csr_spmv<4, T><<<grid, 256, 0, s>>>(A->rows, p, c, v, x, y); break;
-    case 8: k_csr_spmv<8, T><<<grid, 256, 0, s>>>(A->rows, p, c, v, x, y); break;
-    case 16: k_csr_spmv<16, T><<<grid, 256, 0, s>>>(A->rows, p, c, v, x, y); break;
-    case 32: k_csr_spmv<32, T><<<grid, 256, 0, s>>>(A->rows, p, c, v, x, y); break;
-    default: k_csr_spmv<64, T><<<grid, 256, 0, s>>>(A->rows, p, c, v, x, y); break;
+  const SellView<T> v = sell_view<T>(A);
+  const int grid = sell_grid(A->nsl, spmv_cap(A->ctx));
+  switch (A->unroll) {
+    case 1: k_sell_spmv_csr<1, T><<<grid, 256, 0, s>>>(v, x, y); break;
+    case 2: k_sell_spmv_csr<2, T><<<grid, 256, 0, s>>>(v, x, y); break;
+    case 8: k_sell_spmv_csr<8, T><<<grid, 256, 0, s>>>(v, x, y); break;
+    default: k_sell_spmv_csr<4, T><<<grid, 256, 0, s>>>(v, x, y); break;
   }
   HIPCHK(hipGetLastError());
 }
 
 template <class T>
 void launch_bcsr(yafel_hip_matrix_s *A, const T *x, T *y, hipStream_t s) {
-  const int G = A->lanes;
-  const int grid = grid_for(A->ctx, A->brows * (int64_t)G, 256);
-  const int *p = A->ptr.as<int>(), *c = A->col.as<int>();
-  const T *v = (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p);
-  switch (G) {
-    case 8: k_bcsr_spmv<8, T><<<grid, 256, 0, s>>>(A->brows, A->nblk, p, c, v, x, y); break;
-    case 16: k_bcsr_spmv<16, T><<<grid, 256, 0, s>>>(A->brows, A->nblk, p, c, v, x, y); break;
-    case 32: k_bcsr_spmv<32, T><<<grid, 256, 0, s>>>(A->brows, A->nblk, p, c, v, x, y); break;
-    default: k_bcsr_spmv<64, T><<<grid, 256, 0, s>>>(A->brows, A->nblk, p, c, v, x, y); break;
+  const SellView<T> v = sell_view<T>(A);
+  const int grid = sell_grid(A->nsl, spmv_cap(A->ctx));
+  switch (A->unroll) {
+    case 1: k_sell_spmv_bcsr<1, T><<<grid, 256, 0, s>>>(v, x, y); break;
+    case 2: k_sell_spmv_bcsr<2, T><<<grid, 256, 0, s>>>(v, x, y); break;
+    case 8: k_sell_spmv_bcsr<8, T><<<grid, 256, 0, s>>>(v, x, y); break;
+    default: k_sell_spmv_bcsr<4, T><<<grid, 256, 0, s>>>(v, x, y); break;
   }
   HIPCHK(hipGetLastError());
 }
@@ -221,15 +322,70 @@ yafel_hip_matrix_s *new_matrix(yafel_hip_ctx_s *ctx, int format) {
 
 namespace yh {
 
-void matrix_choose_lanes(yafel_hip_matrix_s *A) {
-  const double avg = A->brows > 0 ? (double)A->nblk / (double)A->brows : 0.0;
-  int L;
-  if (A->format == YAFEL_HIP_CSR) {
-    L = avg < 4 ? 2 : avg < 8 ? 4 : avg < 16 ? 8 : avg < 32 ? 16 : avg < 64 ? 32 : 64;
+void matrix_make_sliced(yafel_hip_matrix_s *A, DBuf &ptr, DBuf &col, DBuf &val64) {
+  hipStream_t s = A->ctx->stream;
+  const int64_t brows = A->brows, nblk = A->nblk;
+  const int nval = A->format == YAFEL_HIP_CSR ? 1 : 9;
+  const int64_t nsl = (brows + 63) / 64;
+  A->nsl = nsl;
+  std::swap(A->ptr.p, ptr.p);
+  std::swap(A->ptr.bytes, ptr.bytes);
+  A->perm.release();
+  A->iperm.release();
+  A->sptr.alloc((nsl + 1) * 4);
+  int64_t slots = 0;
+  if (nsl > 0) {
+    // slot counts with natural row order and with rows sorted inside windows
+    DBuf su, ss, perm, off;
+    su.alloc((nsl + 1) * 8);
+    ss.alloc((nsl + 1) * 8);
+    off.alloc((nsl + 1) * 8);
+    perm.alloc(brows * 4);
+    const int *p = A->ptr.as<int>();
+    k_slice_len<<<ceil_div(nsl + 1, 256), 256, 0, s>>>(brows, nsl, p, su.as<int64_t>());
+    k_window_sort<<<ceil_div(brows, kSortWindow), kSortWindow, 0, s>>>(brows, nsl, p, perm.as<int>(), ss.as<int64_t>());
+    HIPCHK(hipGetLastError());
+    int64_t tot[2];
+    exclusive_scan_i64(su.as<int64_t>(), off.as<int64_t>(), nsl + 1, s);
+    HIPCHK(hipMemcpyAsync(&tot[0], off.as<int64_t>() + nsl, 8, hipMemcpyDeviceToHost, s));
+    exclusive_scan_i64(ss.as<int64_t>(), su.as<int64_t>(), nsl + 1, s);
+    HIPCHK(hipMemcpyAsync(&tot[1], su.as<int64_t>() + nsl, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // sorting moves the output rows around; take it only when it saves padding
+    const bool sorted = tot[1] < tot[0] - tot[0] / 50;
+    slots = sorted ? tot[1] : tot[0];
+    REQUIRE(slots < (int64_t)INT32_MAX, "matrix: sliced storage exceeds int32 slot offsets");
+    k_slots_to_i32<<<ceil_div(nsl + 1, 256), 256, 0, s>>>(nsl + 1, sorted ? su.as<int64_t>() : off.as<int64_t>(),
+                                                         A->sptr.as<int>());
+    HIPCHK(hipGetLastError());
+    if (sorted) {
+      std::swap(A->perm.p, perm.p);
+      std::swap(A->perm.bytes, perm.bytes);
+      A->iperm.alloc(brows * 4);
+      k_inverse<<<ceil_div(brows, 256), 256, 0, s>>>(brows, A->perm.as<int>(), A->iperm.as<int>());
+      HIPCHK(hipGetLastError());
+    }
+    A->ntot = slots * 64;
+    A->col.alloc(std::max<int64_t>(A->ntot, 1) * 4);
+    A->val64.alloc(std::max<int64_t>(A->ntot * nval, 1) * 8);
+    k_sell_fill<<<ceil_div(nsl * 64, 256), 256, 0, s>>>(brows, nsl, nblk, A->ntot, nval, A->ptr.as<int>(),
+                                                         col.as<int>(), val64.as<double>(),
+                                                         A->perm.p ? A->perm.as<int>() : nullptr, A->sptr.as<int>(),
+                                                         A->col.as<int>(), A->val64.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
   } else {
-    L = avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
+    HIPCHK(hipMemsetAsync(A->sptr.p, 0, 4, s));
+    A->ntot = 0;
+    A->col.alloc(4);
+    A->val64.alloc(8 * nval);
+    HIPCHK(hipStreamSynchronize(s));
   }
-  A->lanes = L;
+  col.release();
+  val64.release();
+  A->has64 = true;
+  A->has32 = false;
+  A->diag_ok = false;
 }
 
 void spmv_launch(yafel_hip_matrix_s *A, int precision, const void *x, void *y, hipStream_t s) {
@@ -248,26 +404,30 @@ void spmv_launch(yafel_hip_matrix_s *A, int precision, const void *x, void *y, h
 void spmm_launch(yafel_hip_matrix_s *A, int precision, int k, const void *X, void *Y, hipStream_t s) {
   if (A->brows == 0) return;
   const int grid = grid_for(A->ctx, A->brows * 64, 256);
-  const int *p = A->ptr.as<int>(), *c = A->col.as<int>();
+  const int *p = A->ptr.as<int>(), *sp = A->sptr.as<int>(), *c = A->col.as<int>();
+  const int *ip = A->iperm.p ? A->iperm.as<int>() : nullptr;
   if (precision == YAFEL_HIP_FP32) {
     REQUIRE(A->has32, "spmm: matrix has no FP32 copy");
     if (A->format == YAFEL_HIP_CSR)
-      k_csr_spmm<float><<<grid, 256, 0, s>>>(A->rows, k, p, c, A->val32.as<float>(), (const float *)X, (float *)Y);
+      k_csr_spmm<float><<<grid, 256, 0, s>>>(A->rows, k, p, sp, ip, c, A->val32.as<float>(), (const float *)X, (float *)Y);
     else
-      k_bcsr_spmm<float><<<grid, 256, 0, s>>>(A->brows, A->nblk, k, p, c, A->val32.as<float>(), (const float *)X, (float *)Y);
+      k_bcsr_spmm<float><<<grid, 256, 0, s>>>(A->brows, A->ntot, k, p, sp, ip, c, A->val32.as<float>(),
+                                              (const float *)X, (float *)Y);
   } else {
     REQUIRE(A->has64, "spmm: matrix has no FP64 copy");
     if (A->format == YAFEL_HIP_CSR)
-      k_csr_spmm<double><<<grid, 256, 0, s>>>(A->rows, k, p, c, A->val64.as<double>(), (const double *)X, (double *)Y);
+      k_csr_spmm<double><<<grid, 256, 0, s>>>(A->rows, k, p, sp, ip, c, A->val64.as<double>(), (const double *)X,
+                                              (double *)Y);
     else
-      k_bcsr_spmm<double><<<grid, 256, 0, s>>>(A->brows, A->nblk, k, p, c, A->val64.as<double>(), (const double *)X, (double *)Y);
+      k_bcsr_spmm<double><<<grid, 256, 0, s>>>(A->brows, A->ntot, k, p, sp, ip, c, A->val64.as<double>(),
+                                               (const double *)X, (double *)Y);
   }
   HIPCHK(hipGetLastError());
 }
 
 void matrix_build_fp32(yafel_hip_matrix_s *A) {
-  const int64_t n = A->nblk * (A->format == YAFEL_HIP_CSR ? 1 : 9);
-  if (!A->val32.p || A->val32.bytes < (size_t)n * 4) A->val32.alloc((size_t)std::max<int64_t>(n, 1) * 4);
+  const int64_t n = A->ntot * (A->format == YAFEL_HIP_CSR ? 1 : 9);
+  if (!A->val32.p || A->val32.bytes < (size_t)std::max<int64_t>(n, 1) * 4) A->val32.alloc((size_t)std::max<int64_t>(n, 1) * 4);
   if (n > 0) {
     k_to_fp32<<<grid_for(A->ctx, n, 256), 256, 0, A->ctx->stream>>>(A->val64.as<double>(), A->val32.as<float>(), n);
     HIPCHK(hipGetLastError());
@@ -282,25 +442,27 @@ void matrix_build_diag(yafel_hip_matrix_s *A) {
     A->dinv64.alloc(std::max<int64_t>(n, 1) * 8);
     A->dinv32.alloc(std::max<int64_t>(n, 1) * 4);
   }
+  const int *ip = A->iperm.p ? A->iperm.as<int>() : nullptr;
   if (A->brows > 0) {
     if (A->format == YAFEL_HIP_CSR)
       k_diag_csr<<<ceil_div(A->brows, 256), 256, 0, A->ctx->stream>>>(
-          A->ptr.as<int>(), A->col.as<int>(), A->val64.as<double>(), A->brows, A->diag64.as<double>(),
-          A->dinv64.as<double>(), A->dinv32.as<float>());
+          A->brows, A->ptr.as<int>(), A->sptr.as<int>(), ip, A->col.as<int>(), A->val64.as<double>(),
+          A->diag64.as<double>(), A->dinv64.as<double>(), A->dinv32.as<float>());
     else
       k_diag_bcsr<<<ceil_div(A->brows, 256), 256, 0, A->ctx->stream>>>(
-          A->ptr.as<int>(), A->col.as<int>(), A->val64.as<double>(), A->brows, A->nblk,
+          A->brows, A->ntot, A->ptr.as<int>(), A->sptr.as<int>(), ip, A->col.as<int>(), A->val64.as<double>(),
           A->diag64.as<double>(), A->dinv64.as<double>(), A->dinv32.as<float>());
     HIPCHK(hipGetLastError());
   }
   A->diag_ok = true;
 }
 
-// Shared by the triplet constructors: takes ownership of device arrays.
+// Shared by the triplet and pattern constructors: takes the CSR-ordered device
+// arrays and converts them to sliced storage.
 yafel_hip_matrix_s *matrix_from_device(yafel_hip_ctx_s *ctx, int format, int64_t brows, int64_t bcols,
                                        int64_t nblk, int64_t ptr_len, DBuf &ptr, DBuf &col, DBuf &val64,
                                        int precisions) {
-  auto *A = new_matrix(ctx, format);
+  std::unique_ptr<yafel_hip_matrix_s> A(new_matrix(ctx, format));
   const int B = format == YAFEL_HIP_CSR ? 1 : 3;
   A->brows = brows;
   A->bcols = bcols;
@@ -308,13 +470,9 @@ yafel_hip_matrix_s *matrix_from_device(yafel_hip_ctx_s *ctx, int format, int64_t
   A->cols = bcols * B;
   A->nblk = nblk;
   A->ptr_len = ptr_len;
-  std::swap(A->ptr.p, ptr.p); std::swap(A->ptr.bytes, ptr.bytes);
-  std::swap(A->col.p, col.p); std::swap(A->col.bytes, col.bytes);
-  std::swap(A->val64.p, val64.p); std::swap(A->val64.bytes, val64.bytes);
-  A->has64 = true;
-  matrix_choose_lanes(A);
-  if (precisions & YAFEL_HIP_FP32) matrix_build_fp32(A);
-  return A;
+  matrix_make_sliced(A.get(), ptr, col, val64);
+  if (precisions & YAFEL_HIP_FP32) matrix_build_fp32(A.get());
+  return A.release();
 }
 
 }  // namespace yh
@@ -334,23 +492,18 @@ int yafel_hip_csr_create(yafel_hip_ctx ctx, int64_t rows, int64_t cols, const in
   HIPCHK(hipSetDevice(ctx->device));
   auto p32 = to_i32(row_ptr, rows + 1, nnz, "csr_create row_ptr");
   auto c32 = to_i32(col_index, nnz, cols - 1, "csr_create col_index");
-  std::unique_ptr<yafel_hip_matrix_s> A(new_matrix(ctx, YAFEL_HIP_CSR));
-  A->rows = A->brows = rows;
-  A->cols = A->bcols = cols;
-  A->nblk = nnz;
-  A->ptr_len = rows + 1;
-  A->ptr.alloc((rows + 1) * 4);
-  A->col.alloc(std::max<int64_t>(nnz, 1) * 4);
-  A->val64.alloc(std::max<int64_t>(nnz, 1) * 8);
   hipStream_t s = ctx->stream;
-  HIPCHK(hipMemcpyAsync(A->ptr.p, p32.data(), (rows + 1) * 4, hipMemcpyHostToDevice, s));
+  DBuf ptr, col, val;
+  ptr.alloc((rows + 1) * 4);
+  col.alloc(std::max<int64_t>(nnz, 1) * 4);
+  val.alloc(std::max<int64_t>(nnz, 1) * 8);
+  HIPCHK(hipMemcpyAsync(ptr.p, p32.data(), (rows + 1) * 4, hipMemcpyHostToDevice, s));
   if (nnz) {
-    HIPCHK(hipMemcpyAsync(A->col.p, c32.data(), nnz * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(A->val64.p, data, nnz * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(col.p, c32.data(), nnz * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(val.p, data, nnz * 8, hipMemcpyHostToDevice, s));
   }
-  A->has64 = true;
-  matrix_choose_lanes(A.get());
-  if (precisions & YAFEL_HIP_FP32) matrix_build_fp32(A.get());
+  std::unique_ptr<yafel_hip_matrix_s> A(matrix_from_device(ctx, YAFEL_HIP_CSR, rows, cols, nnz, rows + 1, ptr, col,
+                                                           val, precisions));
   HIPCHK(hipStreamSynchronize(s));
   *out = A.release();
   return YAFEL_HIP_OK;
@@ -371,30 +524,23 @@ int yafel_hip_bcsr3_create(yafel_hip_ctx ctx, int64_t brows, int64_t bcols, cons
   HIPCHK(hipSetDevice(ctx->device));
   auto p32 = to_i32(brow_ptr, brows + 1, nb, "bcsr3_create brow_ptr");
   auto c32 = to_i32(bcol_index, nb, bcols - 1, "bcsr3_create bcol_index");
-  std::unique_ptr<yafel_hip_matrix_s> A(new_matrix(ctx, YAFEL_HIP_BCSR3));
-  A->brows = brows;
-  A->bcols = bcols;
-  A->rows = 3 * brows;
-  A->cols = 3 * bcols;
-  A->nblk = nb;
-  A->ptr_len = brows + 1;
-  A->ptr.alloc((brows + 1) * 4);
-  A->col.alloc(std::max<int64_t>(nb, 1) * 4);
-  A->val64.alloc(std::max<int64_t>(nb, 1) * 72);
   hipStream_t s = ctx->stream;
-  HIPCHK(hipMemcpyAsync(A->ptr.p, p32.data(), (brows + 1) * 4, hipMemcpyHostToDevice, s));
+  DBuf ptr, col, val;
+  ptr.alloc((brows + 1) * 4);
+  col.alloc(std::max<int64_t>(nb, 1) * 4);
+  val.alloc(std::max<int64_t>(nb, 1) * 72);
+  HIPCHK(hipMemcpyAsync(ptr.p, p32.data(), (brows + 1) * 4, hipMemcpyHostToDevice, s));
   if (nb) {
     DBuf tmp;
     tmp.alloc(nb * 72);
-    HIPCHK(hipMemcpyAsync(A->col.p, c32.data(), nb * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(col.p, c32.data(), nb * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(tmp.p, data, nb * 72, hipMemcpyHostToDevice, s));
-    k_aos_to_soa<<<grid_for(ctx, nb * 9, 256), 256, 0, s>>>(tmp.as<double>(), A->val64.as<double>(), nb);
+    k_aos_to_soa<<<grid_for(ctx, nb * 9, 256), 256, 0, s>>>(tmp.as<double>(), val.as<double>(), nb);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
   }
-  A->has64 = true;
-  matrix_choose_lanes(A.get());
-  if (precisions & YAFEL_HIP_FP32) matrix_build_fp32(A.get());
+  std::unique_ptr<yafel_hip_matrix_s> A(matrix_from_device(ctx, YAFEL_HIP_BCSR3, brows, bcols, nb, brows + 1, ptr,
+                                                           col, val, precisions));
   HIPCHK(hipStreamSynchronize(s));
   *out = A.release();
   return YAFEL_HIP_OK;
@@ -434,25 +580,22 @@ int yafel_hip_matrix_download(yafel_hip_matrix A, int64_t *row_ptr, int64_t *col
     for (size_t i = 0; i < p.size(); ++i) row_ptr[i] = p[i];
     for (int64_t i = (int64_t)p.size(); i < A->ptr_len; ++i) row_ptr[i] = A->ptr_tail.empty() ? p.back() : A->ptr_tail[i - p.size()];
   }
-  if (col_index && A->nblk) {
-    std::vector<int32_t> c(A->nblk);
-    HIPCHK(hipMemcpyAsync(c.data(), A->col.p, c.size() * 4, hipMemcpyDeviceToHost, s));
+  if ((col_index || data) && A->nblk) {
+    REQUIRE(!data || A->has64, "matrix_download: no FP64 values");
+    const int nval = A->format == YAFEL_HIP_CSR ? 1 : 9;
+    DBuf dc, dv;
+    if (col_index) dc.alloc(A->nblk * 4);
+    if (data) dv.alloc(A->nblk * nval * 8);
+    k_sell_to_csr<<<ceil_div(A->brows, 256), 256, 0, s>>>(
+        A->brows, A->ntot, nval, A->ptr.as<int>(), A->sptr.as<int>(), A->iperm.p ? A->iperm.as<int>() : nullptr,
+        A->col.as<int>(), A->val64.as<double>(), col_index ? dc.as<int>() : nullptr,
+        data ? dv.as<double>() : nullptr);
+    HIPCHK(hipGetLastError());
+    std::vector<int32_t> c(col_index ? A->nblk : 0);
+    if (col_index) HIPCHK(hipMemcpyAsync(c.data(), dc.p, A->nblk * 4, hipMemcpyDeviceToHost, s));
+    if (data) HIPCHK(hipMemcpyAsync(data, dv.p, A->nblk * nval * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     for (size_t i = 0; i < c.size(); ++i) col_index[i] = c[i];
-  }
-  if (data && A->nblk) {
-    REQUIRE(A->has64, "matrix_download: no FP64 values");
-    if (A->format == YAFEL_HIP_CSR) {
-      HIPCHK(hipMemcpyAsync(data, A->val64.p, A->nblk * 8, hipMemcpyDeviceToHost, s));
-    } else {
-      DBuf tmp;
-      tmp.alloc(A->nblk * 72);
-      k_soa_to_aos<<<grid_for(A->ctx, A->nblk * 9, 256), 256, 0, s>>>(A->val64.as<double>(), tmp.as<double>(), A->nblk);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(data, tmp.p, A->nblk * 72, hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-    }
-    HIPCHK(hipStreamSynchronize(s));
   }
   return YAFEL_HIP_OK;
   YH_TRY_END(A->ctx)
@@ -467,13 +610,11 @@ int yafel_hip_matrix_sync_fp32(yafel_hip_matrix A) {
   YH_TRY_END(A ? A->ctx : nullptr)
 }
 
-int yafel_hip_matrix_set_lanes(yafel_hip_matrix A, int lanes) {
+int yafel_hip_matrix_set_unroll(yafel_hip_matrix A, int unroll) {
   YH_TRY_BEGIN
-  REQUIRE(A, "matrix_set_lanes: null matrix");
-  const bool pow2 = lanes > 0 && (lanes & (lanes - 1)) == 0;
-  if (A->format == YAFEL_HIP_CSR) REQUIRE(pow2 && lanes >= 2 && lanes <= 64, "matrix_set_lanes: CSR lanes must be 2..64, a power of two");
-  else REQUIRE(pow2 && lanes >= 8 && lanes <= 64, "matrix_set_lanes: BCSR3 lanes must be 8..64, a power of two");
-  A->lanes = lanes;
+  REQUIRE(A, "matrix_set_unroll: null matrix");
+  REQUIRE(unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8, "matrix_set_unroll: unroll must be 1, 2, 4 or 8");
+  A->unroll = unroll;
   return YAFEL_HIP_OK;
   YH_TRY_END(A ? A->ctx : nullptr)
 }

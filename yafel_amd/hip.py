@@ -18,7 +18,7 @@ if not os.path.exists(LIB_PATH):
 lib = C.CDLL(LIB_PATH)
 
 OK, EINVAL, EDEVICE, ENOMEM, ESTATE = 0, 1, 2, 3, 4
-FP64, FP32, MIXED = 1, 2, 3
+FP64, FP32, MIXED, MIXED_MATRIX = 1, 2, 3, 4
 CSR, BCSR3 = 1, 3
 CG, PCG_JACOBI, BICGSTAB, BICGSTAB_JACOBI = 0, 1, 2, 3
 STOP_REFERENCE, STOP_RELRES = 0, 1
@@ -90,7 +90,7 @@ for _n, _a in [
     ("yafel_hip_bcsr3_from_triplets", [_P, _I64, _P, _P, _P, _I, C.POINTER(_P)]),
     ("yafel_hip_matrix_destroy", [_P]), ("yafel_hip_matrix_info_get", [_P, C.POINTER(MatrixInfo)]),
     ("yafel_hip_matrix_download", [_P, _P, _P, _P]), ("yafel_hip_matrix_sync_fp32", [_P]),
-    ("yafel_hip_matrix_diagonal", [_P, _P]), ("yafel_hip_matrix_set_lanes", [_P, _I]),
+    ("yafel_hip_matrix_diagonal", [_P, _P]), ("yafel_hip_matrix_set_unroll", [_P, _I]),
     ("yafel_hip_spmv", [_P, _I, _P, _P]), ("yafel_hip_spmv_host", [_P, _P, _P]),
     ("yafel_hip_spmm", [_P, _I, _I, _P, _P]), ("yafel_hip_dot", [_P, _I64, _P, _P, C.POINTER(_D)]),
     ("yafel_hip_copy_bandwidth", [_P, C.c_size_t, _I, C.POINTER(_D)]),
@@ -282,8 +282,8 @@ class Matrix:
         check(lib.yafel_hip_matrix_download(self.h, _ptr(ptr), _ptr(col), _ptr(val)), self.ctx.h)
         return ptr, col, val
 
-    def set_lanes(self, lanes):
-        check(lib.yafel_hip_matrix_set_lanes(self.h, lanes), self.ctx.h)
+    def set_unroll(self, unroll):
+        check(lib.yafel_hip_matrix_set_unroll(self.h, unroll), self.ctx.h)
 
     def sync_fp32(self):
         check(lib.yafel_hip_matrix_sync_fp32(self.h), self.ctx.h)
