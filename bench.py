@@ -83,10 +83,10 @@ def cpu_baseline(w, fp64_cg_iters, hip):
     }
 
 
-def run_stream(hip, ctx, mats, arrivals, budget, slots):
+def run_stream(hip, ctx, mats, arrivals, budget, slots, setup_threads=0):
     """Feed requests at their arrival times into a server; returns per-request
     latency (arrival -> finished), makespan and the results."""
-    srv = hip.Server(ctx, budget, slots)
+    srv = hip.Server(ctx, budget, slots, setup_threads)
     n = len(mats)
     t0 = time.perf_counter()
     i, done_t, rid_of = 0, {}, {}
@@ -123,13 +123,13 @@ def serve_bench(args):
     mats = [workloads.c4_materialise(r) for r in reqs]
     log(f"{len(mats)} requests materialised; dofs total {sum(r['ndofs'] for r in reqs)}")
     # warm-up (kernel code objects, allocator)
-    run_stream(hip, ctx, mats[:8], np.zeros(8), budget, args.slots)
+    run_stream(hip, ctx, mats[:8], np.zeros(8), budget, args.slots, args.setup_threads)
     sweep = []
     rates = [float(r) for r in args.rates.split(",")]
     for rate in rates:
         nr = args.requests if rate == max(rates) else min(args.requests, args.sweep_requests)
         arr = workloads.poisson_arrivals(nr, rate, seed=int(rate))
-        lat, span, res, st = run_stream(hip, ctx, mats[:nr], arr, budget, args.slots)
+        lat, span, res, st = run_stream(hip, ctx, mats[:nr], arr, budget, args.slots, args.setup_threads)
         bad = sum(1 for x, inf in res.values() if inf.status != 0 or inf.rel_residual > TOL)
         rec = {"rate": rate, "requests": nr, "solves_per_s": nr / span, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                "p99_ms": float(np.percentile(lat, 99) * 1e3), "failed": bad, "hbm_peak_gb": st["hbm_peak"] / 2**30,
@@ -142,7 +142,7 @@ def serve_bench(args):
     # alone vs in the mix: bit-identical
     diffs = []
     for j in range(min(6, nr)):
-        _, _, r1, _ = run_stream(hip, ctx, [mats[j]], np.zeros(1), budget, args.slots)
+        _, _, r1, _ = run_stream(hip, ctx, [mats[j]], np.zeros(1), budget, args.slots, args.setup_threads)
         diffs.append(float(np.abs(r1[0][0] - res[j][0]).max()))
     # the plain FP64 GPU path on a sample of the same requests, one at a time
     ns = min(args.fp64_sample, nr)
@@ -179,7 +179,7 @@ def serve_bench(args):
         "rate_sweep": sweep, "alone_vs_mix_max_abs_diff": max(diffs) if diffs else None,
         "fp64_gpu_one_at_a_time_solves_per_s": fp64_rate, "fp64_sample": ns,
         "speedup_vs_fp64_gpu": value / fp64_rate,
-        "server": {k: st[k] for k in ("admitted", "completed", "failed", "chunks", "batched_launches")},
+        "server": {k: st[k] for k in ("admitted", "completed", "failed", "chunks", "batched_launches", "setup_ms")},
     }
     print(json.dumps(out), flush=True)
 
@@ -198,6 +198,7 @@ def main():
     ap.add_argument("--rates", default="50,200,500,1000,2000")
     ap.add_argument("--slots", type=int, default=128)
     ap.add_argument("--fp64-sample", type=int, default=100)
+    ap.add_argument("--setup-threads", type=int, default=0, help="server setup workers (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): each solve of this workload "

@@ -286,9 +286,14 @@ typedef struct {
   int64_t submitted, admitted, completed, failed, cancelled, queued, active;
   int64_t chunks, batched_launches;
   size_t hbm_in_use, hbm_peak, hbm_budget;
+  double setup_ms;              /* host time the setup worker spent building requests */
 } yafel_hip_server_stats;
 
 int yafel_hip_server_create(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots, yafel_hip_server *out);
+/* as yafel_hip_server_create, with the number of setup worker threads (each
+ * with its own setup stream; 0: the default of 3) */
+int yafel_hip_server_create_ex(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots, int setup_threads,
+                               yafel_hip_server *out);
 int yafel_hip_server_destroy(yafel_hip_server s);
 /* copies the request's inputs; YAFEL_HIP_ENOMEM when it could never fit the budget */
 int yafel_hip_server_submit(yafel_hip_server s, const yafel_hip_request *req, int64_t *id);

@@ -10,6 +10,7 @@
 #include <memory>
 #include <atomic>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "yafel_hip.h"
@@ -47,17 +48,31 @@ void hip_check(hipError_t e, const char *what, const char *file, int line);
 
 // -------------------------------------------------------- device buffers
 // RAII device allocation (stream-ordered free on the owning stream).
+// Device buffer.  Allocated with hipMalloc, or -- on a thread that has set
+// alloc_stream (the serving setup worker) -- stream-ordered from the device's
+// memory pool on that stream, so that building one request never waits for
+// the kernels other streams are running (hipFree synchronises the device).
 struct DBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  hipStream_t s = nullptr;  // stream of a stream-ordered allocation
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
   ~DBuf() { release(); }
   void alloc(size_t b);
   void release();
+  void swap(DBuf &o) {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
+    std::swap(s, o.s);
+  }
   template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
+
+// Stream for the stream-ordered allocations of the calling thread (nullptr:
+// plain hipMalloc/hipFree).
+void set_alloc_stream(hipStream_t s);
 
 }  // namespace yh
 

@@ -50,7 +50,7 @@ void launch_spmv_op(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
   else k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
   HIPCHK(hipGetLastError());
   if constexpr (Op::NP > 0) {
-    k_finish<Op><<<1, NT, 0, s>>>(op, grid);
+    k_finish<Op><<<1, NTF, 0, s>>>(op, grid);
     HIPCHK(hipGetLastError());
   }
 }
@@ -61,7 +61,7 @@ void launch_elem(yafel_hip_ctx_s *ctx, int64_t n, const Op &op, hipStream_t s) {
   k_elem<Op><<<grid, NT, 0, s>>>(n, op);
   HIPCHK(hipGetLastError());
   if constexpr (Op::NP > 0) {
-    k_finish<Op><<<1, NT, 0, s>>>(op, grid);
+    k_finish<Op><<<1, NTF, 0, s>>>(op, grid);
     HIPCHK(hipGetLastError());
   }
 }

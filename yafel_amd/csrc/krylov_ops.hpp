@@ -56,25 +56,43 @@ __device__ __forceinline__ void finish_partials(Op &op, double (&acc)[NP > 0 ? N
 
 // Fold the nblk partials of each sum in block order (deterministic) and run
 // op.last().  Evaluates op.skip() exactly as the producing kernel did: only
-// op.last() changes the state, so both see the same flags.
+// op.last() changes the state, so both see the same flags.  Runs in one
+// workgroup of NTF threads: the partial loads of all sums are independent, so
+// a wide workgroup keeps them in flight together.
+constexpr int NTF = 1024;
+
 template <class Op>
 __device__ __forceinline__ void finish_body(Op &op, int nblk) {
   if (op.skip()) return;
   op.begin();
   constexpr int NP = Op::NP;
-  __shared__ double lds[NT / 64];
+  __shared__ double lds[NP > 0 ? NP : 1][NTF / 64];
   __shared__ double sums[NP > 0 ? NP : 1];
+  double v[NP > 0 ? NP : 1];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) v[j] = 0;
+  for (int i = threadIdx.x; i < nblk; i += NTF)
+#pragma unroll
+    for (int j = 0; j < NP; ++j) v[j] += op.part[(size_t)j * nblk + i];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
-    double s = dev::reduce_partials<NT>(op.part + (size_t)j * nblk, nblk, 1, lds);
-    if (threadIdx.x == 0) sums[j] = s;
+    const double t = dev::wave_sum(v[j]);
+    if (l == 0) lds[j][w] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < NP) {
+    double t = 0;
+#pragma unroll
+    for (int q = 0; q < NTF / 64; ++q) t += lds[threadIdx.x][q];
+    sums[threadIdx.x] = t;
   }
   __syncthreads();
   if (threadIdx.x == 0) op.last(sums);
 }
 
 template <class Op>
-__global__ __launch_bounds__(NT) void k_finish(Op op0, int nblk) {
+__global__ __launch_bounds__(NTF) void k_finish(Op op0, int nblk) {
   Op op = op0;
   finish_body(op, nblk);
 }

@@ -422,10 +422,8 @@ int yafel_hip_mesh_create_cg(yafel_hip_ctx ctx, int dim, int p, int dpn, int64_t
   if (p == 1) {
     // make_cg_dofs p == 1 (DoFManager.cpp:153-160): the mesh passes through.
     m->n_nodes = n_corner;
-    std::swap(m->elements.p, dcells.p);
-    std::swap(m->elements.bytes, dcells.bytes);
-    std::swap(m->nodes.p, dcorner.p);
-    std::swap(m->nodes.bytes, dcorner.bytes);
+    m->elements.swap(dcells);
+    m->nodes.swap(dcorner);
   } else {
     // make_raw_dofs + recombine_all_duplicates (DoFManager.cpp:176-229, 340-414)
     const int64_t N = ncell * m->nen;

@@ -27,16 +27,29 @@ void record_error(yafel_hip_ctx_s *ctx, const std::string &msg) {
   if (ctx) ctx->err = msg;
 }
 
+static thread_local hipStream_t g_alloc_stream = nullptr;
+
+void set_alloc_stream(hipStream_t s) { g_alloc_stream = s; }
+
 void DBuf::alloc(size_t b) {
   release();
   if (b == 0) return;
-  HIPCHK(hipMalloc(&p, b));
+  if (g_alloc_stream) {
+    HIPCHK(hipMallocAsync(&p, b, g_alloc_stream));
+    s = g_alloc_stream;
+  } else {
+    HIPCHK(hipMalloc(&p, b));
+  }
   bytes = b;
 }
 void DBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p) {
+    if (s) (void)hipFreeAsync(p, s);
+    else (void)hipFree(p);
+  }
   p = nullptr;
   bytes = 0;
+  s = nullptr;
 }
 
 int grid_for(yafel_hip_ctx_s *ctx, int64_t work_items, int block) {

@@ -6,15 +6,24 @@
 // covers every active slot, each slot on its own contiguous block range
 // (block counts fixed by the slot's size alone) with its own FP64 scalars and
 // its own last-block reductions -- so a request's result is bit-identical
-// whether it runs alone or in a mix.  The host loop (progress) admits queued
-// requests under the HBM budget on a separate setup stream while up to two
-// chunks of batched iterations run on the solve stream, and retires slots
-// whose solve finished.
+// whether it runs alone or in a mix.
+//
+// Two host threads drive the device.  A setup worker owned by the server takes
+// queued requests in FIFO order as the HBM budget and the slot limit allow and
+// builds each one (DoF numbering, pattern, assembly, Dirichlet, FP32 copy,
+// diagonal) on its own setup stream, with stream-ordered allocations so that
+// it never waits on the solve stream.  The caller's thread (progress) joins
+// the slots the worker finished, keeps up to two chunks of batched iterations
+// in flight on the solve stream, and retires slots whose solve finished.  A
+// server is driven from one caller thread at a time.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <deque>
 #include <map>
+#include <mutex>
+#include <thread>
 
 #include "krylov_ops.hpp"
 #include "mesh.hpp"
@@ -129,7 +138,7 @@ __global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slot
 
 // one workgroup per slot folds that slot's partials
 template <int PH>
-__global__ __launch_bounds__(NT) void kb_finish(const SlotDesc *__restrict__ slots, int ns) {
+__global__ __launch_bounds__(NTF) void kb_finish(const SlotDesc *__restrict__ slots, int ns) {
   const SlotDesc d = slots[blockIdx.x];
   slot_phase<PH, true>(d, 0);
 }
@@ -192,6 +201,7 @@ struct Slot {
   void *buf = nullptr;  // one stream-ordered allocation for the solve vectors and state
   size_t buf_bytes = 0;
   SlotDesc d{};
+  int64_t ndofs = 0;
   bool joined = false, retired = false;
 };
 
@@ -206,7 +216,7 @@ T *carve(char *&p, size_t count) {
 
 struct yafel_hip_server_s {
   yafel_hip_ctx_s *ctx = nullptr;   // solve stream
-  yafel_hip_ctx_s *sctx = nullptr;  // setup stream (same device)
+  std::vector<yafel_hip_ctx_s *> sctx;  // one setup context (stream) per setup worker
   size_t budget = 0, in_use = 0, peak = 0;
   int max_slots = 0;
   int64_t next_id = 1;
@@ -219,7 +229,7 @@ struct yafel_hip_server_s {
   // at most two calls' work is in flight, so an entry is never rewritten while
   // a launch that reads it can still run
   static constexpr int RING = 8;
-  static constexpr size_t STAGE = 32768;  // pinned bytes per ring entry (table / statuses)
+  static constexpr size_t STAGE = 131072;  // pinned bytes per ring entry (table / statuses)
   DBuf desc[RING], status[RING];
   int ring = 0;
   struct Inflight {
@@ -233,6 +243,16 @@ struct yafel_hip_server_s {
   char *pin = nullptr;  // RING status areas, then RING table staging areas
   int K = 8;
   std::vector<Slot *> joining;
+  hipStream_t copy_stream = nullptr;  // result copies to the host
+  // shared with the setup worker, guarded by mu
+  std::mutex mu;
+  std::condition_variable cv;        // wakes the worker
+  std::condition_variable cv_ready;  // wakes an idle progress call
+  std::vector<std::thread> workers;
+  bool stop = false;
+  int held = 0;                     // slots in setup, set up, joining or active
+  std::vector<Slot *> ready;        // set up by the worker, not yet joined
+  std::vector<int64_t> failed_ids;  // setups that failed, not yet reported
 };
 
 namespace {
@@ -269,11 +289,10 @@ void check_rc(int rc, yafel_hip_ctx_s *c) {
   if (rc != YAFEL_HIP_OK) fail(rc, yafel_hip_last_error(c));
 }
 
-// Build the request's device system on the setup stream.
-Slot *setup_slot(yafel_hip_server_s *S, Req *r) {
+// Build the request's device system on setup context c.
+Slot *setup_slot(yafel_hip_ctx_s *c, Req *r) {
   auto *sl = new Slot();
   sl->req = r;
-  yafel_hip_ctx_s *c = S->sctx;
   yafel_hip_mesh m = nullptr;
   try {
     check_rc(yafel_hip_mesh_create_cg(c, r->dim, r->p, r->dpn, (int64_t)r->nodes.size() / 3, r->nodes.data(),
@@ -282,7 +301,7 @@ Slot *setup_slot(yafel_hip_server_s *S, Req *r) {
     const int fmt = r->dpn == 3 ? YAFEL_HIP_BCSR3 : YAFEL_HIP_CSR;
     check_rc(yafel_hip_mesh_pattern(m, fmt, YAFEL_HIP_FP64, &sl->A), c);
     const int64_t n = m->n_nodes * r->dpn;
-    r->ndofs = n;
+    sl->ndofs = n;
     sl->b.alloc(n * 8);
     HIPCHK(hipMemsetAsync(sl->b.p, 0, n * 8, c->stream));
     yafel_hip_physics ph = r->phys;
@@ -375,11 +394,69 @@ void join_slot(yafel_hip_server_s *S, Slot *sl) {
 void release_slot(yafel_hip_server_s *S, Slot *sl) {
   if (sl->buf) (void)hipFreeAsync(sl->buf, S->ctx->stream);
   sl->buf = nullptr;
-  if (sl->A) yafel_hip_matrix_destroy(sl->A);
+  // the solve stream is done with the slot (its last chunk was drained) and
+  // the setup stream finished building it: free stream-ordered, no sync
+  delete sl->A;
   sl->A = nullptr;
   sl->b.release();
   sl->mask.release();
-  S->in_use -= std::min(S->in_use, sl->req ? sl->req->est_bytes : 0);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->in_use -= std::min(S->in_use, sl->req ? sl->req->est_bytes : 0);
+    S->held -= 1;
+  }
+  S->cv.notify_one();
+}
+
+// A setup worker: FIFO admission under the HBM budget and the slot limit;
+// builds on its own setup context.
+void worker_main(yafel_hip_server_s *S, yafel_hip_ctx_s *c) {
+  (void)hipSetDevice(S->ctx->device);
+  set_alloc_stream(c->stream);
+  std::unique_lock<std::mutex> lk(S->mu);
+  while (true) {
+    S->cv.wait(lk, [S] {
+      if (S->stop) return true;
+      while (!S->queue.empty() && S->queue.front()->state == YAFEL_HIP_REQ_CANCELLED) S->queue.pop_front();
+      return !S->queue.empty() && S->held < S->max_slots && S->in_use + S->queue.front()->est_bytes <= S->budget;
+    });
+    if (S->stop) break;
+    Req *r = S->queue.front();
+    S->queue.pop_front();
+    r->state = YAFEL_HIP_REQ_RUNNING;
+    S->in_use += r->est_bytes;
+    S->peak = std::max(S->peak, S->in_use);
+    S->held += 1;
+    lk.unlock();
+    Slot *sl = nullptr;
+    std::string err;
+    const auto t0 = std::chrono::steady_clock::now();
+    try {
+      sl = setup_slot(c, r);
+    } catch (const Error &e) {
+      err = e.msg;
+    } catch (const std::exception &e) {
+      err = e.what();
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    lk.lock();
+    S->stats.setup_ms += ms;
+    if (sl) {
+      r->ndofs = sl->ndofs;
+      S->ready.push_back(sl);
+      S->stats.admitted++;
+      S->cv_ready.notify_one();
+    } else {
+      r->state = YAFEL_HIP_REQ_FAILED;
+      r->error = err;
+      S->in_use -= std::min(S->in_use, r->est_bytes);
+      S->held -= 1;
+      S->stats.failed++;
+      S->failed_ids.push_back(r->id);
+      S->cv_ready.notify_one();
+    }
+  }
+  set_alloc_stream(nullptr);
 }
 
 // upload a descriptor table for `slots` (offsets recomputed) into ring entry
@@ -414,7 +491,7 @@ void launch_phase(yafel_hip_server_s *S, int e, int ns, int blocks) {
   kb_phase<PH><<<blocks, NT, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns);
   HIPCHK(hipGetLastError());
   if (PH != PH_FOLD) {
-    kb_finish<PH><<<ns, NT, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns);
+    kb_finish<PH><<<ns, NTF, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns);
     HIPCHK(hipGetLastError());
   }
   S->stats.batched_launches++;
@@ -435,11 +512,14 @@ void drain_one(yafel_hip_server_s *S, std::vector<int64_t> &done_ids) {
     r->info.criterion = st[i].crit;
     r->info.rel_residual = st[i].crit;  // the FP64 true residual at the last refinement
     r->x.resize(sl->d.n);
-    // x is final: its chunk completed; copy on the setup stream
-    HIPCHK(hipMemcpyAsync(r->x.data(), sl->d.x, sl->d.n * 8, hipMemcpyDeviceToHost, S->sctx->stream));
-    HIPCHK(hipStreamSynchronize(S->sctx->stream));
-    r->state = YAFEL_HIP_REQ_DONE;
-    S->stats.completed++;
+    // x is final: its chunk completed
+    HIPCHK(hipMemcpyAsync(r->x.data(), sl->d.x, sl->d.n * 8, hipMemcpyDeviceToHost, S->copy_stream));
+    HIPCHK(hipStreamSynchronize(S->copy_stream));
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      r->state = YAFEL_HIP_REQ_DONE;
+      S->stats.completed++;
+    }
     done_ids.push_back(r->id);
     sl->retired = true;
   }
@@ -450,18 +530,30 @@ void drain_one(yafel_hip_server_s *S, std::vector<int64_t> &done_ids) {
 extern "C" {
 
 int yafel_hip_server_create(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots, yafel_hip_server *out) {
+  return yafel_hip_server_create_ex(ctx, hbm_budget, max_slots, 0, out);
+}
+
+int yafel_hip_server_create_ex(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots, int setup_threads,
+                               yafel_hip_server *out) {
   YH_TRY_BEGIN
   REQUIRE(ctx && out && max_slots > 0 && hbm_budget > 0, "server_create: bad argument");
+  REQUIRE(setup_threads >= 0 && setup_threads <= 16, "server_create: setup_threads must be 0..16");
   REQUIRE((size_t)max_slots * sizeof(SlotDesc) <= yafel_hip_server_s::STAGE, "server_create: max_slots too large");
   HIPCHK(hipSetDevice(ctx->device));
+  if (setup_threads == 0) setup_threads = 3;
   auto *S = new yafel_hip_server_s();
   ctx_retain(ctx);
   S->ctx = ctx;
-  int rc = yafel_hip_ctx_create(ctx->device, &S->sctx);
-  if (rc != YAFEL_HIP_OK) {
-    delete S;
-    ctx_release(ctx);
-    fail(rc, yafel_hip_last_error(nullptr));
+  for (int i = 0; i < setup_threads; ++i) {
+    yafel_hip_ctx_s *c = nullptr;
+    int rc = yafel_hip_ctx_create(ctx->device, &c);
+    if (rc != YAFEL_HIP_OK) {
+      for (auto *o : S->sctx) yafel_hip_ctx_destroy(o);
+      delete S;
+      ctx_release(ctx);
+      fail(rc, yafel_hip_last_error(nullptr));
+    }
+    S->sctx.push_back(c);
   }
   S->budget = hbm_budget;
   S->max_slots = max_slots;
@@ -469,6 +561,8 @@ int yafel_hip_server_create(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots,
   for (auto &e : S->evs) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipHostMalloc((void **)&S->pin, 2 * yafel_hip_server_s::RING * yafel_hip_server_s::STAGE,
                        hipHostMallocDefault));
+  HIPCHK(hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking));
+  for (auto *c : S->sctx) S->workers.emplace_back(worker_main, S, c);
   *out = S;
   return YAFEL_HIP_OK;
   YH_TRY_END(ctx)
@@ -476,7 +570,16 @@ int yafel_hip_server_create(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots,
 
 int yafel_hip_server_destroy(yafel_hip_server S) {
   if (!S) return YAFEL_HIP_OK;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv.notify_all();
+  for (auto &w : S->workers)
+    if (w.joinable()) w.join();
   (void)hipStreamSynchronize(S->ctx->stream);
+  for (Slot *sl : S->ready) S->joining.push_back(sl);
+  S->ready.clear();
   for (Slot *sl : S->active) {
     release_slot(S, sl);
     delete sl;
@@ -489,7 +592,9 @@ int yafel_hip_server_destroy(yafel_hip_server S) {
   (void)hipStreamSynchronize(S->ctx->stream);
   for (auto &e : S->evs) (void)hipEventDestroy(e);
   if (S->pin) (void)hipHostFree(S->pin);
-  yafel_hip_ctx_destroy(S->sctx);
+  for (auto *c : S->sctx) (void)hipStreamSynchronize(c->stream);
+  if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
+  for (auto *c : S->sctx) yafel_hip_ctx_destroy(c);
   yafel_hip_ctx_s *ctx = S->ctx;
   delete S;
   ctx_release(ctx);
@@ -520,6 +625,7 @@ int yafel_hip_server_submit(yafel_hip_server S, const yafel_hip_request *q, int6
   r->phys.coef = nullptr;
   r->est_bytes = estimate_bytes(*r);
   if (r->est_bytes > S->budget) {
+    std::lock_guard<std::mutex> lk(S->mu);
     S->stats.failed++;
     fail(YAFEL_HIP_ENOMEM, "server_submit: the request alone needs " + std::to_string(r->est_bytes) +
                                 " bytes, more than the HBM budget");
@@ -529,10 +635,14 @@ int yafel_hip_server_submit(yafel_hip_server S, const yafel_hip_request *q, int6
     r->load.assign(q->load, q->load + q->load_len);
   }
   *id = r->id;
-  S->queue.push_back(r.get());
-  S->reqs[r->id] = std::move(r);
-  S->stats.submitted++;
-  S->stats.queued = (int64_t)S->queue.size();
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->queue.push_back(r.get());
+    S->reqs[r->id] = std::move(r);
+    S->stats.submitted++;
+    S->stats.queued = (int64_t)S->queue.size();
+  }
+  S->cv.notify_one();
   return YAFEL_HIP_OK;
   YH_TRY_END(S ? S->ctx : nullptr)
 }
@@ -542,28 +652,18 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
   REQUIRE(S && n_done, "server_progress: null argument");
   HIPCHK(hipSetDevice(S->ctx->device));
   std::vector<int64_t> done;
-  // 1. admission (setup stream; overlaps the chunks in flight)
-  while (!S->queue.empty() && (int)(S->active.size() + S->joining.size()) < S->max_slots) {
-    Req *r = S->queue.front();
-    if (r->state == YAFEL_HIP_REQ_CANCELLED) {
-      S->queue.pop_front();
-      continue;
-    }
-    if (S->in_use + r->est_bytes > S->budget) break;  // FIFO: wait for memory
-    S->queue.pop_front();
-    try {
-      Slot *sl = setup_slot(S, r);
-      S->in_use += r->est_bytes;
-      S->peak = std::max(S->peak, S->in_use);
-      S->joining.push_back(sl);
-      r->state = YAFEL_HIP_REQ_RUNNING;
-      S->stats.admitted++;
-    } catch (const Error &e) {
-      r->state = YAFEL_HIP_REQ_FAILED;
-      r->error = e.msg;
-      S->stats.failed++;
-      done.push_back(r->id);
-    }
+  // 1. pick up what the setup worker finished; with nothing on the device,
+  //    wait briefly for it rather than return at once
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (S->active.empty() && S->joining.empty() && S->inflight.empty() && S->ready.empty() &&
+        S->failed_ids.empty() && (S->held > 0 || !S->queue.empty()))
+      S->cv_ready.wait_for(lk, std::chrono::milliseconds(2),
+                           [S] { return !S->ready.empty() || !S->failed_ids.empty(); });
+    for (Slot *sl : S->ready) S->joining.push_back(sl);
+    S->ready.clear();
+    for (int64_t id : S->failed_ids) done.push_back(id);
+    S->failed_ids.clear();
   }
   // 2. drain the oldest chunk when two are in flight
   while (S->inflight.size() >= 2) drain_one(S, done);
@@ -621,10 +721,13 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
     for (Slot *sl : S->grave) delete sl;
     S->grave.clear();
   }
-  S->stats.queued = (int64_t)S->queue.size();
-  S->stats.active = (int64_t)S->active.size();
-  S->stats.hbm_in_use = S->in_use;
-  S->stats.hbm_peak = S->peak;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stats.queued = (int64_t)S->queue.size();
+    S->stats.active = (int64_t)S->active.size();
+    S->stats.hbm_in_use = S->in_use;
+    S->stats.hbm_peak = S->peak;
+  }
   *n_done = 0;
   for (int64_t id : done) {
     if (done_ids && *n_done < max_ids) done_ids[*n_done] = id;
@@ -637,6 +740,7 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
 int yafel_hip_server_query(yafel_hip_server S, int64_t id, int *state, int64_t *ndofs) {
   YH_TRY_BEGIN
   REQUIRE(S, "server_query: null server");
+  std::lock_guard<std::mutex> lk(S->mu);
   auto it = S->reqs.find(id);
   REQUIRE(it != S->reqs.end(), "server_query: unknown request id");
   if (state) *state = it->second->state;
@@ -648,6 +752,7 @@ int yafel_hip_server_query(yafel_hip_server S, int64_t id, int *state, int64_t *
 int yafel_hip_server_result(yafel_hip_server S, int64_t id, double *x, int64_t n, yafel_hip_solve_info *info) {
   YH_TRY_BEGIN
   REQUIRE(S, "server_result: null server");
+  std::lock_guard<std::mutex> lk(S->mu);
   auto it = S->reqs.find(id);
   REQUIRE(it != S->reqs.end(), "server_result: unknown request id");
   Req *r = it->second.get();
@@ -664,6 +769,7 @@ int yafel_hip_server_result(yafel_hip_server S, int64_t id, double *x, int64_t n
 int yafel_hip_server_cancel(yafel_hip_server S, int64_t id) {
   YH_TRY_BEGIN
   REQUIRE(S, "server_cancel: null server");
+  std::lock_guard<std::mutex> lk(S->mu);
   auto it = S->reqs.find(id);
   REQUIRE(it != S->reqs.end(), "server_cancel: unknown request id");
   Req *r = it->second.get();
@@ -680,6 +786,7 @@ int yafel_hip_server_cancel(yafel_hip_server S, int64_t id) {
 int yafel_hip_server_stats_get(yafel_hip_server S, yafel_hip_server_stats *st) {
   YH_TRY_BEGIN
   REQUIRE(S && st, "server_stats: null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
   *st = S->stats;
   return YAFEL_HIP_OK;
   YH_TRY_END(S ? S->ctx : nullptr)

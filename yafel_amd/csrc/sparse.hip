@@ -328,8 +328,7 @@ void matrix_make_sliced(yafel_hip_matrix_s *A, DBuf &ptr, DBuf &col, DBuf &val64
   const int nval = A->format == YAFEL_HIP_CSR ? 1 : 9;
   const int64_t nsl = (brows + 63) / 64;
   A->nsl = nsl;
-  std::swap(A->ptr.p, ptr.p);
-  std::swap(A->ptr.bytes, ptr.bytes);
+  A->ptr.swap(ptr);
   A->perm.release();
   A->iperm.release();
   A->sptr.alloc((nsl + 1) * 4);
@@ -359,8 +358,7 @@ void matrix_make_sliced(yafel_hip_matrix_s *A, DBuf &ptr, DBuf &col, DBuf &val64
                                                          A->sptr.as<int>());
     HIPCHK(hipGetLastError());
     if (sorted) {
-      std::swap(A->perm.p, perm.p);
-      std::swap(A->perm.bytes, perm.bytes);
+      A->perm.swap(perm);
       A->iperm.alloc(brows * 4);
       k_inverse<<<ceil_div(brows, 256), 256, 0, s>>>(brows, A->perm.as<int>(), A->iperm.as<int>());
       HIPCHK(hipGetLastError());

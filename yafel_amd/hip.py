@@ -65,7 +65,7 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", _I64), ("admitted", _I64), ("completed", _I64), ("failed", _I64),
                 ("cancelled", _I64), ("queued", _I64), ("active", _I64), ("chunks", _I64),
                 ("batched_launches", _I64), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t),
-                ("hbm_budget", C.c_size_t)]
+                ("hbm_budget", C.c_size_t), ("setup_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -109,6 +109,7 @@ for _n, _a in [
     ("yafel_hip_structured_mesh", [_I, _I, _I, _I, _I, _P, _P]),
     ("yafel_hip_normal_fill", [C.c_uint64, _I64, _D, _D, _P]),
     ("yafel_hip_server_create", [_P, C.c_size_t, _I, C.POINTER(_P)]),
+    ("yafel_hip_server_create_ex", [_P, C.c_size_t, _I, _I, C.POINTER(_P)]),
     ("yafel_hip_server_destroy", [_P]),
     ("yafel_hip_server_submit", [_P, C.POINTER(Request), C.POINTER(_I64)]),
     ("yafel_hip_server_progress", [_P, _P, _I, C.POINTER(_I)]),
@@ -432,10 +433,10 @@ REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
 class Server:
     """Device-resident request loop with continuous batching (yafel_hip_server_*)."""
 
-    def __init__(self, ctx, hbm_budget, max_slots=64):
+    def __init__(self, ctx, hbm_budget, max_slots=64, setup_threads=0):
         self.ctx = ctx
         h = _P()
-        check(lib.yafel_hip_server_create(ctx.h, int(hbm_budget), max_slots, C.byref(h)), ctx.h)
+        check(lib.yafel_hip_server_create_ex(ctx.h, int(hbm_budget), max_slots, setup_threads, C.byref(h)), ctx.h)
         self.h = h
         self._ids = (_I64 * 4096)()
 
