@@ -209,6 +209,7 @@ int yafel_hip_mesh_pattern(yafel_hip_mesh m, int format, int precisions, yafel_h
 #define YAFEL_HIP_PHYS_POISSON 0     /* apps/BasicPoisson/main.cpp:25-61 */
 #define YAFEL_HIP_PHYS_ELASTICITY 1  /* apps/BasicLinearElasticity/main.cpp:41-113 */
 #define YAFEL_HIP_PHYS_SUPG 2        /* apps/supg_learning/supg_advection.hpp:19-132 */
+#define YAFEL_HIP_PHYS_DG_ADVECTION 3 /* apps/BasicAdvection_DG/main.cpp:14-98 (velocity U, source f) */
 
 /* The Physics template parameter of CGAssembly as a descriptor (a GPU cannot
  * call a host functor). */
@@ -247,6 +248,36 @@ int yafel_hip_dirichlet_apply(yafel_hip_matrix A, const uint8_t *mask_dev, const
  * (or all components when comp < 0) of every node with |x_axis - value| < tol. */
 int yafel_hip_bc_mask_plane(yafel_hip_mesh m, int axis, double value, double tol, int comp,
                             uint8_t *mask_dev);
+
+/* -------------------------------------------------------------------- DG */
+/* DoFManager(M, DG, p, 1) + Mesh::buildInternalFaces (src/utils/DoFManager.cpp:
+ * 166-293, src/mesh/build_faces.cpp:25-96) for 2-D Quad4 meshes with gmsh
+ * (counter-clockwise) corners and p = 1..4, plus the advection physics of
+ * apps/BasicAdvection_DG/main.cpp:14-98 (phys->kind YAFEL_HIP_PHYS_DG_ADVECTION,
+ * nsd 2, constant velocity phys->U, source phys->f).  Dof d = e * (p+1)^2 + a
+ * belongs to element e alone.  The element mass inverses (the reference's
+ * cached PartialPivLU, DGAssembly.hpp:274-277) are built here, once. */
+typedef struct yafel_hip_dg_s *yafel_hip_dg;
+int yafel_hip_dg_create(yafel_hip_ctx ctx, int p, int64_t n_corner, const double *corner_nodes,
+                        int64_t ncell, const int32_t *cells, const yafel_hip_physics *phys,
+                        yafel_hip_dg *out);
+int yafel_hip_dg_destroy(yafel_hip_dg dg);
+/* info[0] dofs, info[1] faces, info[2] boundary faces, info[3] dofs per element */
+int yafel_hip_dg_info(yafel_hip_dg dg, int64_t *info);
+/* dof node coordinates (dofs * 3 doubles, device) */
+int yafel_hip_dg_nodes(yafel_hip_dg dg, double *nodes_dev);
+/* the face list in the reference's order (host arrays of `faces` entries; -1 = none) */
+int yafel_hip_dg_faces(yafel_hip_dg dg, int32_t *left, int32_t *right, int32_t *left_flocal,
+                       int32_t *right_flocal);
+/* DGAssembly<Physics>(FESystem&, P, {Residual, DtMass}) (include/assembly/
+ * DGAssembly.hpp:32-305): R_dev = R_face + M^-1 (R_face + R_vol), the
+ * reference's result; Rface_dev (may be NULL) receives R_face alone. */
+int yafel_hip_dg_residual(yafel_hip_dg dg, double time, const double *U_dev, double *R_dev,
+                          double *Rface_dev);
+/* nsteps of DGRK4::step (include/time_integration/DGRK4.hpp:33-69) on U_dev in
+ * place; *time advances by nsteps * dt.  A step is captured once into a HIP
+ * graph (per dt and U_dev) and replayed. */
+int yafel_hip_dg_rk4(yafel_hip_dg dg, double dt, int64_t nsteps, double *time, double *U_dev);
 
 /* ---------------------------------------------------------------- serving */
 /* No reference counterpart: a device-resident request loop.  Requests are

@@ -924,6 +924,285 @@ int64_t orc_bicgstab_jacobi_solve(const orc_csr *A, const double *b, double rtol
 
 // splitmix64 (Steele, Lea, Flood 2014) and Box-Muller: the seeded generator
 // shared by the synthetic workloads.
+/* ------------------------------------------------------------------------ */
+/* DG: include/assembly/DGAssembly.hpp:32-305 for Quad4 cells, dpn = 1,      */
+/* with the advection physics of apps/BasicAdvection_DG/main.cpp:14-98.      */
+/* ------------------------------------------------------------------------ */
+}  // extern "C"
+
+namespace {
+
+struct DGFace {
+  int32_t a, b;                 // global corner ids, a < b after orient()
+  int32_t left, right, lf, rf;  // cells and local faces (-1: none)
+};
+
+// Mesh::buildInternalFaces (build_faces.cpp:25-96) with CellFace::orient
+// (CellFace.cpp:74-83) for 2-node faces {f, (f+1) % 4} (CellFace.cpp:18-21).
+std::vector<DGFace> dg_faces(i64 ncell, const int32_t *cells) {
+  std::vector<DGFace> F;
+  F.reserve(4 * ncell);
+  for (i64 c = 0; c < ncell; ++c)
+    for (int f = 0; f < 4; ++f) {
+      DGFace x{cells[4 * c + f], cells[4 * c + (f + 1) % 4], (int32_t)c, -1, f, -1};
+      if (x.b < x.a) {
+        std::swap(x.a, x.b);
+        std::swap(x.left, x.right);
+        std::swap(x.lf, x.rf);
+      }
+      F.push_back(x);
+    }
+  std::sort(F.begin(), F.end(), [](const DGFace &l, const DGFace &r) {
+    return l.a != r.a ? l.a < r.a : l.b < r.b;
+  });
+  for (size_t i = 1; i < F.size(); ++i)
+    if (F[i].a == F[i - 1].a && F[i].b == F[i - 1].b) {
+      DGFace &P = F[i - 1];
+      P.left = std::max(P.left, F[i].left);
+      P.right = std::max(P.right, F[i].right);
+      P.lf = std::max(P.lf, F[i].lf);
+      P.rf = std::max(P.rf, F[i].rf);
+    }
+  auto e = std::unique(F.begin(), F.end(), [](const DGFace &l, const DGFace &r) {
+    return l.a == r.a && l.b == r.b;
+  });
+  F.erase(e, F.end());
+  return F;
+}
+
+struct DGSetup {
+  int p, n1, nen;
+  i64 ncell;
+  ElementTables vol, bnd;
+  std::vector<double> X;             // dof node coordinates, ncell*nen*3
+  std::vector<std::vector<int>> fn;  // face_nodes[f][i]
+  std::vector<DGFace> faces;
+  std::vector<std::vector<int>> fl, fr;  // per face: left / right local node lists
+};
+
+// Element::build_element_faces for TensorProduct topoDim 2
+// (element_boundary.cpp:10-43, 82-93): the element node on face f whose face
+// coordinate equals boundary node i.
+std::vector<std::vector<int>> dg_face_nodes(int p) {
+  const int n1 = p + 1, nen = n1 * n1;
+  std::vector<double> xl(nen * 3), g(n1), gw(n1);
+  local_nodes(2, p, xl.data());
+  gauss_lobatto(n1, g.data(), gw.data());
+  std::vector<std::vector<int>> out(4, std::vector<int>(n1, -1));
+  for (int f = 0; f < 4; ++f)
+    for (int a = 0; a < nen; ++a) {
+      const double x = xl[3 * a], y = xl[3 * a + 1];
+      bool on;
+      double sc;
+      if (f == 0) { on = std::abs(y + 1) < 1e-6; sc = x; }
+      else if (f == 1) { on = std::abs(x - 1) < 1e-6; sc = y; }
+      else if (f == 2) { on = std::abs(y - 1) < 1e-6; sc = -x; }
+      else { on = std::abs(x + 1) < 1e-6; sc = -y; }
+      if (!on) continue;
+      for (int i = 0; i < n1; ++i)
+        if (std::abs(sc - g[i]) < 1e-6) out[f][i] = a;
+    }
+  return out;
+}
+
+DGSetup dg_setup(int p, const double *corners, i64 ncell, const int32_t *cells) {
+  DGSetup S;
+  S.p = p;
+  S.n1 = p + 1;
+  S.nen = S.n1 * S.n1;
+  S.ncell = ncell;
+  S.vol = make_tables(2, p);
+  S.bnd = make_tables(1, p);
+  // make_raw_dofs (DoFManager.cpp:176-229): every cell gets its own GLL nodes
+  std::vector<double> xl(S.nen * 3);
+  local_nodes(2, p, xl.data());
+  S.X.resize(ncell * S.nen * 3);
+  for (i64 c = 0; c < ncell; ++c) {
+    double cc[12];
+    for (int k = 0; k < 4; ++k)
+      for (int d = 0; d < 3; ++d) cc[3 * k + d] = corners[3 * cells[4 * c + k] + d];
+    for (int a = 0; a < S.nen; ++a) interp_corners(2, &xl[3 * a], cc, &S.X[3 * (c * S.nen + a)]);
+  }
+  S.fn = dg_face_nodes(p);
+  S.faces = dg_faces(ncell, cells);
+  // match_face_nodes (DoFManager.cpp:231-293)
+  for (const DGFace &F : S.faces) {
+    std::vector<int> L, R;
+    if (F.left < 0) {
+      for (int nr : S.fn[F.rf]) { L.push_back(-1); R.push_back(nr); }
+    } else if (F.right < 0) {
+      for (int nl : S.fn[F.lf]) { L.push_back(nl); R.push_back(-1); }
+    } else {
+      for (int nl : S.fn[F.lf]) {
+        L.push_back(nl);
+        const double *xl_ = &S.X[3 * ((i64)F.left * S.nen + nl)];
+        for (int nr : S.fn[F.rf]) {
+          const double *xr = &S.X[3 * ((i64)F.right * S.nen + nr)];
+          const double d0 = xl_[0] - xr[0], d1 = xl_[1] - xr[1], d2 = xl_[2] - xr[2];
+          if (std::sqrt(d0 * d0 + d1 * d1 + d2 * d2) < 1.0e-6) {
+            R.push_back(nr);
+            break;
+          }
+        }
+      }
+    }
+    S.fl.push_back(L);
+    S.fr.push_back(R);
+  }
+  return S;
+}
+
+// Element::face_update<2> (Element.hpp:151-199): unit outward normal
+// (J(1,0), -J(0,0)) / |.| and jxw = |.| * w.
+void dg_face_update(const DGSetup &S, i64 el, int fq, const std::vector<int> &lnodes, double n[2],
+                    double &jxw) {
+  double J0 = 0, J1 = 0;
+  for (size_t A = 0; A < lnodes.size(); ++A) {
+    const double *x = &S.X[3 * (el * S.nen + lnodes[A])];
+    const double g = S.bnd.dN[fq * S.n1 + A];
+    J0 += x[0] * g;
+    J1 += x[1] * g;
+  }
+  double nx = J1, ny = -J0;
+  const double det = std::sqrt(nx * nx + ny * ny);
+  n[0] = nx / det;
+  n[1] = ny / det;
+  jxw = det * S.bnd.w[fq];
+}
+
+void dg_residual(const DGSetup &S, const orc_physics *ph, double t, const double *U, double *R,
+                 double *R_face) {
+  (void)t;
+  const int n1 = S.n1, nen = S.nen;
+  const double c0 = ph->U[0], c1 = ph->U[1];
+  const i64 nd = S.ncell * nen;
+  for (i64 i = 0; i < nd; ++i) R[i] = 0;  // GlobalResidual *= 0 (DGAssembly.hpp:43)
+  // face loop (DGAssembly.hpp:106-189)
+  for (size_t fi = 0; fi < S.faces.size(); ++fi) {
+    const DGFace &F = S.faces[fi];
+    if (F.left < 0 || F.right < 0) {
+      const i64 el = F.left >= 0 ? F.left : F.right;
+      const std::vector<int> &fnodes = F.left >= 0 ? S.fl[fi] : S.fr[fi];
+      for (int fq = 0; fq < n1; ++fq) {
+        double n[2], jxw;
+        dg_face_update(S, el, fq, fnodes, n, jxw);
+        double u = 0;
+        for (int i = 0; i < n1; ++i) u += U[el * nen + fnodes[i]] * S.bnd.N[fq * n1 + i];
+        // AdvectionPhysics::BoundaryFlux: outflow only
+        const double vn = n[0] * c0 + n[1] * c1;
+        const double flux = vn > 0 ? vn * u : 0 * vn * u;
+        for (int A = 0; A < n1; ++A) R[el * nen + fnodes[A]] -= S.bnd.N[fq * n1 + A] * flux * jxw;
+      }
+    } else {
+      const std::vector<int> &L = S.fl[fi], &Rn = S.fr[fi];
+      for (int fq = 0; fq < n1; ++fq) {
+        double nl[2], nr[2], jl, jr;
+        dg_face_update(S, F.left, fq, L, nl, jl);
+        dg_face_update(S, F.right, fq, Rn, nr, jr);
+        double ul = 0, ur = 0;
+        for (int i = 0; i < n1; ++i) {
+          const double sv = S.bnd.N[fq * n1 + i];
+          ul += U[(i64)F.left * nen + L[i]] * sv;
+          ur += U[(i64)F.right * nen + Rn[i]] * sv;
+        }
+        // AdvectionPhysics::Flux (upwind)
+        const double vn = nl[0] * c0 + nl[1] * c1;
+        const double flux = 0.5 * (vn * (ul + ur) + std::abs(vn) * (ul - ur));
+        for (int i = 0; i < n1; ++i) {
+          const double sv = S.bnd.N[fq * n1 + i];
+          R[(i64)F.left * nen + L[i]] -= sv * flux * jl;
+          R[(i64)F.right * nen + Rn[i]] += sv * flux * jr;
+        }
+      }
+    }
+  }
+  if (R_face)
+    for (i64 i = 0; i < nd; ++i) R_face[i] = R[i];
+  // element loop (DGAssembly.hpp:194-299): local = M^-1 (R_face + R_vol), R += local
+  std::vector<double> loc(nen), M(nen * nen), grad(nen * 2);
+  for (i64 e = 0; e < S.ncell; ++e) {
+    for (int a = 0; a < nen; ++a) loc[a] = R[e * nen + a];
+    std::fill(M.begin(), M.end(), 0.0);
+    for (int q = 0; q < S.vol.nqp; ++q) {
+      double J[4] = {0, 0, 0, 0};
+      for (int A = 0; A < nen; ++A) {
+        const double *x = &S.X[3 * (e * nen + A)];
+        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < 2; ++j) J[2 * i + j] += x[i] * S.vol.dN[(q * nen + A) * 2 + j];
+      }
+      double det, Ji[4];
+      jac_inv(2, J, det, Ji);
+      const double jxw = det * S.vol.w[q];
+      for (int A = 0; A < nen; ++A) {
+        const double g0 = S.vol.dN[(q * nen + A) * 2], g1 = S.vol.dN[(q * nen + A) * 2 + 1];
+        grad[2 * A] = Ji[0] * g0 + Ji[2] * g1;      // (J^-T g)_0
+        grad[2 * A + 1] = Ji[1] * g0 + Ji[3] * g1;  // (J^-T g)_1
+      }
+      // LocalResidual (main.cpp:53-71)
+      double u = 0;
+      for (int A = 0; A < nen; ++A) u += U[e * nen + A] * S.vol.N[q * nen + A];
+      for (int i = 0; i < nen; ++i) {
+        const double val = u * (grad[2 * i] * c0 + grad[2 * i + 1] * c1) + ph->f * S.vol.N[q * nen + i];
+        loc[i] += val * jxw;
+      }
+      // LocalMass (main.cpp:73-78)
+      for (int i = 0; i < nen; ++i)
+        for (int j = 0; j < nen; ++j) M[i * nen + j] += (jxw * S.vol.N[q * nen + i]) * S.vol.N[q * nen + j];
+    }
+    lu_solve(nen, M, 1, loc);
+    for (int a = 0; a < nen; ++a) R[e * nen + a] += loc[a];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t orc_dg_faces(int64_t ncell, const int32_t *cells, int32_t *left, int32_t *right,
+                     int32_t *left_flocal, int32_t *right_flocal) {
+  auto F = dg_faces(ncell, cells);
+  for (size_t i = 0; i < F.size(); ++i) {
+    left[i] = F[i].left;
+    right[i] = F[i].right;
+    left_flocal[i] = F[i].lf;
+    right_flocal[i] = F[i].rf;
+  }
+  return (int64_t)F.size();
+}
+
+void orc_dg_nodes(int p, const double *corner_nodes, int64_t ncell, const int32_t *cells,
+                  double *dof_nodes) {
+  DGSetup S = dg_setup(p, corner_nodes, ncell, cells);
+  std::memcpy(dof_nodes, S.X.data(), S.X.size() * sizeof(double));
+}
+
+void orc_dg_residual(int p, const double *corner_nodes, int64_t ncell, const int32_t *cells,
+                     const orc_physics *phys, double t, const double *U, double *R, double *R_face) {
+  DGSetup S = dg_setup(p, corner_nodes, ncell, cells);
+  dg_residual(S, phys, t, U, R, R_face);
+}
+
+void orc_dg_rk4(int p, const double *corner_nodes, int64_t ncell, const int32_t *cells,
+                const orc_physics *phys, double dt, int64_t nsteps, double t0, double *U) {
+  DGSetup S = dg_setup(p, corner_nodes, ncell, cells);
+  const i64 nd = ncell * S.nen;
+  std::vector<double> U0(nd), k1(nd), k2(nd), k3(nd), k4(nd), W(nd);
+  double t = t0;
+  for (i64 s = 0; s < nsteps; ++s) {
+    for (i64 i = 0; i < nd; ++i) U0[i] = U[i];
+    dg_residual(S, phys, t, U0.data(), k1.data(), nullptr);
+    for (i64 i = 0; i < nd; ++i) W[i] = U0[i] + (dt / 2) * k1[i];
+    dg_residual(S, phys, t + dt / 2, W.data(), k2.data(), nullptr);
+    for (i64 i = 0; i < nd; ++i) W[i] = U0[i] + (dt / 2) * k2[i];
+    dg_residual(S, phys, t + dt / 2, W.data(), k3.data(), nullptr);
+    for (i64 i = 0; i < nd; ++i) W[i] = U0[i] + dt * k3[i];
+    dg_residual(S, phys, t + dt, W.data(), k4.data(), nullptr);
+    for (i64 i = 0; i < nd; ++i)
+      U[i] = U0[i] + (dt / 6) * k1[i] + (dt / 3) * k2[i] + (dt / 3) * k3[i] + (dt / 6) * k4[i];
+    t += dt;
+  }
+}
+
 uint64_t orc_splitmix64(uint64_t *state) {
   uint64_t z = (*state += 0x9E3779B97F4A7C15ull);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;

@@ -106,6 +106,25 @@ int64_t orc_bicgstab_solve(const orc_csr *A, const double *b, const double *x0, 
 int64_t orc_bicgstab_jacobi_solve(const orc_csr *A, const double *b, double rtol, int64_t max_iter,
                                   double *x);
 
+/* ---- DG (include/assembly/DGAssembly.hpp:32-305): Quad4 cells, dpn = 1 ---- */
+/* Mesh::buildInternalFaces (src/mesh/build_faces.cpp:25-96) for Quad4 cells;
+ * returns the face count, the arrays hold up to 4*ncell entries */
+int64_t orc_dg_faces(int64_t ncell, const int32_t *cells, int32_t *left, int32_t *right,
+                     int32_t *left_flocal, int32_t *right_flocal);
+/* DG numbering (DoFManager::make_raw_dofs, DoFManager.cpp:176-229): element e
+ * owns dofs e*nen .. e*nen+nen-1, nen = (p+1)^2; dof_nodes is ncell*nen*3 */
+void orc_dg_nodes(int p, const double *corner_nodes, int64_t ncell, const int32_t *cells,
+                  double *dof_nodes);
+/* DGAssembly<AdvectionPhysics> with {Residual, DtMass}
+ * (apps/BasicAdvection_DG/main.cpp:14-98): velocity phys->U, source phys->f.
+ * R (ncell*nen) is overwritten; R_face, when not NULL, receives the residual
+ * after the face loop alone. */
+void orc_dg_residual(int p, const double *corner_nodes, int64_t ncell, const int32_t *cells,
+                     const orc_physics *phys, double t, const double *U, double *R, double *R_face);
+/* nsteps of DGRK4::step (include/time_integration/DGRK4.hpp:33-69); U in place */
+void orc_dg_rk4(int p, const double *corner_nodes, int64_t ncell, const int32_t *cells,
+                const orc_physics *phys, double dt, int64_t nsteps, double t0, double *U);
+
 /* ---- seeded synthetic inputs ------------------------------------------- */
 uint64_t orc_splitmix64(uint64_t *state);
 void orc_normal_fill(uint64_t seed, int64_t n, double mean, double sd, double *out);

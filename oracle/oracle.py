@@ -80,6 +80,11 @@ class Oracle:
         L.orc_gauss_legendre_1d.argtypes = [C.c_int, _P, _P]
         L.orc_gauss_lobatto_1d.argtypes = [C.c_int, _P, _P]
         L.orc_normal_fill.argtypes = [C.c_uint64, _I64, _D, _D, _P]
+        L.orc_dg_faces.restype = _I64
+        L.orc_dg_faces.argtypes = [_I64, _P, _P, _P, _P, _P]
+        L.orc_dg_nodes.argtypes = [C.c_int, _P, _I64, _P, _P]
+        L.orc_dg_residual.argtypes = [C.c_int, _P, _I64, _P, _P, _D, _P, _P, _P]
+        L.orc_dg_rk4.argtypes = [C.c_int, _P, _I64, _P, _P, _D, _I64, _D, _P]
 
     # -- quadrature / shape functions -------------------------------------
     def gauss_legendre(self, n):
@@ -178,6 +183,37 @@ class Oracle:
         out = np.zeros(n)
         self.lib.orc_normal_fill(seed, n, mean, sd, _ptr(out))
         return out
+
+    # -- DG (Quad4 cells, dpn = 1) -------------------------------------------
+    def dg_faces(self, cells):
+        cells = np.ascontiguousarray(cells, np.int32)
+        n = 4 * cells.shape[0]
+        l, r, lf, rf = (np.zeros(n, np.int32) for _ in range(4))
+        nf = self.lib.orc_dg_faces(cells.shape[0], _ptr(cells), _ptr(l), _ptr(r), _ptr(lf), _ptr(rf))
+        return l[:nf].copy(), r[:nf].copy(), lf[:nf].copy(), rf[:nf].copy()
+
+    def dg_nodes(self, p, nodes, cells):
+        cells = np.ascontiguousarray(cells, np.int32)
+        X = np.zeros((cells.shape[0] * (p + 1) ** 2, 3))
+        self.lib.orc_dg_nodes(p, _ptr(np.ascontiguousarray(nodes, np.float64)), cells.shape[0], _ptr(cells),
+                              _ptr(X))
+        return X
+
+    def dg_residual(self, p, nodes, cells, ph, U, t=0.0, face=False):
+        cells = np.ascontiguousarray(cells, np.int32)
+        U = np.ascontiguousarray(U, np.float64)
+        R = np.zeros_like(U)
+        Rf = np.zeros_like(U) if face else None
+        self.lib.orc_dg_residual(p, _ptr(np.ascontiguousarray(nodes, np.float64)), cells.shape[0], _ptr(cells),
+                                 C.byref(ph), t, _ptr(U), _ptr(R), _ptr(Rf))
+        return (R, Rf) if face else R
+
+    def dg_rk4(self, p, nodes, cells, ph, U, dt, nsteps, t0=0.0):
+        cells = np.ascontiguousarray(cells, np.int32)
+        U = np.array(U, dtype=np.float64)
+        self.lib.orc_dg_rk4(p, _ptr(np.ascontiguousarray(nodes, np.float64)), cells.shape[0], _ptr(cells),
+                            C.byref(ph), dt, nsteps, t0, _ptr(U))
+        return U
 
 
 class OMatrix:

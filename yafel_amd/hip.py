@@ -23,7 +23,7 @@ CSR, BCSR3 = 1, 3
 CG, PCG_JACOBI, BICGSTAB, BICGSTAB_JACOBI = 0, 1, 2, 3
 STOP_REFERENCE, STOP_RELRES = 0, 1
 STATUS = {0: "converged", 1: "maxiter", 2: "breakdown", 3: "nan"}
-POISSON, ELASTICITY, SUPG = 0, 1, 2
+POISSON, ELASTICITY, SUPG, DG_ADVECTION = 0, 1, 2, 3
 TANGENT, RESIDUAL = 1, 2
 
 _P, _I64, _D, _I = C.c_void_p, C.c_int64, C.c_double, C.c_int
@@ -108,6 +108,11 @@ for _n, _a in [
     ("yafel_hip_bc_mask_plane", [_P, _I, _D, _D, _I, _P]),
     ("yafel_hip_structured_mesh", [_I, _I, _I, _I, _I, _P, _P]),
     ("yafel_hip_normal_fill", [C.c_uint64, _I64, _D, _D, _P]),
+    ("yafel_hip_dg_create", [_P, _I, _I64, _P, _I64, _P, C.POINTER(Physics), C.POINTER(_P)]),
+    ("yafel_hip_dg_destroy", [_P]), ("yafel_hip_dg_info", [_P, _P]), ("yafel_hip_dg_nodes", [_P, _P]),
+    ("yafel_hip_dg_faces", [_P, _P, _P, _P, _P]),
+    ("yafel_hip_dg_residual", [_P, _D, _P, _P, _P]),
+    ("yafel_hip_dg_rk4", [_P, _D, _I64, C.POINTER(_D), _P]),
     ("yafel_hip_server_create", [_P, C.c_size_t, _I, C.POINTER(_P)]),
     ("yafel_hip_server_create_ex", [_P, C.c_size_t, _I, _I, C.POINTER(_P)]),
     ("yafel_hip_server_destroy", [_P]),
@@ -368,6 +373,52 @@ def physics(kind, nsd, coef=None, coef_dev=None, f=0.0, lam=0.0, mu=0.0, body=(0
     ph.U[:] = list(U)
     ph.kdiff, ph.h, ph.alpha, ph.Q = kdiff, h, alpha, Q
     return ph
+
+
+class DG:
+    """DoFManager(M, DG, p, 1) + DGAssembly / DGRK4 on the device (2-D Quad4)."""
+
+    def __init__(self, ctx, p, corner_nodes, cells, ph):
+        self.ctx, self.p = ctx, p
+        h = _P()
+        corner_nodes = np.ascontiguousarray(corner_nodes, np.float64)
+        cells = np.ascontiguousarray(cells, np.int32)
+        check(lib.yafel_hip_dg_create(ctx.h, p, corner_nodes.shape[0], _ptr(corner_nodes), cells.shape[0],
+                                      _ptr(cells), C.byref(ph), C.byref(h)), ctx.h)
+        self.h = h
+        info = np.zeros(4, np.int64)
+        check(lib.yafel_hip_dg_info(h, _ptr(info)), ctx.h)
+        self.ndofs, self.nfaces, self.nboundary, self.nen = (int(v) for v in info)
+
+    def close(self):
+        if self.h:
+            lib.yafel_hip_dg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def nodes(self):
+        d = self.ctx.alloc(self.ndofs * 3)
+        check(lib.yafel_hip_dg_nodes(self.h, d.ptr), self.ctx.h)
+        return d.download().reshape(-1, 3)
+
+    def faces(self):
+        out = [np.zeros(self.nfaces, np.int32) for _ in range(4)]
+        check(lib.yafel_hip_dg_faces(self.h, *[_ptr(a) for a in out]), self.ctx.h)
+        return tuple(out)
+
+    def residual(self, U_dev, R_dev, Rface_dev=None, time=0.0):
+        check(lib.yafel_hip_dg_residual(self.h, time, U_dev.ptr, R_dev.ptr,
+                                        None if Rface_dev is None else Rface_dev.ptr), self.ctx.h)
+
+    def rk4(self, U_dev, dt, nsteps, time=0.0):
+        t = C.c_double(time)
+        check(lib.yafel_hip_dg_rk4(self.h, dt, nsteps, C.byref(t), U_dev.ptr), self.ctx.h)
+        return t.value
 
 
 class Mesh:
