@@ -83,6 +83,24 @@ def cpu_baseline(w, fp64_cg_iters, hip):
     }
 
 
+KERNEL_A = "k_spmv_fused<3, 4, float, OpCgA<float>>"
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "c2_cg_kernel_a_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprof counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on the copy kernel), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def run_stream(hip, ctx, mats, arrivals, budget, slots, setup_threads=0):
     """Feed requests at their arrival times into a server; returns per-request
     latency (arrival -> finished), makespan and the results."""
@@ -184,6 +202,81 @@ def serve_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def c3_bench(args):
+    """c3: 2-D Poisson Q4 256^2 with lognormal coefficients -- the high-order
+    element kernel (FP64 scalar vs FP32 matrix cores), assembly, and the solve."""
+    import ctypes as C
+    from yafel_amd import hip, workloads
+    ctx = hip.Context(0)
+    w = workloads.c3(ctx)
+    nel, nen = w.mesh.nel, w.mesh.nen
+    cd = ctx.upload(w.coef)
+    ph_dev = hip.physics(hip.POISSON, 2, coef_dev=cd)
+    K64 = ctx.alloc(nel * nen * nen)
+    K32 = ctx.alloc(nel * nen * nen, np.float32)
+    lib = hip.lib
+
+    def timed(fn, reps):
+        fn()
+        ctx.sync()
+        ctx.timer_begin()
+        for _ in range(reps):
+            fn()
+        return ctx.timer_end() / reps
+
+    t64 = timed(lambda: hip.check(lib.yafel_hip_element_matrices(w.mesh.h, C.byref(ph_dev), K64.ptr), ctx.h), 10)
+    t32 = timed(lambda: w.mesh.element_matrices_fp32(ph_dev, K32), 20)
+    issued = nel * (2 * nen // 2) * 32 * 32 * 2 * 2      # 32x32x2 steps, nen padded to 32
+    useful = nel * nen * nen * 2 * nen * 2              # (2 nqp) x nen x nen, nqp = nen
+    a64 = timed(lambda: w.assemble(), 5)
+    w.extra_req = hip.FP32_ELEMENTS
+    a32 = timed(lambda: w.assemble(), 5)
+    w.extra_req = 0
+    w.assemble()
+    res = {}
+    for name, prec in (("fp64_pcg", hip.FP64), ("mixed_pcg", hip.MIXED)):
+        info = w.solve(method=hip.PCG_JACOBI, precision=prec, stop=hip.STOP_RELRES, tol=1e-8)
+        info = w.solve(method=hip.PCG_JACOBI, precision=prec, stop=hip.STOP_RELRES, tol=1e-8)
+        res[name] = {"ms": info.ms, "iterations": info.iterations, "rel_residual": info.rel_residual,
+                     "status": info.status}
+    out = {
+        "workload": "c3: 2-D Poisson Q4 (GLL) 256^2, 5x5 Gauss, k_e = exp(N(0, 0.5^2)), rhs N(0,1), "
+                    "homogeneous Dirichlet on all edges",
+        "element_kernel_fp64_us": t64 * 1e3, "element_kernel_fp32_mfma_us": t32 * 1e3,
+        "mfma_tflops_issued": issued / (t32 * 1e-3) / 1e12, "mfma_tflops_useful": useful / (t32 * 1e-3) / 1e12,
+        "mfma_peak_fp32_tflops": 157.3, "mfma_frac_issued": issued / (t32 * 1e-3) / 157.3e12,
+        "fp32_staging_write_gbs": nel * nen * nen * 4 / (t32 * 1e-3) / 1e9,
+        "assembly_fp64_ms": a64, "assembly_fp32_mfma_ms": a32, "solves": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def spmv_bench(args):
+    """The c2 BCSR3 SpMV alone (FP32 and FP64), plus one copy-kernel pass of
+    known size: the command the rocprof counter runs profile, the copy kernel
+    being the calibration of their byte counters."""
+    from yafel_amd import hip, workloads
+    ctx = hip.Context(0)
+    w = workloads.c2(ctx, n=args.n)
+    w.assemble()
+    A = w.A
+    A.sync_fp32()
+    n, inf = w.n, A.info()
+    out = {"workload": f"c2 BCSR3 SpMV, {args.n}^3 Q1 elasticity", "blocks": inf.nblocks}
+    for name, prec, dt, s in (("fp32", hip.FP32, np.float32, 4), ("fp64", hip.FP64, np.float64, 8)):
+        x = ctx.upload(np.random.default_rng(0).standard_normal(n).astype(dt))
+        y = ctx.alloc(n, dt)
+        A.spmv(x, y, prec)
+        ctx.timer_begin()
+        for _ in range(args.steps):
+            A.spmv(x, y, prec)
+        ms = ctx.timer_end() / args.steps
+        byts = inf.nblocks * (9 * s + 4) + (n // 3 + 1) * 4 + 2 * n * s
+        out[name] = {"us": ms * 1e3, "gbs": byts / (ms * 1e-3) / 1e9, "algorithmic_bytes": byts}
+    out["copy_gbs"] = ctx.copy_bandwidth(1 << 30, 1)  # two launches of a 1 GiB copy
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,7 +285,7 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="cells per axis (64 = the benchmark config)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "serve"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "serve", "c3", "spmv"])
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--sweep-requests", type=int, default=300)
     ap.add_argument("--rates", default="50,200,500,1000,2000")
@@ -206,6 +299,12 @@ def main():
         sys.exit(2)
     if args.workload == "serve":
         serve_bench(args)
+        return
+    if args.workload == "c3":
+        c3_bench(args)
+        return
+    if args.workload == "spmv":
+        spmv_bench(args)
         return
 
     from yafel_amd import hip, workloads
@@ -248,11 +347,20 @@ def main():
             log(f"fp64 {name}: {fp64[name]}")
         ctx.sync()
 
-    # roofline of the dominant kernel: FP32 BCSR SpMV of this matrix (the
-    # product inside every inner iteration), HIP events on the context stream
+    # roofline of the dominant kernel: the CG iteration's fused FP32 SpMV
+    # (kernel A: p = z + beta p rebuilt in the gather, q = A p, p.q), every
+    # launch of one more step timed with HIP events on the context stream
     A = w.A
     inf = A.info()
     nbr = inf.rows // 3
+    ctx.kernel_timing(True)
+    w.step(**mixed)
+    ka_ms, ka_n = ctx.kernel_time()
+    ctx.kernel_timing(False)
+    ka_us = ka_ms / max(ka_n, 1) * 1e3
+    ka_bytes = bcsr_bytes(inf.nblocks, nbr, w.n, 4) + 2 * w.n * 4  # + z gather and p write
+    ka_gbs = ka_bytes / (ka_us * 1e-6) / 1e9
+    log(f"kernel A: {ka_n} launches, {ka_us:.1f} us each, {ka_gbs:.0f} GB/s")
     x32 = ctx.upload(np.random.default_rng(0).standard_normal(w.n).astype(np.float32))
     y32 = ctx.alloc(w.n, np.float32)
     A.sync_fp32()
@@ -298,11 +406,11 @@ def main():
         "rel_residual": infos[-1].rel_residual, "setup_s": setup_s,
         "fp64_gpu": fp64,
         "speedup_vs_fp64_cg": (value / fp64["cg"]["solves_per_s"]) if "cg" in fp64 else None,
-        "spmv_fp64_gbs": gbs64, "copy_gbs": copy_gbs,
-        "roofline": {"bound": "hbm", "achieved": gbs32, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": gbs32 / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "k_bcsr_spmv<32,float>", "bytes_per_launch": bytes32,
-                     "us_per_launch": ms32 * 1e3},
+        "spmv_fp32_gbs": gbs32, "spmv_fp32_us": ms32 * 1e3, "spmv_fp64_gbs": gbs64, "copy_gbs": copy_gbs,
+        "roofline": {"bound": "hbm", "achieved": ka_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": ka_gbs / PEAK_HBM_GBS, "traffic": measured_traffic(KERNEL_A),
+                     "kernel": KERNEL_A, "bytes_per_launch": ka_bytes, "us_per_launch": ka_us,
+                     "launches_timed": ka_n},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -1,10 +1,13 @@
 // YAFEL's FE layer over the HIP C ABI: Mesh, DoFManager, FESystem,
-// CGAssembly<Physics> and DirichletBC with the reference's names.
+// CGAssembly<Physics>, DGAssembly<Physics>, DGRK4 and DirichletBC with the
+// reference's names.
 //
-//   DoFManager(M, CG, p, dof_per_node)   src/utils/DoFManager.cpp:11-36 (numbering on the device)
-//   FESystem(dofm)                       include/fe_system/FESystem.hpp:18-56
-//   CGAssembly<Physics>(fesys, reqs)     include/assembly/CGAssembly.hpp:27-186
-//   DirichletBC(dofm, value, comp)       include/boundary_conditions/DirichletBC.hpp:20-89
+//   DoFManager(M, CG|DG, p, dof_per_node) src/utils/DoFManager.cpp:11-36 (numbering on the device)
+//   FESystem(dofm)                        include/fe_system/FESystem.hpp:18-56
+//   CGAssembly<Physics>(fesys, reqs)      include/assembly/CGAssembly.hpp:27-186
+//   DGAssembly<Physics>(fesys, P, reqs)   include/assembly/DGAssembly.hpp:32-305
+//   DGRK4(dt).step(fesys, P)              include/time_integration/DGRK4.hpp:33-69
+//   DirichletBC(dofm, value, comp)        include/boundary_conditions/DirichletBC.hpp:20-89
 //
 // A GPU cannot call a host Physics functor, so a Physics type provides
 //     static yafel_hip_physics device_descriptor();
@@ -59,9 +62,34 @@ class DoFManager {
   enum class ManagerType { CG, DG };
 
   DoFManager(const Mesh &M, ManagerType type, int polyOrder, int dof_per_node)
-      : dof_per_node(dof_per_node), polyOrder(polyOrder) {
-    if (type != ManagerType::CG) throw std::invalid_argument("DoFManager: only CG numbering runs on the device");
+      : dof_per_node(dof_per_node), polyOrder(polyOrder), type_(type) {
     auto ctx = hipdetail::context();
+    if (type == ManagerType::DG) {
+      // make_dg_dofs (DoFManager.cpp:166-174): element-private nodes, matched faces
+      if (dof_per_node != 1 || M.dim() != 2)
+        throw std::invalid_argument("DoFManager: the device DG numbering covers 2-D meshes with 1 dof per node");
+      yafel_hip_dg g = nullptr;
+      hipdetail::check(yafel_hip_dg_create(ctx, polyOrder, (int64_t)M.nNodes(), M.nodes().data(),
+                                           (int64_t)M.nCells(), M.cells().data(), &g),
+                       ctx);
+      dg_ = std::shared_ptr<yafel_hip_dg_s>(g, [](yafel_hip_dg_s *p) { yafel_hip_dg_destroy(p); });
+      int64_t info[4];
+      hipdetail::check(yafel_hip_dg_info(g, info));
+      nen_ = (int)info[3];
+      dim_ = 2;
+      void *xd = nullptr;
+      hipdetail::check(yafel_hip_malloc(ctx, sizeof(double) * 3 * (size_t)info[0], &xd), ctx);
+      std::vector<double> xyz((std::size_t)info[0] * 3);
+      int rc = yafel_hip_dg_nodes(g, (double *)xd);
+      if (rc == YAFEL_HIP_OK) rc = yafel_hip_memcpy_d2h(ctx, xyz.data(), xd, sizeof(double) * xyz.size());
+      yafel_hip_free(ctx, xd);
+      hipdetail::check(rc, ctx);
+      const std::size_t ncell = (std::size_t)info[0] / nen_;
+      elements.resize((std::size_t)info[0]);
+      for (std::size_t i = 0; i < elements.size(); ++i) elements[i] = (int)i;
+      set_nodes(xyz, ncell);
+      return;
+    }
     yafel_hip_mesh m = nullptr;
     hipdetail::check(yafel_hip_mesh_create_cg(ctx, M.dim(), polyOrder, dof_per_node, (int64_t)M.nNodes(),
                                               M.nodes().data(), (int64_t)M.nCells(), M.cells().data(), &m),
@@ -74,15 +102,12 @@ class DoFManager {
     elements.resize((std::size_t)info[1] * nen_);
     std::vector<double> xyz((std::size_t)info[0] * 3);
     hipdetail::check(yafel_hip_mesh_download(m, elements.data(), xyz.data()));
-    dof_nodes.resize((std::size_t)info[0]);
-    for (std::size_t i = 0; i < dof_nodes.size(); ++i)
-      for (int d = 0; d < 3; ++d) dof_nodes[i].x[d] = xyz[3 * i + d];
-    element_offsets.resize((std::size_t)info[1] + 1);
-    for (std::size_t e = 0; e < element_offsets.size(); ++e) element_offsets[e] = (int)(e * nen_);
+    set_nodes(xyz, (std::size_t)info[1]);
   }
 
   int nCells() const { return (int)(element_offsets.size() - 1); }
   int nsd() const { return dim_; }
+  ManagerType managerType() const { return type_; }
   // DoFManager::getGlobalDofs (DoFManager.cpp:39-52): dof = node * dpn + comp
   void getGlobalDofs(int elnum, std::vector<int> &c) const {
     c.clear();
@@ -93,6 +118,7 @@ class DoFManager {
     c.assign(elements.begin() + element_offsets[elnum], elements.begin() + element_offsets[elnum + 1]);
   }
   yafel_hip_mesh handle() const { return m_.get(); }
+  yafel_hip_dg dg_handle() const { return dg_.get(); }
 
   int dof_per_node;
   int polyOrder;
@@ -101,7 +127,16 @@ class DoFManager {
   std::vector<coordinate> dof_nodes;
 
  private:
+  void set_nodes(const std::vector<double> &xyz, std::size_t ncell) {
+    dof_nodes.resize(xyz.size() / 3);
+    for (std::size_t i = 0; i < dof_nodes.size(); ++i)
+      for (int d = 0; d < 3; ++d) dof_nodes[i].x[d] = xyz[3 * i + d];
+    element_offsets.resize(ncell + 1);
+    for (std::size_t e = 0; e < element_offsets.size(); ++e) element_offsets[e] = (int)(e * nen_);
+  }
+  ManagerType type_;
   std::shared_ptr<yafel_hip_mesh_s> m_;
+  std::shared_ptr<yafel_hip_dg_s> dg_;
   int nen_ = 0, dim_ = 0;
 };
 
@@ -113,9 +148,12 @@ class FESystem {
  public:
   explicit FESystem(DoFManager &dofm, int format = YAFEL_HIP_CSR) : dofm_(dofm) {
     auto ctx = hipdetail::context();
-    yafel_hip_matrix A = nullptr;
-    hipdetail::check(yafel_hip_mesh_pattern(dofm.handle(), format, YAFEL_HIP_FP64, &A), ctx);
-    A_ = hipdetail::MatrixPtr(A, hipdetail::MatrixDeleter());
+    if (dofm.managerType() == DoFManager::ManagerType::CG) {
+      // DG systems are explicit: no tangent
+      yafel_hip_matrix A = nullptr;
+      hipdetail::check(yafel_hip_mesh_pattern(dofm.handle(), format, YAFEL_HIP_FP64, &A), ctx);
+      A_ = hipdetail::MatrixPtr(A, hipdetail::MatrixDeleter());
+    }
     n_ = dofm.dof_nodes.size() * dofm.dof_per_node;
     void *r = nullptr;
     hipdetail::check(yafel_hip_malloc(ctx, sizeof(double) * n_, &r), ctx);
@@ -188,6 +226,64 @@ void CGAssembly(FESystem &fes, std::vector<AssemblyRequirement> requirements = {
                                       fes.residual_device()),
                    hipdetail::context());
 }
+
+// DGAssembly<Physics> (include/assembly/DGAssembly.hpp:32-305) with
+// {Residual, DtMass}: the residual of the host solution into the device
+// residual (overwritten, as GlobalResidual *= 0 does); the element mass
+// inverses live with the DoFManager's device state, built once.
+template <typename Physics>
+void DGAssembly(FESystem &fes, Physics &, std::vector<AssemblyRequirement> requirements =
+                                               {AssemblyRequirement::Residual}) {
+  for (auto r : requirements)
+    if (r == AssemblyRequirement::Tangent || r == AssemblyRequirement::DtDtMass)
+      throw std::invalid_argument("DGAssembly: the device path assembles the explicit residual");
+  auto ctx = hipdetail::context();
+  const yafel_hip_physics ph = Physics::device_descriptor();
+  const std::size_t n = fes.ndofs();
+  void *u = nullptr;
+  hipdetail::check(yafel_hip_malloc(ctx, sizeof(double) * n, &u), ctx);
+  int rc = yafel_hip_memcpy_h2d(ctx, u, fes.getSolution().data(), sizeof(double) * n);
+  if (rc == YAFEL_HIP_OK)
+    rc = yafel_hip_dg_residual(fes.getDoFManager().dg_handle(), &ph, fes.currentTime(), (const double *)u,
+                               fes.residual_device(), nullptr);
+  if (rc == YAFEL_HIP_OK) rc = yafel_hip_ctx_sync(ctx);
+  yafel_hip_free(ctx, u);
+  hipdetail::check(rc, ctx);
+}
+
+// DGRK4 (include/time_integration/DGRK4.hpp:22-69): classic RK4 on
+// dU/dt = DGAssembly residual.  step() advances the host solution by dt;
+// run() keeps the solution on the device for nsteps replays of the captured
+// step graph.
+class DGRK4 {
+ public:
+  DGRK4() = delete;
+  explicit DGRK4(double dt) : dt(dt) {}
+
+  template <typename Physics>
+  void step(FESystem &fes, Physics &P) {
+    run(fes, P, 1);
+  }
+
+  template <typename Physics>
+  void run(FESystem &fes, Physics &, int64_t nsteps) {
+    auto ctx = hipdetail::context();
+    const yafel_hip_physics ph = Physics::device_descriptor();
+    const std::size_t n = fes.ndofs();
+    void *u = nullptr;
+    hipdetail::check(yafel_hip_malloc(ctx, sizeof(double) * n, &u), ctx);
+    double t = fes.currentTime();
+    int rc = yafel_hip_memcpy_h2d(ctx, u, fes.getSolution().data(), sizeof(double) * n);
+    if (rc == YAFEL_HIP_OK) rc = yafel_hip_dg_rk4(fes.getDoFManager().dg_handle(), &ph, dt, nsteps, &t, (double *)u);
+    if (rc == YAFEL_HIP_OK) rc = yafel_hip_memcpy_d2h(ctx, fes.getSolution().data(), u, sizeof(double) * n);
+    yafel_hip_free(ctx, u);
+    hipdetail::check(rc, ctx);
+    fes.currentTime() = t;
+  }
+
+ private:
+  double dt;
+};
 
 // DirichletBC (DirichletBC.hpp:20-89, DirichletBC.cpp:27-56)
 class DirichletBC {

@@ -134,6 +134,13 @@ int yafel_hip_dot(yafel_hip_ctx ctx, int64_t n, const double *a_dev, const doubl
  * (read + write bytes / time) over `bytes` per buffer, `reps` launches. */
 int yafel_hip_copy_bandwidth(yafel_hip_ctx ctx, size_t bytes, int reps, double *gbs);
 
+/* No reference counterpart: HIP-event timing of every launch of the CG
+ * iteration's fused SpMV kernel (kernel A) on ctx, for roofline figures.
+ * Enabling (or disabling) resets the record; kernel_time returns the summed
+ * duration and the launch count (stream-synchronising). */
+int yafel_hip_kernel_timing(yafel_hip_ctx ctx, int enable);
+int yafel_hip_kernel_time(yafel_hip_ctx ctx, double *ms_total, int64_t *launches);
+
 /* ---------------------------------------------------------------- solvers */
 #define YAFEL_HIP_CG 0              /* cg_solve.hpp:33-81 */
 #define YAFEL_HIP_PCG_JACOBI 1      /* pcg_solve.hpp:126-178 + JacobiPreconditioner.hpp */
@@ -227,6 +234,10 @@ typedef struct {
 
 #define YAFEL_HIP_ASSEMBLE_TANGENT 1  /* AssemblyRequirement::Tangent */
 #define YAFEL_HIP_ASSEMBLE_RESIDUAL 2 /* AssemblyRequirement::Residual */
+/* No reference counterpart: element tangents computed in FP32 on the matrix
+ * cores (G^T G as 32x32x2 FP32 MFMA steps) and gathered in FP64 -- values
+ * carry FP32 rounding (about 1e-7 relative).  2-D Poisson with p = 3 or 4. */
+#define YAFEL_HIP_ASSEMBLE_FP32_ELEMENTS 4
 
 /* CGAssembly<Physics>(FESystem&, reqs) (include/assembly/CGAssembly.hpp:27-186):
  * overwrites A's values (A from yafel_hip_mesh_pattern of this mesh, FP64
@@ -237,6 +248,9 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
 /* element matrices only: Ke_dev holds nel*(nen*dpn)^2 doubles, element-major,
  * row-major (the local_tangent buffer of CGAssembly.hpp:117-118). */
 int yafel_hip_element_matrices(yafel_hip_mesh m, const yafel_hip_physics *phys, double *Ke_dev);
+/* the FP32 matrix-core element matrices of YAFEL_HIP_ASSEMBLE_FP32_ELEMENTS
+ * (same layout, float; coefficients, if any, on the device); stream-ordered */
+int yafel_hip_element_matrices_fp32(yafel_hip_mesh m, const yafel_hip_physics *phys, float *Ke_dev);
 
 /* DirichletBC::apply (src/boundary_conditions/DirichletBC.cpp:27-56):
  * rhs -= A g; entries in BC rows or columns zeroed, diagonal set to 1,
@@ -252,15 +266,12 @@ int yafel_hip_bc_mask_plane(yafel_hip_mesh m, int axis, double value, double tol
 /* -------------------------------------------------------------------- DG */
 /* DoFManager(M, DG, p, 1) + Mesh::buildInternalFaces (src/utils/DoFManager.cpp:
  * 166-293, src/mesh/build_faces.cpp:25-96) for 2-D Quad4 meshes with gmsh
- * (counter-clockwise) corners and p = 1..4, plus the advection physics of
- * apps/BasicAdvection_DG/main.cpp:14-98 (phys->kind YAFEL_HIP_PHYS_DG_ADVECTION,
- * nsd 2, constant velocity phys->U, source phys->f).  Dof d = e * (p+1)^2 + a
- * belongs to element e alone.  The element mass inverses (the reference's
- * cached PartialPivLU, DGAssembly.hpp:274-277) are built here, once. */
+ * (counter-clockwise) corners and p = 1..4.  Dof d = e * (p+1)^2 + a belongs
+ * to element e alone.  The element mass inverses (the reference's cached
+ * PartialPivLU, DGAssembly.hpp:274-277) are built here, once. */
 typedef struct yafel_hip_dg_s *yafel_hip_dg;
 int yafel_hip_dg_create(yafel_hip_ctx ctx, int p, int64_t n_corner, const double *corner_nodes,
-                        int64_t ncell, const int32_t *cells, const yafel_hip_physics *phys,
-                        yafel_hip_dg *out);
+                        int64_t ncell, const int32_t *cells, yafel_hip_dg *out);
 int yafel_hip_dg_destroy(yafel_hip_dg dg);
 /* info[0] dofs, info[1] faces, info[2] boundary faces, info[3] dofs per element */
 int yafel_hip_dg_info(yafel_hip_dg dg, int64_t *info);
@@ -270,14 +281,17 @@ int yafel_hip_dg_nodes(yafel_hip_dg dg, double *nodes_dev);
 int yafel_hip_dg_faces(yafel_hip_dg dg, int32_t *left, int32_t *right, int32_t *left_flocal,
                        int32_t *right_flocal);
 /* DGAssembly<Physics>(FESystem&, P, {Residual, DtMass}) (include/assembly/
- * DGAssembly.hpp:32-305): R_dev = R_face + M^-1 (R_face + R_vol), the
- * reference's result; Rface_dev (may be NULL) receives R_face alone. */
-int yafel_hip_dg_residual(yafel_hip_dg dg, double time, const double *U_dev, double *R_dev,
-                          double *Rface_dev);
+ * DGAssembly.hpp:32-305) with the advection physics of apps/BasicAdvection_DG/
+ * main.cpp:14-98 (phys->kind YAFEL_HIP_PHYS_DG_ADVECTION, nsd 2, constant
+ * velocity phys->U, source phys->f): R_dev = R_face + M^-1 (R_face + R_vol),
+ * the reference's result; Rface_dev (may be NULL) receives R_face alone. */
+int yafel_hip_dg_residual(yafel_hip_dg dg, const yafel_hip_physics *phys, double time,
+                          const double *U_dev, double *R_dev, double *Rface_dev);
 /* nsteps of DGRK4::step (include/time_integration/DGRK4.hpp:33-69) on U_dev in
  * place; *time advances by nsteps * dt.  A step is captured once into a HIP
- * graph (per dt and U_dev) and replayed. */
-int yafel_hip_dg_rk4(yafel_hip_dg dg, double dt, int64_t nsteps, double *time, double *U_dev);
+ * graph (per dt, physics and U_dev) and replayed. */
+int yafel_hip_dg_rk4(yafel_hip_dg dg, const yafel_hip_physics *phys, double dt, int64_t nsteps,
+                     double *time, double *U_dev);
 
 /* ---------------------------------------------------------------- serving */
 /* No reference counterpart: a device-resident request loop.  Requests are

@@ -15,7 +15,8 @@ DG = 3
 def make(ctx, orc, p, nx, ny, c=(-1.0, 0.35), f=0.0, jitter=0.2, seed=11):
     from yafel_amd import hip as H
     nodes, cells = rect_mesh(orc, nx, ny, 1.25, 0.8, jitter=jitter, seed=seed)
-    dg = H.DG(ctx, p, nodes, cells, H.physics(H.DG_ADVECTION, 2, f=f, U=(c[0], c[1], 0.0)))
+    dg = H.DG(ctx, p, nodes, cells)
+    dg.ph = H.physics(H.DG_ADVECTION, 2, f=f, U=(c[0], c[1], 0.0))
     return dg, nodes, cells, ophysics(DG, 2, f=f, U=(c[0], c[1], 0.0))
 
 
@@ -35,7 +36,7 @@ def test_residual_matches_the_oracle(ctx, orc, p):
     dg, nodes, cells, oph = make(ctx, orc, p, 9, 6, f=0.3)
     U = np.random.default_rng(p).standard_normal(dg.ndofs)
     Ud, Rd, Fd = ctx.upload(U), ctx.alloc(dg.ndofs), ctx.alloc(dg.ndofs)
-    dg.residual(Ud, Rd, Fd)
+    dg.residual(dg.ph, Ud, Rd, Fd)
     R, Rf = orc.dg_residual(p, nodes, cells, oph, U, face=True)
     r, f = Rd.download(), Fd.download()
     scale = np.abs(R).max()
@@ -48,12 +49,12 @@ def test_closed_forms_on_the_device(ctx, orc):
     dg, *_ = make(ctx, orc, 3, 5, 4, c=(0.0, 0.0), f=-0.75)
     U = np.random.default_rng(1).standard_normal(dg.ndofs)
     Rd = ctx.alloc(dg.ndofs)
-    dg.residual(ctx.upload(U), Rd)
+    dg.residual(dg.ph, ctx.upload(U), Rd)
     np.testing.assert_allclose(Rd.download(), -0.75, rtol=0, atol=1e-13)
     # constant field: face terms sum to minus the outflow, here (1.0 * 0.8 + 0.35 * 1.25)
     dg, *_ = make(ctx, orc, 2, 6, 5, c=(-1.0, 0.35))
     Fd, Rd = ctx.alloc(dg.ndofs), ctx.alloc(dg.ndofs)
-    dg.residual(ctx.upload(np.ones(dg.ndofs)), Rd, Fd)
+    dg.residual(dg.ph, ctx.upload(np.ones(dg.ndofs)), Rd, Fd)
     assert abs(Fd.download().sum() + (1.0 * 0.8 + 0.35 * 1.25)) < 1e-12
 
 
@@ -64,7 +65,7 @@ def test_rk4_graph_matches_the_oracle_and_replays_consistently(ctx, orc):
     U0 = np.sin(3 * X[:, 0]) * np.cos(2 * X[:, 1])
     exp = orc.dg_rk4(p, nodes, cells, oph, U0, dt, 6)
     Ud = ctx.upload(U0)
-    t = dg.rk4(Ud, dt, 6)
+    t = dg.rk4(dg.ph, Ud, dt, 6)
     assert abs(t - 6 * dt) < 1e-15
     got = Ud.download()
     assert np.abs(got - exp).max() <= 1e-12 * np.abs(exp).max()
@@ -73,8 +74,8 @@ def test_rk4_graph_matches_the_oracle_and_replays_consistently(ctx, orc):
     scratch = ctx.alloc(dg.ndofs)
     t = 0.0
     for k in range(6):
-        t = dg.rk4(Ud2, dt, 1, t)
-        dg.residual(Ud, scratch)
+        t = dg.rk4(dg.ph, Ud2, dt, 1, t)
+        dg.residual(dg.ph, Ud, scratch)
     assert np.array_equal(Ud2.download(), got)
     assert abs(t - 6 * dt) < 1e-15
 
@@ -83,8 +84,10 @@ def test_errors_are_reported(ctx, orc):
     from yafel_amd import hip as H
     nodes, cells = rect_mesh(orc, 2, 2)
     with pytest.raises(H.YafelHipError) as e:
-        H.DG(ctx, 5, nodes, cells, H.physics(H.DG_ADVECTION, 2, U=(1, 0, 0)))
+        H.DG(ctx, 5, nodes, cells)
     assert e.value.code == 1
+    dg = H.DG(ctx, 2, nodes, cells)
+    U, R = ctx.alloc(dg.ndofs), ctx.alloc(dg.ndofs)
     with pytest.raises(H.YafelHipError) as e:
-        H.DG(ctx, 2, nodes, cells, H.physics(H.POISSON, 2))
+        dg.residual(H.physics(H.POISSON, 2), U, R)
     assert e.value.code == 1

@@ -205,11 +205,95 @@ __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__re
   }
 }
 
+// FP32 element matrices of the Poisson tangent for 2-D elements of order
+// p = 3, 4 on the matrix cores.  With G[q*2+d][a] = sqrt(jxw_q) (J^-T grad N_a)_d
+// the element matrix is K_e = -k_e G^T G (apps/BasicPoisson/main.cpp:51-58): a
+// (2 nqp) x nen -> nen x nen contraction, run as 32x32x2 FP32 MFMA steps with
+// nen padded to 32.  G feeds both operands: A[m][k] = G[k][m] and B[k][n] =
+// G[k][n] are the same register on every lane.  One wavefront per element,
+// four per workgroup; the residual stays FP64.
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+template <int P>
+__global__ __launch_bounds__(256) void k_element_mfma(int64_t nel, const int32_t *__restrict__ el,
+                                                      const double *__restrict__ X,
+                                                      const double *__restrict__ tabN,
+                                                      const double *__restrict__ tabdN,
+                                                      const double *__restrict__ tabw, double f,
+                                                      const double *__restrict__ coef, float *__restrict__ Ke,
+                                                      double *__restrict__ Re, int want_k, int want_r) {
+  constexpr int N1 = P + 1, NEN = N1 * N1, NQP = NEN, KD = 2 * NQP;
+  static_assert(NEN <= 32 && KD % 2 == 0, "one 32x32 tile per element");
+  __shared__ float sG[4][KD][32];
+  __shared__ double sX[4][NEN][2];
+  __shared__ double sJ[4][NQP][5];  // J^-1 (row-major) and jxw
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t e = (int64_t)blockIdx.x * 4 + w;
+  const bool live = e < nel;  // uniform over the wavefront
+  if (live && l < NEN) {
+    const int64_t node = el[e * NEN + l];
+    sX[w][l][0] = X[3 * node];
+    sX[w][l][1] = X[3 * node + 1];
+  }
+  __syncthreads();
+  if (live && l < NQP) {
+    // Element::update (Element.hpp:116-130)
+    double J[4] = {0, 0, 0, 0};
+    for (int A = 0; A < NEN; ++A) {
+      const double g0 = tabdN[(l * NEN + A) * 2], g1 = tabdN[(l * NEN + A) * 2 + 1];
+      J[0] += sX[w][A][0] * g0;
+      J[1] += sX[w][A][0] * g1;
+      J[2] += sX[w][A][1] * g0;
+      J[3] += sX[w][A][1] * g1;
+    }
+    double det, Ji[4];
+    jac_inv<2>(J, det, Ji);
+    for (int k = 0; k < 4; ++k) sJ[w][l][k] = Ji[k];
+    sJ[w][l][4] = det * tabw[l];
+  }
+  __syncthreads();
+  if (live) {
+    for (int t = l; t < KD * 32; t += 64) {
+      const int k = t >> 5, a = t & 31, q = k >> 1, d = k & 1;
+      float v = 0.f;
+      if (a < NEN) {
+        const double g0 = tabdN[(q * NEN + a) * 2], g1 = tabdN[(q * NEN + a) * 2 + 1];
+        const double gd = sJ[w][q][d] * g0 + sJ[w][q][2 + d] * g1;  // (J^-T grad_xi N_a)_d
+        v = (float)(sqrt(sJ[w][q][4]) * gd);
+      }
+      sG[w][k][a] = v;
+    }
+  }
+  __syncthreads();
+  if (live && want_k) {
+    v16f acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 5
+    for (int kk = 0; kk < KD / 2; ++kk) {
+      const float v = sG[w][2 * kk + (l >> 5)][l & 31];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(v, v, acc, 0, 0, 0);
+    }
+    const float ke = (float)(coef ? coef[e] : 1.0);
+    float *Kout = Ke + e * (int64_t)NEN * NEN;
+    const int col = l & 31;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      // accumulator i of lane l holds C[8 (i/4) + 4 (l/32) + i%4][l%32]
+      const int row = 8 * (i >> 2) + 4 * (l >> 5) + (i & 3);
+      if (row < NEN && col < NEN) Kout[row * NEN + col] = -ke * acc[i];
+    }
+  }
+  if (live && want_r && l < NEN) {
+    double r = 0;
+    for (int q = 0; q < NQP; ++q) r += (f * sJ[w][q][4]) * tabN[q * NEN + l];
+    Re[e * NEN + l] = r;
+  }
+}
+
 // one thread per stored node pair: sum the element contributions in order
-template <int DPN, int FMT>
+template <int DPN, int FMT, class KT = double>
 __global__ __launch_bounds__(NT) void k_gather(int64_t nruns, int nen, const int32_t *__restrict__ cptr,
                                                const uint32_t *__restrict__ contrib,
-                                               const double *__restrict__ Ke,
+                                               const KT *__restrict__ Ke,
                                                const int32_t *__restrict__ run_row,
                                                const int32_t *__restrict__ node_ptr,
                                                const int32_t *__restrict__ sptr,
@@ -230,7 +314,7 @@ __global__ __launch_bounds__(NT) void k_gather(int64_t nruns, int nen, const int
     const int64_t e = t / nen2;
     const int r = (int)(t - e * nen2);
     const int a = r / nen, b = r - (r / nen) * nen;
-    const double *k = Ke + e * (int64_t)nld * nld + (int64_t)(a * DPN) * nld + b * DPN;
+    const KT *k = Ke + e * (int64_t)nld * nld + (int64_t)(a * DPN) * nld + b * DPN;
 #pragma unroll
     for (int i = 0; i < DPN; ++i)
 #pragma unroll
@@ -418,6 +502,29 @@ int yafel_hip_element_matrices(yafel_hip_mesh m, const yafel_hip_physics *phys, 
   YH_TRY_END(m ? m->ctx : nullptr)
 }
 
+int yafel_hip_element_matrices_fp32(yafel_hip_mesh m, const yafel_hip_physics *phys, float *Ke_dev) {
+  YH_TRY_BEGIN
+  REQUIRE(m && Ke_dev, "element_matrices_fp32: null argument");
+  check_phys(m, phys);
+  REQUIRE(m->dim == 2 && (m->p == 3 || m->p == 4) && m->dpn == 1 && phys->kind == YAFEL_HIP_PHYS_POISSON,
+          "element_matrices_fp32: matrix-core element matrices cover 2-D Poisson with p = 3 or 4");
+  HIPCHK(hipSetDevice(m->ctx->device));
+  REQUIRE(!phys->coef || phys->coef_on_device, "element_matrices_fp32: coefficients must be device data");
+  const unsigned g = (unsigned)((m->nel + 3) / 4);
+  hipStream_t s = m->ctx->stream;
+  if (m->p == 3)
+    k_element_mfma<3><<<g, 256, 0, s>>>(m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(), m->tabN.as<double>(),
+                                         m->tabdN.as<double>(), m->tabw.as<double>(), phys->f, phys->coef, Ke_dev,
+                                         nullptr, 1, 0);
+  else
+    k_element_mfma<4><<<g, 256, 0, s>>>(m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(), m->tabN.as<double>(),
+                                         m->tabdN.as<double>(), m->tabw.as<double>(), phys->f, phys->coef, Ke_dev,
+                                         nullptr, 1, 0);
+  HIPCHK(hipGetLastError());
+  return YAFEL_HIP_OK;
+  YH_TRY_END(m ? m->ctx : nullptr)
+}
+
 int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requirements,
                        yafel_hip_matrix A, double *R_dev) {
   YH_TRY_BEGIN
@@ -442,10 +549,31 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
       coef = coefd.as<double>();
     }
   }
-  if (wk && m->stageK.bytes < (size_t)m->nel * nld * nld * 8) m->stageK.alloc((size_t)m->nel * nld * nld * 8);
+  const bool f32 = (requirements & YAFEL_HIP_ASSEMBLE_FP32_ELEMENTS) != 0;
+  REQUIRE(!f32 || (m->dim == 2 && (m->p == 3 || m->p == 4) && m->dpn == 1 && phys->kind == YAFEL_HIP_PHYS_POISSON),
+          "assemble: FP32 matrix-core element matrices cover 2-D Poisson with p = 3 or 4");
+  if (wk && !f32 && m->stageK.bytes < (size_t)m->nel * nld * nld * 8) m->stageK.alloc((size_t)m->nel * nld * nld * 8);
+  if (wk && f32 && m->stageK32.bytes < (size_t)m->nel * nld * nld * 4)
+    m->stageK32.alloc((size_t)m->nel * nld * nld * 4);
   if (wr && m->stageR.bytes < (size_t)m->nel * nld * 8) m->stageR.alloc((size_t)m->nel * nld * 8);
   Phys ph = make_phys(phys);
-  launch_element(m, ph, coef, wk ? m->stageK.as<double>() : nullptr, wr ? m->stageR.as<double>() : nullptr, wk, wr);
+  if (f32) {
+    const unsigned g = (unsigned)((m->nel + 3) / 4);
+    float *k32 = wk ? m->stageK32.as<float>() : nullptr;
+    double *r64 = wr ? m->stageR.as<double>() : nullptr;
+    if (m->p == 3)
+      k_element_mfma<3><<<g, 256, 0, s>>>(m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(),
+                                           m->tabN.as<double>(), m->tabdN.as<double>(), m->tabw.as<double>(), ph.f,
+                                           coef, k32, r64, wk, wr);
+    else
+      k_element_mfma<4><<<g, 256, 0, s>>>(m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(),
+                                           m->tabN.as<double>(), m->tabdN.as<double>(), m->tabw.as<double>(), ph.f,
+                                           coef, k32, r64, wk, wr);
+    HIPCHK(hipGetLastError());
+  } else {
+    launch_element(m, ph, coef, wk ? m->stageK.as<double>() : nullptr, wr ? m->stageR.as<double>() : nullptr, wk,
+                   wr);
+  }
   if (wk) {
     YhPlan *pl = A->plan;
     const int g = ceil_div(pl->nruns, NT);
@@ -457,6 +585,11 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
         k_gather<3, YAFEL_HIP_BCSR3><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
                                                       m->stageK.as<double>(), pl->run_row.as<int32_t>(),
                                                       pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val);
+      else if (f32)
+        k_gather<1, YAFEL_HIP_CSR, float><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(),
+                                                           pl->contrib.as<uint32_t>(), m->stageK32.as<float>(),
+                                                           pl->run_row.as<int32_t>(), pl->node_ptr.as<int32_t>(), sp,
+                                                           ip, A->ntot, val);
       else if (m->dpn == 1)
         k_gather<1, YAFEL_HIP_CSR><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
                                                     m->stageK.as<double>(), pl->run_row.as<int32_t>(),

@@ -21,6 +21,7 @@
 // HIP graph and replayed.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "internal.hpp"
 #include "mesh.hpp"
@@ -43,21 +44,20 @@ struct yafel_hip_dg_s {
   yafel_hip_ctx_s *ctx = nullptr;
   int p = 1, n1 = 2, nen = 4, nqp = 4;
   int64_t nel = 0, nfaces = 0, nbnd = 0;
-  double c[2] = {0, 0}, src = 0;
   DBuf X;                 // dof node coordinates, nel*nen*3
   DBuf tabN, tabdN, tabw; // volume tables (GL points x GLL nodes)
   DBuf bN, bdN, bw;       // face tables (1-D)
   DBuf fnodes;            // int32 [4][n1] local face nodes
   DBuf faces;             // DGFaceRec [nfaces]
   DBuf lnodes, rnodes;    // int32 [nfaces][n1] matched local node lists (-1: no side)
-  DBuf geo;               // double [nfaces][n1][3]: c.n, jxw left list, jxw right list
+  DBuf geo;               // double [nfaces][n1][4]: unit normal (x, y), jxw left list, jxw right list
   DBuf ef;                // int32 [nel][4]: (face << 1) | side (0 left, 1 right)
   DBuf minv;              // double [nel][nen][nen]
   DBuf flux;              // double [nfaces][n1] scratch of one evaluation
   DBuf rk;                // U0, k1..k4, W (6 * ndofs)
   DBuf tdev;              // device time
   hipGraphExec_t graph = nullptr;
-  double graph_dt = 0;
+  double graph_key[4] = {0, 0, 0, 0};  // dt, velocity, source the graph was captured with
   double *graph_U = nullptr;
   ~yafel_hip_dg_s() {
     if (graph) (void)hipGraphExecDestroy(graph);
@@ -190,7 +190,7 @@ __device__ void face_geom(int n1, int nen, int64_t el, const int32_t *nodes, con
 
 __global__ void k_dg_geom(int64_t nf, int n1, int nen, const DGFaceRec *__restrict__ F, const int32_t *__restrict__ L,
                           const int32_t *__restrict__ R, const double *__restrict__ X, const double *__restrict__ bdN,
-                          const double *__restrict__ bw, double c0, double c1, double *__restrict__ geo) {
+                          const double *__restrict__ bw, double *__restrict__ geo) {
   const int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x;
   if (i >= nf * n1) return;
   const int64_t f = i / n1;
@@ -199,7 +199,7 @@ __global__ void k_dg_geom(int64_t nf, int n1, int nen, const DGFaceRec *__restri
   // unmatched node lists are reported by k_dg_match; leave their geometry NaN
   for (int k = 0; k < n1; ++k)
     if ((r.left >= 0 && L[f * n1 + k] < 0) || (r.right >= 0 && R[f * n1 + k] < 0) || (r.left < 0 && r.right < 0)) {
-      geo[3 * i] = geo[3 * i + 1] = geo[3 * i + 2] = __builtin_nan("");
+      geo[4 * i] = geo[4 * i + 1] = geo[4 * i + 2] = geo[4 * i + 3] = __builtin_nan("");
       return;
     }
   double nx, ny, jl = 0, jr = 0;
@@ -212,9 +212,10 @@ __global__ void k_dg_geom(int64_t nf, int n1, int nen, const DGFaceRec *__restri
   } else {
     face_geom(n1, nen, r.right, &R[f * n1], X, bdN, bw, fq, nx, ny, jr);
   }
-  geo[3 * i] = nx * c0 + ny * c1;
-  geo[3 * i + 1] = jl;
-  geo[3 * i + 2] = jr;
+  geo[4 * i] = nx;
+  geo[4 * i + 1] = ny;
+  geo[4 * i + 2] = jl;
+  geo[4 * i + 3] = jr;
 }
 
 __global__ void k_dg_elem_faces(int64_t nf, const DGFaceRec *__restrict__ F, int32_t *__restrict__ ef) {
@@ -311,13 +312,14 @@ __global__ __launch_bounds__(64) void k_dg_minv(int64_t nel, int nqp, const doub
 // only) on boundary faces, at every face point.
 __global__ void k_dg_flux(int64_t nf, int n1, int nen, const DGFaceRec *__restrict__ F, const int32_t *__restrict__ L,
                           const int32_t *__restrict__ R, const double *__restrict__ bN,
-                          const double *__restrict__ geo, const double *__restrict__ U, double *__restrict__ flux) {
+                          const double *__restrict__ geo, double c0, double c1, const double *__restrict__ U,
+                          double *__restrict__ flux) {
   const int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x;
   if (i >= nf * n1) return;
   const int64_t f = i / n1;
   const int fq = (int)(i - f * n1);
   const DGFaceRec r = F[f];
-  const double vn = geo[3 * i];
+  const double vn = geo[4 * i] * c0 + geo[4 * i + 1] * c1;  // dot(n, convection_velocity)
   double fl;
   if (r.left >= 0 && r.right >= 0) {
     double ul = 0, ur = 0;
@@ -373,7 +375,7 @@ __global__ __launch_bounds__(NT) void k_dg_elem(int64_t nel, const DGFaceRec *__
     for (int fq = 0; fq < n1; ++fq) {
       const int64_t gi = f * n1 + fq;
       const double fl = flux[gi];
-      const double jw = side == 0 ? geo[3 * gi + 1] : geo[3 * gi + 2];
+      const double jw = side == 0 ? geo[4 * gi + 2] : geo[4 * gi + 3];
       for (int i = 0; i < n1; ++i) {
         const int a = nodes[i];
         const double v = bN[fq * n1 + i] * fl * jw;
@@ -436,18 +438,29 @@ __global__ void k_dg_rk4_combine(int64_t n, const double *__restrict__ u0, const
   if (i == 0) *t += dt;
 }
 
-void dg_eval(yafel_hip_dg_s *D, const double *U, double *R, double *Rface, hipStream_t s) {
+struct DGPhys {
+  double c0, c1, src;
+};
+
+DGPhys dg_phys(const yafel_hip_physics *phys) {
+  REQUIRE(phys, "dg: null physics");
+  REQUIRE(phys->kind == YAFEL_HIP_PHYS_DG_ADVECTION && phys->nsd == 2,
+          "dg: physics must be the 2-D DG advection descriptor");
+  return DGPhys{phys->U[0], phys->U[1], phys->f};
+}
+
+void dg_eval(yafel_hip_dg_s *D, const DGPhys &ph, const double *U, double *R, double *Rface, hipStream_t s) {
   const int64_t nfq = D->nfaces * D->n1;
   k_dg_flux<<<ceil_div(nfq, NT), NT, 0, s>>>(D->nfaces, D->n1, D->nen, D->faces.as<DGFaceRec>(),
                                              D->lnodes.as<int32_t>(), D->rnodes.as<int32_t>(), D->bN.as<double>(),
-                                             D->geo.as<double>(), U, D->flux.as<double>());
+                                             D->geo.as<double>(), ph.c0, ph.c1, U, D->flux.as<double>());
   HIPCHK(hipGetLastError());
   const int g = ceil_div(D->nel, NT);
 #define DG_ELEM(P)                                                                                               \
   k_dg_elem<P><<<g, NT, 0, s>>>(D->nel, D->faces.as<DGFaceRec>(), D->lnodes.as<int32_t>(), D->rnodes.as<int32_t>(), \
                                 D->ef.as<int32_t>(), D->geo.as<double>(), D->flux.as<double>(), D->bN.as<double>(), \
                                 D->X.as<double>(), D->tabN.as<double>(), D->tabdN.as<double>(), D->tabw.as<double>(), \
-                                D->minv.as<double>(), D->c[0], D->c[1], D->src, U, R, Rface)
+                                D->minv.as<double>(), ph.c0, ph.c1, ph.src, U, R, Rface)
   switch (D->p) {
     case 1: DG_ELEM(1); break;
     case 2: DG_ELEM(2); break;
@@ -459,18 +472,18 @@ void dg_eval(yafel_hip_dg_s *D, const double *U, double *R, double *Rface, hipSt
 }
 
 // One DGRK4::step on U (stream-ordered kernels only, so it can be captured).
-void dg_rk4_step(yafel_hip_dg_s *D, double dt, double *U, hipStream_t s) {
+void dg_rk4_step(yafel_hip_dg_s *D, const DGPhys &ph, double dt, double *U, hipStream_t s) {
   const int64_t n = D->nel * D->nen;
   double *U0 = D->rk.as<double>(), *k1 = U0 + n, *k2 = k1 + n, *k3 = k2 + n, *k4 = k3 + n, *W = k4 + n;
   const int g = ceil_div(n, NT);
   k_dg_copy<<<g, NT, 0, s>>>(n, U, U0);
-  dg_eval(D, U0, k1, nullptr, s);
+  dg_eval(D, ph, U0, k1, nullptr, s);
   k_dg_axpy<<<g, NT, 0, s>>>(n, U0, k1, dt / 2, W);
-  dg_eval(D, W, k2, nullptr, s);
+  dg_eval(D, ph, W, k2, nullptr, s);
   k_dg_axpy<<<g, NT, 0, s>>>(n, U0, k2, dt / 2, W);
-  dg_eval(D, W, k3, nullptr, s);
+  dg_eval(D, ph, W, k3, nullptr, s);
   k_dg_axpy<<<g, NT, 0, s>>>(n, U0, k3, dt, W);
-  dg_eval(D, W, k4, nullptr, s);
+  dg_eval(D, ph, W, k4, nullptr, s);
   k_dg_rk4_combine<<<g, NT, 0, s>>>(n, U0, k1, k2, k3, k4, dt, U, D->tdev.as<double>());
   HIPCHK(hipGetLastError());
 }
@@ -486,13 +499,11 @@ void upload(DBuf &b, const std::vector<T> &v, hipStream_t s) {
 extern "C" {
 
 int yafel_hip_dg_create(yafel_hip_ctx ctx, int p, int64_t n_corner, const double *corner_nodes, int64_t ncell,
-                        const int32_t *cells, const yafel_hip_physics *phys, yafel_hip_dg *out) {
+                        const int32_t *cells, yafel_hip_dg *out) {
   YH_TRY_BEGIN
-  REQUIRE(ctx && out && corner_nodes && cells && phys, "dg_create: null argument");
+  REQUIRE(ctx && out && corner_nodes && cells, "dg_create: null argument");
   REQUIRE(p >= 1 && p <= PMAX, "dg_create: polynomial order must be 1..4");
   REQUIRE(n_corner > 0 && ncell > 0, "dg_create: empty mesh");
-  REQUIRE(phys->kind == YAFEL_HIP_PHYS_DG_ADVECTION && phys->nsd == 2,
-          "dg_create: physics must be the 2-D DG advection descriptor");
   REQUIRE(ncell * 4 < (int64_t)INT32_MAX / 2 && n_corner < (int64_t)INT32_MAX, "dg_create: mesh too large");
   for (int64_t i = 0; i < 4 * ncell; ++i)
     REQUIRE(cells[i] >= 0 && cells[i] < n_corner, "dg_create: cell references a node outside corner_nodes");
@@ -506,9 +517,6 @@ int yafel_hip_dg_create(yafel_hip_ctx ctx, int p, int64_t n_corner, const double
   D->nen = D->n1 * D->n1;
   D->nqp = D->nen;
   D->nel = ncell;
-  D->c[0] = phys->U[0];
-  D->c[1] = phys->U[1];
-  D->src = phys->f;
   const int n1 = D->n1, nen = D->nen;
   // element and face tables
   std::vector<double> N, dN, w, xl, bN, bdN, bw, bxl;
@@ -582,11 +590,11 @@ int yafel_hip_dg_create(yafel_hip_ctx ctx, int p, int64_t n_corner, const double
                                              D->X.as<double>(), D->lnodes.as<int32_t>(), D->rnodes.as<int32_t>(),
                                              bad.as<int>());
   HIPCHK(hipGetLastError());
-  D->geo.alloc((int64_t)nf * n1 * 24);
+  D->geo.alloc((int64_t)nf * n1 * 32);
   k_dg_geom<<<ceil_div((int64_t)nf * n1, NT), NT, 0, s>>>(nf, n1, nen, D->faces.as<DGFaceRec>(),
                                                            D->lnodes.as<int32_t>(), D->rnodes.as<int32_t>(),
                                                            D->X.as<double>(), D->bdN.as<double>(), D->bw.as<double>(),
-                                                           D->c[0], D->c[1], D->geo.as<double>());
+                                                           D->geo.as<double>());
   HIPCHK(hipGetLastError());
   D->ef.alloc(ncell * 16);
   HIPCHK(hipMemsetAsync(D->ef.p, 0xff, ncell * 16, s));
@@ -666,25 +674,30 @@ int yafel_hip_dg_faces(yafel_hip_dg D, int32_t *left, int32_t *right, int32_t *l
   YH_TRY_END(D ? D->ctx : nullptr)
 }
 
-int yafel_hip_dg_residual(yafel_hip_dg D, double time, const double *U_dev, double *R_dev, double *Rface_dev) {
+int yafel_hip_dg_residual(yafel_hip_dg D, const yafel_hip_physics *phys, double time, const double *U_dev,
+                          double *R_dev, double *Rface_dev) {
   YH_TRY_BEGIN
   REQUIRE(D && U_dev && R_dev, "dg_residual: null argument");
   REQUIRE(U_dev != R_dev && (void *)U_dev != (void *)Rface_dev, "dg_residual: U must not alias R");
+  const DGPhys ph = dg_phys(phys);
   (void)time;  // the advection physics is time-independent
   HIPCHK(hipSetDevice(D->ctx->device));
-  dg_eval(D, U_dev, R_dev, Rface_dev, D->ctx->stream);
+  dg_eval(D, ph, U_dev, R_dev, Rface_dev, D->ctx->stream);
   return YAFEL_HIP_OK;
   YH_TRY_END(D ? D->ctx : nullptr)
 }
 
-int yafel_hip_dg_rk4(yafel_hip_dg D, double dt, int64_t nsteps, double *time, double *U_dev) {
+int yafel_hip_dg_rk4(yafel_hip_dg D, const yafel_hip_physics *phys, double dt, int64_t nsteps, double *time,
+                     double *U_dev) {
   YH_TRY_BEGIN
   REQUIRE(D && U_dev && nsteps >= 0, "dg_rk4: bad argument");
+  const DGPhys ph = dg_phys(phys);
+  const double key[4] = {dt, ph.c0, ph.c1, ph.src};
   HIPCHK(hipSetDevice(D->ctx->device));
   hipStream_t s = D->ctx->stream;
   const double t0 = time ? *time : 0.0;
   HIPCHK(hipMemcpyAsync(D->tdev.p, &t0, 8, hipMemcpyHostToDevice, s));
-  if (nsteps > 0 && (!D->graph || D->graph_dt != dt || D->graph_U != U_dev)) {
+  if (nsteps > 0 && (!D->graph || std::memcmp(D->graph_key, key, sizeof(key)) != 0 || D->graph_U != U_dev)) {
     if (D->graph) {
       HIPCHK(hipGraphExecDestroy(D->graph));
       D->graph = nullptr;
@@ -692,7 +705,7 @@ int yafel_hip_dg_rk4(yafel_hip_dg D, double dt, int64_t nsteps, double *time, do
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
-      dg_rk4_step(D, dt, U_dev, s);
+      dg_rk4_step(D, ph, dt, U_dev, s);
     } catch (...) {
       (void)hipStreamEndCapture(s, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -702,7 +715,7 @@ int yafel_hip_dg_rk4(yafel_hip_dg D, double dt, int64_t nsteps, double *time, do
     hipError_t ie = hipGraphInstantiate(&D->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HIPCHK(ie);
-    D->graph_dt = dt;
+    std::memcpy(D->graph_key, key, sizeof(key));
     D->graph_U = U_dev;
   }
   for (int64_t k = 0; k < nsteps; ++k) HIPCHK(hipGraphLaunch(D->graph, s));

@@ -107,7 +107,17 @@ struct yafel_hip_ctx_s {
   void *pinned = nullptr;  // host-pinned status words
   size_t pinned_bytes = 0;
   std::atomic<int> refs{1};
+  // per-launch timing of the solvers' inner SpMV kernel (off unless asked for)
+  bool ktime = false;
+  std::vector<hipEvent_t> kev;  // start/stop pairs of the recorded launches
+  size_t kused = 0;             // events in use
 };
+
+namespace yh {
+// When kernel timing is on, returns a (start, stop) event pair to record
+// around one launch on ctx->stream; nullptr pair otherwise.
+void ktime_pair(yafel_hip_ctx_s *ctx, hipEvent_t *start, hipEvent_t *stop);
+}  // namespace yh
 
 struct YhPlan;  // assembly gather plan (per mesh & format)
 

@@ -46,9 +46,13 @@ template <class T, class Op>
 void launch_spmv_op(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
   const int grid = sell_grid(A->nsl, A->ctx->num_cus * 8);
   const SellView<T> v = view<T>(A);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
+  if (e0) HIPCHK(hipEventRecord(e0, s));
   if (A->format == YAFEL_HIP_CSR) k_spmv_fused<YAFEL_HIP_CSR, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
   else k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
   HIPCHK(hipGetLastError());
+  if (e1) HIPCHK(hipEventRecord(e1, s));
   if constexpr (Op::NP > 0) {
     k_finish<Op><<<1, NTF, 0, s>>>(op, grid);
     HIPCHK(hipGetLastError());

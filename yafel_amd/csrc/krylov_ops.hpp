@@ -165,6 +165,7 @@ __global__ __launch_bounds__(NT) void k_elem(int64_t n, Op op0) {
 
 // --------------------------------------------------------------------- ops
 struct OpBase {
+  static constexpr bool kTimed = false;  // recorded by the optional kernel timing
   KState *st;
   double *part;
   unsigned int *counter;
@@ -221,6 +222,7 @@ __device__ void after_iteration(KState *st, double rho_new, double rr) {
 template <class T>
 struct OpCgA : OpBase {
   using value_type = T;
+  static constexpr bool kTimed = true;  // the CG iteration's SpMV kernel
   static constexpr int NP = 1;
   const T *z;
   T *pbuf0, *pbuf1;

@@ -34,6 +34,7 @@ struct yafel_hip_mesh_s {
   bool inc_ok = false;
   std::unique_ptr<YhPlan> plan_csr, plan_bcsr;
   yh::DBuf stageK, stageR;  // element matrix / vector staging
+  yh::DBuf stageK32;        // FP32 element matrix staging (matrix-core path)
   ~yafel_hip_mesh_s() { yh::ctx_release(ctx); }
 };
 

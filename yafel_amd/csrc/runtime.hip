@@ -65,8 +65,22 @@ int grid_for(yafel_hip_ctx_s *ctx, int64_t work_items, int block) {
 
 namespace yh {
 
+void ktime_pair(yafel_hip_ctx_s *ctx, hipEvent_t *start, hipEvent_t *stop) {
+  *start = *stop = nullptr;
+  if (!ctx->ktime) return;
+  while (ctx->kev.size() < ctx->kused + 2) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    ctx->kev.push_back(e);
+  }
+  *start = ctx->kev[ctx->kused];
+  *stop = ctx->kev[ctx->kused + 1];
+  ctx->kused += 2;
+}
+
 static void ctx_free(yafel_hip_ctx_s *ctx) {
   (void)hipSetDevice(ctx->device);
+  for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
   (void)hipStreamSynchronize(ctx->stream);
   for (auto &b : ctx->ws.vec64) b.release();
   for (auto &b : ctx->ws.vec32) b.release();
@@ -202,6 +216,32 @@ int yafel_hip_timer_end(yafel_hip_ctx ctx, double *ms) {
   float f = 0;
   HIPCHK(hipEventElapsedTime(&f, ctx->t0, ctx->t1));
   *ms = f;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(ctx)
+}
+
+int yafel_hip_kernel_timing(yafel_hip_ctx ctx, int enable) {
+  YH_TRY_BEGIN
+  REQUIRE(ctx, "kernel_timing: null context");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->ktime = enable != 0;
+  ctx->kused = 0;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(ctx)
+}
+
+int yafel_hip_kernel_time(yafel_hip_ctx ctx, double *ms_total, int64_t *launches) {
+  YH_TRY_BEGIN
+  REQUIRE(ctx && ms_total && launches, "kernel_time: null argument");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double t = 0;
+  for (size_t i = 0; i + 1 < ctx->kused; i += 2) {
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, ctx->kev[i], ctx->kev[i + 1]));
+    t += f;
+  }
+  *ms_total = t;
+  *launches = (int64_t)(ctx->kused / 2);
   return YAFEL_HIP_OK;
   YH_TRY_END(ctx)
 }

@@ -24,7 +24,7 @@ CG, PCG_JACOBI, BICGSTAB, BICGSTAB_JACOBI = 0, 1, 2, 3
 STOP_REFERENCE, STOP_RELRES = 0, 1
 STATUS = {0: "converged", 1: "maxiter", 2: "breakdown", 3: "nan"}
 POISSON, ELASTICITY, SUPG, DG_ADVECTION = 0, 1, 2, 3
-TANGENT, RESIDUAL = 1, 2
+TANGENT, RESIDUAL, FP32_ELEMENTS = 1, 2, 4
 
 _P, _I64, _D, _I = C.c_void_p, C.c_int64, C.c_double, C.c_int
 
@@ -94,6 +94,7 @@ for _n, _a in [
     ("yafel_hip_spmv", [_P, _I, _P, _P]), ("yafel_hip_spmv_host", [_P, _P, _P]),
     ("yafel_hip_spmm", [_P, _I, _I, _P, _P]), ("yafel_hip_dot", [_P, _I64, _P, _P, C.POINTER(_D)]),
     ("yafel_hip_copy_bandwidth", [_P, C.c_size_t, _I, C.POINTER(_D)]),
+    ("yafel_hip_kernel_timing", [_P, _I]), ("yafel_hip_kernel_time", [_P, C.POINTER(_D), C.POINTER(_I64)]),
     ("yafel_hip_solve_opts_default", [C.POINTER(SolveOpts)]),
     ("yafel_hip_solve", [_P, C.POINTER(SolveOpts), _P, _P, C.POINTER(SolveInfo)]),
     ("yafel_hip_solve_host", [_P, C.POINTER(SolveOpts), _P, _P, _P, C.POINTER(SolveInfo)]),
@@ -104,15 +105,16 @@ for _n, _a in [
     ("yafel_hip_mesh_pattern", [_P, _I, _I, C.POINTER(_P)]),
     ("yafel_hip_assemble", [_P, C.POINTER(Physics), _I, _P, _P]),
     ("yafel_hip_element_matrices", [_P, C.POINTER(Physics), _P]),
+    ("yafel_hip_element_matrices_fp32", [_P, C.POINTER(Physics), _P]),
     ("yafel_hip_dirichlet_apply", [_P, _P, _P, _P]),
     ("yafel_hip_bc_mask_plane", [_P, _I, _D, _D, _I, _P]),
     ("yafel_hip_structured_mesh", [_I, _I, _I, _I, _I, _P, _P]),
     ("yafel_hip_normal_fill", [C.c_uint64, _I64, _D, _D, _P]),
-    ("yafel_hip_dg_create", [_P, _I, _I64, _P, _I64, _P, C.POINTER(Physics), C.POINTER(_P)]),
+    ("yafel_hip_dg_create", [_P, _I, _I64, _P, _I64, _P, C.POINTER(_P)]),
     ("yafel_hip_dg_destroy", [_P]), ("yafel_hip_dg_info", [_P, _P]), ("yafel_hip_dg_nodes", [_P, _P]),
     ("yafel_hip_dg_faces", [_P, _P, _P, _P, _P]),
-    ("yafel_hip_dg_residual", [_P, _D, _P, _P, _P]),
-    ("yafel_hip_dg_rk4", [_P, _D, _I64, C.POINTER(_D), _P]),
+    ("yafel_hip_dg_residual", [_P, C.POINTER(Physics), _D, _P, _P, _P]),
+    ("yafel_hip_dg_rk4", [_P, C.POINTER(Physics), _D, _I64, C.POINTER(_D), _P]),
     ("yafel_hip_server_create", [_P, C.c_size_t, _I, C.POINTER(_P)]),
     ("yafel_hip_server_create_ex", [_P, C.c_size_t, _I, _I, C.POINTER(_P)]),
     ("yafel_hip_server_destroy", [_P]),
@@ -180,6 +182,15 @@ class Context:
         d = DeviceArray(self, a.size, a.dtype)
         d.upload(a)
         return d
+
+    def kernel_timing(self, enable=True):
+        check(lib.yafel_hip_kernel_timing(self.h, 1 if enable else 0), self.h)
+
+    def kernel_time(self):
+        """(summed ms, launches) of the CG kernel-A launches since timing was enabled."""
+        ms, n = C.c_double(), _I64()
+        check(lib.yafel_hip_kernel_time(self.h, C.byref(ms), C.byref(n)), self.h)
+        return ms.value, n.value
 
     def copy_bandwidth(self, nbytes=1 << 30, reps=20):
         g = _D()
@@ -378,13 +389,13 @@ def physics(kind, nsd, coef=None, coef_dev=None, f=0.0, lam=0.0, mu=0.0, body=(0
 class DG:
     """DoFManager(M, DG, p, 1) + DGAssembly / DGRK4 on the device (2-D Quad4)."""
 
-    def __init__(self, ctx, p, corner_nodes, cells, ph):
+    def __init__(self, ctx, p, corner_nodes, cells):
         self.ctx, self.p = ctx, p
         h = _P()
         corner_nodes = np.ascontiguousarray(corner_nodes, np.float64)
         cells = np.ascontiguousarray(cells, np.int32)
         check(lib.yafel_hip_dg_create(ctx.h, p, corner_nodes.shape[0], _ptr(corner_nodes), cells.shape[0],
-                                      _ptr(cells), C.byref(ph), C.byref(h)), ctx.h)
+                                      _ptr(cells), C.byref(h)), ctx.h)
         self.h = h
         info = np.zeros(4, np.int64)
         check(lib.yafel_hip_dg_info(h, _ptr(info)), ctx.h)
@@ -411,13 +422,13 @@ class DG:
         check(lib.yafel_hip_dg_faces(self.h, *[_ptr(a) for a in out]), self.ctx.h)
         return tuple(out)
 
-    def residual(self, U_dev, R_dev, Rface_dev=None, time=0.0):
-        check(lib.yafel_hip_dg_residual(self.h, time, U_dev.ptr, R_dev.ptr,
+    def residual(self, ph, U_dev, R_dev, Rface_dev=None, time=0.0):
+        check(lib.yafel_hip_dg_residual(self.h, C.byref(ph), time, U_dev.ptr, R_dev.ptr,
                                         None if Rface_dev is None else Rface_dev.ptr), self.ctx.h)
 
-    def rk4(self, U_dev, dt, nsteps, time=0.0):
+    def rk4(self, ph, U_dev, dt, nsteps, time=0.0):
         t = C.c_double(time)
-        check(lib.yafel_hip_dg_rk4(self.h, dt, nsteps, C.byref(t), U_dev.ptr), self.ctx.h)
+        check(lib.yafel_hip_dg_rk4(self.h, C.byref(ph), dt, nsteps, C.byref(t), U_dev.ptr), self.ctx.h)
         return t.value
 
 
@@ -472,6 +483,13 @@ class Mesh:
         K = self.ctx.alloc(self.nel * nld * nld)
         check(lib.yafel_hip_element_matrices(self.h, C.byref(ph), K.ptr), self.ctx.h)
         return K.download().reshape(self.nel, nld, nld)
+
+    def element_matrices_fp32(self, ph, K_dev=None):
+        """FP32 matrix-core element matrices into K_dev (allocated when None)."""
+        nld = self.nen * self.dpn
+        K = K_dev if K_dev is not None else self.ctx.alloc(self.nel * nld * nld, np.float32)
+        check(lib.yafel_hip_element_matrices_fp32(self.h, C.byref(ph), K.ptr), self.ctx.h)
+        return K
 
     def bc_mask_plane(self, mask_dev, axis, value, tol=1e-6, comp=-1):
         check(lib.yafel_hip_bc_mask_plane(self.h, axis, value, tol, comp, mask_dev.ptr), self.ctx.h)

@@ -56,6 +56,7 @@ class Workload:
         self.b = ctx.alloc(self.n)
         self.x = ctx.alloc(self.n)
         self.req = hip.TANGENT | (hip.RESIDUAL if rhs is None else 0)
+        self.extra_req = 0  # e.g. hip.FP32_ELEMENTS
 
     def assemble(self):
         """CGAssembly + DirichletBC::apply; leaves A and b ready to solve."""
@@ -63,7 +64,7 @@ class Workload:
             self.b.upload(np.zeros(self.n))
         else:
             self.b.copy_from(self.rhs_given)
-        self.mesh.assemble(self.ph, self.A, self.b if self.req & hip.RESIDUAL else None, self.req)
+        self.mesh.assemble(self.ph, self.A, self.b if self.req & hip.RESIDUAL else None, self.req | self.extra_req)
         self.A.dirichlet(self.mask, self.b)
 
     def solve(self, **kw):
