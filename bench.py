@@ -277,6 +277,43 @@ def spmv_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def dg_bench(args):
+    """The DG residual and the graph-replayed RK4 step: upwind advection
+    (apps/BasicAdvection_DG, velocity (-1, 0)) on a 256 x 256 Quad4 mesh, p = 3."""
+    from yafel_amd import hip
+    ctx = hip.Context(0)
+    n, p = 256, 3
+    nodes, cells = hip.structured_mesh(2, n, n, 0, 2)  # gmsh (counter-clockwise) corners
+    dg = hip.DG(ctx, p, nodes, cells)
+    ph = hip.physics(hip.DG_ADVECTION, 2, U=(-1.0, 0.0, 0.0))
+    X = dg.nodes()
+    dx = X[:, 0] - 0.5
+    U = ctx.upload(np.where(np.abs(dx) < 0.25, np.sin(4 * np.pi * dx) ** 2, 0.0))
+    R = ctx.alloc(dg.ndofs)
+    dg.residual(ph, U, R)
+    reps = 50
+    ctx.timer_begin()
+    for _ in range(reps):
+        dg.residual(ph, U, R)
+    t_res = ctx.timer_end() / reps
+    nel, nen, nf, n1 = cells.shape[0], dg.nen, dg.nfaces, p + 1
+    # unique bytes of one evaluation: U, X, element mass inverses, face records,
+    # node lists and geometry, the face flux written and read back, R written
+    byts = (dg.ndofs * 8 + dg.ndofs * 24 + nel * nen * nen * 8 + nf * (16 + 2 * n1 * 4 + n1 * 32)
+            + 2 * nf * n1 * 8 + nel * 16 + dg.ndofs * 8)
+    dt, steps = 1e-4, 20
+    dg.rk4(ph, U, dt, 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    dg.rk4(ph, U, dt, steps)
+    t_rk = (time.perf_counter() - t0) / steps * 1e3
+    out = {"workload": f"DG upwind advection, Quad4 {n}^2, p = {p} ({dg.ndofs} dofs, {nf} faces)",
+           "residual_us": t_res * 1e3, "residual_gbs": byts / (t_res * 1e-3) / 1e9,
+           "residual_bytes": byts, "elements_per_s": nel / (t_res * 1e-3),
+           "rk4_step_ms": t_rk, "rk4_steps_per_s": 1e3 / t_rk}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,7 +322,7 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="cells per axis (64 = the benchmark config)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "serve", "c3", "spmv"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "serve", "c3", "spmv", "dg"])
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--sweep-requests", type=int, default=300)
     ap.add_argument("--rates", default="50,200,500,1000,2000")
@@ -305,6 +342,9 @@ def main():
         return
     if args.workload == "spmv":
         spmv_bench(args)
+        return
+    if args.workload == "dg":
+        dg_bench(args)
         return
 
     from yafel_amd import hip, workloads
