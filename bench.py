@@ -83,22 +83,26 @@ def cpu_baseline(w, fp64_cg_iters, hip):
     }
 
 
-KERNEL_A = "k_spmv_fused<3, 4, float, OpCgA<float>>"
+KERNEL_A = "k_spmv_fused<3, 4, float, OpCgQ<float>>"
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "c2_cg_kernel_a_hbm_traffic.md")
 
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprof counter
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on the copy kernel), or None."""
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on the copy kernel), or None
+    when the summary covers another kernel."""
+    found, value = None, None
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
+                if line.startswith("traffic_kernel:"):
+                    found = line.split(":", 1)[1].strip()
                 if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                    value = float(line.split(":", 1)[1])
     except OSError:
-        pass
-    return None
+        return None
+    return value if found == kernel else None
 
 
 def run_stream(hip, ctx, mats, arrivals, budget, slots, setup_threads=0):
@@ -387,9 +391,9 @@ def main():
             log(f"fp64 {name}: {fp64[name]}")
         ctx.sync()
 
-    # roofline of the dominant kernel: the CG iteration's fused FP32 SpMV
-    # (kernel A: p = z + beta p rebuilt in the gather, q = A p, p.q), every
-    # launch of one more step timed with HIP events on the context stream
+    # roofline of the dominant kernel: the CG iteration's FP32 SpMV (q = A p
+    # and the p.q partials), every launch of one more step timed with HIP
+    # events on the context stream
     A = w.A
     inf = A.info()
     nbr = inf.rows // 3
@@ -398,7 +402,7 @@ def main():
     ka_ms, ka_n = ctx.kernel_time()
     ctx.kernel_timing(False)
     ka_us = ka_ms / max(ka_n, 1) * 1e3
-    ka_bytes = bcsr_bytes(inf.nblocks, nbr, w.n, 4) + 2 * w.n * 4  # + z gather and p write
+    ka_bytes = bcsr_bytes(inf.nblocks, nbr, w.n, 4)  # matrix, p gathered, q written
     ka_gbs = ka_bytes / (ka_us * 1e-6) / 1e9
     log(f"kernel A: {ka_n} launches, {ka_us:.1f} us each, {ka_gbs:.0f} GB/s")
     x32 = ctx.upload(np.random.default_rng(0).standard_normal(w.n).astype(np.float32))

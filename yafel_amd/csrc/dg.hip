@@ -13,7 +13,7 @@
 //
 // One residual evaluation is two kernels:
 //   face kernel, a thread per (face, face point): upwind or outflow flux;
-//   element kernel, a thread per element: the face terms of its own dofs in
+//   element kernel, a lane per (element, dof): the face terms of the dof in
 //   face order, the volume term, then R = R_face + M^-1 (R_face + R_vol) --
 //   the reference seeds the element's local residual with the face terms and
 //   adds its M^-1 solve on top of them.
@@ -52,7 +52,7 @@ struct yafel_hip_dg_s {
   DBuf lnodes, rnodes;    // int32 [nfaces][n1] matched local node lists (-1: no side)
   DBuf geo;               // double [nfaces][n1][4]: unit normal (x, y), jxw left list, jxw right list
   DBuf ef;                // int32 [nel][4]: (face << 1) | side (0 left, 1 right)
-  DBuf minv;              // double [nel][nen][nen]
+  DBuf minv;              // double [nel][nen][nen], transposed: minv[e][j][i] = (M_e^-1)_ij
   DBuf flux;              // double [nfaces][n1] scratch of one evaluation
   DBuf rk;                // U0, k1..k4, W (6 * ndofs)
   DBuf tdev;              // device time
@@ -303,8 +303,10 @@ __global__ __launch_bounds__(64) void k_dg_minv(int64_t nel, int nqp, const doub
     }
     __syncthreads();
   }
+  // stored transposed: lane i of the residual kernel reads (M^-1)_ij at
+  // minv[e][j][i], so the lanes of one element read consecutive addresses
   if (t < NEN)
-    for (int j = 0; j < NEN; ++j) minv[(e * NEN + t) * NEN + j] = A[t][NEN + j];
+    for (int j = 0; j < NEN; ++j) minv[(e * NEN + j) * NEN + t] = A[t][NEN + j];
 }
 
 // ------------------------------------------------------------ residual
@@ -339,6 +341,12 @@ __global__ void k_dg_flux(int64_t nf, int n1, int nen, const DGFaceRec *__restri
   flux[i] = fl;
 }
 
+// Element pass, a lane per (element, dof): the 64 / nen elements of a
+// wavefront are handled by nen lanes each (p = 3: 4 elements of 16 dofs).
+// Lane (element, q) first forms the geometry and u at volume point q into
+// LDS; lane (element, a) then adds the face terms of dof a in face order
+// (DGAssembly.hpp:106-189), the volume term (LocalResidual, main.cpp:53-71)
+// and row a of M^-1 (R_face + R_vol).
 template <int P>
 __global__ __launch_bounds__(NT) void k_dg_elem(int64_t nel, const DGFaceRec *__restrict__ F,
                                                const int32_t *__restrict__ L, const int32_t *__restrict__ R,
@@ -346,74 +354,114 @@ __global__ __launch_bounds__(NT) void k_dg_elem(int64_t nel, const DGFaceRec *__
                                                const double *__restrict__ flux, const double *__restrict__ bN,
                                                const double *__restrict__ X, const double *__restrict__ N,
                                                const double *__restrict__ dN, const double *__restrict__ w,
-                                               const double *__restrict__ minv, double c0, double c1, double src,
+                                               const double *__restrict__ minvT, double c0, double c1, double src,
                                                const double *__restrict__ U, double *__restrict__ Rout,
                                                double *__restrict__ Rface) {
-  constexpr int n1 = P + 1, nen = n1 * n1, nqp = nen;
-  const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
-  if (e >= nel) return;
-  double loc[nen];
-#pragma unroll
-  for (int a = 0; a < nen; ++a) loc[a] = 0;
-  // face terms of this element's dofs in face order (DGAssembly.hpp:106-189)
-  int fe[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) fe[k] = ef[4 * e + k];
-  for (int k = 1; k < 4; ++k)
-    for (int j = k; j > 0 && (fe[j - 1] >> 1) > (fe[j] >> 1); --j) {
-      const int t = fe[j];
-      fe[j] = fe[j - 1];
-      fe[j - 1] = t;
-    }
-  for (int k = 0; k < 4; ++k) {
-    if (fe[k] < 0) continue;  // a face merged away by a non-conforming mesh
-    const int64_t f = fe[k] >> 1;
-    const int side = fe[k] & 1;
-    const DGFaceRec r = F[f];
-    const bool interior = r.left >= 0 && r.right >= 0;
-    const int32_t *nodes = side == 0 ? &L[f * n1] : &R[f * n1];
-    for (int fq = 0; fq < n1; ++fq) {
-      const int64_t gi = f * n1 + fq;
-      const double fl = flux[gi];
-      const double jw = side == 0 ? geo[4 * gi + 2] : geo[4 * gi + 3];
-      for (int i = 0; i < n1; ++i) {
-        const int a = nodes[i];
-        const double v = bN[fq * n1 + i] * fl * jw;
-#pragma unroll
-        for (int b = 0; b < nen; ++b)
-          if (b == a) loc[b] = (interior && side == 1) ? loc[b] + v : loc[b] - v;
-      }
-    }
+  constexpr int n1 = P + 1, nen = n1 * n1, nqp = nen, EPW = 64 / nen, WPB = NT / 64;
+  constexpr int LD = nen + 1;  // padded rows: lanes reading a row or a column hit distinct banks
+  __shared__ double sN[nqp][LD];
+  __shared__ double sdN[nqp][LD][2];
+  __shared__ double sU[WPB][EPW][nen];
+  __shared__ double sX[WPB][EPW][nen][2];
+  __shared__ double sG[WPB][EPW][nqp][6];  // jxw, J^-T (4), u at each volume point
+  __shared__ double sL[WPB][EPW][nen];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int el = l / nen, a = l - el * nen;
+  const int64_t e = ((int64_t)blockIdx.x * WPB + wv) * EPW + el;
+  const bool act = el < EPW && e < nel;
+  for (int t = threadIdx.x; t < nqp * nen; t += NT) {
+    const int q = t / nen, A = t - q * nen;
+    sN[q][A] = N[t];
+    sdN[q][A][0] = dN[2 * t];
+    sdN[q][A][1] = dN[2 * t + 1];
   }
-  double rf[nen];
+  // row a of M^-1, loaded early so its latency hides behind the face and
+  // volume work
+  double mrow[nen];
+  if (act) {
+    const double *Mt = &minvT[e * nen * nen];
 #pragma unroll
-  for (int a = 0; a < nen; ++a) rf[a] = loc[a];
-  // volume term (LocalResidual, main.cpp:53-71)
-  double ue[nen];
+    for (int j = 0; j < nen; ++j) mrow[j] = Mt[j * nen + a];
+    sU[wv][el][a] = U[e * nen + a];
+    sX[wv][el][a][0] = X[3 * (e * nen + a)];
+    sX[wv][el][a][1] = X[3 * (e * nen + a) + 1];
+  }
+  __syncthreads();
+  if (act) {
+    const int q = a;
+    double J[4] = {0, 0, 0, 0};
 #pragma unroll
-  for (int a = 0; a < nen; ++a) ue[a] = U[e * nen + a];
-  for (int q = 0; q < nqp; ++q) {
-    double jxw, Ji[4];
-    vol_geom(nen, e, q, X, dN, w, jxw, Ji);
+    for (int A = 0; A < nen; ++A) {
+      const double x0 = sX[wv][el][A][0], x1 = sX[wv][el][A][1];
+      const double g0 = sdN[q][A][0], g1 = sdN[q][A][1];
+      J[0] += x0 * g0;
+      J[1] += x0 * g1;
+      J[2] += x1 * g0;
+      J[3] += x1 * g1;
+    }
+    const double det = J[0] * J[3] - J[1] * J[2];
     double u = 0;
 #pragma unroll
-    for (int a = 0; a < nen; ++a) u += ue[a] * N[q * nen + a];
-#pragma unroll
-    for (int i = 0; i < nen; ++i) {
-      const double g0 = dN[(q * nen + i) * 2], g1 = dN[(q * nen + i) * 2 + 1];
-      const double gx = Ji[0] * g0 + Ji[2] * g1, gy = Ji[1] * g0 + Ji[3] * g1;
-      loc[i] += (u * (gx * c0 + gy * c1) + src * N[q * nen + i]) * jxw;
-    }
+    for (int A = 0; A < nen; ++A) u += sU[wv][el][A] * sN[q][A];
+    double *g = sG[wv][el][q];
+    g[0] = det * w[q];
+    g[1] = J[3] / det;
+    g[2] = -J[1] / det;
+    g[3] = -J[2] / det;
+    g[4] = J[0] / det;
+    g[5] = u;
   }
-  // R = R_face + M^-1 (R_face + R_vol)
-  const double *Mi = &minv[e * nen * nen];
+  __syncthreads();
+  double loc = 0, rf = 0;
+  if (act) {
+    // face terms of dof a, faces in the reference's order
+    int fe[4];
 #pragma unroll
-  for (int i = 0; i < nen; ++i) {
+    for (int k = 0; k < 4; ++k) fe[k] = ef[4 * e + k];
+    for (int k = 1; k < 4; ++k)
+      for (int j = k; j > 0 && (fe[j - 1] >> 1) > (fe[j] >> 1); --j) {
+        const int t = fe[j];
+        fe[j] = fe[j - 1];
+        fe[j - 1] = t;
+      }
+    for (int k = 0; k < 4; ++k) {
+      if (fe[k] < 0) continue;  // a face merged away by a non-conforming mesh
+      const int64_t f = fe[k] >> 1;
+      const int side = fe[k] & 1;
+      const int32_t *nodes = side == 0 ? &L[f * n1] : &R[f * n1];
+      int i = -1;
+#pragma unroll
+      for (int ii = 0; ii < n1; ++ii)
+        if (nodes[ii] == a) i = ii;
+      if (i < 0) continue;
+      const DGFaceRec r = F[f];
+      const bool plus = r.left >= 0 && r.right >= 0 && side == 1;
+#pragma unroll
+      for (int fq = 0; fq < n1; ++fq) {
+        const int64_t gi = f * n1 + fq;
+        const double v = bN[fq * n1 + i] * flux[gi] * geo[4 * gi + 2 + side];
+        loc = plus ? loc + v : loc - v;
+      }
+    }
+    rf = loc;
+    // volume term
+#pragma unroll 4
+    for (int q = 0; q < nqp; ++q) {
+      const double *g = sG[wv][el][q];
+      const double g0 = sdN[q][a][0], g1 = sdN[q][a][1];
+      const double gx = g[1] * g0 + g[3] * g1, gy = g[2] * g0 + g[4] * g1;
+      loc += (g[5] * (gx * c0 + gy * c1) + src * sN[q][a]) * g[0];
+    }
+    sL[wv][el][a] = loc;
+  }
+  __syncthreads();
+  if (act) {
+    // R = R_face + M^-1 (R_face + R_vol), row a
     double s = 0;
 #pragma unroll
-    for (int j = 0; j < nen; ++j) s += Mi[i * nen + j] * loc[j];
-    Rout[e * nen + i] = rf[i] + s;
-    if (Rface) Rface[e * nen + i] = rf[i];
+    for (int j = 0; j < nen; ++j) s += mrow[j] * sL[wv][el][j];
+    Rout[e * nen + a] = rf + s;
+    if (Rface) Rface[e * nen + a] = rf;
   }
 }
 
@@ -455,9 +503,8 @@ void dg_eval(yafel_hip_dg_s *D, const DGPhys &ph, const double *U, double *R, do
                                              D->lnodes.as<int32_t>(), D->rnodes.as<int32_t>(), D->bN.as<double>(),
                                              D->geo.as<double>(), ph.c0, ph.c1, U, D->flux.as<double>());
   HIPCHK(hipGetLastError());
-  const int g = ceil_div(D->nel, NT);
 #define DG_ELEM(P)                                                                                               \
-  k_dg_elem<P><<<g, NT, 0, s>>>(D->nel, D->faces.as<DGFaceRec>(), D->lnodes.as<int32_t>(), D->rnodes.as<int32_t>(), \
+  k_dg_elem<P><<<ceil_div(D->nel, (NT / 64) * (64 / ((P + 1) * (P + 1)))), NT, 0, s>>>(D->nel, D->faces.as<DGFaceRec>(), D->lnodes.as<int32_t>(), D->rnodes.as<int32_t>(), \
                                 D->ef.as<int32_t>(), D->geo.as<double>(), D->flux.as<double>(), D->bN.as<double>(), \
                                 D->X.as<double>(), D->tabN.as<double>(), D->tabdN.as<double>(), D->tabw.as<double>(), \
                                 D->minv.as<double>(), ph.c0, ph.c1, ph.src, U, R, Rface)

@@ -2,17 +2,18 @@
 // in FP64 or with FP32 matrix streams inside FP64 refinement.
 //
 // Every scalar recurrence (alpha, beta, omega, rho, the stopping test) runs
-// on the device in FP64, in a one-workgroup kernel that folds the producing
-// kernel's per-workgroup partials in a fixed order (deterministic), so a solve
-// needs no host round trip per iteration.  The host launches chunks of
-// iterations and polls a status word once per chunk; kernels after
-// convergence return immediately.
+// on the device in FP64 from per-workgroup partials folded in a fixed order
+// (deterministic), so a solve needs no host round trip per iteration.  The
+// host launches chunks of iterations and polls a status word once per chunk;
+// kernels after convergence return immediately.
 //
-// One CG iteration is two kernels (each followed by its fold):
-//   A: p_new = z + beta p_old (reconstructed in the SpMV gather, written by the
-//      row owner), q = A p_new, partial p_new.q  -> alpha
-//   B: x += alpha p_new, r -= alpha q, z = D^-1 r, partials r.z, r.r
-//      -> beta, stopping test
+// One CG iteration is three kernels and no separate fold:
+//   Q: q = A p, partials p.q
+//   U: alpha = rho / p.q folded by every workgroup; x += alpha p,
+//      r -= alpha q, z = D^-1 r, partials r.z, r.r
+//   P: beta and the stopping test folded by every workgroup; p = z + beta p
+// BiCGSTAB folds each reduction in a one-workgroup kernel launched right
+// after its producer.
 // MIXED: vectors and matrix are FP32 (row sums and dots accumulate in FP64);
 // MIXED_MATRIX: only the matrix is streamed in FP32, vectors stay FP64.  In
 // both, when the recurrence residual has dropped by refine_delta (or claims
@@ -59,6 +60,21 @@ void launch_spmv_op(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
   }
 }
 
+// The CG iteration's SpMV: partials only, folded by the next kernel.
+template <class T, class Op>
+int launch_spmv_nofold(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
+  const int grid = sell_grid(A->nsl, A->ctx->num_cus * 8);
+  const SellView<T> v = view<T>(A);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
+  if (e0) HIPCHK(hipEventRecord(e0, s));
+  if (A->format == YAFEL_HIP_CSR) k_spmv_fused<YAFEL_HIP_CSR, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
+  else k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
+  HIPCHK(hipGetLastError());
+  if (e1) HIPCHK(hipEventRecord(e1, s));
+  return grid;
+}
+
 template <class Op>
 void launch_elem(yafel_hip_ctx_s *ctx, int64_t n, const Op &op, hipStream_t s) {
   const int grid = grid_for(ctx, n, NT);
@@ -96,19 +112,44 @@ struct Solver {
   yafel_hip_solve_opts o;
   int64_t n;
   bool prec, mixed, cgfam;
-  KState *st;
+  KState *st;        // two versions: st[0], st[1]
+  int64_t ver = 0;   // st[ver & 1] is current in stream order
   double *part;
-  unsigned int *cnt;
   const double *b;
   double *x;
   hipStream_t s;
 
+  KState *cur() const { return st + (ver & 1); }
+  // partial regions: the CG SpMV's at 0, the update pass's at 2 * cap
+  int cap() const { return ctx->num_cus * 8; }
+
   template <class O>
   O base(O op, int counter) const {
-    op.st = st;
+    op.st = cur();
     op.part = part;
-    op.counter = cnt + counter;
+    op.counter = cur()->counter + counter;
     return op;
+  }
+
+  // launch a versioned elementwise pass: reads cur(), publishes the next
+  // version; returns its workgroup count (its partials' stride)
+  template <class O>
+  int vlaunch(O op, const double *pin, int nin, double *ppart) {
+    op.in = cur();
+    op.out = st + ((ver + 1) & 1);
+    op.pin = pin;
+    op.nin = nin;
+    op.part = ppart;
+    // VE elements per thread; a multiple of 8 workgroups keeps the XCD-aware
+    // block order, and rounding up keeps each workgroup's chunk <= VE * NT
+    int64_t g = (n + (int64_t)NT * VE - 1) / ((int64_t)NT * VE);
+    if (g >= 8) g = (g + 7) / 8 * 8;
+    if (g > cap()) g = cap();
+    if (g < 1) g = 1;
+    k_velem<O><<<(int)g, NT, 0, s>>>(n, op);
+    HIPCHK(hipGetLastError());
+    ++ver;
+    return (int)g;
   }
 
   template <class TL>
@@ -137,28 +178,37 @@ struct Solver {
   // TM: stored matrix type of the inner SpMV; T: inner vector type
   template <class TM, class T>
   void cg_iteration(const Vecs &v) {
-    OpCgA<T> a{};
+    OpCgQ<T> a{};
     a = base(a, 1);
-    a.z = (const T *)(prec ? v.z : v.r);
-    a.pbuf0 = (T *)v.p0;
-    a.pbuf1 = (T *)v.p1;
+    a.p = (const T *)v.p0;
     a.q = (T *)v.q;
-    launch_spmv_op<TM>(A, a, s);
+    const int ga = launch_spmv_nofold<TM>(A, a, s);
     const T *dinv = (const T *)(sizeof(T) == 8 ? A->dinv64.p : A->dinv32.p);
     T *xx = mixed ? (T *)v.xl : (T *)x;
+    double *pu = part + 2 * (size_t)cap();
+    int gu;
     if (prec) {
-      OpCgB<T, true> bb{};
-      bb = base(bb, 2);
-      bb.x = xx; bb.r = (T *)v.r; bb.z = (T *)v.z; bb.pbuf0 = (const T *)v.p0; bb.pbuf1 = (const T *)v.p1;
-      bb.q = (const T *)v.q; bb.dinv = dinv;
-      launch_elem(ctx, n, bb, s);
+      OpCgU<T, true> u{};
+      u.x = xx; u.r = (T *)v.r; u.z = (T *)v.z; u.p = (const T *)v.p0; u.q = (const T *)v.q; u.dinv = dinv;
+      gu = vlaunch(u, part, ga, pu);
     } else {
-      OpCgB<T, false> bb{};
-      bb = base(bb, 2);
-      bb.x = xx; bb.r = (T *)v.r; bb.z = (T *)v.z; bb.pbuf0 = (const T *)v.p0; bb.pbuf1 = (const T *)v.p1;
-      bb.q = (const T *)v.q; bb.dinv = dinv;
-      launch_elem(ctx, n, bb, s);
+      OpCgU<T, false> u{};
+      u.x = xx; u.r = (T *)v.r; u.z = (T *)v.z; u.p = (const T *)v.p0; u.q = (const T *)v.q; u.dinv = dinv;
+      gu = vlaunch(u, part, ga, pu);
     }
+    OpCgP<T> pp{};
+    pp.p = (T *)v.p0;
+    pp.z = (const T *)(prec ? v.z : v.r);
+    vlaunch(pp, pu, gu, nullptr);
+  }
+
+  // p = z + beta p after the initial residual or a refinement
+  template <class T>
+  void cg_restart(const Vecs &v) {
+    OpCgP0<T> p0{};
+    p0.p = (T *)v.p0;
+    p0.z = (const T *)(prec ? v.z : v.r);
+    vlaunch(p0, nullptr, 0, nullptr);
   }
 
   template <class TM, class T>
@@ -213,6 +263,7 @@ struct Solver {
       launch_elem(ctx, n, f, s);
     }
     residual<T>(1, v);
+    if (cgfam) cg_restart<T>(v);
   }
 
   template <class TM, class T>
@@ -243,14 +294,17 @@ struct Solver {
     h.method = o.method;
     h.mixed = mixed ? 1 : 0;
     std::memcpy(ctx->pinned, &h, sizeof(KState));
-    HIPCHK(hipMemcpyAsync(st, ctx->pinned, sizeof(KState), hipMemcpyHostToDevice, s));
+    ver = 0;
+    HIPCHK(hipMemcpyAsync(cur(), ctx->pinned, sizeof(KState), hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
     residual<T>(0, v);
     if (!cgfam) HIPCHK(hipMemcpyAsync(v.rhat, v.r, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    if (cgfam) cg_restart<T>(v);
     const int K = o.check_every > 0 ? o.check_every : 16;
     const bool with_refine = mixed || !cgfam;
     char *pin = (char *)ctx->pinned;
     const size_t slot = 256;
+    static_assert(sizeof(KState) <= slot && 2 * sizeof(KState) <= 4096, "KState slots");
     hipEvent_t ev[2];
     HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
@@ -263,7 +317,7 @@ struct Solver {
         }
       }
       if (with_refine) refine<T>(v);
-      HIPCHK(hipMemcpyAsync(pin + slot * (1 + slot_i), st, sizeof(KState), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(pin + slot * (1 + slot_i), cur(), sizeof(KState), hipMemcpyDeviceToHost, s));
       HIPCHK(hipEventRecord(ev[slot_i], s));
     };
     chunk(0);
@@ -288,7 +342,7 @@ struct Solver {
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     KState fs;
-    HIPCHK(hipMemcpy(&fs, st, sizeof(KState), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&fs, cur(), sizeof(KState), hipMemcpyDeviceToHost));
     if (fs.special == 1) HIPCHK(hipMemcpyAsync(x, b, n * 8, hipMemcpyDeviceToDevice, s));
     if (fs.special == 2) HIPCHK(hipMemsetAsync(x, 0, n * 8, s));
     // report the FP64 true residual
@@ -296,7 +350,7 @@ struct Solver {
     rep = base(rep, 7);
     rep.xh = x;
     rep.b = b;
-    rep.out = part + 4 * (size_t)ctx->num_cus * 8;  // past the NP x grid partials
+    rep.out = part + 4 * (size_t)cap();  // past the NP x grid partials
     launch_spmv_op<double>(A, rep, s);
     double rb[2];
     HIPCHK(hipMemcpyAsync(rb, rep.out, 16, hipMemcpyDeviceToHost, s));
@@ -360,7 +414,6 @@ int yafel_hip_solve(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const 
   if (S.mixed && !A->has32) matrix_build_fp32(A);
   ensure_ws(ctx, n);
   S.st = ctx->ws.state.as<KState>();
-  S.cnt = S.st->counter;
   S.part = ctx->ws.partials.as<double>();
   S.b = b_dev;
   S.x = x_dev;

@@ -27,7 +27,8 @@ struct KState {
   long long iter, max_iter, refinements;
   int done, status, paused, first;
   int method, stop, mixed, special;
-  int pidx, pad0;
+  int pidx;       // serving CG: which p buffer holds the current p
+  int pupd;       // single-solve CG: p = z + beta p pending (after a residual)
   unsigned int counter[8];
 };
 
@@ -339,6 +340,7 @@ struct OpResidual : OpBase {
       st->beta = 0.0;
       st->paused = 0;
       st->pidx = 0;
+      st->pupd = cgfam ? 1 : 0;
       // early exits of the reference
       if (cgfam && st->stop == YAFEL_HIP_STOP_REFERENCE && s[0] == 0.0) {
         st->done = 1;  // cg_solve.hpp:52-54 / pcg_solve.hpp:148-150: x0 solves it
@@ -382,6 +384,7 @@ struct OpResidual : OpBase {
     if (cgfam) {
       st->beta = s[0] / st->rho;
       st->rho = s[0];
+      st->pupd = 1;
     } else {
       // rho_prev holds rho of the finished iteration's predecessor
       st->beta = (s[0] / st->rho_prev) * (st->alpha / st->omega);
@@ -546,6 +549,220 @@ struct OpReport : OpBase {
   }
 };
 
+// --------------------------------------------- CG with folds in the consumer
+// The single-solve CG runs three kernels per iteration and no one-workgroup
+// fold:
+//   Q: q = A p, partial p.q                                (sliced SpMV)
+//   U: alpha from the p.q partials; x += alpha p, r -= alpha q, z = D^-1 r,
+//      partials r.z, r.r
+//   P: beta and the stopping test from the r.z, r.r partials; p = z + beta p
+// Every workgroup of U and P folds the producer's partials itself, in one
+// fixed order, so all workgroups derive bit-identical scalars and take the
+// same branch.  The state is versioned: U and P read KState `in` and their
+// workgroup 0 writes the updated copy to `out` (the other slot of a pair), so
+// no workgroup reads a field that another workgroup of the same kernel
+// writes.  The kernel boundary makes `out` and the partials visible to the
+// next kernel.
+
+// Sum of each of NPIN partial arrays part[j * nblk + i], i < nblk, in a fixed
+// order; the result is the same in every thread of every workgroup.
+template <int NPIN>
+__device__ __forceinline__ void refold(const double *__restrict__ part, int nblk, double (&s)[NPIN]) {
+  __shared__ double lds[NPIN][NT / 64];
+  double v[NPIN];
+#pragma unroll
+  for (int j = 0; j < NPIN; ++j) v[j] = 0;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nblk; i += NT)
+#pragma unroll
+    for (int j = 0; j < NPIN; ++j) v[j] += part[(size_t)j * nblk + i];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < NPIN; ++j) {
+    const double t = dev::wave_sum(v[j]);
+    if (l == 0) lds[j][w] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NPIN; ++j) {
+    double t = 0;
+#pragma unroll
+    for (int q = 0; q < NT / 64; ++q) t += lds[j][q];
+    s[j] = t;
+  }
+}
+
+// Versioned elementwise pass: fold the producer's NPIN partials, let the op
+// update the state (decide), publish it from workgroup 0, then finish the
+// elementwise body with up to NP partial sums of its own.  Each thread loads
+// its first VE elements' inputs before the fold, so their latency overlaps
+// the fold's; the grid is sized (velem_grid) so that VE elements per thread
+// cover the vector.  Thread 0 of each workgroup keeps the state in LDS and
+// runs decide(); the other threads read the outcome from there (no
+// per-thread copy of the state, which would land in scratch memory).
+constexpr int VE = 4;
+
+// the state moves between global memory and LDS as 8-byte words, a word per
+// thread
+constexpr int KSW = (int)(sizeof(KState) / 8);
+static_assert(sizeof(KState) % 8 == 0 && KSW <= NT, "KState words");
+
+template <class Op>
+__global__ __launch_bounds__(NT) void k_velem(int64_t n, Op op0) {
+  Op op = op0;
+  __shared__ KState ss;
+  __shared__ int sgo;
+  const int t = threadIdx.x;
+  const bool b0 = blockIdx.x == 0;
+  const int blk = dev::xcd_block(), nblk = gridDim.x;
+  const int64_t chunk = ((n + nblk - 1) / nblk + 63) & ~(int64_t)63;
+  const int64_t i0 = (int64_t)blk * chunk + t;
+  const int64_t i1 = (int64_t)blk * chunk + chunk < n ? (int64_t)blk * chunk + chunk : n;
+  typename Op::In v[VE];
+#pragma unroll
+  for (int k = 0; k < VE; ++k)
+    if (i0 + k * NT < i1) op.load(i0 + k * NT, v[k]);
+  if (t < KSW) reinterpret_cast<long long *>(&ss)[t] = reinterpret_cast<const long long *>(op.in)[t];
+  __syncthreads();
+  if (!op.active(ss)) {
+    if (b0 && t < KSW) reinterpret_cast<long long *>(op.out)[t] = reinterpret_cast<const long long *>(&ss)[t];
+    return;
+  }
+  constexpr int NPIN = Op::NPIN;
+  double f[NPIN > 0 ? NPIN : 1];
+  if constexpr (NPIN > 0) refold<NPIN>(op.pin, op.nin, f);
+  __syncthreads();  // every wave has read ss before thread 0 changes it
+  if (t == 0) sgo = op.decide(ss, f) ? 1 : 0;
+  __syncthreads();
+  if (b0 && t < KSW) reinterpret_cast<long long *>(op.out)[t] = reinterpret_cast<const long long *>(&ss)[t];
+  if (!sgo) return;
+  op.begin(ss);
+  constexpr int NP = Op::NP;
+  double acc[NP > 0 ? NP : 1];
+#pragma unroll
+  for (int j = 0; j < (NP > 0 ? NP : 1); ++j) acc[j] = 0;
+#pragma unroll
+  for (int k = 0; k < VE; ++k)
+    if (i0 + k * NT < i1) op.compute(i0 + k * NT, v[k], acc);
+  for (int64_t i = i0 + VE * NT; i < i1; i += NT) {
+    typename Op::In tmp;
+    op.load(i, tmp);
+    op.compute(i, tmp, acc);
+  }
+  finish_partials<NP>(op, acc, blk, nblk);
+}
+
+struct VOpBase {
+  const KState *in;
+  KState *out;
+  const double *pin;  // the producer's partials
+  int nin;            // the producer's workgroup count
+  double *part;       // this pass's partials
+  static __device__ bool running(const KState &s) { return !s.done && !s.paused; }
+};
+
+// Q: q = A p and the p.q partials; reads the state, never writes it.
+template <class T>
+struct OpCgQ : OpBase {
+  using value_type = T;
+  static constexpr bool kTimed = true;  // the CG iteration's SpMV kernel
+  static constexpr int NP = 1;
+  const T *p;
+  T *q;
+  __device__ bool skip() const { return !running(); }
+  __device__ void begin() {}
+  __device__ T x(int64_t c) const { return p[c]; }
+  __device__ void row(int64_t i, T y, double *acc) const {
+    q[i] = y;
+    acc[0] += (double)p[i] * (double)y;
+  }
+  __device__ void last(const double *) const {}
+};
+
+// U: alpha = rho / p.q; x += alpha p, r -= alpha q, z = D^-1 r; r.z, r.r.
+template <class T, bool PREC>
+struct OpCgU : VOpBase {
+  static constexpr int NPIN = 1, NP = 2;
+  T *x, *r, *z;
+  const T *p, *q, *dinv;
+  T alpha;
+  struct In {
+    T p, q, r, x, d;
+  };
+  __device__ bool active(const KState &s) const { return running(s); }
+  __device__ bool decide(KState &s, const double *f) const {
+    const double a = s.rho / f[0];
+    if (f[0] == 0.0 || bad(a)) {
+      s.status = YAFEL_HIP_STATUS_BREAKDOWN;
+      s.done = 1;
+      return false;
+    }
+    s.alpha = a;
+    s.first = 0;
+    return true;
+  }
+  __device__ void begin(const KState &s) { alpha = (T)s.alpha; }
+  __device__ void load(int64_t i, In &v) const {
+    v.p = p[i];
+    v.q = q[i];
+    v.r = r[i];
+    v.x = x[i];
+    if (PREC) v.d = dinv[i];
+  }
+  __device__ void compute(int64_t i, const In &v, double *acc) const {
+    x[i] = fma(alpha, v.p, v.x);
+    const T ri = fma(-alpha, v.q, v.r);
+    r[i] = ri;
+    if (PREC) {
+      const T zi = v.d * ri;
+      z[i] = zi;
+      acc[0] += (double)ri * (double)zi;
+    } else {
+      acc[0] += (double)ri * (double)ri;
+    }
+    acc[1] += (double)ri * (double)ri;
+  }
+};
+
+// p = z + beta p: the body shared by P and P0.
+template <class T>
+struct OpPUpdate : VOpBase {
+  T *p;
+  const T *z;
+  T beta;
+  struct In {
+    T z, p;
+  };
+  __device__ void begin(const KState &s) { beta = (T)s.beta; }
+  __device__ void load(int64_t i, In &v) const {
+    v.z = z[i];
+    v.p = p[i];
+  }
+  __device__ void compute(int64_t i, const In &v, double *) const { p[i] = fma(beta, v.p, v.z); }
+};
+
+// P: the stopping test and beta from r.z, r.r; p = z + beta p unless the
+// solve stopped or paused for a refinement.
+template <class T>
+struct OpCgP : OpPUpdate<T> {
+  static constexpr int NPIN = 2, NP = 0;
+  __device__ bool active(const KState &s) const { return VOpBase::running(s); }
+  __device__ bool decide(KState &s, const double *f) const {
+    after_iteration(&s, f[0], f[1]);
+    return VOpBase::running(s);
+  }
+};
+
+// p = z + beta p after the initial or a refinement residual (KState::pupd).
+template <class T>
+struct OpCgP0 : OpPUpdate<T> {
+  static constexpr int NPIN = 0, NP = 0;
+  __device__ bool active(const KState &s) const { return VOpBase::running(s) && s.pupd; }
+  __device__ bool decide(KState &s, const double *) const {
+    s.pupd = 0;
+    return true;
+  }
+};
 
 }  // namespace kr
 }  // namespace yh
