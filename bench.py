@@ -281,6 +281,68 @@ def spmv_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def c5_bench(args):
+    """c5: SUPG advection-diffusion, Q2 512^2 (cell Peclet 100), 64 seeded
+    N(0,1) right-hand sides (zero on the Dirichlet edges), right-Jacobi
+    BiCGSTAB to 1e-8: all 64 together through the multi-RHS path (one SpMM per
+    product) against one right-hand side at a time (a bounded sample of 8,
+    scaled to 64)."""
+    from yafel_amd import hip, workloads
+    ctx = hip.Context(0)
+    w = workloads.c5(ctx)
+    w.assemble()
+    A, n, k = w.A, w.n, w.nrhs
+    A.sync_fp32()
+    mask = w.mask.download().astype(bool)
+    B = w.B_host.copy()
+    B[mask, :] = 0.0
+    Bd, Xd = ctx.upload(B.ravel()), ctx.alloc(n * k)
+    out = {"workload": f"c5: SUPG Q2 512^2 ({n} dofs), {k} right-hand sides, right-Jacobi BiCGSTAB to 1e-8"}
+    for name, prec in (("mixed", hip.MIXED), ("fp64", hip.FP64)):
+        kw = dict(method=hip.BICGSTAB_JACOBI, precision=prec, stop=hip.STOP_RELRES, tol=TOL)
+        Xd.upload(np.zeros(n * k))
+        A.solve_multi(Bd, Xd, k, **kw)  # warm-up
+        Xd.upload(np.zeros(n * k))
+        ctx.sync()
+        t0 = time.perf_counter()
+        infos = A.solve_multi(Bd, Xd, k, **kw)
+        ctx.sync()
+        t_multi = time.perf_counter() - t0
+        bad = [j for j, i in enumerate(infos) if i.status != 0 or i.rel_residual >= TOL]
+        # one right-hand side at a time: 8 of the 64
+        b, x = ctx.alloc(n), ctx.alloc(n)
+        its, t_one = [], 0.0
+        for j in range(8):
+            b.upload(B[:, j])
+            x.upload(np.zeros(n))
+            ctx.sync()
+            t0 = time.perf_counter()
+            info = A.solve(b, x, **kw)
+            ctx.sync()
+            t_one += time.perf_counter() - t0
+            its.append(info.iterations)
+        out[name] = {"multi_s": t_multi, "multi_rhs_per_s": k / t_multi,
+                     "multi_iterations_max": max(i.iterations for i in infos),
+                     "multi_iterations_mean": float(np.mean([i.iterations for i in infos])),
+                     "failed": len(bad),
+                     "one_at_a_time_rhs_per_s": 8 / t_one, "one_at_a_time_iterations_mean": float(np.mean(its)),
+                     "speedup_multi_vs_one": (k / t_multi) / (8 / t_one)}
+        log(f"c5 {name}: {out[name]}")
+    # the SpMM of the multi-RHS products alone (FP32, k = 64)
+    inf = A.info()
+    X32 = ctx.upload(np.random.default_rng(0).standard_normal(n * k).astype(np.float32))
+    Y32 = ctx.alloc(n * k, np.float32)
+    A.spmm(X32, Y32, k, hip.FP32)
+    reps = 20
+    ctx.timer_begin()
+    for _ in range(reps):
+        A.spmm(X32, Y32, k, hip.FP32)
+    ms = ctx.timer_end() / reps
+    byts = inf.nnz * 8 + (n + 1) * 4 + 2 * k * n * 4
+    out["spmm_fp32_k64"] = {"us": ms * 1e3, "gbs": byts / (ms * 1e-3) / 1e9, "algorithmic_bytes": byts}
+    print(json.dumps(out), flush=True)
+
+
 def dg_bench(args):
     """The DG residual and the graph-replayed RK4 step: upwind advection
     (apps/BasicAdvection_DG, velocity (-1, 0)) on a 256 x 256 Quad4 mesh, p = 3."""
@@ -326,7 +388,7 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="cells per axis (64 = the benchmark config)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "serve", "c3", "spmv", "dg"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "serve", "c3", "spmv", "dg", "c5"])
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--sweep-requests", type=int, default=300)
     ap.add_argument("--rates", default="50,200,500,1000,2000")
@@ -346,6 +408,9 @@ def main():
         return
     if args.workload == "spmv":
         spmv_bench(args)
+        return
+    if args.workload == "c5":
+        c5_bench(args)
         return
     if args.workload == "dg":
         dg_bench(args)

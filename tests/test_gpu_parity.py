@@ -117,6 +117,28 @@ def test_spmm_matches_columnwise_spmv(H, ctx, golden):
             assert np.abs(Y[:, j] - yj).max() <= 1e-13 * np.abs(yj).max()
 
 
+@pytest.mark.parametrize("k", [1, 37, 64, 70])
+def test_spmm_sorted_rows_matches_columnwise_spmv(H, ctx, k):
+    """A Q2 pattern (rows sorted by length inside the slices, so stored
+    positions differ from rows) with k right-hand sides, FP64 and FP32."""
+    m = H.Mesh.structured(ctx, 2, 2, 1, 20)
+    A = m.pattern(H.CSR, H.FP64 | H.FP32)
+    m.assemble(H.physics(H.POISSON, 2, coef=np.linspace(0.5, 2.0, 400)), A, None, H.TANGENT)
+    A.sync_fp32()
+    n = A.info().rows
+    X = np.random.default_rng(k).standard_normal((n, k))
+    Yd = ctx.alloc(n * k)
+    A.spmm(ctx.upload(X.ravel()), Yd, k)
+    Y = Yd.download().reshape(n, k)
+    Y32d = ctx.alloc(n * k, np.float32)
+    A.spmm(ctx.upload(X.astype(np.float32).ravel()), Y32d, k, H.FP32)
+    Y32 = Y32d.download().reshape(n, k)
+    for j in range(k):
+        yj = A.spmv_host(X[:, j])
+        assert np.abs(Y[:, j] - yj).max() <= 1e-13 * np.abs(yj).max()
+        assert np.abs(Y32[:, j] - yj).max() <= 1e-5 * np.abs(yj).max()
+
+
 def test_dot_matches_oracle(H, ctx, orc):
     rng = np.random.default_rng(9)
     a, b = rng.standard_normal(100003), rng.standard_normal(100003)

@@ -60,3 +60,4 @@ def test_elasticity_48_repeats_bit_identically(H, ctx, prec):
     for it, rf, x in runs[1:]:
         assert it == runs[0][0] and rf == runs[0][1]
         assert np.array_equal(x, runs[0][2])
+

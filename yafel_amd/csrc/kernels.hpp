@@ -109,5 +109,63 @@ __device__ __forceinline__ void sell_row_bcsr(const SellView<T> &A, int64_t s, i
   a[2] = a2;
 }
 
+
+// Value of `v` in lane l (l wave-uniform), for 32- and 64-bit types.
+__device__ __forceinline__ int lane_bcast(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float lane_bcast(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Multi-RHS rows of one slice: rows h*RB .. h*RB+RB-1 of slice s, with the
+// wavefront's lanes as right-hand sides.  For each slot the 64 lanes load the
+// slice's 64 (column, value) pairs in one coalesced step (the next slot's
+// pairs are loaded before this slot's gathers are issued, so only the gathers
+// wait on memory); each of the RB rows' pair is then broadcast (readlane) and
+// every lane gathers its right-hand side's x at that column, so one slot
+// issues RB independent 64-wide gathers.  y[r] accumulates row h*RB + r in
+// slot order (padding adds exact zeros).  xf(c) returns this lane's x at
+// column c and must be safe to call in every lane (callers clamp the lanes
+// beyond their right-hand sides to a valid one); those lanes, `on` false,
+// accumulate zeros.
+// With a column scale cs (nullptr: none) the product is A diag(cs) x, the
+// scale applied to each value as it is loaded (one gather per slot, not per
+// row).
+template <int RB, class T, class XF>
+__device__ __forceinline__ void sell_rows_multi(const int *__restrict__ sptr, const int *__restrict__ col,
+                                                const T *__restrict__ val, const T *__restrict__ cs, int64_t s,
+                                                int h, int lane, bool on, XF xf, T (&y)[RB]) {
+  const int p0 = sptr[s], ns = sptr[s + 1] - p0;
+#pragma unroll
+  for (int r = 0; r < RB; ++r) y[r] = 0;
+  if (ns == 0) return;
+  int64_t q = (int64_t)p0 * 64 + lane;
+  int cl = __builtin_nontemporal_load(&col[q]);
+  T vl = __builtin_nontemporal_load(&val[q]);
+  if (cs) vl *= cs[cl];
+  for (int t = 0; t < ns; ++t) {
+    int cn = 0;
+    T vn = 0;
+    if (t + 1 < ns) {
+      q += 64;
+      cn = __builtin_nontemporal_load(&col[q]);
+      vn = __builtin_nontemporal_load(&val[q]);
+      if (cs) vn *= cs[cn];
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int c = lane_bcast(cl, h * RB + r);
+      const T v = lane_bcast(vl, h * RB + r);
+      const T x = xf(c);
+      y[r] = fma(v, on ? x : T(0), y[r]);
+    }
+    cl = cn;
+    vl = vn;
+  }
+}
 }  // namespace dev
 }  // namespace yh

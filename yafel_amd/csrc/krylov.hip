@@ -9,9 +9,10 @@
 //
 // One CG iteration is three kernels and no separate fold:
 //   Q: q = A p, partials p.q
-//   U: alpha = rho / p.q folded by every workgroup; x += alpha p,
-//      r -= alpha q, z = D^-1 r, partials r.z, r.r
-//   P: beta and the stopping test folded by every workgroup; p = z + beta p
+//   U: alpha = rho / p.q folded by every workgroup; r -= alpha q,
+//      z = D^-1 r, partials r.z, r.r
+//   P: beta and the stopping test folded by every workgroup; x += alpha p,
+//      p = z + beta p
 // BiCGSTAB folds each reduction in a one-workgroup kernel launched right
 // after its producer.
 // MIXED: vectors and matrix are FP32 (row sums and dots accumulate in FP64);
@@ -189,15 +190,16 @@ struct Solver {
     int gu;
     if (prec) {
       OpCgU<T, true> u{};
-      u.x = xx; u.r = (T *)v.r; u.z = (T *)v.z; u.p = (const T *)v.p0; u.q = (const T *)v.q; u.dinv = dinv;
+      u.r = (T *)v.r; u.z = (T *)v.z; u.q = (const T *)v.q; u.dinv = dinv;
       gu = vlaunch(u, part, ga, pu);
     } else {
       OpCgU<T, false> u{};
-      u.x = xx; u.r = (T *)v.r; u.z = (T *)v.z; u.p = (const T *)v.p0; u.q = (const T *)v.q; u.dinv = dinv;
+      u.r = (T *)v.r; u.z = (T *)v.z; u.q = (const T *)v.q; u.dinv = dinv;
       gu = vlaunch(u, part, ga, pu);
     }
     OpCgP<T> pp{};
     pp.p = (T *)v.p0;
+    pp.x = xx;
     pp.z = (const T *)(prec ? v.z : v.r);
     vlaunch(pp, pu, gu, nullptr);
   }

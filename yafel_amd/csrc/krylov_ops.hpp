@@ -553,9 +553,10 @@ struct OpReport : OpBase {
 // The single-solve CG runs three kernels per iteration and no one-workgroup
 // fold:
 //   Q: q = A p, partial p.q                                (sliced SpMV)
-//   U: alpha from the p.q partials; x += alpha p, r -= alpha q, z = D^-1 r,
-//      partials r.z, r.r
-//   P: beta and the stopping test from the r.z, r.r partials; p = z + beta p
+//   U: alpha from the p.q partials; r -= alpha q, z = D^-1 r, partials r.z,
+//      r.r
+//   P: beta and the stopping test from the r.z, r.r partials; x += alpha p,
+//      p = z + beta p
 // Every workgroup of U and P folds the producer's partials itself, in one
 // fixed order, so all workgroups derive bit-identical scalars and take the
 // same branch.  The state is versioned: U and P read KState `in` and their
@@ -635,7 +636,8 @@ __global__ __launch_bounds__(NT) void k_velem(int64_t n, Op op0) {
   if (t == 0) sgo = op.decide(ss, f) ? 1 : 0;
   __syncthreads();
   if (b0 && t < KSW) reinterpret_cast<long long *>(op.out)[t] = reinterpret_cast<const long long *>(&ss)[t];
-  if (!sgo) return;
+  const bool go = sgo != 0;
+  if (!go && !Op::kOnStop) return;
   op.begin(ss);
   constexpr int NP = Op::NP;
   double acc[NP > 0 ? NP : 1];
@@ -643,16 +645,17 @@ __global__ __launch_bounds__(NT) void k_velem(int64_t n, Op op0) {
   for (int j = 0; j < (NP > 0 ? NP : 1); ++j) acc[j] = 0;
 #pragma unroll
   for (int k = 0; k < VE; ++k)
-    if (i0 + k * NT < i1) op.compute(i0 + k * NT, v[k], acc);
+    if (i0 + k * NT < i1) op.compute(i0 + k * NT, v[k], acc, go);
   for (int64_t i = i0 + VE * NT; i < i1; i += NT) {
     typename Op::In tmp;
     op.load(i, tmp);
-    op.compute(i, tmp, acc);
+    op.compute(i, tmp, acc, go);
   }
   finish_partials<NP>(op, acc, blk, nblk);
 }
 
 struct VOpBase {
+  static constexpr bool kOnStop = false;  // compute() also runs when decide() says stop
   const KState *in;
   KState *out;
   const double *pin;  // the producer's partials
@@ -679,15 +682,15 @@ struct OpCgQ : OpBase {
   __device__ void last(const double *) const {}
 };
 
-// U: alpha = rho / p.q; x += alpha p, r -= alpha q, z = D^-1 r; r.z, r.r.
+// U: alpha = rho / p.q; r -= alpha q, z = D^-1 r; r.z, r.r.
 template <class T, bool PREC>
 struct OpCgU : VOpBase {
   static constexpr int NPIN = 1, NP = 2;
-  T *x, *r, *z;
-  const T *p, *q, *dinv;
+  T *r, *z;
+  const T *q, *dinv;
   T alpha;
   struct In {
-    T p, q, r, x, d;
+    T q, r, d;
   };
   __device__ bool active(const KState &s) const { return running(s); }
   __device__ bool decide(KState &s, const double *f) const {
@@ -703,14 +706,11 @@ struct OpCgU : VOpBase {
   }
   __device__ void begin(const KState &s) { alpha = (T)s.alpha; }
   __device__ void load(int64_t i, In &v) const {
-    v.p = p[i];
     v.q = q[i];
     v.r = r[i];
-    v.x = x[i];
     if (PREC) v.d = dinv[i];
   }
-  __device__ void compute(int64_t i, const In &v, double *acc) const {
-    x[i] = fma(alpha, v.p, v.x);
+  __device__ void compute(int64_t i, const In &v, double *acc, bool) const {
     const T ri = fma(-alpha, v.q, v.r);
     r[i] = ri;
     if (PREC) {
@@ -724,44 +724,60 @@ struct OpCgU : VOpBase {
   }
 };
 
-// p = z + beta p: the body shared by P and P0.
+// P: the stopping test and beta from r.z, r.r; x += alpha p (the iteration
+// U finished) and, unless the solve stopped or paused for a refinement,
+// p = z + beta p.
 template <class T>
-struct OpPUpdate : VOpBase {
+struct OpCgP : VOpBase {
+  static constexpr int NPIN = 2, NP = 0;
+  static constexpr bool kOnStop = true;
+  T *p, *x;
+  const T *z;
+  T alpha, beta;
+  struct In {
+    T z, p, x;
+  };
+  __device__ bool active(const KState &s) const { return running(s); }
+  __device__ bool decide(KState &s, const double *f) const {
+    after_iteration(&s, f[0], f[1]);
+    return running(s);
+  }
+  __device__ void begin(const KState &s) {
+    alpha = (T)s.alpha;
+    beta = (T)s.beta;
+  }
+  __device__ void load(int64_t i, In &v) const {
+    v.z = z[i];
+    v.p = p[i];
+    v.x = x[i];
+  }
+  __device__ void compute(int64_t i, const In &v, double *, bool go) const {
+    x[i] = fma(alpha, v.p, v.x);
+    if (go) p[i] = fma(beta, v.p, v.z);
+  }
+};
+
+// p = z + beta p after the initial or a refinement residual (KState::pupd).
+template <class T>
+struct OpCgP0 : VOpBase {
+  static constexpr int NPIN = 0, NP = 0;
   T *p;
   const T *z;
   T beta;
   struct In {
     T z, p;
   };
+  __device__ bool active(const KState &s) const { return running(s) && s.pupd; }
+  __device__ bool decide(KState &s, const double *) const {
+    s.pupd = 0;
+    return true;
+  }
   __device__ void begin(const KState &s) { beta = (T)s.beta; }
   __device__ void load(int64_t i, In &v) const {
     v.z = z[i];
     v.p = p[i];
   }
-  __device__ void compute(int64_t i, const In &v, double *) const { p[i] = fma(beta, v.p, v.z); }
-};
-
-// P: the stopping test and beta from r.z, r.r; p = z + beta p unless the
-// solve stopped or paused for a refinement.
-template <class T>
-struct OpCgP : OpPUpdate<T> {
-  static constexpr int NPIN = 2, NP = 0;
-  __device__ bool active(const KState &s) const { return VOpBase::running(s); }
-  __device__ bool decide(KState &s, const double *f) const {
-    after_iteration(&s, f[0], f[1]);
-    return VOpBase::running(s);
-  }
-};
-
-// p = z + beta p after the initial or a refinement residual (KState::pupd).
-template <class T>
-struct OpCgP0 : OpPUpdate<T> {
-  static constexpr int NPIN = 0, NP = 0;
-  __device__ bool active(const KState &s) const { return VOpBase::running(s) && s.pupd; }
-  __device__ bool decide(KState &s, const double *) const {
-    s.pupd = 0;
-    return true;
-  }
+  __device__ void compute(int64_t i, const In &v, double *, bool) const { p[i] = fma(beta, v.p, v.z); }
 };
 
 }  // namespace kr

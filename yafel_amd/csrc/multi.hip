@@ -1,14 +1,16 @@
 // k right-hand sides advanced together (k <= 64 per batch; larger k runs in
-// batches of 64).  Vectors are row-major n x k, so one wavefront owns a matrix
-// row and its 64 lanes are the right-hand sides: the row's values and column
-// indices are wave-uniform (scalar loads, read once for all k), the gathered
-// rows of X are 64-wide coalesced reads, and every per-RHS dot product is a
+// batches of 64).  Vectors are row-major n x k, so the 64 lanes of a
+// wavefront are the right-hand sides: in the products a wavefront takes a
+// slice's rows, reads the slice's columns and values once per slot
+// (coalesced) and broadcasts each row's entry to all lanes; the gathered rows
+// of X are 64-wide coalesced reads, and every per-RHS dot product is a
 // per-lane accumulation.  Each right-hand side keeps its own FP64 scalars and
 // its own stopping decision; a right-hand side that has converged is frozen.
 //
 // Methods: BiCGSTAB with right Jacobi preconditioning (ph = D^-1 p and
 // sh = D^-1 s are formed on the fly, never stored) and CG / Jacobi-PCG, in
 // FP64 or as FP32 inner iterations with FP64 residual replacement.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 
@@ -36,12 +38,11 @@ struct MState {
 
 __device__ __forceinline__ bool bad(double v) { return !(fabs(v) <= 1.7976931348623157e308); }
 
-// Sliced CSR storage addressed by row (kernels.hpp): entry t of row r is at
-// sell_pos(sptr, iperm, r, t).
+// Sliced CSR storage (kernels.hpp): slices of 64 stored rows; perm maps a
+// stored position to its row (nullptr: identity).
 template <class T> struct CsrView {
-  const int *ptr;
   const int *sptr;
-  const int *iperm;
+  const int *perm;
   const int *col;
   const T *val;
   int64_t rows;
@@ -71,20 +72,23 @@ __device__ __forceinline__ void finish(Op &op, double (&acc)[NP > 0 ? NP : 1]) {
   }
 }
 
+constexpr int NTM = 1024;  // threads of the fold: 16 wavefronts share the blocks
+
 template <class Op>
-__global__ __launch_bounds__(NT) void k_mfinish(Op op0, int nblk) {
+__global__ __launch_bounds__(NTM) void k_mfinish(Op op0, int nblk) {
   Op op = op0;
   if (op.skip()) return;
   op.begin();
-  constexpr int NP = Op::NP;
-  __shared__ double lds[NP][NT / 64][64];
+  constexpr int NP = Op::NP, NW = NTM / 64;
+  __shared__ double lds[NP][NW][64];
   __shared__ double sums[NP][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  // thread (w, lane) sums blocks w, w+4, ... ; then the 4 quarters in order
+  // thread (w, lane) sums blocks w, w+16, ... ; then the 16 parts in order
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     double s = 0;
-    for (int b = w; b < nblk; b += NT / 64) s += op.part[((size_t)j * nblk + b) * 64 + lane];
+#pragma unroll 8
+    for (int b = w; b < nblk; b += NW) s += op.part[((size_t)j * nblk + b) * 64 + lane];
     lds[j][w][lane] = s;
   }
   __syncthreads();
@@ -93,7 +97,7 @@ __global__ __launch_bounds__(NT) void k_mfinish(Op op0, int nblk) {
     for (int j = 0; j < NP; ++j) {
       double s = 0;
 #pragma unroll
-      for (int q = 0; q < NT / 64; ++q) s += lds[j][q][lane];
+      for (int q = 0; q < NW; ++q) s += lds[j][q][lane];
       sums[j][lane] = s;
     }
   }
@@ -108,9 +112,13 @@ __global__ __launch_bounds__(NT) void k_mfinish(Op op0, int nblk) {
   if (threadIdx.x == 0) op.last_all();
 }
 
-// Y-producing kernel: wave per row, y[lane] = sum_q val[q] x(col[q], lane).
+// Y-producing kernel: a wavefront takes RB rows of a slice (32 for FP64, whose
+// accumulators take twice the registers), lanes = right-hand sides; the
+// slice's columns and values are read once per slot, coalesced, and each
+// row's gather is a 64-wide read of X (sell_rows_multi).
 template <class T, class Op>
 __global__ __launch_bounds__(NT) void k_mspmm(CsrView<T> A, Op op0) {
+  constexpr int RB = sizeof(T) == 4 ? 64 : 32, H = 64 / RB;
   Op op = op0;
   if (op.skip()) return;
   op.begin();
@@ -120,20 +128,20 @@ __global__ __launch_bounds__(NT) void k_mspmm(CsrView<T> A, Op op0) {
   for (int j = 0; j < (NP > 0 ? NP : 1); ++j) acc[j] = 0;
   const int lane = threadIdx.x & 63;
   const int k = op.st->k;
-  const int64_t chunk = (A.rows + gridDim.x - 1) / gridDim.x;
-  const int64_t r0 = (int64_t)dev::xcd_block() * chunk;
-  const int64_t r1 = r0 + chunk < A.rows ? r0 + chunk : A.rows;
-  for (int64_t rr = r0 + (threadIdx.x >> 6); rr < r1; rr += NT / 64) {
-    const int r = __builtin_amdgcn_readfirstlane((int)rr);
-    const int len = A.ptr[r + 1] - A.ptr[r];
-    const int64_t base = dev::sell_pos(A.sptr, A.iperm, r, 0);
-    if (lane < k) {
-      T y = 0;
-      for (int t = 0; t < len; ++t) {
-        const int64_t q = base + (int64_t)t * 64;
-        y = fma(A.val[q], op.x(A.col[q], lane), y);
-      }
-      op.row(r, lane, y, acc);
+  const bool on = lane < k;
+  const int64_t units = ((A.rows + 63) >> 6) * H;
+  const int64_t nw = (int64_t)gridDim.x * (NT / 64);
+  for (int64_t u = (int64_t)dev::xcd_block() * (NT / 64) + (threadIdx.x >> 6); u < units; u += nw) {
+    const int64_t s = u / H;
+    const int h = (int)(u - s * H);
+    T y[RB];
+    const int jl = on ? lane : 0;
+    dev::sell_rows_multi<RB, T>(A.sptr, A.col, A.val, (const T *)op.cscale(), s, h, lane, on,
+                                [&](int c) { return op.x(c, jl); }, y);
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int64_t i = s * 64 + h * RB + r;
+      if (i < A.rows && on) op.row(A.perm ? (int64_t)A.perm[i] : i, lane, y[r], acc);
     }
   }
   finish<NP>(op, acc);
@@ -165,6 +173,8 @@ struct MBase {
   int k;
   __device__ bool running() const { return !st->done && !st->paused; }
   __device__ void last_all() const {}
+  // product ops: no column scale unless the op supplies one
+  __device__ const void *cscale() const { return nullptr; }
 };
 
 __device__ void request_pause_or_advance(MState *st, int j, double rho_new, double rr, bool cgfam) {
@@ -214,7 +224,7 @@ struct MP : MBase {
   __device__ void last_lane(int, const double *) const {}
 };
 
-// v = A (D^-1 p), rhat.v -> alpha
+// v = A (D^-1 p), rhat.v -> alpha; D^-1 scales the matrix columns
 template <class T>
 struct MV : MBase {
   static constexpr int NP = 1;
@@ -222,7 +232,8 @@ struct MV : MBase {
   T *v;
   __device__ bool skip() const { return !running(); }
   __device__ void begin() {}
-  __device__ T x(int c, int j) const { return dinv[c] * p[(int64_t)c * k + j]; }
+  __device__ const T *cscale() const { return dinv; }
+  __device__ T x(int c, int j) const { return p[(int64_t)c * k + j]; }
   __device__ void row(int64_t i, int j, T y, double *acc) const {
     if (!st->active[j]) return;
     v[i * k + j] = y;
@@ -259,7 +270,7 @@ struct MS : MBase {
   __device__ void last_lane(int, const double *) const {}
 };
 
-// t = A (D^-1 s), t.t, t.s -> omega
+// t = A (D^-1 s), t.t, t.s -> omega; D^-1 scales the matrix columns
 template <class T>
 struct MT : MBase {
   static constexpr int NP = 2;
@@ -267,7 +278,8 @@ struct MT : MBase {
   T *t;
   __device__ bool skip() const { return !running(); }
   __device__ void begin() {}
-  __device__ T x(int c, int j) const { return dinv[c] * s[(int64_t)c * k + j]; }
+  __device__ const T *cscale() const { return dinv; }
+  __device__ T x(int c, int j) const { return s[(int64_t)c * k + j]; }
   __device__ void row(int64_t i, int j, T y, double *acc) const {
     if (!st->active[j]) return;
     t[i * k + j] = y;
@@ -524,8 +536,8 @@ struct MReport : MBase {
 
 template <class T>
 CsrView<T> cview(yafel_hip_matrix_s *A) {
-  return CsrView<T>{A->ptr.as<int>(), A->sptr.as<int>(), A->iperm.p ? A->iperm.as<int>() : nullptr,
-                    A->col.as<int>(), (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->rows};
+  return CsrView<T>{A->sptr.as<int>(), A->perm.p ? A->perm.as<int>() : nullptr, A->col.as<int>(),
+                    (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->rows};
 }
 
 struct Multi {
@@ -550,10 +562,14 @@ struct Multi {
     return op;
   }
   template <class T, class O> void spmm(const O &op) {
-    k_mspmm<T, O><<<grid, NT, 0, s>>>(cview<T>(A), op);
+    // RB-row units of the slices, four per workgroup, within the partials'
+    // grid
+    const int64_t units = A->nsl * (sizeof(T) == 4 ? 1 : 2);
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((units + 3) / 4, grid));
+    k_mspmm<T, O><<<g, NT, 0, s>>>(cview<T>(A), op);
     HIPCHK(hipGetLastError());
     if constexpr (O::NP > 0) {
-      k_mfinish<O><<<1, NT, 0, s>>>(op, grid);
+      k_mfinish<O><<<1, NTM, 0, s>>>(op, g);
       HIPCHK(hipGetLastError());
     }
   }
@@ -561,7 +577,7 @@ struct Multi {
     k_melem<O><<<grid, NT, 0, s>>>(n, op);
     HIPCHK(hipGetLastError());
     if constexpr (O::NP > 0) {
-      k_mfinish<O><<<1, NT, 0, s>>>(op, grid);
+      k_mfinish<O><<<1, NTM, 0, s>>>(op, grid);
       HIPCHK(hipGetLastError());
     }
   }

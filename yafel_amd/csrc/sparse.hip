@@ -201,32 +201,33 @@ __global__ __launch_bounds__(256) void k_sell_spmv_bcsr(SellView<T> A, const T *
   }
 }
 
-// Y = A X for k right-hand sides, row-major X (n x k).  One wavefront per row;
-// lanes run over right-hand sides, so the X/Y rows are read/written coalesced
-// and each matrix entry is one wave-uniform load.
+// Y = A X for k right-hand sides, row-major X (n x k).  A wavefront takes RB
+// rows of a slice (two passes of 32 rows for FP64, whose accumulators take
+// twice the registers) and 64 right-hand sides at a time: the slice's
+// columns and values are read once per slot, coalesced, and each row's X
+// gather is a 64-wide coalesced read.
 template <class T>
-__global__ __launch_bounds__(256) void k_csr_spmm(int64_t rows, int k, const int *__restrict__ ptr,
-                                                  const int *__restrict__ sptr, const int *__restrict__ iperm,
-                                                  const int *__restrict__ col, const T *__restrict__ val,
-                                                  const T *__restrict__ X, T *__restrict__ Y) {
+__global__ __launch_bounds__(256) void k_csr_spmm(int64_t rows, int k, const int *__restrict__ sptr,
+                                                  const int *__restrict__ perm, const int *__restrict__ col,
+                                                  const T *__restrict__ val, const T *__restrict__ X,
+                                                  T *__restrict__ Y) {
+  constexpr int RB = sizeof(T) == 4 ? 64 : 32, H = 64 / RB;
   const int lane = threadIdx.x & 63;
-  const int64_t wpb = 4;
-  const int64_t chunk = (rows + gridDim.x - 1) / gridDim.x;
-  const int64_t r0 = (int64_t)dev::xcd_block() * chunk;
-  const int64_t r1 = r0 + chunk < rows ? r0 + chunk : rows;
-  for (int64_t rr = r0 + threadIdx.x / 64; rr < r1; rr += wpb) {
-    const int r = __builtin_amdgcn_readfirstlane((int)rr);
-    const int len = ptr[r + 1] - ptr[r];
-    const int64_t base = dev::sell_pos(sptr, iperm, r, 0);
+  const int64_t units = ((rows + 63) >> 6) * H;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  for (int64_t u = (int64_t)dev::xcd_block() * 4 + (threadIdx.x >> 6); u < units; u += nw) {
+    const int64_t s = u / H;
+    const int h = (int)(u - s * H);
     for (int j0 = 0; j0 < k; j0 += 64) {
-      const int j = j0 + lane;
-      T acc = 0;
-      if (j < k)
-        for (int t = 0; t < len; ++t) {
-          const int64_t q = base + (int64_t)t * 64;
-          acc = fma(val[q], X[(int64_t)col[q] * k + j], acc);
-        }
-      if (j < k) Y[r * (int64_t)k + j] = acc;
+      const int j = j0 + lane, jj = j < k ? j : j0;
+      T y[RB];
+      dev::sell_rows_multi<RB, T>(sptr, col, val, nullptr, s, h, lane, j < k,
+                                  [&](int c) { return X[(int64_t)c * k + jj]; }, y);
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int64_t i = s * 64 + h * RB + r;
+        if (i < rows && j < k) Y[(perm ? (int64_t)perm[i] : i) * k + j] = y[r];
+      }
     }
   }
 }
@@ -404,17 +405,21 @@ void spmm_launch(yafel_hip_matrix_s *A, int precision, int k, const void *X, voi
   const int grid = grid_for(A->ctx, A->brows * 64, 256);
   const int *p = A->ptr.as<int>(), *sp = A->sptr.as<int>(), *c = A->col.as<int>();
   const int *ip = A->iperm.p ? A->iperm.as<int>() : nullptr;
+  const int *pm = A->perm.p ? A->perm.as<int>() : nullptr;
+  // CSR: RB-row units of the slices, four per workgroup
+  const int64_t units = A->nsl * (precision == YAFEL_HIP_FP32 ? 1 : 2);
+  const int gcsr = (int)std::min<int64_t>((units + 3) / 4, (int64_t)A->ctx->num_cus * 8);
   if (precision == YAFEL_HIP_FP32) {
     REQUIRE(A->has32, "spmm: matrix has no FP32 copy");
     if (A->format == YAFEL_HIP_CSR)
-      k_csr_spmm<float><<<grid, 256, 0, s>>>(A->rows, k, p, sp, ip, c, A->val32.as<float>(), (const float *)X, (float *)Y);
+      k_csr_spmm<float><<<gcsr, 256, 0, s>>>(A->rows, k, sp, pm, c, A->val32.as<float>(), (const float *)X, (float *)Y);
     else
       k_bcsr_spmm<float><<<grid, 256, 0, s>>>(A->brows, A->ntot, k, p, sp, ip, c, A->val32.as<float>(),
                                               (const float *)X, (float *)Y);
   } else {
     REQUIRE(A->has64, "spmm: matrix has no FP64 copy");
     if (A->format == YAFEL_HIP_CSR)
-      k_csr_spmm<double><<<grid, 256, 0, s>>>(A->rows, k, p, sp, ip, c, A->val64.as<double>(), (const double *)X,
+      k_csr_spmm<double><<<gcsr, 256, 0, s>>>(A->rows, k, sp, pm, c, A->val64.as<double>(), (const double *)X,
                                               (double *)Y);
     else
       k_bcsr_spmm<double><<<grid, 256, 0, s>>>(A->brows, A->ntot, k, p, sp, ip, c, A->val64.as<double>(),
