@@ -566,17 +566,22 @@ struct OpReport : OpBase {
 // next kernel.
 
 // Sum of each of NPIN partial arrays part[j * nblk + i], i < nblk, in a fixed
-// order; the result is the same in every thread of every workgroup.
+// order; the result is the same in every thread of every workgroup.  Split in
+// two so that the loads (refold_load, this thread's share) can be issued
+// before other waits: refold_reduce combines the shares across the workgroup.
 template <int NPIN>
-__device__ __forceinline__ void refold(const double *__restrict__ part, int nblk, double (&s)[NPIN]) {
-  __shared__ double lds[NPIN][NT / 64];
-  double v[NPIN];
+__device__ __forceinline__ void refold_load(const double *__restrict__ part, int nblk, double (&v)[NPIN]) {
 #pragma unroll
   for (int j = 0; j < NPIN; ++j) v[j] = 0;
 #pragma unroll 4
   for (int i = threadIdx.x; i < nblk; i += NT)
 #pragma unroll
     for (int j = 0; j < NPIN; ++j) v[j] += part[(size_t)j * nblk + i];
+}
+
+template <int NPIN>
+__device__ __forceinline__ void refold_reduce(const double (&v)[NPIN], double (&s)[NPIN]) {
+  __shared__ double lds[NPIN][NT / 64];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
   for (int j = 0; j < NPIN; ++j) {
@@ -623,15 +628,18 @@ __global__ __launch_bounds__(NT) void k_velem(int64_t n, Op op0) {
 #pragma unroll
   for (int k = 0; k < VE; ++k)
     if (i0 + k * NT < i1) op.load(i0 + k * NT, v[k]);
+  // the producer's partials are loaded before the state, so that the vector,
+  // partial and state loads are in flight together
+  constexpr int NPIN = Op::NPIN;
+  double f[NPIN > 0 ? NPIN : 1];
+  if constexpr (NPIN > 0) refold_load<NPIN>(op.pin, op.nin, f);
   if (t < KSW) reinterpret_cast<long long *>(&ss)[t] = reinterpret_cast<const long long *>(op.in)[t];
   __syncthreads();
   if (!op.active(ss)) {
     if (b0 && t < KSW) reinterpret_cast<long long *>(op.out)[t] = reinterpret_cast<const long long *>(&ss)[t];
     return;
   }
-  constexpr int NPIN = Op::NPIN;
-  double f[NPIN > 0 ? NPIN : 1];
-  if constexpr (NPIN > 0) refold<NPIN>(op.pin, op.nin, f);
+  if constexpr (NPIN > 0) refold_reduce<NPIN>(f, f);
   __syncthreads();  // every wave has read ss before thread 0 changes it
   if (t == 0) sgo = op.decide(ss, f) ? 1 : 0;
   __syncthreads();
