@@ -106,8 +106,10 @@ int yafel_hip_matrix_download(yafel_hip_matrix A, int64_t *row_ptr, int64_t *col
                               double *data);
 /* refresh the FP32 copy from the FP64 values (allocating it if needed) */
 int yafel_hip_matrix_sync_fp32(yafel_hip_matrix A);
-/* slots per step (1, 2, 4 or 8; default 4) of the sliced SpMV kernel behind
- * yafel_hip_spmv; a tuning knob, results are identical for every value */
+/* slots per step (1, 2, 4 or 8) of the sliced SpMV kernel behind
+ * yafel_hip_spmv (the Krylov solvers' SpMV uses 8 when set, else 4); the
+ * default is 8 for CSR matrices averaging 24 or more slots per slice and 4
+ * otherwise.  A tuning knob: results are identical for every value. */
 int yafel_hip_matrix_set_unroll(yafel_hip_matrix A, int unroll);
 /* device pointer to the FP64 diagonal (computed on first use) */
 int yafel_hip_matrix_diagonal(yafel_hip_matrix A, double *diag_dev);

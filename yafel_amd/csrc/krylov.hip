@@ -69,8 +69,12 @@ int launch_spmv_nofold(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
   if (e0) HIPCHK(hipEventRecord(e0, s));
-  if (A->format == YAFEL_HIP_CSR) k_spmv_fused<YAFEL_HIP_CSR, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
-  else k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
+  if (A->format == YAFEL_HIP_CSR && A->unroll == 8)
+    k_spmv_fused<YAFEL_HIP_CSR, 8, T, Op><<<grid, NT, 0, s>>>(v, op);
+  else if (A->format == YAFEL_HIP_CSR)
+    k_spmv_fused<YAFEL_HIP_CSR, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
+  else
+    k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
   HIPCHK(hipGetLastError());
   if (e1) HIPCHK(hipEventRecord(e1, s));
   return grid;

@@ -365,6 +365,10 @@ void matrix_make_sliced(yafel_hip_matrix_s *A, DBuf &ptr, DBuf &col, DBuf &val64
       HIPCHK(hipGetLastError());
     }
     A->ntot = slots * 64;
+    // long scalar rows (Q4 CSR: about 36 slots per slice) keep more loads in
+    // flight with 8 slots per step; shorter rows and 3x3 blocks (ten loads
+    // per slot already) run best at 4
+    A->unroll = A->format == YAFEL_HIP_CSR && slots >= 24 * nsl ? 8 : 4;
     A->col.alloc(std::max<int64_t>(A->ntot, 1) * 4);
     A->val64.alloc(std::max<int64_t>(A->ntot * nval, 1) * 8);
     k_sell_fill<<<ceil_div(nsl * 64, 256), 256, 0, s>>>(brows, nsl, nblk, A->ntot, nval, A->ptr.as<int>(),
