@@ -69,7 +69,68 @@ __device__ __forceinline__ void jac_inv(const double *J, double &det, double *Ji
   }
 }
 
-template <int DIM, int P, int DPN>
+// Block (A, B) of the element tangent, k[a][b] = K_e[A*DPN + a][B*DPN + b],
+// from the physical gradients and jxw staged in LDS.
+template <int DIM, int DPN, int NEN, int NQP>
+__device__ __forceinline__ void tangent_block(const Phys &ph, double ke, const double (*sG)[NEN][DIM],
+                                              const double *sW, const double *__restrict__ tabN, int A, int B,
+                                              double (&k)[DPN][DPN]) {
+#pragma unroll
+  for (int a = 0; a < DPN; ++a)
+#pragma unroll
+    for (int b = 0; b < DPN; ++b) k[a][b] = 0;
+  for (int q = 0; q < NQP; ++q) {
+    const double jxw = sW[q];
+    const double *gA = sG[q][A];
+    const double *gB = sG[q][B];
+    if (ph.kind == YAFEL_HIP_PHYS_POISSON) {
+      // apps/BasicPoisson/main.cpp:51-58
+      double dt = 0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) dt += gA[d] * gB[d];
+      k[0][0] -= ke * dt * jxw;
+    } else if (ph.kind == YAFEL_HIP_PHYS_ELASTICITY) {
+      // apps/BasicLinearElasticity/main.cpp:70-110
+      const double lam = ke * ph.lambda, mu = ke * ph.mu;
+#pragma unroll
+      for (int ii = 0; ii < DPN; ++ii)
+#pragma unroll
+        for (int kk = 0; kk < DPN; ++kk) {
+          double s = 0;
+#pragma unroll
+          for (int j = 0; j < DIM; ++j)
+#pragma unroll
+            for (int l = 0; l < DIM; ++l) {
+              const double C = lam * (ii == j) * (kk == l) + mu * ((ii == kk) * (j == l) + (ii == l) * (j == kk));
+              s += C * (gA[j] * gB[l]);
+            }
+          k[ii][kk] -= s * jxw;
+        }
+    } else {
+      // apps/supg_learning/supg_advection.hpp:90-103 (advection added once per A component)
+      const double NA = tabN[q * NEN + A];
+      double s = 0;
+#pragma unroll
+      for (int Ac = 0; Ac < DIM; ++Ac)
+#pragma unroll
+        for (int Bc = 0; Bc < DIM; ++Bc)
+          s += (NA * ph.U[Bc] * gB[Bc] + gA[Ac] * gB[Bc] * ph.Kt[Ac * DIM + Bc]) * jxw;
+      k[0][0] += s;
+    }
+  }
+}
+
+// Element tangents and residuals, EPB elements per workgroup: coordinates to
+// LDS; a thread per (element, qp) forms J, det, J^-1 and the physical
+// gradients into LDS; then the tangent blocks and the residual rows.
+// SYM (Poisson, elasticity, whose tangents the reference fills for B >= A and
+// mirrors): only the blocks B >= A are formed, a thread per (element, block),
+// and staged block by block in the packed upper triangle,
+//   Ke[(e * NB + idx(A, B)) * DPN^2 + a * DPN + b],  NB = NEN (NEN + 1) / 2,
+// so consecutive threads write consecutive blocks; the gather reads block
+// (B, A) as the transpose.  Otherwise every block is formed, a thread per
+// (element, row block A), into the row-major element matrix.
+template <int DIM, int P, int DPN, bool SYM>
 __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__restrict__ el,
                                                 const double *__restrict__ X,
                                                 const double *__restrict__ tabN,
@@ -123,58 +184,36 @@ __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__re
     }
   }
   __syncthreads();
+  if (want_k && SYM) {
+    constexpr int NB = NEN * (NEN + 1) / 2;
+    for (int i = threadIdx.x; i < EPB * NB; i += NT) {
+      const int le = i / NB, bi = i - le * NB;
+      const int64_t e = e0 + le;
+      if (e >= nel) continue;
+      int A = 0, rem = bi;
+      while (rem >= NEN - A) {
+        rem -= NEN - A;
+        ++A;
+      }
+      double k[DPN][DPN];
+      tangent_block<DIM, DPN, NEN, NQP>(ph, coef ? coef[e] : 1.0, sG[le], sW[le], tabN, A, A + rem, k);
+      double *Kb = Ke + (e * NB + bi) * (DPN * DPN);
+#pragma unroll
+      for (int a = 0; a < DPN; ++a)
+#pragma unroll
+        for (int b = 0; b < DPN; ++b) Kb[a * DPN + b] = k[a][b];
+    }
+  }
   for (int i = threadIdx.x; i < EPB * NEN; i += NT) {
     const int le = i / NEN, A = i - (i / NEN) * NEN;
     const int64_t e = e0 + le;
     if (e >= nel) continue;
     const double ke = coef ? coef[e] : 1.0;
-    if (want_k) {
+    if (want_k && !SYM) {
       double *Krow = Ke + e * (int64_t)NLD * NLD;
       for (int B = 0; B < NEN; ++B) {
         double k[DPN][DPN];
-#pragma unroll
-        for (int a = 0; a < DPN; ++a)
-#pragma unroll
-          for (int b = 0; b < DPN; ++b) k[a][b] = 0;
-        for (int q = 0; q < NQP; ++q) {
-          const double jxw = sW[le][q];
-          const double *gA = sG[le][q][A];
-          const double *gB = sG[le][q][B];
-          if (ph.kind == YAFEL_HIP_PHYS_POISSON) {
-            // apps/BasicPoisson/main.cpp:51-58
-            double dt = 0;
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) dt += gA[d] * gB[d];
-            k[0][0] -= ke * dt * jxw;
-          } else if (ph.kind == YAFEL_HIP_PHYS_ELASTICITY) {
-            // apps/BasicLinearElasticity/main.cpp:70-110
-            const double lam = ke * ph.lambda, mu = ke * ph.mu;
-#pragma unroll
-            for (int ii = 0; ii < DPN; ++ii)
-#pragma unroll
-              for (int kk = 0; kk < DPN; ++kk) {
-                double s = 0;
-#pragma unroll
-                for (int j = 0; j < DIM; ++j)
-#pragma unroll
-                  for (int l = 0; l < DIM; ++l) {
-                    const double C = lam * (ii == j) * (kk == l) + mu * ((ii == kk) * (j == l) + (ii == l) * (j == kk));
-                    s += C * (gA[j] * gB[l]);
-                  }
-                k[ii][kk] -= s * jxw;
-              }
-          } else {
-            // apps/supg_learning/supg_advection.hpp:90-103 (advection added once per A component)
-            const double NA = tabN[q * NEN + A];
-            double s = 0;
-#pragma unroll
-            for (int Ac = 0; Ac < DIM; ++Ac)
-#pragma unroll
-              for (int Bc = 0; Bc < DIM; ++Bc)
-                s += (NA * ph.U[Bc] * gB[Bc] + gA[Ac] * gB[Bc] * ph.Kt[Ac * DIM + Bc]) * jxw;
-            k[0][0] += s;
-          }
-        }
+        tangent_block<DIM, DPN, NEN, NQP>(ph, ke, sG[le], sW[le], tabN, A, B, k);
 #pragma unroll
         for (int a = 0; a < DPN; ++a)
 #pragma unroll
@@ -290,7 +329,9 @@ __global__ __launch_bounds__(256) void k_element_mfma(int64_t nel, const int32_t
 }
 
 // one thread per stored node pair: sum the element contributions in order
-template <int DPN, int FMT, class KT = double>
+// (SYM: packed upper-triangle staging, block (a, b) with a > b read as the
+// transpose of block (b, a))
+template <int DPN, int FMT, class KT = double, bool SYM = false>
 __global__ __launch_bounds__(NT) void k_gather(int64_t nruns, int nen, const int32_t *__restrict__ cptr,
                                                const uint32_t *__restrict__ contrib,
                                                const KT *__restrict__ Ke,
@@ -314,11 +355,22 @@ __global__ __launch_bounds__(NT) void k_gather(int64_t nruns, int nen, const int
     const int64_t e = t / nen2;
     const int r = (int)(t - e * nen2);
     const int a = r / nen, b = r - (r / nen) * nen;
-    const KT *k = Ke + e * (int64_t)nld * nld + (int64_t)(a * DPN) * nld + b * DPN;
+    if (SYM) {
+      const int lo = a <= b ? a : b, hi = a <= b ? b : a;
+      const int64_t nb = (int64_t)nen * (nen + 1) / 2;
+      const int64_t bi = (int64_t)lo * nen - (int64_t)lo * (lo - 1) / 2 + (hi - lo);
+      const KT *k = Ke + (e * nb + bi) * (DPN * DPN);
 #pragma unroll
-    for (int i = 0; i < DPN; ++i)
+      for (int i = 0; i < DPN; ++i)
 #pragma unroll
-      for (int jj = 0; jj < DPN; ++jj) acc[i][jj] += k[i * nld + jj];
+        for (int jj = 0; jj < DPN; ++jj) acc[i][jj] += a <= b ? k[i * DPN + jj] : k[jj * DPN + i];
+    } else {
+      const KT *k = Ke + e * (int64_t)nld * nld + (int64_t)(a * DPN) * nld + b * DPN;
+#pragma unroll
+      for (int i = 0; i < DPN; ++i)
+#pragma unroll
+        for (int jj = 0; jj < DPN; ++jj) acc[i][jj] += k[i * nld + jj];
+    }
   }
   // sliced position of the run's entries: run j is entry t of node row A
   const int A = run_row[j];
@@ -432,37 +484,46 @@ Phys make_phys(const yafel_hip_physics *p) {
 
 template <int DIM, int P, int DPN>
 void launch_elem_dpn(yafel_hip_mesh_s *m, const Phys &ph, const double *coef, double *Ke, double *Re, int wk,
-                     int wr) {
+                     int wr, bool sym) {
   using S = ElemShape<DIM, P>;
   const int grid = ceil_div(m->nel, S::EPB);
-  k_element<DIM, P, DPN><<<grid, NT, 0, m->ctx->stream>>>(m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(),
-                                                          m->tabN.as<double>(), m->tabdN.as<double>(),
-                                                          m->tabw.as<double>(), ph, coef, Ke, Re, wk, wr);
+  if (sym)
+    k_element<DIM, P, DPN, true><<<grid, NT, 0, m->ctx->stream>>>(
+        m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(), m->tabN.as<double>(), m->tabdN.as<double>(),
+        m->tabw.as<double>(), ph, coef, Ke, Re, wk, wr);
+  else
+    k_element<DIM, P, DPN, false><<<grid, NT, 0, m->ctx->stream>>>(
+        m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(), m->tabN.as<double>(), m->tabdN.as<double>(),
+        m->tabw.as<double>(), ph, coef, Ke, Re, wk, wr);
   HIPCHK(hipGetLastError());
 }
 
 template <int DIM, int P>
-void launch_elem_p(yafel_hip_mesh_s *m, const Phys &ph, const double *coef, double *Ke, double *Re, int wk, int wr) {
+void launch_elem_p(yafel_hip_mesh_s *m, const Phys &ph, const double *coef, double *Ke, double *Re, int wk, int wr,
+                   bool sym) {
   switch (m->dpn) {
-    case 1: launch_elem_dpn<DIM, P, 1>(m, ph, coef, Ke, Re, wk, wr); break;
-    case 2: launch_elem_dpn<DIM, P, 2>(m, ph, coef, Ke, Re, wk, wr); break;
-    default: launch_elem_dpn<DIM, P, 3>(m, ph, coef, Ke, Re, wk, wr); break;
+    case 1: launch_elem_dpn<DIM, P, 1>(m, ph, coef, Ke, Re, wk, wr, sym); break;
+    case 2: launch_elem_dpn<DIM, P, 2>(m, ph, coef, Ke, Re, wk, wr, sym); break;
+    default: launch_elem_dpn<DIM, P, 3>(m, ph, coef, Ke, Re, wk, wr, sym); break;
   }
 }
 
-void launch_element(yafel_hip_mesh_s *m, const Phys &ph, const double *coef, double *Ke, double *Re, int wk, int wr) {
+// sym: stage only the blocks B >= A (packed upper triangle); the public
+// element-matrix entry point keeps full row-major matrices
+void launch_element(yafel_hip_mesh_s *m, const Phys &ph, const double *coef, double *Ke, double *Re, int wk, int wr,
+                    bool sym = false) {
   if (m->dim == 2) {
     switch (m->p) {
-      case 1: launch_elem_p<2, 1>(m, ph, coef, Ke, Re, wk, wr); break;
-      case 2: launch_elem_p<2, 2>(m, ph, coef, Ke, Re, wk, wr); break;
-      case 3: launch_elem_p<2, 3>(m, ph, coef, Ke, Re, wk, wr); break;
-      case 4: launch_elem_p<2, 4>(m, ph, coef, Ke, Re, wk, wr); break;
+      case 1: launch_elem_p<2, 1>(m, ph, coef, Ke, Re, wk, wr, sym); break;
+      case 2: launch_elem_p<2, 2>(m, ph, coef, Ke, Re, wk, wr, sym); break;
+      case 3: launch_elem_p<2, 3>(m, ph, coef, Ke, Re, wk, wr, sym); break;
+      case 4: launch_elem_p<2, 4>(m, ph, coef, Ke, Re, wk, wr, sym); break;
       default: fail(YAFEL_HIP_EINVAL, "assemble: 2-D elements are supported for p = 1..4");
     }
   } else {
     switch (m->p) {
-      case 1: launch_elem_p<3, 1>(m, ph, coef, Ke, Re, wk, wr); break;
-      case 2: launch_elem_p<3, 2>(m, ph, coef, Ke, Re, wk, wr); break;
+      case 1: launch_elem_p<3, 1>(m, ph, coef, Ke, Re, wk, wr, sym); break;
+      case 2: launch_elem_p<3, 2>(m, ph, coef, Ke, Re, wk, wr, sym); break;
       default: fail(YAFEL_HIP_EINVAL, "assemble: 3-D elements are supported for p = 1..2");
     }
   }
@@ -557,6 +618,8 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
     m->stageK32.alloc((size_t)m->nel * nld * nld * 4);
   if (wr && m->stageR.bytes < (size_t)m->nel * nld * 8) m->stageR.alloc((size_t)m->nel * nld * 8);
   Phys ph = make_phys(phys);
+  // symmetric tangents (the reference mirrors B < A) stage half the blocks
+  const bool sym = !f32 && phys->kind != YAFEL_HIP_PHYS_SUPG;
   if (f32) {
     const unsigned g = (unsigned)((m->nel + 3) / 4);
     float *k32 = wk ? m->stageK32.as<float>() : nullptr;
@@ -572,7 +635,7 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
     HIPCHK(hipGetLastError());
   } else {
     launch_element(m, ph, coef, wk ? m->stageK.as<double>() : nullptr, wr ? m->stageR.as<double>() : nullptr, wk,
-                   wr);
+                   wr, sym);
   }
   if (wk) {
     YhPlan *pl = A->plan;
@@ -581,27 +644,21 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
     const int32_t *ip = A->iperm.p ? A->iperm.as<int32_t>() : nullptr;
     double *val = A->val64.as<double>();
     if (pl->nruns > 0) {
-      if (A->format == YAFEL_HIP_BCSR3)
-        k_gather<3, YAFEL_HIP_BCSR3><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
-                                                      m->stageK.as<double>(), pl->run_row.as<int32_t>(),
-                                                      pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val);
-      else if (f32)
-        k_gather<1, YAFEL_HIP_CSR, float><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(),
-                                                           pl->contrib.as<uint32_t>(), m->stageK32.as<float>(),
-                                                           pl->run_row.as<int32_t>(), pl->node_ptr.as<int32_t>(), sp,
-                                                           ip, A->ntot, val);
-      else if (m->dpn == 1)
-        k_gather<1, YAFEL_HIP_CSR><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
-                                                    m->stageK.as<double>(), pl->run_row.as<int32_t>(),
-                                                    pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val);
-      else if (m->dpn == 2)
-        k_gather<2, YAFEL_HIP_CSR><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
-                                                    m->stageK.as<double>(), pl->run_row.as<int32_t>(),
-                                                    pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val);
-      else
-        k_gather<3, YAFEL_HIP_CSR><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
-                                                    m->stageK.as<double>(), pl->run_row.as<int32_t>(),
-                                                    pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val);
+#define GATHER(DPN_, FMT_, KT_, SYM_, K_)                                                                        \
+  k_gather<DPN_, FMT_, KT_, SYM_><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(),                     \
+                                                   pl->contrib.as<uint32_t>(), K_, pl->run_row.as<int32_t>(),     \
+                                                   pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val)
+      const double *k64 = m->stageK.as<double>();
+      if (f32) GATHER(1, YAFEL_HIP_CSR, float, false, m->stageK32.as<float>());
+      else if (A->format == YAFEL_HIP_BCSR3 && sym) GATHER(3, YAFEL_HIP_BCSR3, double, true, k64);
+      else if (A->format == YAFEL_HIP_BCSR3) GATHER(3, YAFEL_HIP_BCSR3, double, false, k64);
+      else if (m->dpn == 1 && sym) GATHER(1, YAFEL_HIP_CSR, double, true, k64);
+      else if (m->dpn == 1) GATHER(1, YAFEL_HIP_CSR, double, false, k64);
+      else if (m->dpn == 2 && sym) GATHER(2, YAFEL_HIP_CSR, double, true, k64);
+      else if (m->dpn == 2) GATHER(2, YAFEL_HIP_CSR, double, false, k64);
+      else if (sym) GATHER(3, YAFEL_HIP_CSR, double, true, k64);
+      else GATHER(3, YAFEL_HIP_CSR, double, false, k64);
+#undef GATHER
       HIPCHK(hipGetLastError());
     }
     A->has32 = false;
