@@ -64,6 +64,9 @@ int yafel_hip_free(yafel_hip_ctx ctx, void *dptr);
 int yafel_hip_memcpy_h2d(yafel_hip_ctx ctx, void *dst_dev, const void *src, size_t bytes);
 int yafel_hip_memcpy_d2h(yafel_hip_ctx ctx, void *dst, const void *src_dev, size_t bytes);
 int yafel_hip_memcpy_d2d(yafel_hip_ctx ctx, void *dst_dev, const void *src_dev, size_t bytes);
+/* zero `bytes` of device memory (the Vector(n) zero fill, old_handmade_linalg/
+ * Vector.hpp:29-33, kept on the device) */
+int yafel_hip_memzero(yafel_hip_ctx ctx, void *dst_dev, size_t bytes);
 /* HIP-event timer on the context stream: begin, then end returns elapsed ms */
 int yafel_hip_timer_begin(yafel_hip_ctx ctx);
 int yafel_hip_timer_end(yafel_hip_ctx ctx, double *ms);

@@ -52,10 +52,14 @@ void hip_check(hipError_t e, const char *what, const char *file, int line);
 // alloc_stream (the serving setup worker) -- stream-ordered from the device's
 // memory pool on that stream, so that building one request never waits for
 // the kernels other streams are running (hipFree synchronises the device).
+struct TempPool;
+
 struct DBuf {
   void *p = nullptr;
   size_t bytes = 0;
-  hipStream_t s = nullptr;  // stream of a stream-ordered allocation
+  hipStream_t s = nullptr;   // stream of a stream-ordered allocation
+  size_t cap = 0;            // allocated bytes (size class) of a stream-ordered allocation
+  TempPool *pool = nullptr;  // the allocating thread's pool (stream-ordered allocations)
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
@@ -66,12 +70,16 @@ struct DBuf {
     std::swap(p, o.p);
     std::swap(bytes, o.bytes);
     std::swap(s, o.s);
+    std::swap(cap, o.cap);
+    std::swap(pool, o.pool);
   }
   template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
 
 // Stream for the stream-ordered allocations of the calling thread (nullptr:
-// plain hipMalloc/hipFree).
+// plain hipMalloc/hipFree).  With a stream set, buffers the thread releases
+// itself are kept in a per-thread pool for its next allocations; resetting
+// frees the pool.
 void set_alloc_stream(hipStream_t s);
 
 }  // namespace yh

@@ -2,6 +2,8 @@
 // generators of the C ABI.
 #include <cmath>
 #include <stdexcept>
+#include <unordered_map>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -29,14 +31,63 @@ void record_error(yafel_hip_ctx_s *ctx, const std::string &msg) {
 
 static thread_local hipStream_t g_alloc_stream = nullptr;
 
-void set_alloc_stream(hipStream_t s) { g_alloc_stream = s; }
+// A thread with an allocation stream (a serving setup worker) keeps the
+// buffers it releases itself in size classes and hands them out again: its
+// work is ordered on that one stream, so a buffer freed by one request's
+// set-up is ready for the next one's without a sync.  Buffers released on
+// another thread (a finished request's matrix) are freed stream-ordered.
+struct TempPool {
+  hipStream_t s = nullptr;
+  std::unordered_map<size_t, std::vector<void *>> free;
+  size_t cached = 0;
+  static constexpr size_t kMaxCached = (size_t)4 << 30;
+  void *take(size_t sc) {
+    auto it = free.find(sc);
+    if (it == free.end() || it->second.empty()) return nullptr;
+    void *p = it->second.back();
+    it->second.pop_back();
+    cached -= sc;
+    return p;
+  }
+  bool put(void *p, size_t sc) {
+    if (cached + sc > kMaxCached) return false;
+    free[sc].push_back(p);
+    cached += sc;
+    return true;
+  }
+  ~TempPool() {
+    for (auto &kv : free)
+      for (void *p : kv.second) (void)hipFreeAsync(p, s);
+  }
+};
+static thread_local TempPool *g_pool = nullptr;
+
+static size_t size_class(size_t b) {
+  size_t sc = 4096;
+  while (sc < b) sc <<= 1;
+  return sc;
+}
+
+void set_alloc_stream(hipStream_t s) {
+  delete g_pool;
+  g_pool = nullptr;
+  g_alloc_stream = s;
+  if (s) {
+    g_pool = new TempPool();
+    g_pool->s = s;
+  }
+}
 
 void DBuf::alloc(size_t b) {
   release();
   if (b == 0) return;
   if (g_alloc_stream) {
-    HIPCHK(hipMallocAsync(&p, b, g_alloc_stream));
+    const size_t sc = size_class(b);
+    p = g_pool ? g_pool->take(sc) : nullptr;
+    if (!p) HIPCHK(hipMallocAsync(&p, sc, g_alloc_stream));
     s = g_alloc_stream;
+    cap = sc;
+    pool = g_pool;
   } else {
     HIPCHK(hipMalloc(&p, b));
   }
@@ -44,12 +95,19 @@ void DBuf::alloc(size_t b) {
 }
 void DBuf::release() {
   if (p) {
-    if (s) (void)hipFreeAsync(p, s);
-    else (void)hipFree(p);
+    if (pool && pool == g_pool && s == g_pool->s && g_pool->put(p, cap)) {
+      // kept for this thread's next allocation of the same class
+    } else if (s) {
+      (void)hipFreeAsync(p, s);
+    } else {
+      (void)hipFree(p);
+    }
   }
   p = nullptr;
   bytes = 0;
   s = nullptr;
+  cap = 0;
+  pool = nullptr;
 }
 
 int grid_for(yafel_hip_ctx_s *ctx, int64_t work_items, int block) {
@@ -196,6 +254,14 @@ int yafel_hip_memcpy_d2d(yafel_hip_ctx ctx, void *dst, const void *src, size_t b
   YH_TRY_BEGIN
   REQUIRE(ctx && (bytes == 0 || (dst && src)), "yafel_hip_memcpy_d2d: null argument");
   if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return YAFEL_HIP_OK;
+  YH_TRY_END(ctx)
+}
+
+int yafel_hip_memzero(yafel_hip_ctx ctx, void *dst, size_t bytes) {
+  YH_TRY_BEGIN
+  REQUIRE(ctx && (bytes == 0 || dst), "yafel_hip_memzero: null argument");
+  if (bytes) HIPCHK(hipMemsetAsync(dst, 0, bytes, ctx->stream));
   return YAFEL_HIP_OK;
   YH_TRY_END(ctx)
 }

@@ -9,6 +9,13 @@
 
 namespace yh {
 
+// The temporaries are released when a wrapper returns.  Stream-ordered ones
+// (a setup thread's allocation stream, the sort's stream) are freed in
+// stream order; plain ones need the sort to have finished first.
+static void finish_temps(const DBuf &t, hipStream_t s) {
+  if (t.s != s) HIPCHK(hipStreamSynchronize(s));
+}
+
 void sort_pairs_u64_u32(uint64_t *keys, uint32_t *vals, int64_t n, int end_bit, hipStream_t s) {
   if (n <= 1) return;
   DBuf kt, vt, tmp;
@@ -22,7 +29,7 @@ void sort_pairs_u64_u32(uint64_t *keys, uint32_t *vals, int64_t n, int end_bit, 
                                    (size_t)n, 0u, (unsigned)end_bit, s));
   HIPCHK(hipMemcpyAsync(keys, kt.p, n * 8, hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(vals, vt.p, n * 4, hipMemcpyDeviceToDevice, s));
-  HIPCHK(hipStreamSynchronize(s));
+  finish_temps(kt, s);
 }
 
 void sort_pairs_u64_u64(uint64_t *keys, uint64_t *vals, int64_t n, int end_bit, hipStream_t s) {
@@ -38,7 +45,7 @@ void sort_pairs_u64_u64(uint64_t *keys, uint64_t *vals, int64_t n, int end_bit, 
                                    (size_t)n, 0u, (unsigned)end_bit, s));
   HIPCHK(hipMemcpyAsync(keys, kt.p, n * 8, hipMemcpyDeviceToDevice, s));
   HIPCHK(hipMemcpyAsync(vals, vt.p, n * 8, hipMemcpyDeviceToDevice, s));
-  HIPCHK(hipStreamSynchronize(s));
+  finish_temps(kt, s);
 }
 
 void exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s) {
@@ -48,7 +55,7 @@ void exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t 
   HIPCHK(rocprim::exclusive_scan(nullptr, tb, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), s));
   tmp.alloc(tb + 16);
   HIPCHK(rocprim::exclusive_scan(tmp.p, tb, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), s));
-  HIPCHK(hipStreamSynchronize(s));
+  finish_temps(tmp, s);
 }
 
 void inclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n, hipStream_t s) {
@@ -58,7 +65,7 @@ void inclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n, hipStream_t 
   HIPCHK(rocprim::inclusive_scan(nullptr, tb, in, out, (size_t)n, rocprim::plus<int32_t>(), s));
   tmp.alloc(tb + 16);
   HIPCHK(rocprim::inclusive_scan(tmp.p, tb, in, out, (size_t)n, rocprim::plus<int32_t>(), s));
-  HIPCHK(hipStreamSynchronize(s));
+  finish_temps(tmp, s);
 }
 
 }  // namespace yh

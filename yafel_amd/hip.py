@@ -82,7 +82,7 @@ for _n, _a in [
     ("yafel_hip_ctx_sync", [_P]), ("yafel_hip_malloc", [_P, C.c_size_t, C.POINTER(_P)]),
     ("yafel_hip_free", [_P, _P]), ("yafel_hip_memcpy_h2d", [_P, _P, _P, C.c_size_t]),
     ("yafel_hip_memcpy_d2h", [_P, _P, _P, C.c_size_t]),
-    ("yafel_hip_memcpy_d2d", [_P, _P, _P, C.c_size_t]),
+    ("yafel_hip_memcpy_d2d", [_P, _P, _P, C.c_size_t]), ("yafel_hip_memzero", [_P, _P, C.c_size_t]),
     ("yafel_hip_timer_begin", [_P]), ("yafel_hip_timer_end", [_P, C.POINTER(_D)]),
     ("yafel_hip_csr_create", [_P, _I64, _I64, _P, _P, _P, _I, C.POINTER(_P)]),
     ("yafel_hip_bcsr3_create", [_P, _I64, _I64, _P, _P, _P, _I, C.POINTER(_P)]),
@@ -227,6 +227,10 @@ class DeviceArray:
 
     def copy_from(self, other):
         check(lib.yafel_hip_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, self.nbytes), self.ctx.h)
+
+    def zero(self):
+        check(lib.yafel_hip_memzero(self.ctx.h, self.ptr, self.nbytes), self.ctx.h)
+        return self
 
     def free(self):
         if self.ptr:

@@ -61,14 +61,14 @@ class Workload:
     def assemble(self):
         """CGAssembly + DirichletBC::apply; leaves A and b ready to solve."""
         if self.rhs_given is None:
-            self.b.upload(np.zeros(self.n))
+            self.b.zero()
         else:
             self.b.copy_from(self.rhs_given)
         self.mesh.assemble(self.ph, self.A, self.b if self.req & hip.RESIDUAL else None, self.req | self.extra_req)
         self.A.dirichlet(self.mask, self.b)
 
     def solve(self, **kw):
-        self.x.upload(np.zeros(self.n))
+        self.x.zero()
         return self.A.solve(self.b, self.x, **kw)
 
     def step(self, **kw):
