@@ -1,7 +1,7 @@
 // Krylov building blocks shared by the single-solve driver (krylov.hip) and
 // the batched serving driver (serve.hip): the device solver state, the fused
-// SpMV / elementwise kernel bodies with deterministic last-block reductions,
-// and the per-method operations.
+// SpMV / elementwise kernel bodies with deterministic reductions (fixed-order
+// folds of per-workgroup partials), and the per-method operations.
 #pragma once
 #include <cmath>
 
@@ -27,7 +27,7 @@ struct KState {
   long long iter, max_iter, refinements;
   int done, status, paused, first;
   int method, stop, mixed, special;
-  int pidx;       // serving CG: which p buffer holds the current p
+  int pad0;
   int pupd;       // single-solve CG: p = z + beta p pending (after a residual)
   unsigned int counter[8];
 };
@@ -217,76 +217,6 @@ __device__ void after_iteration(KState *st, double rho_new, double rr) {
   }
 }
 
-// CG kernel A: q = A p_new with p_new = z + beta p_old.  The two p buffers
-// alternate; st->pidx names the one holding the current p, so iterations
-// skipped after a pause or convergence never desynchronise them.
-template <class T>
-struct OpCgA : OpBase {
-  using value_type = T;
-  static constexpr bool kTimed = true;  // the CG iteration's SpMV kernel
-  static constexpr int NP = 1;
-  const T *z;
-  T *pbuf0, *pbuf1;
-  T *q;
-  const T *pold;
-  T *pnew;
-  int cur;
-  T beta;
-  __device__ bool skip() const { return !running(); }
-  __device__ void begin() {
-    beta = st->first ? T(0) : (T)st->beta;
-    cur = st->pidx;
-    pold = cur ? pbuf1 : pbuf0;
-    pnew = cur ? pbuf0 : pbuf1;
-  }
-  __device__ T x(int64_t c) const { return fma(beta, pold[c], z[c]); }
-  __device__ void row(int64_t i, T y, double *acc) const {
-    const T pn = fma(beta, pold[i], z[i]);
-    pnew[i] = pn;
-    q[i] = y;
-    acc[0] += (double)pn * (double)y;
-  }
-  __device__ void last(const double *s) const {
-    const double a = st->rho / s[0];
-    st->first = 0;
-    st->alpha = a;
-    st->pidx = cur ^ 1;
-    if (s[0] == 0.0 || bad(a)) {
-      st->status = YAFEL_HIP_STATUS_BREAKDOWN;
-      st->done = 1;
-    }
-  }
-};
-
-// CG kernel B: x += alpha p, r -= alpha q, z = D^-1 r.
-template <class T, bool PREC>
-struct OpCgB : OpBase {
-  static constexpr int NP = 2;
-  T *x, *r, *z;
-  const T *pbuf0, *pbuf1, *q, *dinv;
-  const T *p;
-  T alpha;
-  __device__ bool skip() const { return !running(); }
-  __device__ void begin() {
-    alpha = (T)st->alpha;
-    p = st->pidx ? pbuf1 : pbuf0;
-  }
-  __device__ void elem(int64_t i, double *acc) const {
-    x[i] = fma(alpha, p[i], x[i]);
-    const T ri = fma(-alpha, q[i], r[i]);
-    r[i] = ri;
-    if (PREC) {
-      const T zi = dinv[i] * ri;
-      z[i] = zi;
-      acc[0] += (double)ri * (double)zi;
-    } else {
-      acc[0] += (double)ri * (double)ri;
-    }
-    acc[1] += (double)ri * (double)ri;
-  }
-  __device__ void last(const double *s) const { after_iteration(st, s[0], s[1]); }
-};
-
 // FP64 residual r = b - A x_hi; stores r (in the inner precision TL) and z,
 // and folds r.w (w = z, r or rhat), r.r, the FP64 true r.r and b.b.
 // MODE 0 = initialisation, 1 = refinement / verification.
@@ -339,7 +269,6 @@ struct OpResidual : OpBase {
       st->omega = 1.0;
       st->beta = 0.0;
       st->paused = 0;
-      st->pidx = 0;
       st->pupd = cgfam ? 1 : 0;
       // early exits of the reference
       if (cgfam && st->stop == YAFEL_HIP_STOP_REFERENCE && s[0] == 0.0) {
@@ -613,14 +542,15 @@ constexpr int VE = 4;
 constexpr int KSW = (int)(sizeof(KState) / 8);
 static_assert(sizeof(KState) % 8 == 0 && KSW <= NT, "KState words");
 
+// The body takes its block id and block count explicitly, so that a batched
+// launch can run several solves on their own block ranges; block 0 of the
+// range publishes the state.
 template <class Op>
-__global__ __launch_bounds__(NT) void k_velem(int64_t n, Op op0) {
-  Op op = op0;
+__device__ __forceinline__ void velem_body(int64_t n, Op &op, int blk, int nblk) {
   __shared__ KState ss;
   __shared__ int sgo;
   const int t = threadIdx.x;
-  const bool b0 = blockIdx.x == 0;
-  const int blk = dev::xcd_block(), nblk = gridDim.x;
+  const bool b0 = blk == 0;
   const int64_t chunk = ((n + nblk - 1) / nblk + 63) & ~(int64_t)63;
   const int64_t i0 = (int64_t)blk * chunk + t;
   const int64_t i1 = (int64_t)blk * chunk + chunk < n ? (int64_t)blk * chunk + chunk : n;
@@ -660,6 +590,12 @@ __global__ __launch_bounds__(NT) void k_velem(int64_t n, Op op0) {
     op.compute(i, tmp, acc, go);
   }
   finish_partials<NP>(op, acc, blk, nblk);
+}
+
+template <class Op>
+__global__ __launch_bounds__(NT) void k_velem(int64_t n, Op op0) {
+  Op op = op0;
+  velem_body(n, op, dev::xcd_block(), gridDim.x);
 }
 
 struct VOpBase {

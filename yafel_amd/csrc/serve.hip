@@ -5,8 +5,11 @@
 // device.  A table of slot descriptors drives batched launches: one launch
 // covers every active slot, each slot on its own contiguous block range
 // (block counts fixed by the slot's size alone) with its own FP64 scalars and
-// its own last-block reductions -- so a request's result is bit-identical
-// whether it runs alone or in a mix.
+// its own folds of its partial sums -- so a request's result is bit-identical
+// whether it runs alone or in a mix.  A CG iteration is the single-solve
+// three-pass iteration (Q: SpMV, U and P: folds in every workgroup of the
+// slot) with each slot's state in a versioned pair; every slot of the table
+// takes part in every versioned launch, so one parity serves them all.
 //
 // Two host threads drive the device.  A setup worker owned by the server takes
 // queued requests in FIFO order as the HBM budget and the slot limit allow and
@@ -44,10 +47,10 @@ struct SlotDesc {
   const float *v32;
   const double *v64;
   const float *dinv;
-  float *r, *z, *p0, *p1, *q, *xl;
+  float *r, *z, *p, *q, *xl;
   double *x;
   const double *b;
-  KState *st;
+  KState *st;  // two versions: st[0], st[1]
   double *part;
   double tol;
   long long max_iter;
@@ -64,13 +67,34 @@ __device__ __forceinline__ int find_slot(const SlotDesc *d, int ns, int b, bool 
   return lo;
 }
 
-enum { PH_INIT = 0, PH_A = 1, PH_B = 2, PH_FOLD = 3, PH_REF = 4 };
+enum { PH_INIT = 0, PH_Q = 1, PH_U = 2, PH_P = 3, PH_P0 = 4, PH_FOLD = 5, PH_REF = 6 };
+
+template <int PH>
+constexpr bool spmv_phase() {
+  return PH == PH_INIT || PH == PH_Q || PH == PH_REF;
+}
 
 template <class O>
-__device__ __forceinline__ void bind(O &op, const SlotDesc &d, int counter) {
-  op.st = d.st;
+__device__ __forceinline__ void bind(O &op, const SlotDesc &d, int par) {
+  op.st = d.st + par;
   op.part = d.part;
-  op.counter = d.st->counter + counter;
+  op.counter = d.st[par].counter;
+}
+
+// a versioned pass of the slot: state version par in, par ^ 1 out
+template <class O>
+__device__ __forceinline__ void vbind(O &op, const SlotDesc &d, int par, const double *pin, int nin,
+                                      double *ppart) {
+  op.in = d.st + par;
+  op.out = d.st + (par ^ 1);
+  op.pin = pin;
+  op.nin = nin;
+  op.part = ppart;
+}
+
+// partial regions of a slot: the SpMV's at 0, the update pass's at 2 gmax
+__device__ __forceinline__ double *upart(const SlotDesc &d) {
+  return d.part + 2 * (size_t)(d.g_spmv > d.g_elem ? d.g_spmv : d.g_elem);
 }
 
 template <class T, class O>
@@ -81,14 +105,15 @@ __device__ __forceinline__ void run_spmv(const SlotDesc &d, const T *val, O &op,
 }
 
 // Builds the phase's operation for slot d and either runs the phase body on
-// block blk (FIN = false) or folds the slot's partials (FIN = true).
+// block blk (FIN = false) or folds the slot's partials (FIN = true; the
+// residual phases only).  par is the state version of the launch.
 template <int PH, bool FIN>
-__device__ __forceinline__ void slot_phase(const SlotDesc &d, int blk) {
-  constexpr bool SP = PH == PH_INIT || PH == PH_A || PH == PH_REF;
+__device__ __forceinline__ void slot_phase(const SlotDesc &d, int blk, int par) {
+  constexpr bool SP = spmv_phase<PH>();
   const int nblk = SP ? d.g_spmv : d.g_elem;
   if (PH == PH_INIT || PH == PH_REF) {
     OpResidual<float, PH == PH_INIT ? 0 : 1> op{};
-    bind(op, d, 0);
+    bind(op, d, par);
     op.xh = d.x;
     op.b = d.b;
     op.r = d.r;
@@ -98,30 +123,36 @@ __device__ __forceinline__ void slot_phase(const SlotDesc &d, int blk) {
     op.wkind = 1;
     if (FIN) finish_body(op, nblk);
     else run_spmv<double>(d, d.v64, op, blk);
-  } else if (PH == PH_A) {
-    OpCgA<float> op{};
-    bind(op, d, 1);
-    op.z = d.z;
-    op.pbuf0 = d.p0;
-    op.pbuf1 = d.p1;
+  } else if (PH == PH_Q) {
+    OpCgQ<float> op{};
+    bind(op, d, par);
+    op.p = d.p;
     op.q = d.q;
-    if (FIN) finish_body(op, nblk);
-    else run_spmv<float>(d, d.v32, op, blk);
-  } else if (PH == PH_B) {
-    OpCgB<float, true> op{};
-    bind(op, d, 2);
-    op.x = d.xl;
+    if (!FIN) run_spmv<float>(d, d.v32, op, blk);
+  } else if (PH == PH_U) {
+    OpCgU<float, true> op{};
+    vbind(op, d, par, d.part, d.g_spmv, upart(d));
     op.r = d.r;
     op.z = d.z;
-    op.pbuf0 = d.p0;
-    op.pbuf1 = d.p1;
     op.q = d.q;
     op.dinv = d.dinv;
-    if (FIN) finish_body(op, nblk);
-    else elem_body(d.n, op, blk, nblk);
+    if (!FIN) velem_body(d.n, op, blk, nblk);
+  } else if (PH == PH_P) {
+    OpCgP<float> op{};
+    vbind(op, d, par, upart(d), d.g_elem, nullptr);
+    op.p = d.p;
+    op.x = d.xl;
+    op.z = d.z;
+    if (!FIN) velem_body(d.n, op, blk, nblk);
+  } else if (PH == PH_P0) {
+    OpCgP0<float> op{};
+    vbind(op, d, par, nullptr, 0, nullptr);
+    op.p = d.p;
+    op.z = d.z;
+    if (!FIN) velem_body(d.n, op, blk, nblk);
   } else if (!FIN) {
     OpFold<float> op{};
-    bind(op, d, 6);
+    bind(op, d, par);
     op.xh = d.x;
     op.xl = d.xl;
     elem_body(d.n, op, blk, nblk);
@@ -129,25 +160,25 @@ __device__ __forceinline__ void slot_phase(const SlotDesc &d, int blk) {
 }
 
 template <int PH>
-__global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slots, int ns) {
-  constexpr bool SP = PH == PH_INIT || PH == PH_A || PH == PH_REF;
+__global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slots, int ns, int par) {
+  constexpr bool SP = spmv_phase<PH>();
   const int s = find_slot(slots, ns, blockIdx.x, SP);
   const SlotDesc d = slots[s];
-  slot_phase<PH, false>(d, blockIdx.x - (SP ? d.off_spmv : d.off_elem));
+  slot_phase<PH, false>(d, blockIdx.x - (SP ? d.off_spmv : d.off_elem), par);
 }
 
 // one workgroup per slot folds that slot's partials
 template <int PH>
-__global__ __launch_bounds__(NTF) void kb_finish(const SlotDesc *__restrict__ slots, int ns) {
+__global__ __launch_bounds__(NTF) void kb_finish(const SlotDesc *__restrict__ slots, int ns, int par) {
   const SlotDesc d = slots[blockIdx.x];
-  slot_phase<PH, true>(d, 0);
+  slot_phase<PH, true>(d, 0, par);
 }
 
-// zero a joining slot's solver state and set its configuration
-__global__ void kb_init_state(const SlotDesc *slots, int ns) {
+// zero a joining slot's solver state (version par) and set its configuration
+__global__ void kb_init_state(const SlotDesc *slots, int ns, int par) {
   const int j = blockIdx.x;
   if (j >= ns) return;
-  KState *st = slots[j].st;
+  KState *st = slots[j].st + par;
   unsigned int *w = (unsigned int *)st;
   for (int i = threadIdx.x; i < (int)(sizeof(KState) / 4); i += blockDim.x) w[i] = 0u;
   __syncthreads();
@@ -167,10 +198,10 @@ struct SlotStatus {
   double crit;
 };
 
-__global__ void kb_status(const SlotDesc *slots, int ns, SlotStatus *out) {
+__global__ void kb_status(const SlotDesc *slots, int ns, int par, SlotStatus *out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ns) return;
-  const KState *st = slots[j].st;
+  const KState *st = slots[j].st + par;
   out[j].done = st->done;
   out[j].status = st->status;
   out[j].iter = st->iter;
@@ -242,6 +273,7 @@ struct yafel_hip_server_s {
   hipEvent_t evs[RING];
   char *pin = nullptr;  // RING status areas, then RING table staging areas
   int K = 8;
+  int64_t ver = 0;  // state version parity of the next launch (every slot's)
   std::vector<Slot *> joining;
   hipStream_t copy_stream = nullptr;  // result copies to the host
   // shared with the setup worker, guarded by mu
@@ -354,8 +386,8 @@ void join_slot(yafel_hip_server_s *S, Slot *sl) {
   Req *r = sl->req;
   const int g_spmv = sell_grid(A->nsl, GCAP);
   const int g_elem = std::max(1, std::min<int>(GCAP, ceil_div(n, NT)));
-  const size_t nbytes = 6 * (((size_t)n * 4 + 255) & ~(size_t)255) + (((size_t)n * 8 + 255) & ~(size_t)255) +
-                        ((sizeof(KState) + 255) & ~(size_t)255) + (size_t)4 * std::max(g_spmv, g_elem) * 8 + 256;
+  const size_t nbytes = 5 * (((size_t)n * 4 + 255) & ~(size_t)255) + (((size_t)n * 8 + 255) & ~(size_t)255) +
+                        ((2 * sizeof(KState) + 255) & ~(size_t)255) + (size_t)4 * std::max(g_spmv, g_elem) * 8 + 256;
   HIPCHK(hipMallocAsync(&sl->buf, nbytes, S->ctx->stream));
   sl->buf_bytes = nbytes;
   char *p = (char *)sl->buf;
@@ -375,18 +407,17 @@ void join_slot(yafel_hip_server_s *S, Slot *sl) {
   d.dinv = A->dinv32.as<float>();
   d.r = carve<float>(p, n);
   d.z = carve<float>(p, n);
-  d.p0 = carve<float>(p, n);
-  d.p1 = carve<float>(p, n);
+  d.p = carve<float>(p, n);
   d.q = carve<float>(p, n);
   d.xl = carve<float>(p, n);
   d.x = carve<double>(p, n);
-  d.st = carve<KState>(p, 1);
+  d.st = carve<KState>(p, 2);
   d.part = carve<double>(p, (size_t)4 * std::max(g_spmv, g_elem));
   d.b = sl->b.as<double>();
   hipStream_t s = S->ctx->stream;
   d.tol = r->tol;
   d.max_iter = r->max_iter > 0 ? r->max_iter : n;
-  HIPCHK(hipMemsetAsync(d.p0, 0, n * 4, s));
+  HIPCHK(hipMemsetAsync(d.p, 0, n * 4, s));
   HIPCHK(hipMemsetAsync(d.xl, 0, n * 4, s));
   HIPCHK(hipMemsetAsync(d.x, 0, n * 8, s));
 }
@@ -485,15 +516,20 @@ int upload_table(yafel_hip_server_s *S, const std::vector<Slot *> &slots, int *t
   return e;
 }
 
+// One batched launch of phase PH over the table; the residual phases are
+// followed by their per-slot folds, the versioned passes advance the state
+// version of every slot in the table.
 template <int PH>
 void launch_phase(yafel_hip_server_s *S, int e, int ns, int blocks) {
   if (ns == 0 || blocks == 0) return;
-  kb_phase<PH><<<blocks, NT, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns);
+  const int par = (int)(S->ver & 1);
+  kb_phase<PH><<<blocks, NT, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns, par);
   HIPCHK(hipGetLastError());
-  if (PH != PH_FOLD) {
-    kb_finish<PH><<<ns, NTF, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns);
+  if (PH == PH_INIT || PH == PH_REF) {
+    kb_finish<PH><<<ns, NTF, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns, par);
     HIPCHK(hipGetLastError());
   }
+  if (PH == PH_U || PH == PH_P || PH == PH_P0) ++S->ver;
   S->stats.batched_launches++;
 }
 
@@ -684,7 +720,7 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
     int ts, te;
     const int e = upload_table(S, S->joining, &ts, &te);
     kb_init_state<<<(int)S->joining.size(), 64, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(),
-                                                                     (int)S->joining.size());
+                                                                     (int)S->joining.size(), (int)(S->ver & 1));
     HIPCHK(hipGetLastError());
     launch_phase<PH_INIT>(S, e, (int)S->joining.size(), ts);
     for (Slot *sl : S->joining) S->active.push_back(sl);
@@ -695,13 +731,16 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
     int ts, te;
     const int e = upload_table(S, S->active, &ts, &te);
     const int ns = (int)S->active.size();
+    // p = z + beta p after a slot's initial or refinement residual
+    launch_phase<PH_P0>(S, e, ns, te);
     for (int k = 0; k < S->K; ++k) {
-      launch_phase<PH_A>(S, e, ns, ts);
-      launch_phase<PH_B>(S, e, ns, te);
+      launch_phase<PH_Q>(S, e, ns, ts);
+      launch_phase<PH_U>(S, e, ns, te);
+      launch_phase<PH_P>(S, e, ns, te);
     }
     launch_phase<PH_FOLD>(S, e, ns, te);
     launch_phase<PH_REF>(S, e, ns, ts);
-    kb_status<<<ceil_div(ns, 64), 64, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns,
+    kb_status<<<ceil_div(ns, 64), 64, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns, (int)(S->ver & 1),
                                                             S->status[e].as<SlotStatus>());
     HIPCHK(hipGetLastError());
     yafel_hip_server_s::Inflight f;
