@@ -141,8 +141,10 @@ int yafel_hip_copy_bandwidth(yafel_hip_ctx ctx, size_t bytes, int reps, double *
 
 /* No reference counterpart: HIP-event timing of every launch of the CG
  * iteration's fused SpMV kernel (kernel A) on ctx, for roofline figures.
- * Enabling (or disabling) resets the record; kernel_time returns the summed
- * duration and the launch count (stream-synchronising). */
+ * Each timed launch carries its start/stop event pair (hipExtLaunchKernel),
+ * so an event pair spans that dispatch alone.  Enabling (or disabling)
+ * resets the record; kernel_time returns the summed duration and the launch
+ * count (stream-synchronising). */
 int yafel_hip_kernel_timing(yafel_hip_ctx ctx, int enable);
 int yafel_hip_kernel_time(yafel_hip_ctx ctx, double *ms_total, int64_t *launches);
 

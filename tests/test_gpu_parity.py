@@ -117,7 +117,7 @@ def test_spmm_matches_columnwise_spmv(H, ctx, golden):
             assert np.abs(Y[:, j] - yj).max() <= 1e-13 * np.abs(yj).max()
 
 
-@pytest.mark.parametrize("k", [1, 37, 64, 70])
+@pytest.mark.parametrize("k", [1, 4, 37, 64, 68, 70])
 def test_spmm_sorted_rows_matches_columnwise_spmv(H, ctx, k):
     """A Q2 pattern (rows sorted by length inside the slices, so stored
     positions differ from rows) with k right-hand sides, FP64 and FP32."""
@@ -375,8 +375,12 @@ def test_error_paths(H, ctx):
 # ------------------------------------------------------------ multi-RHS
 @pytest.mark.parametrize("method", ["BICGSTAB_JACOBI", "PCG_JACOBI"])
 @pytest.mark.parametrize("prec", ["FP64", "MIXED"])
-def test_multi_rhs_alone_vs_together(H, ctx, orc, method, prec):
-    """k right-hand sides together reach the same solutions as one at a time."""
+@pytest.mark.parametrize("k", [70, 68])
+def test_multi_rhs_alone_vs_together(H, ctx, orc, method, prec, k):
+    """k right-hand sides together reach the same solutions as one at a time:
+    batches of 64 + 6 (FP32 products four right-hand sides per lane, then one)
+    and 64 + 4 with a zero right-hand side (inactive from the start, so the
+    products store only some of a lane's four columns)."""
     n = 12
     if method == "BICGSTAB_JACOBI":
         kw = dict(U=(0.8660254037844387, 0.5, 0.0), kdiff=1.0 / n / 200, h=1.0 / n, alpha=0.99)
@@ -389,16 +393,20 @@ def test_multi_rhs_alone_vs_together(H, ctx, orc, method, prec):
     _, dn = m.download()
     mask = edge_mask(dn, 2)
     A.dirichlet(ctx.upload(mask), ctx.upload(np.zeros(m.ndofs)))
-    k = 70  # two batches (64 + 6)
     B = np.stack([H.normal(100 + j, m.ndofs) for j in range(k)], axis=1)
     B[mask == 1, :] = 0.0
+    if k == 68:
+        B[:, 2] = 0.0
     Bd, Xd = ctx.upload(B.ravel()), ctx.upload(np.zeros(m.ndofs * k))
     infos = A.solve_multi(Bd, Xd, k, method=getattr(H, method), precision=getattr(H, prec),
                           stop=H.STOP_RELRES, tol=1e-8)
     X = Xd.download().reshape(m.ndofs, k)
     ptr, col, val = A.arrays()
     O = orc.wrap(1, m.ndofs, m.ndofs, ptr, col, val)
-    for j in (0, 1, 37, 63, 64, 69):
+    if k == 68:
+        assert infos[2].status == 0 and infos[2].iterations == 0
+        assert not X[:, 2].any()
+    for j in (0, 1, 37, 63, 64, k - 1):
         assert infos[j].status == 0 and infos[j].rel_residual < 1e-8
         r = B[:, j] - O.spmv(X[:, j])
         assert np.linalg.norm(r) / np.linalg.norm(B[:, j]) < 1e-8

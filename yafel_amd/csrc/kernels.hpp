@@ -167,5 +167,50 @@ __device__ __forceinline__ void sell_rows_multi(const int *__restrict__ sptr, co
     vl = vn;
   }
 }
+// FP32 multi-RHS rows of one slice, four rows per gather instruction: lane
+// (rs = lane >> 4, g = lane & 15) takes right-hand sides 4g .. 4g+3 of rows
+// h*RB + 4i + rs, so a 16-byte gather per lane covers four rows' 256-byte X
+// rows at once.  Entries and their order as in sell_rows_multi; xf4(c) returns
+// this lane's four x values at column c and must be safe in every lane.
+template <int RB, class XF4>
+__device__ __forceinline__ void sell_rows_multi4(const int *__restrict__ sptr, const int *__restrict__ col,
+                                                 const float *__restrict__ val, const float *__restrict__ cs,
+                                                 int64_t s, int h, int lane, bool on, XF4 xf4,
+                                                 float4 (&y)[RB / 4]) {
+  const int rs = lane >> 4;
+  const int p0 = sptr[s], ns = sptr[s + 1] - p0;
+#pragma unroll
+  for (int i = 0; i < RB / 4; ++i) y[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (ns == 0) return;
+  int64_t q = (int64_t)p0 * 64 + lane;
+  int cl = __builtin_nontemporal_load(&col[q]);
+  float vl = __builtin_nontemporal_load(&val[q]);
+  if (cs) vl *= cs[cl];
+  for (int t = 0; t < ns; ++t) {
+    int cn = 0;
+    float vn = 0;
+    if (t + 1 < ns) {
+      q += 64;
+      cn = __builtin_nontemporal_load(&col[q]);
+      vn = __builtin_nontemporal_load(&val[q]);
+      if (cs) vn *= cs[cn];
+    }
+#pragma unroll
+    for (int i = 0; i < RB / 4; ++i) {
+      const int src = h * RB + 4 * i + rs;
+      const int c = __shfl(cl, src, 64);
+      const float v = __shfl(vl, src, 64);
+      float4 x = xf4(c);
+      if (!on) x = make_float4(0.f, 0.f, 0.f, 0.f);
+      y[i].x = fmaf(v, x.x, y[i].x);
+      y[i].y = fmaf(v, x.y, y[i].y);
+      y[i].z = fmaf(v, x.z, y[i].z);
+      y[i].w = fmaf(v, x.w, y[i].w);
+    }
+    cl = cn;
+    vl = vn;
+  }
+}
+
 }  // namespace dev
 }  // namespace yh

@@ -23,6 +23,7 @@
 // direction), and the FP64 true residual decides convergence.
 #include <chrono>
 #include <cmath>
+#include <hip/hip_ext.h>
 
 #include "internal.hpp"
 #include "kernels.hpp"
@@ -50,11 +51,11 @@ void launch_spmv_op(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
   const SellView<T> v = view<T>(A);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
-  if (e0) HIPCHK(hipEventRecord(e0, s));
-  if (A->format == YAFEL_HIP_CSR) k_spmv_fused<YAFEL_HIP_CSR, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
-  else k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
+  auto kern = A->format == YAFEL_HIP_CSR ? k_spmv_fused<YAFEL_HIP_CSR, kSellUnroll, T, Op>
+                                         : k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op>;
+  if (e0) hipExtLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, s, e0, e1, 0, v, op);
+  else kern<<<grid, NT, 0, s>>>(v, op);
   HIPCHK(hipGetLastError());
-  if (e1) HIPCHK(hipEventRecord(e1, s));
   if constexpr (Op::NP > 0) {
     k_finish<Op><<<1, NTF, 0, s>>>(op, grid);
     HIPCHK(hipGetLastError());
@@ -68,15 +69,14 @@ int launch_spmv_nofold(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
   const SellView<T> v = view<T>(A);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
-  if (e0) HIPCHK(hipEventRecord(e0, s));
-  if (A->format == YAFEL_HIP_CSR && A->unroll == 8)
-    k_spmv_fused<YAFEL_HIP_CSR, 8, T, Op><<<grid, NT, 0, s>>>(v, op);
-  else if (A->format == YAFEL_HIP_CSR)
-    k_spmv_fused<YAFEL_HIP_CSR, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
-  else
-    k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op><<<grid, NT, 0, s>>>(v, op);
+  auto kern = A->format == YAFEL_HIP_CSR && A->unroll == 8 ? k_spmv_fused<YAFEL_HIP_CSR, 8, T, Op>
+              : A->format == YAFEL_HIP_CSR               ? k_spmv_fused<YAFEL_HIP_CSR, kSellUnroll, T, Op>
+                                                         : k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op>;
+  // timed launches carry their own start/stop events: the events then hold the
+  // dispatch's begin and end time stamps, not those of separate markers
+  if (e0) hipExtLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, s, e0, e1, 0, v, op);
+  else kern<<<grid, NT, 0, s>>>(v, op);
   HIPCHK(hipGetLastError());
-  if (e1) HIPCHK(hipEventRecord(e1, s));
   return grid;
 }
 

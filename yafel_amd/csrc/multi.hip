@@ -10,6 +10,12 @@
 // Methods: BiCGSTAB with right Jacobi preconditioning (ph = D^-1 p and
 // sh = D^-1 s are formed on the fly, never stored) and CG / Jacobi-PCG, in
 // FP64 or as FP32 inner iterations with FP64 residual replacement.
+//
+// Residual replacement is one FP64 product for the whole batch, so the
+// right-hand sides share it: one whose recurrence residual has dropped by
+// refine_delta marks itself and keeps iterating, and the batch pauses for
+// the replacement once half of the active right-hand sides are marked, when
+// one has dropped by refine_delta^2, claims convergence or reaches max_iter.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -32,7 +38,7 @@ struct MState {
   unsigned int counter[8];
   double rho[KMAX], rho_prev[KMAX], rho0[KMAX], alpha[KMAX], omega[KMAX], beta[KMAX];
   double rr[KMAX], rr_ref[KMAX], bb[KMAX], true_rr[KMAX], crit[KMAX];
-  int active[KMAX], status[KMAX];
+  int active[KMAX], status[KMAX], want[KMAX];
   long long it[KMAX];
 };
 
@@ -147,6 +153,68 @@ __global__ __launch_bounds__(NT) void k_mspmm(CsrView<T> A, Op op0) {
   finish<NP>(op, acc);
 }
 
+// FP32 with k a multiple of 4: a wavefront takes 32 rows of a slice, lane
+// (rs = lane >> 4, g = lane & 15) right-hand sides 4g .. 4g+3 of rows
+// 4i + rs, so each gather is a 16-byte read per lane (sell_rows_multi4).  The
+// per-lane sums of four right-hand sides are regrouped by right-hand side in
+// LDS before the same per-block partials as finish() writes.
+template <int NP, class Op>
+__device__ __forceinline__ void finish4(Op &op, double (&acc)[NP][4]) {
+  __shared__ double lds[NP][NT / 64][4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane & 15, rs = lane >> 4;
+  const int blk = dev::xcd_block();
+#pragma unroll
+  for (int j = 0; j < NP; ++j)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) lds[j][w][rs][4 * g + c] = acc[j][c];
+  __syncthreads();
+  if (w == 0) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      double s = 0;
+#pragma unroll
+      for (int q = 0; q < NT / 64; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s += lds[j][q][r][lane];
+      op.part[((size_t)j * gridDim.x + blk) * 64 + lane] = s;
+    }
+  }
+}
+
+template <class Op>
+__global__ __launch_bounds__(NT) void k_mspmm4(CsrView<float> A, Op op0) {
+  constexpr int RB = 32, H = 64 / RB, NP = Op::NP;
+  static_assert(NP > 0, "product ops reduce at least one sum");
+  Op op = op0;
+  if (op.skip()) return;
+  op.begin();
+  double acc[NP][4];
+#pragma unroll
+  for (int j = 0; j < NP; ++j)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[j][c] = 0;
+  const int lane = threadIdx.x & 63, g = lane & 15, rs = lane >> 4;
+  const int k4 = op.st->k >> 2;
+  const bool on = g < k4;
+  const int gl = on ? g : 0;
+  op.bind4(gl);
+  const int64_t units = ((A.rows + 63) >> 6) * H;
+  const int64_t nw = (int64_t)gridDim.x * (NT / 64);
+  for (int64_t u = (int64_t)dev::xcd_block() * (NT / 64) + (threadIdx.x >> 6); u < units; u += nw) {
+    const int64_t s = u / H;
+    const int h = (int)(u - s * H);
+    float4 y[RB / 4];
+    dev::sell_rows_multi4<RB>(A.sptr, A.col, A.val, (const float *)op.cscale(), s, h, lane, on,
+                              [&](int c) { return op.x4(c, gl); }, y);
+#pragma unroll
+    for (int i = 0; i < RB / 4; ++i) {
+      const int64_t r = s * 64 + h * RB + 4 * i + rs;
+      if (r < A.rows && on) op.row4(A.perm ? (int64_t)A.perm[r] : r, 4 * gl, y[i], acc);
+    }
+  }
+  finish4<NP>(op, acc);
+}
+
 template <class Op>
 __global__ __launch_bounds__(NT) void k_melem(int64_t n, Op op0) {
   Op op = op0;
@@ -175,7 +243,27 @@ struct MBase {
   __device__ void last_all() const {}
   // product ops: no column scale unless the op supplies one
   __device__ const void *cscale() const { return nullptr; }
+  // the four right-hand sides 4g .. 4g+3 of one lane (FP32 products)
+  __device__ void bind4(int) {}
+  __device__ void act4(int j0, bool (&a)[4]) const {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = st->active[j0 + c] != 0;
+  }
+  template <class T>
+  __device__ static void store4(T *dst, int64_t o, const float4 &v, const bool (&a)[4]) {
+    if (a[0] && a[1] && a[2] && a[3]) {
+      *reinterpret_cast<float4 *>(dst + o) = v;
+    } else {
+      if (a[0]) dst[o] = v.x;
+      if (a[1]) dst[o + 1] = v.y;
+      if (a[2]) dst[o + 2] = v.z;
+      if (a[3]) dst[o + 3] = v.w;
+    }
+  }
 };
+
+__device__ __forceinline__ float4 ld4(const float *p, int64_t o) { return *reinterpret_cast<const float4 *>(p + o); }
+__device__ __forceinline__ float comp(const float4 &v, int c) { return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w; }
 
 __device__ void request_pause_or_advance(MState *st, int j, double rho_new, double rr, bool cgfam) {
   st->it[j] += 1;
@@ -187,11 +275,12 @@ __device__ void request_pause_or_advance(MState *st, int j, double rho_new, doub
     st->active[j] = 0;
     return;
   }
-  if (crit < st->tol || rr < st->delta2 * st->rr_ref[j] || st->it[j] >= st->max_iter) {
+  if (crit < st->tol || rr < st->delta2 * st->delta2 * st->rr_ref[j] || st->it[j] >= st->max_iter) {
     st->paused = 1;  // plain store from several lanes, same value
     st->rho_prev[j] = st->rho[j];
     return;
   }
+  if (rr < st->delta2 * st->rr_ref[j]) st->want[j] = 1;
   if (cgfam) {
     st->beta[j] = rho_new / st->rho[j];
   } else {
@@ -199,6 +288,18 @@ __device__ void request_pause_or_advance(MState *st, int j, double rho_new, doub
   }
   st->rho_prev[j] = st->rho[j];
   st->rho[j] = rho_new;
+}
+
+// after an iteration: pause for the shared residual replacement once half of
+// the active right-hand sides want one
+__device__ void pause_when_half_want(MState *st) {
+  int na = 0, nw = 0;
+  for (int j = 0; j < st->k; ++j)
+    if (st->active[j]) {
+      ++na;
+      nw += st->want[j];
+    }
+  if (nw > 0 && 2 * nw >= na) st->paused = 1;
 }
 
 __device__ void all_inactive_done(MState *st) {
@@ -238,6 +339,17 @@ struct MV : MBase {
     if (!st->active[j]) return;
     v[i * k + j] = y;
     acc[0] += (double)rhat[i * k + j] * (double)y;
+  }
+  __device__ float4 x4(int c, int g) const { return ld4((const float *)p, (int64_t)c * k + 4 * g); }
+  __device__ void row4(int64_t i, int j0, const float4 &y, double (&acc)[NP][4]) const {
+    bool a[4];
+    act4(j0, a);
+    const int64_t o = i * k + j0;
+    store4(v, o, y, a);
+    const float4 rh = ld4((const float *)rhat, o);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (a[c]) acc[0][c] += (double)comp(rh, c) * (double)comp(y, c);
   }
   __device__ void last_lane(int j, const double *s) const {
     if (!st->active[j]) return;
@@ -286,6 +398,21 @@ struct MT : MBase {
     acc[0] += (double)y * (double)y;
     acc[1] += (double)y * (double)s[i * k + j];
   }
+  __device__ float4 x4(int c, int g) const { return ld4((const float *)s, (int64_t)c * k + 4 * g); }
+  __device__ void row4(int64_t i, int j0, const float4 &y, double (&acc)[NP][4]) const {
+    bool a[4];
+    act4(j0, a);
+    const int64_t o = i * k + j0;
+    store4(t, o, y, a);
+    const float4 sv = ld4((const float *)s, o);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (a[c]) {
+        const double yc = comp(y, c);
+        acc[0][c] += yc * yc;
+        acc[1][c] += yc * (double)comp(sv, c);
+      }
+  }
   __device__ void last_lane(int j, const double *sm) const {
     if (!st->active[j]) return;
     const double w = sm[0] == 0.0 ? 0.0 : sm[1] / sm[0];
@@ -322,6 +449,7 @@ struct MX : MBase {
   }
   __device__ void last_all() const {
     st->iter += 1;
+    pause_when_half_want(st);
     all_inactive_done(st);
   }
 };
@@ -353,6 +481,28 @@ struct MCgA : MBase {
     pnew[o] = pn;
     q[o] = y;
     acc[0] += (double)pn * (double)y;
+  }
+  float b4[4];  // beta of this lane's four right-hand sides (bind4)
+  __device__ void bind4(int g) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) b4[c] = (float)bet(4 * g + c);
+  }
+  __device__ float4 pn4(int64_t o) const {
+    const float4 po = ld4((const float *)pold, o), zz = ld4((const float *)z, o);
+    return make_float4(fmaf(b4[0], po.x, zz.x), fmaf(b4[1], po.y, zz.y), fmaf(b4[2], po.z, zz.z),
+                       fmaf(b4[3], po.w, zz.w));
+  }
+  __device__ float4 x4(int c, int g) const { return pn4((int64_t)c * k + 4 * g); }
+  __device__ void row4(int64_t i, int j0, const float4 &y, double (&acc)[NP][4]) const {
+    bool a[4];
+    act4(j0, a);
+    const int64_t o = i * k + j0;
+    const float4 pn = pn4(o);
+    store4(pnew, o, pn, a);
+    store4(q, o, y, a);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (a[c]) acc[0][c] += (double)comp(pn, c) * (double)comp(y, c);
   }
   __device__ void last_lane(int j, const double *s) const {
     if (!st->active[j]) return;
@@ -401,6 +551,7 @@ struct MCgB : MBase {
   }
   __device__ void last_all() const {
     st->iter += 1;
+    pause_when_half_want(st);
     all_inactive_done(st);
   }
 };
@@ -441,6 +592,7 @@ struct MRes : MBase {
   }
   __device__ void last_lane(int j, const double *s) const {
     st->true_rr[j] = s[2];
+    st->want[j] = 0;
     if (MODE == 0) {
       st->bb[j] = s[3];
       st->rho[j] = s[0];
@@ -564,9 +716,15 @@ struct Multi {
   template <class T, class O> void spmm(const O &op) {
     // RB-row units of the slices, four per workgroup, within the partials'
     // grid
-    const int64_t units = A->nsl * (sizeof(T) == 4 ? 1 : 2);
+    const bool v4 = sizeof(T) == 4 && k % 4 == 0;
+    const int64_t units = A->nsl * (sizeof(T) == 4 && !v4 ? 1 : 2);
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>((units + 3) / 4, grid));
-    k_mspmm<T, O><<<g, NT, 0, s>>>(cview<T>(A), op);
+    if constexpr (sizeof(T) == 4) {
+      if (v4) k_mspmm4<O><<<g, NT, 0, s>>>(cview<T>(A), op);
+      else k_mspmm<T, O><<<g, NT, 0, s>>>(cview<T>(A), op);
+    } else {
+      k_mspmm<T, O><<<g, NT, 0, s>>>(cview<T>(A), op);
+    }
     HIPCHK(hipGetLastError());
     if constexpr (O::NP > 0) {
       k_mfinish<O><<<1, NTM, 0, s>>>(op, g);

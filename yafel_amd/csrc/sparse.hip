@@ -232,6 +232,37 @@ __global__ __launch_bounds__(256) void k_csr_spmm(int64_t rows, int k, const int
   }
 }
 
+// FP32 with k a multiple of 4: RB = 32 rows of a slice per wavefront, four
+// rows per 16-byte gather (sell_rows_multi4).
+__global__ __launch_bounds__(256) void k_csr_spmm4(int64_t rows, int k, const int *__restrict__ sptr,
+                                                   const int *__restrict__ perm, const int *__restrict__ col,
+                                                   const float *__restrict__ val, const float *__restrict__ X,
+                                                   float *__restrict__ Y) {
+  constexpr int RB = 32, H = 64 / RB;
+  const int lane = threadIdx.x & 63, g = lane & 15, rs = lane >> 4;
+  const int k4 = k >> 2;
+  const float4 *X4 = reinterpret_cast<const float4 *>(X);
+  float4 *Y4 = reinterpret_cast<float4 *>(Y);
+  const int64_t units = ((rows + 63) >> 6) * H;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  for (int64_t u = (int64_t)dev::xcd_block() * 4 + (threadIdx.x >> 6); u < units; u += nw) {
+    const int64_t s = u / H;
+    const int h = (int)(u - s * H);
+    for (int j0 = 0; j0 < k4; j0 += 16) {
+      const bool on = j0 + g < k4;
+      const int jj = on ? j0 + g : j0;
+      float4 y[RB / 4];
+      dev::sell_rows_multi4<RB>(sptr, col, val, nullptr, s, h, lane, on,
+                                [&](int c) { return X4[(int64_t)c * k4 + jj]; }, y);
+#pragma unroll
+      for (int i = 0; i < RB / 4; ++i) {
+        const int64_t r = s * 64 + h * RB + 4 * i + rs;
+        if (r < rows && on) Y4[(perm ? (int64_t)perm[r] : r) * k4 + jj] = y[i];
+      }
+    }
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_bcsr_spmm(int64_t brows, int64_t ntot, int k, const int *__restrict__ ptr,
                                                    const int *__restrict__ sptr, const int *__restrict__ iperm,
@@ -415,7 +446,10 @@ void spmm_launch(yafel_hip_matrix_s *A, int precision, int k, const void *X, voi
   const int gcsr = (int)std::min<int64_t>((units + 3) / 4, (int64_t)A->ctx->num_cus * 8);
   if (precision == YAFEL_HIP_FP32) {
     REQUIRE(A->has32, "spmm: matrix has no FP32 copy");
-    if (A->format == YAFEL_HIP_CSR)
+    if (A->format == YAFEL_HIP_CSR && k % 4 == 0 && ((uintptr_t)X % 16) == 0 && ((uintptr_t)Y % 16) == 0)
+      k_csr_spmm4<<<(int)std::min<int64_t>((A->nsl * 2 + 3) / 4, (int64_t)A->ctx->num_cus * 8), 256, 0, s>>>(
+          A->rows, k, sp, pm, c, A->val32.as<float>(), (const float *)X, (float *)Y);
+    else if (A->format == YAFEL_HIP_CSR)
       k_csr_spmm<float><<<gcsr, 256, 0, s>>>(A->rows, k, sp, pm, c, A->val32.as<float>(), (const float *)X, (float *)Y);
     else
       k_bcsr_spmm<float><<<grid, 256, 0, s>>>(A->brows, A->ntot, k, p, sp, ip, c, A->val32.as<float>(),
