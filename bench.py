@@ -457,8 +457,8 @@ def main():
         ctx.sync()
 
     # roofline of the dominant kernel: the CG iteration's FP32 SpMV (q = A p
-    # and the p.q partials), every launch of one more step timed with HIP
-    # events on the context stream
+    # and the p.q partials), every launch of one more step timed with its own
+    # HIP event pair on the context stream
     A = w.A
     inf = A.info()
     nbr = inf.rows // 3

@@ -114,6 +114,13 @@ int yafel_hip_matrix_sync_fp32(yafel_hip_matrix A);
  * default is 8 for CSR matrices averaging 24 or more slots per slice and 4
  * otherwise.  A tuning knob: results are identical for every value. */
 int yafel_hip_matrix_set_unroll(yafel_hip_matrix A, int unroll);
+/* kernels per iteration of the single-solve CG / Jacobi-PCG on this matrix:
+ * 3 (SpMV; alpha and r, z; beta, x and p) or 2 (p = z + beta p formed in the
+ * SpMV's gather, with x += alpha p, in the same kernel); 0 (the default)
+ * picks 2 for MIXED solves up to 16 Mi stored scalars and FP64-vector solves
+ * on CSR up to 1.5 Mi, else 3.  A tuning knob: the iterates are
+ * bit-identical. */
+int yafel_hip_matrix_set_cg_passes(yafel_hip_matrix A, int passes);
 /* device pointer to the FP64 diagonal (computed on first use) */
 int yafel_hip_matrix_diagonal(yafel_hip_matrix A, double *diag_dev);
 

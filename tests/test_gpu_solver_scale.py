@@ -61,3 +61,30 @@ def test_elasticity_48_repeats_bit_identically(H, ctx, prec):
         assert it == runs[0][0] and rf == runs[0][1]
         assert np.array_equal(x, runs[0][2])
 
+
+
+@pytest.mark.parametrize("fmt", ["BCSR3", "CSR"])
+@pytest.mark.parametrize("prec", ["FP64", "MIXED", "MIXED_MATRIX"])
+@pytest.mark.parametrize("method", ["CG", "PCG_JACOBI"])
+@pytest.mark.parametrize("stop", ["STOP_REFERENCE", "STOP_RELRES"])
+def test_cg_two_and_three_passes_are_bit_identical(H, ctx, fmt, prec, method, stop):
+    """The two-pass CG iteration (beta, x += alpha p and p = z + beta p formed
+    inside the next SpMV) runs the same arithmetic as the three-pass one, so
+    its iteration counts and iterates are bit-identical: elasticity Q1 20^3
+    (27,783 dofs) in both storage formats, every precision, both stopping
+    rules."""
+    from yafel_amd import workloads
+    w = workloads.c2(ctx, n=20, fmt=getattr(H, fmt))
+    w.assemble()
+    tol = 1e-8 if stop == "STOP_RELRES" else 1e-14
+    kw = dict(method=getattr(H, method), precision=getattr(H, prec), stop=getattr(H, stop), tol=tol)
+    out = {}
+    for passes in (3, 2):
+        w.A.set_cg_passes(passes)
+        info = w.solve(**kw)
+        out[passes] = (info.status, info.iterations, info.refinements, w.x.download())
+    w.A.set_cg_passes(0)
+    print(f"{fmt} {prec} {method} {stop}: iterations {out[3][1]} / {out[2][1]}", flush=True)
+    assert out[3][0] == 0 and out[3][1] > 20
+    assert out[2][:3] == out[3][:3]
+    assert np.array_equal(out[2][3], out[3][3])

@@ -157,6 +157,7 @@ struct yafel_hip_matrix_s {
   void *mesh = nullptr;         // mesh this pattern came from (assembly)
   YhPlan *plan = nullptr;
   int unroll = 4;               // slots per step of the plain SpMV kernel
+  int cg_passes = 0;            // kernels per single-solve CG iteration (2, 3; 0: by size)
   // reference-layout extras: the host row_ptr may be longer than brows+1
   std::vector<int64_t> ptr_tail;
   ~yafel_hip_matrix_s() { yh::ctx_release(ctx); }

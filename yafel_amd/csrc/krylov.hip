@@ -13,6 +13,8 @@
 //      z = D^-1 r, partials r.z, r.r
 //   P: beta and the stopping test folded by every workgroup; x += alpha p,
 //      p = z + beta p
+// or, by default on smaller matrices, two: P moves into the next SpMV, which
+// forms p = z + beta p in its gather (k_spmv_qp, OpCgQP).
 // BiCGSTAB folds each reduction in a one-workgroup kernel launched right
 // after its producer.
 // MIXED: vectors and matrix are FP32 (row sums and dots accumulate in FP64);
@@ -80,6 +82,23 @@ int launch_spmv_nofold(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
   return grid;
 }
 
+// The two-pass CG's SpMV (k_spmv_qp): versioned like the update passes, with
+// the SpMV grid.
+template <class T, class Op>
+int launch_qp(yafel_hip_matrix_s *A, const Op &op, int64_t n, hipStream_t s) {
+  const int grid = sell_grid(A->nsl, A->ctx->num_cus * 8);
+  const SellView<T> v = view<T>(A);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
+  auto kern = A->format == YAFEL_HIP_CSR && A->unroll == 8 ? k_spmv_qp<YAFEL_HIP_CSR, 8, T, Op>
+              : A->format == YAFEL_HIP_CSR               ? k_spmv_qp<YAFEL_HIP_CSR, kSellUnroll, T, Op>
+                                                         : k_spmv_qp<YAFEL_HIP_BCSR3, kSellUnroll, T, Op>;
+  if (e0) hipExtLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, s, e0, e1, 0, v, op, n);
+  else kern<<<grid, NT, 0, s>>>(v, op, n);
+  HIPCHK(hipGetLastError());
+  return grid;
+}
+
 template <class Op>
 void launch_elem(yafel_hip_ctx_s *ctx, int64_t n, const Op &op, hipStream_t s) {
   const int grid = grid_for(ctx, n, NT);
@@ -89,6 +108,19 @@ void launch_elem(yafel_hip_ctx_s *ctx, int64_t n, const Op &op, hipStream_t s) {
     k_finish<Op><<<1, NTF, 0, s>>>(op, grid);
     HIPCHK(hipGetLastError());
   }
+}
+
+// Default number of CG passes.  The two-pass form saves a launch and the P
+// pass (about 7 us an iteration) but gathers z and p instead of p, which
+// costs in proportion to the stored entries and more with FP64 vectors.
+// Measured per iteration (MI355X): FP32 vectors gain 14-21 % up to 8 M stored
+// scalars (2-D Poisson, 3-D elasticity 32^3 in CSR and BCSR3) and lose 8 %
+// at 65 M (elasticity 64^3); FP64 vectors gain on CSR up to about 1.5 M
+// entries and lose on BCSR3 from 16^3 on.
+bool two_pass_pays(const yafel_hip_matrix_s *A, int precision) {
+  const int64_t stored = A->format == YAFEL_HIP_BCSR3 ? 9 * A->nblk : A->nblk;
+  if (precision == YAFEL_HIP_MIXED) return stored <= ((int64_t)16 << 20);
+  return A->format == YAFEL_HIP_CSR && stored <= ((int64_t)3 << 19);
 }
 
 // ------------------------------------------------------------- solve driver
@@ -124,6 +156,7 @@ struct Solver {
   double *x;
   hipStream_t s;
 
+  bool two_pass = false;  // CG iterations as QP, U (k_spmv_qp)
   KState *cur() const { return st + (ver & 1); }
   // partial regions: the CG SpMV's at 0, the update pass's at 2 * cap
   int cap() const { return ctx->num_cus * 8; }
@@ -145,15 +178,19 @@ struct Solver {
     op.pin = pin;
     op.nin = nin;
     op.part = ppart;
-    // VE elements per thread; a multiple of 8 workgroups keeps the XCD-aware
-    // block order, and rounding up keeps each workgroup's chunk <= VE * NT
+    const int g = vgrid();
+    k_velem<O><<<g, NT, 0, s>>>(n, op);
+    HIPCHK(hipGetLastError());
+    ++ver;
+    return g;
+  }
+  // VE elements per thread; a multiple of 8 workgroups keeps the XCD-aware
+  // block order, and rounding up keeps each workgroup's chunk <= VE * NT
+  int vgrid() const {
     int64_t g = (n + (int64_t)NT * VE - 1) / ((int64_t)NT * VE);
     if (g >= 8) g = (g + 7) / 8 * 8;
     if (g > cap()) g = cap();
     if (g < 1) g = 1;
-    k_velem<O><<<(int)g, NT, 0, s>>>(n, op);
-    HIPCHK(hipGetLastError());
-    ++ver;
     return (int)g;
   }
 
@@ -206,6 +243,35 @@ struct Solver {
     pp.x = xx;
     pp.z = (const T *)(prec ? v.z : v.r);
     vlaunch(pp, pu, gu, nullptr);
+  }
+
+  // QP (versioned SpMV: beta, x += alpha p, q = A (z + beta p)), then U
+  template <class TM, class T>
+  void cg_iteration2(const Vecs &v) {
+    double *pu = part + 2 * (size_t)cap();
+    OpCgQP<T> a{};
+    a.in = cur();
+    a.out = st + ((ver + 1) & 1);
+    a.pin = pu;
+    a.nin = vgrid();
+    a.part = part;
+    a.p0 = (T *)v.p0;
+    a.p1 = (T *)v.p1;
+    a.q = (T *)v.q;
+    a.xv = mixed ? (T *)v.xl : (T *)x;
+    a.z = (const T *)(prec ? v.z : v.r);
+    const int ga = launch_qp<TM>(A, a, n, s);
+    ++ver;
+    const T *dinv = (const T *)(sizeof(T) == 8 ? A->dinv64.p : A->dinv32.p);
+    if (prec) {
+      OpCgU<T, true> u{};
+      u.r = (T *)v.r; u.z = (T *)v.z; u.q = (const T *)v.q; u.dinv = dinv;
+      vlaunch(u, part, ga, pu);
+    } else {
+      OpCgU<T, false> u{};
+      u.r = (T *)v.r; u.z = (T *)v.z; u.q = (const T *)v.q; u.dinv = dinv;
+      vlaunch(u, part, ga, pu);
+    }
   }
 
   // p = z + beta p after the initial residual or a refinement
@@ -269,7 +335,7 @@ struct Solver {
       launch_elem(ctx, n, f, s);
     }
     residual<T>(1, v);
-    if (cgfam) cg_restart<T>(v);
+    if (cgfam && !two_pass) cg_restart<T>(v);
   }
 
   template <class TM, class T>
@@ -305,7 +371,7 @@ struct Solver {
     HIPCHK(hipStreamSynchronize(s));
     residual<T>(0, v);
     if (!cgfam) HIPCHK(hipMemcpyAsync(v.rhat, v.r, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-    if (cgfam) cg_restart<T>(v);
+    if (cgfam && !two_pass) cg_restart<T>(v);
     const int K = o.check_every > 0 ? o.check_every : 16;
     const bool with_refine = mixed || !cgfam;
     char *pin = (char *)ctx->pinned;
@@ -316,7 +382,9 @@ struct Solver {
     HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     auto chunk = [&](int slot_i) {
       for (int k = 0; k < K; ++k) {
-        if (cgfam) {
+        if (cgfam && two_pass) {
+          cg_iteration2<TM, T>(v);
+        } else if (cgfam) {
           cg_iteration<TM, T>(v);
         } else {
           bicg_iteration<TM, T>(v);
@@ -413,6 +481,7 @@ int yafel_hip_solve(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const 
   if (S.o.max_iter <= 0) S.o.max_iter = S.cgfam ? n : 10 * n;
   if (S.o.refine_delta <= 0) S.o.refine_delta = 0.1;
   if (S.o.check_every <= 0) S.o.check_every = 16;
+  S.two_pass = S.cgfam && (A->cg_passes == 2 || (A->cg_passes == 0 && two_pass_pays(A, opts->precision)));
   S.s = ctx->stream;
   if (S.prec || true) {
     if (!A->diag_ok) matrix_build_diag(A);

@@ -27,8 +27,11 @@ struct KState {
   long long iter, max_iter, refinements;
   int done, status, paused, first;
   int method, stop, mixed, special;
-  int pad0;
+  int upd;        // single-solve CG, two-pass form: U ran since the last QP
+                  // (x += alpha p pending, U's partials valid)
   int pupd;       // single-solve CG: p = z + beta p pending (after a residual)
+  int pidx;       // single-solve CG, two-pass form: p is p[pidx]
+  int pad1;
   unsigned int counter[8];
 };
 
@@ -646,6 +649,7 @@ struct OpCgU : VOpBase {
     }
     s.alpha = a;
     s.first = 0;
+    s.upd = 1;
     return true;
   }
   __device__ void begin(const KState &s) { alpha = (T)s.alpha; }
@@ -722,6 +726,125 @@ struct OpCgP0 : VOpBase {
     v.p = p[i];
   }
   __device__ void compute(int64_t i, const In &v, double *, bool) const { p[i] = fma(beta, v.p, v.z); }
+};
+
+// ----------------------------------------------------- CG in two passes
+// On smaller matrices the launch and the fixed cost of the P pass weigh more
+// than gathering two vectors, so P is folded into the next SpMV and an
+// iteration is two kernels:
+//   QP: fold U's r.z, r.r partials, beta and the stopping test; x += alpha p
+//       for every row; q = A (z + beta p), with p_new = z + beta p formed in
+//       the gather and written by each row's owner into the other p buffer;
+//       partial p_new.q
+//   U:  as in the three-pass form
+// The arithmetic is that of Q, U, P: the same fma forms p_new and x, so the
+// iterates are bit-identical.  After the initial or a refinement residual
+// (no U since), QP only forms p = z + beta p.  When the test stops or pauses
+// the iteration, QP applies x += alpha p over the vector and skips the SpMV.
+template <int FMT, int U, class T, class Op>
+__device__ __forceinline__ void spmv_qp_body(const SellView<T> &A, Op &op, int blk, int nblk, int64_t n) {
+  __shared__ KState ss;
+  __shared__ int sflags;
+  const int t = threadIdx.x;
+  const bool b0 = blk == 0;
+  double f[2];
+  refold_load<2>(op.pin, op.nin, f);
+  if (t < KSW) reinterpret_cast<long long *>(&ss)[t] = reinterpret_cast<const long long *>(op.in)[t];
+  __syncthreads();
+  if (!op.active(ss)) {
+    if (b0 && t < KSW) reinterpret_cast<long long *>(op.out)[t] = reinterpret_cast<const long long *>(&ss)[t];
+    return;
+  }
+  refold_reduce<2>(f, f);
+  __syncthreads();  // every wave has read ss before thread 0 changes it
+  if (t == 0) sflags = op.decide(ss, f);
+  __syncthreads();
+  if (b0 && t < KSW) reinterpret_cast<long long *>(op.out)[t] = reinterpret_cast<const long long *>(&ss)[t];
+  const int flags = sflags;
+  op.begin(ss, flags);
+  if (!(flags & Op::kGo)) {
+    if (flags & Op::kXupd) {
+      const int64_t chunk = (n + nblk - 1) / nblk;
+      const int64_t i1 = (int64_t)blk * chunk + chunk < n ? (int64_t)blk * chunk + chunk : n;
+      for (int64_t i = (int64_t)blk * chunk + t; i < i1; i += NT) op.xonly(i);
+    }
+    return;
+  }
+  const int lane = t & 63;
+  double acc[1] = {0.0};
+  const int64_t nsl = A.nsl(), nw = (int64_t)nblk * (NT / 64);
+  for (int64_t sl = (int64_t)blk * (NT / 64) + (t >> 6); sl < nsl; sl += nw) {
+    const int64_t i = sl * 64 + lane;
+    if (FMT == YAFEL_HIP_CSR) {
+      const double y = dev::sell_row_csr<U, T, double>(A, sl, lane, [&](int c) { return (double)op.x(c); });
+      if (i < A.brows) op.row(A.row_of(i), y, acc);
+    } else {
+      double a[3];
+      dev::sell_row_bcsr<U, T, double>(A, sl, lane, a, [&](int64_t c) { return (double)op.x(c); });
+      if (i < A.brows) {
+        const int64_t r = A.row_of(i);
+        op.row(3 * r, a[0], acc);
+        op.row(3 * r + 1, a[1], acc);
+        op.row(3 * r + 2, a[2], acc);
+      }
+    }
+  }
+  finish_partials<1>(op, acc, blk, nblk);
+}
+
+template <int FMT, int U, class T, class Op>
+__global__ __launch_bounds__(NT) void k_spmv_qp(SellView<T> A, Op op0, int64_t n) {
+  Op op = op0;
+  spmv_qp_body<FMT, U, T, Op>(A, op, dev::xcd_block(), gridDim.x, n);
+}
+
+template <class T>
+struct OpCgQP : VOpBase {
+  using value_type = T;
+  static constexpr bool kTimed = true;  // the CG iteration's SpMV kernel
+  static constexpr int kGo = 1, kXupd = 2;
+  T *p0, *p1, *q, *xv;
+  const T *z;
+  const T *pold;
+  T *pnew;
+  T alpha, beta;
+  int xupd;
+  __device__ bool active(const KState &s) const { return running(s); }
+  // flags: kGo (form p_new, q = A p_new), kXupd (x += alpha p first)
+  __device__ int decide(KState &s, const double *f) const {
+    int fl = 0;
+    if (s.upd) {
+      s.upd = 0;
+      fl |= kXupd;
+      after_iteration(&s, f[0], f[1]);
+    }
+    s.pupd = 0;
+    if (running(s)) {
+      fl |= kGo;
+      s.pidx ^= 1;
+    }
+    return fl;
+  }
+  __device__ void begin(const KState &s, int fl) {
+    alpha = (T)s.alpha;
+    beta = (T)s.beta;
+    xupd = fl & kXupd;
+    // s is the published state: with kGo its pidx already names p_new
+    const int pi = (fl & kGo) ? (s.pidx ^ 1) : s.pidx;
+    pold = pi ? p1 : p0;
+    pnew = pi ? p0 : p1;
+  }
+  __device__ T x(int64_t c) const { return fma(beta, pold[c], z[c]); }
+  __device__ void row(int64_t i, double y, double *acc) const {
+    const T po = pold[i];
+    const T pn = fma(beta, po, z[i]);
+    const T yq = (T)y;
+    pnew[i] = pn;
+    q[i] = yq;
+    if (xupd) xv[i] = fma(alpha, po, xv[i]);
+    acc[0] += (double)pn * (double)yq;
+  }
+  __device__ void xonly(int64_t i) const { xv[i] = fma(alpha, pold[i], xv[i]); }
 };
 
 }  // namespace kr

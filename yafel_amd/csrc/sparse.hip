@@ -660,6 +660,15 @@ int yafel_hip_matrix_set_unroll(yafel_hip_matrix A, int unroll) {
   YH_TRY_END(A ? A->ctx : nullptr)
 }
 
+int yafel_hip_matrix_set_cg_passes(yafel_hip_matrix A, int passes) {
+  YH_TRY_BEGIN
+  REQUIRE(A, "matrix_set_cg_passes: null matrix");
+  REQUIRE(passes == 0 || passes == 2 || passes == 3, "matrix_set_cg_passes: passes must be 0, 2 or 3");
+  A->cg_passes = passes;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(A ? A->ctx : nullptr)
+}
+
 int yafel_hip_matrix_diagonal(yafel_hip_matrix A, double *diag_dev) {
   YH_TRY_BEGIN
   REQUIRE(A && diag_dev, "matrix_diagonal: null argument");
