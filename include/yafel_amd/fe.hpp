@@ -8,6 +8,8 @@
 //   DGAssembly<Physics>(fesys, P, reqs)   include/assembly/DGAssembly.hpp:32-305
 //   DGRK4(dt).step(fesys, P)              include/time_integration/DGRK4.hpp:33-69
 //   DirichletBC(dofm, value, comp)        include/boundary_conditions/DirichletBC.hpp:20-89
+//   LocalSmoothingGradient<Physics>(fesys) include/assembly/LocalSmoothingGradient.hpp:20-94
+//   ZZGradientRecovery<Physics>(fesys)     include/assembly/ZZGradientRecovery.hpp:33-141
 //
 // A GPU cannot call a host Physics functor, so a Physics type provides
 //     static yafel_hip_physics device_descriptor();
@@ -179,6 +181,9 @@ class FESystem {
     return r;
   }
   Vector<double> &getSolution() { return solution_; }
+  // RowMajor n_dofs x nsd (n_nodes x nsd after ZZ recovery), as the
+  // reference's solution_gradient (include/fe_system/FESystem.hpp:37, 53)
+  std::vector<double> &getSolutionGradient() { return gradient_; }
 
   // solve tangent * U = residual on the device (b stays resident)
   yafel_hip_solve_info solve(int method = YAFEL_HIP_PCG_JACOBI, int precision = YAFEL_HIP_FP64,
@@ -207,8 +212,53 @@ class FESystem {
   std::shared_ptr<double> R_;
   std::size_t n_ = 0;
   Vector<double> solution_;
+  std::vector<double> gradient_;
   double time_ = 0;
 };
+
+namespace hipdetail {
+// the host solution up, the recovered gradient (rows x nsd) back
+template <class Fn>
+void recover_gradient(FESystem &fes, std::size_t rows, int nsd, Fn &&fn) {
+  auto ctx = context();
+  const std::size_t n = fes.ndofs();
+  void *u = nullptr, *g = nullptr;
+  check(yafel_hip_malloc(ctx, sizeof(double) * n, &u), ctx);
+  int rc = yafel_hip_malloc(ctx, sizeof(double) * rows * nsd + 8, &g);
+  if (rc == YAFEL_HIP_OK) rc = yafel_hip_memcpy_h2d(ctx, u, fes.getSolution().data(), sizeof(double) * n);
+  if (rc == YAFEL_HIP_OK) rc = fn((const double *)u, (double *)g);
+  auto &out = fes.getSolutionGradient();
+  out.assign(rows * nsd, 0.0);
+  if (rc == YAFEL_HIP_OK) rc = yafel_hip_memcpy_d2h(ctx, out.data(), g, sizeof(double) * rows * nsd);
+  if (g) yafel_hip_free(ctx, g);
+  yafel_hip_free(ctx, u);
+  check(rc, ctx);
+}
+}  // namespace hipdetail
+
+// LocalSmoothingGradient<Physics>(FESystem&) (include/assembly/
+// LocalSmoothingGradient.hpp:20-94): the volume-weighted nodal average of the
+// solution gradient, per dof, into getSolutionGradient() (n_dofs x nsd).
+template <typename Physics>
+void LocalSmoothingGradient(FESystem &fes) {
+  constexpr int NSD = Physics::nsd();
+  yafel_hip_mesh m = fes.getDoFManager().handle();
+  hipdetail::recover_gradient(fes, fes.ndofs(), NSD, [m](const double *u, double *g) {
+    return yafel_hip_gradient_local_smoothing(m, u, g);
+  });
+}
+
+// ZZGradientRecovery<Physics>(FESystem&) (include/assembly/ZZGradientRecovery.hpp:
+// 33-141): p = 1 scalar fields; the nodal least-squares patch fit of the
+// quadrature-point gradients into getSolutionGradient() (n_nodes x nsd).
+template <typename Physics>
+void ZZGradientRecovery(FESystem &fes) {
+  constexpr int NSD = Physics::nsd();
+  yafel_hip_mesh m = fes.getDoFManager().handle();
+  hipdetail::recover_gradient(fes, fes.getDoFManager().dof_nodes.size(), NSD, [m](const double *u, double *g) {
+    return yafel_hip_gradient_zz(m, u, g);
+  });
+}
 
 // CGAssembly<Physics> (include/assembly/CGAssembly.hpp:27-186): tangent values
 // into the fixed pattern (overwritten), residual ACCUMULATED.

@@ -277,6 +277,20 @@ int yafel_hip_dirichlet_apply(yafel_hip_matrix A, const uint8_t *mask_dev, const
 int yafel_hip_bc_mask_plane(yafel_hip_mesh m, int axis, double value, double tol, int comp,
                             uint8_t *mask_dev);
 
+/* ------------------------------------------------------ gradient recovery */
+/* LocalSmoothingGradient<Physics>(FESystem&) (include/assembly/
+ * LocalSmoothingGradient.hpp:20-94): grad_dev (n_dofs x dim, row-major, the
+ * reference's RowMajor SolutionGradient) gets, for each dof, the volume-
+ * weighted average over its node's elements and quadrature points of the
+ * gradient of that dof's component: sum N_a jxw grad u_c / sum N_a jxw.
+ * u_dev is the solution (n_dofs). */
+int yafel_hip_gradient_local_smoothing(yafel_hip_mesh m, const double *u_dev, double *grad_dev);
+/* ZZGradientRecovery<Physics>(FESystem&) (include/assembly/ZZGradientRecovery.hpp:
+ * 33-141), p = 1 and one dof per node: at every node, the least-squares
+ * linear fit of the gradient samples at the quadrature points of the adjacent
+ * elements, evaluated at the node; grad_dev is n_nodes x dim, row-major. */
+int yafel_hip_gradient_zz(yafel_hip_mesh m, const double *u_dev, double *grad_dev);
+
 /* -------------------------------------------------------------------- DG */
 /* DoFManager(M, DG, p, 1) + Mesh::buildInternalFaces (src/utils/DoFManager.cpp:
  * 166-293, src/mesh/build_faces.cpp:25-96) for 2-D Quad4 meshes with gmsh

@@ -1223,3 +1223,170 @@ void orc_normal_fill(uint64_t seed, int64_t n, double mean, double sd, double *o
 }
 
 }  // extern "C"
+
+/* ------------------------------------------------------------------------ */
+/* Gradient recovery: include/assembly/LocalSmoothingGradient.hpp:20-94 and  */
+/* include/assembly/ZZGradientRecovery.hpp:33-141.                            */
+/* ------------------------------------------------------------------------ */
+namespace {
+
+// Element::update at one qp (Element.hpp:109-149): jxw and shapeGrad (nen x nsd)
+void element_update(const ElementTables &T, const double *X, const int32_t *conn, int q, double &jxw,
+                    std::vector<double> &G) {
+  const int nsd = T.dim, nen = T.nen;
+  double J[9] = {0};
+  for (int A = 0; A < nen; ++A) {
+    const double *x = &X[3 * (i64)conn[A]];
+    for (int i = 0; i < nsd; ++i)
+      for (int j = 0; j < nsd; ++j) J[i * nsd + j] += x[i] * T.dN[(q * nen + A) * nsd + j];
+  }
+  double det, Ji[9];
+  jac_inv(nsd, J, det, Ji);
+  jxw = det * T.w[q];
+  for (int A = 0; A < nen; ++A)
+    for (int d = 0; d < nsd; ++d) {
+      double s = 0;
+      for (int k = 0; k < nsd; ++k) s += Ji[k * nsd + d] * T.dN[(q * nen + A) * nsd + k];
+      G[A * nsd + d] = s;
+    }
+}
+
+// Least squares min ||P a - b|| (P m x n, full column rank, row-major; b m x
+// nr) by Householder QR with column pivoting (Businger & Golub 1965), the
+// algorithm of Eigen's ColPivHouseholderQR that ZZGradientRecovery.hpp:127
+// calls: at step k the remaining column of largest norm is swapped in, a
+// reflector zeroes it below the diagonal and is applied to the rest of P and
+// to b; R a = Q^T b is then solved by back substitution and un-permuted.
+void lsq_colpiv_qr(int m, int n, std::vector<double> P, int nr, std::vector<double> b, double *a) {
+  std::vector<int> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  for (int k = 0; k < n && k < m; ++k) {
+    int best = k;
+    double bn = -1;
+    for (int j = k; j < n; ++j) {
+      double s = 0;
+      for (int i = k; i < m; ++i) s += P[i * n + j] * P[i * n + j];
+      if (s > bn) { bn = s; best = j; }
+    }
+    if (best != k) {
+      for (int i = 0; i < m; ++i) std::swap(P[i * n + k], P[i * n + best]);
+      std::swap(perm[k], perm[best]);
+    }
+    double nrm = std::sqrt(bn);
+    if (nrm == 0) continue;
+    const double alpha = P[k * n + k] > 0 ? -nrm : nrm;
+    std::vector<double> v(m, 0.0);
+    for (int i = k; i < m; ++i) v[i] = P[i * n + k];
+    v[k] -= alpha;
+    double vv = 0;
+    for (int i = k; i < m; ++i) vv += v[i] * v[i];
+    if (vv == 0) continue;
+    for (int j = k; j < n; ++j) {
+      double s = 0;
+      for (int i = k; i < m; ++i) s += v[i] * P[i * n + j];
+      s = 2 * s / vv;
+      for (int i = k; i < m; ++i) P[i * n + j] -= s * v[i];
+    }
+    for (int c = 0; c < nr; ++c) {
+      double s = 0;
+      for (int i = k; i < m; ++i) s += v[i] * b[i * nr + c];
+      s = 2 * s / vv;
+      for (int i = k; i < m; ++i) b[i * nr + c] -= s * v[i];
+    }
+  }
+  std::vector<double> y(n * nr);
+  for (int c = 0; c < nr; ++c)
+    for (int r = n - 1; r >= 0; --r) {
+      double s = b[r * nr + c];
+      for (int j = r + 1; j < n; ++j) s -= P[r * n + j] * y[j * nr + c];
+      y[r * nr + c] = s / P[r * n + r];
+    }
+  for (int j = 0; j < n; ++j)
+    for (int c = 0; c < nr; ++c) a[perm[j] * nr + c] = y[j * nr + c];
+}
+
+}  // namespace
+
+extern "C" {
+
+// LocalSmoothingGradient.hpp:20-94: element loop, qp loop; qp_grad(comp, i)
+// = sum_A u(A) shapeGrad(node(A), i) over local dofs A (node-major); then
+// Volume(g(A)) += N jxw and VolTimesGrad(g(A), i) += N qp_grad(comp, i) jxw;
+// finally SolutionGradient(A, i) = VolTimesGrad(A, i) / Volume(A).
+void orc_grad_local_smoothing(int dim, int p, int dpn, int64_t n_nodes, const double *dof_nodes, int64_t nel,
+                              const int32_t *elements, const double *u, double *grad) {
+  ElementTables T = make_tables(dim, p);
+  const int nsd = dim, nen = T.nen, nld = nen * dpn;
+  const i64 ndofs = n_nodes * dpn;
+  std::vector<double> vtg(ndofs * nsd, 0.0), vol(ndofs, 0.0), G(nen * nsd), qg(dpn * nsd), ul(nld);
+  for (i64 e = 0; e < nel; ++e) {
+    const int32_t *conn = &elements[e * nen];
+    for (int A = 0; A < nld; ++A) ul[A] = u[(i64)conn[A / dpn] * dpn + A % dpn];
+    for (int q = 0; q < T.nqp; ++q) {
+      std::fill(qg.begin(), qg.end(), 0.0);
+      double jxw;
+      element_update(T, dof_nodes, conn, q, jxw, G);
+      for (int A = 0; A < nld; ++A) {
+        const int comp = A % dpn, node = A / dpn;
+        for (int i = 0; i < nsd; ++i) qg[comp * nsd + i] += ul[A] * G[node * nsd + i];
+      }
+      for (int A = 0; A < nld; ++A) {
+        const int comp = A % dpn, node = A / dpn;
+        const i64 g = (i64)conn[node] * dpn + comp;
+        const double N = T.N[q * nen + node];
+        vol[g] += N * jxw;
+        for (int i = 0; i < nsd; ++i) vtg[g * nsd + i] += N * qg[comp * nsd + i] * jxw;
+      }
+    }
+  }
+  for (i64 A = 0; A < ndofs; ++A)
+    for (int i = 0; i < nsd; ++i) grad[A * nsd + i] = vtg[A * nsd + i] / vol[A];
+}
+
+// ZZGradientRecovery.hpp:33-141 (p = 1, scalar field): node -> adjacent
+// element sets (ascending), samples at every qp of the patch, P = [1 x (y (z))],
+// b = grad u at the qp, a = colPivHouseholderQr(P).solve(b), grad(n) = [1 x_n] a.
+void orc_grad_zz(int dim, int64_t n_nodes, const double *dof_nodes, int64_t nel, const int32_t *elements,
+                 const double *u, double *grad) {
+  ElementTables T = make_tables(dim, 1);
+  const int nsd = dim, nen = T.nen, nb = nsd + 1;
+  std::vector<std::vector<i64>> adj(n_nodes);
+  for (i64 e = 0; e < nel; ++e)
+    for (int A = 0; A < nen; ++A) {
+      auto &v = adj[elements[e * nen + A]];
+      if (v.empty() || v.back() != e) v.push_back(e);
+    }
+  std::vector<double> G(nen * nsd);
+  for (i64 nd = 0; nd < n_nodes; ++nd) {
+    const auto &ae = adj[nd];
+    const int m = (int)ae.size() * T.nqp;
+    std::vector<double> P(m * nb), b(m * nsd);
+    int idx = 0;
+    for (i64 e : ae) {
+      const int32_t *conn = &elements[e * nen];
+      for (int q = 0; q < T.nqp; ++q) {
+        double jxw, xq[3] = {0, 0, 0}, fg[3] = {0, 0, 0};
+        element_update(T, dof_nodes, conn, q, jxw, G);
+        for (int A = 0; A < nen; ++A) {
+          for (int d = 0; d < 3; ++d) xq[d] += T.N[q * nen + A] * dof_nodes[3 * (i64)conn[A] + d];
+          for (int d = 0; d < nsd; ++d) fg[d] += u[conn[A]] * G[A * nsd + d];
+        }
+        P[idx * nb] = 1;
+        for (int i = 0; i < nsd; ++i) {
+          P[idx * nb + i + 1] = xq[i];
+          b[idx * nsd + i] = fg[i];
+        }
+        ++idx;
+      }
+    }
+    std::vector<double> a(nb * nsd);
+    lsq_colpiv_qr(m, nb, P, nsd, b, a.data());
+    for (int i = 0; i < nsd; ++i) {
+      double g = a[i];
+      for (int j = 0; j < nsd; ++j) g += dof_nodes[3 * nd + j] * a[(j + 1) * nsd + i];
+      grad[nd * nsd + i] = g;
+    }
+  }
+}
+
+}  // extern "C"

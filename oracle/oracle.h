@@ -129,6 +129,14 @@ void orc_dg_rk4(int p, const double *corner_nodes, int64_t ncell, const int32_t 
 uint64_t orc_splitmix64(uint64_t *state);
 void orc_normal_fill(uint64_t seed, int64_t n, double mean, double sd, double *out);
 
+/* ---- gradient recovery ------------------------------------------------- */
+/* LocalSmoothingGradient.hpp:20-94: grad is (n_nodes*dpn) x dim, row-major. */
+void orc_grad_local_smoothing(int dim, int p, int dpn, int64_t n_nodes, const double *dof_nodes, int64_t nel,
+                              const int32_t *elements, const double *u, double *grad);
+/* ZZGradientRecovery.hpp:33-141 (p = 1, dpn = 1): grad is n_nodes x dim. */
+void orc_grad_zz(int dim, int64_t n_nodes, const double *dof_nodes, int64_t nel, const int32_t *elements,
+                 const double *u, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,8 @@ class Oracle:
         L.orc_dg_nodes.argtypes = [C.c_int, _P, _I64, _P, _P]
         L.orc_dg_residual.argtypes = [C.c_int, _P, _I64, _P, _P, _D, _P, _P, _P]
         L.orc_dg_rk4.argtypes = [C.c_int, _P, _I64, _P, _P, _D, _I64, _D, _P]
+        L.orc_grad_local_smoothing.argtypes = [C.c_int, C.c_int, C.c_int, _I64, _P, _I64, _P, _P, _P]
+        L.orc_grad_zz.argtypes = [C.c_int, _I64, _P, _I64, _P, _P, _P]
 
     # -- quadrature / shape functions -------------------------------------
     def gauss_legendre(self, n):
@@ -153,6 +155,27 @@ class Oracle:
                                  _ptr(elements), C.byref(ph), _ptr(rows), _ptr(cols), _ptr(vals),
                                  _ptr(R))
         return rows, cols, vals, R
+
+    # -- gradient recovery -------------------------------------------------
+    def grad_local_smoothing(self, dim, p, dpn, dof_nodes, elements, u):
+        """LocalSmoothingGradient: (n_nodes*dpn, dim)."""
+        dof_nodes = np.ascontiguousarray(dof_nodes, np.float64)
+        elements = np.ascontiguousarray(elements, np.int32)
+        u = np.ascontiguousarray(u, np.float64)
+        G = np.zeros((dof_nodes.shape[0] * dpn, dim))
+        self.lib.orc_grad_local_smoothing(dim, p, dpn, dof_nodes.shape[0], _ptr(dof_nodes), elements.shape[0],
+                                          _ptr(elements), _ptr(u), _ptr(G))
+        return G
+
+    def grad_zz(self, dim, dof_nodes, elements, u):
+        """ZZGradientRecovery (p = 1, scalar): (n_nodes, dim)."""
+        dof_nodes = np.ascontiguousarray(dof_nodes, np.float64)
+        elements = np.ascontiguousarray(elements, np.int32)
+        u = np.ascontiguousarray(u, np.float64)
+        G = np.zeros((dof_nodes.shape[0], dim))
+        self.lib.orc_grad_zz(dim, dof_nodes.shape[0], _ptr(dof_nodes), elements.shape[0], _ptr(elements),
+                             _ptr(u), _ptr(G))
+        return G
 
     def element_matrix(self, dim, p, dpn, dof_nodes, elements, e, ph):
         nld = elements.shape[1] * dpn

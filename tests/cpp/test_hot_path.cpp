@@ -98,6 +98,7 @@ static bool bcsr_readback() {
 // edges, solve, compare with the discrete maximum of -lap u = f (u < 0 under
 // the reference's tangent sign) and the residual.
 struct Poisson {
+  static constexpr int nsd() { return 2; }
   static yafel_hip_physics device_descriptor() {
     yafel_hip_physics p{};
     p.kind = YAFEL_HIP_PHYS_POISSON;
@@ -123,6 +124,25 @@ static bool poisson_flow() {
   double umin = 0;
   for (std::size_t i = 0; i < fes.ndofs(); ++i) umin = std::min(umin, fes.getSolution()(i));
   if (std::abs(-umin - 0.0737) > 0.002) return false;
+  // apps/BasicPoisson/main.cpp:123 recovers the gradient after the solve: the
+  // normal derivative at the middle of the x = 0 edge is about -0.3375 (the
+  // square torsion value, halved for -lap u = 1), zero at the centre
+  auto mid_edge_and_centre = [&](const std::vector<double> &G, double &gx, double &gc) {
+    for (std::size_t i = 0; i < dofm.dof_nodes.size(); ++i) {
+      const auto &x = dofm.dof_nodes[i];
+      if (std::abs(x(0)) < 1e-9 && std::abs(x(1) - 0.5) < 1e-9) gx = G[2 * i];
+      if (std::abs(x(0) - 0.5) < 1e-9 && std::abs(x(1) - 0.5) < 1e-9) gc = std::abs(G[2 * i]) + std::abs(G[2 * i + 1]);
+    }
+  };
+  double gx = 0, gc = 1;
+  ZZGradientRecovery<Poisson>(fes);
+  if (fes.getSolutionGradient().size() != 2 * dofm.dof_nodes.size()) return false;
+  mid_edge_and_centre(fes.getSolutionGradient(), gx, gc);
+  if (!(gx < -0.30 && gx > -0.36 && gc < 1e-6)) return false;
+  gx = 0, gc = 1;
+  LocalSmoothingGradient<Poisson>(fes);
+  mid_edge_and_centre(fes.getSolutionGradient(), gx, gc);
+  if (!(gx < -0.30 && gx > -0.36 && gc < 1e-6)) return false;
   // the host copies agree with a device product
   auto K = fes.getGlobalTangent();
   Vector<> r = K * fes.getSolution() - fes.getGlobalResidual();

@@ -362,3 +362,95 @@ def test_bicgstab_jacobi_nonsymmetric(orc):
 def test_rng_box_muller_moments(orc):
     z = orc.normal(42, 200001)
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+
+
+# ------------------------------------------------------------ gradient recovery
+def _graded(dn, dim):
+    """x -> x + 0.15 x (1 - x) sin(3x) per axis: rectangles stay rectangles
+    (p = 1), with unequal neighbouring cells."""
+    X = dn.copy()
+    for d in range(dim):
+        X[:, d] = dn[:, d] + 0.15 * dn[:, d] * (1 - dn[:, d]) * np.sin(3 * dn[:, d])
+    return X
+
+
+@pytest.mark.parametrize("dim,p,dpn", [(2, 1, 1), (2, 2, 1), (2, 3, 1), (3, 1, 1), (3, 1, 3), (3, 2, 3), (2, 1, 2)])
+def test_local_smoothing_recovers_linear_fields_exactly(orc, dim, p, dpn):
+    """u_c = a_c + B_c . x lies in every element space, so every gradient
+    sample is B_c and the weighted average is B_c at every dof."""
+    el, dn = orc.mesh(dim, p, 3)
+    if p == 1:
+        dn = _graded(dn, dim)
+    rng = np.random.default_rng(dim * 10 + p + dpn)
+    a, B = rng.standard_normal(dpn), rng.standard_normal((dpn, dim))
+    u = (a[None, :] + dn[:, :dim] @ B.T).ravel()
+    G = orc.grad_local_smoothing(dim, p, dpn, dn, el, u)
+    expect = np.tile(B, (dn.shape[0], 1))
+    assert np.abs(G - expect).max() < 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_local_smoothing_q1_quadratic_closed_form(orc, dim):
+    """u = x^2 on a uniform Q1 grid (h = 1/n): the interpolant's x-derivative
+    is 2 x_mid in each cell column.  The two cells around an interior node
+    carry equal weights (sum N jxw), so the average is 2 x_n; at x = 0 only the
+    right cells count (h), at x = 1 only the left ones (2 - h)."""
+    n = 4
+    h = 1.0 / n
+    el, dn = orc.mesh(dim, 1, n)
+    G = orc.grad_local_smoothing(dim, 1, 1, dn, el, dn[:, 0] ** 2)
+    x = dn[:, 0]
+    expect = np.where(x < 1e-12, h, np.where(x > 1 - 1e-12, 2 - h, 2 * x))
+    assert np.abs(G[:, 0] - expect).max() < 1e-13
+    assert np.abs(G[:, 1:]).max() < 1e-13
+
+
+def test_local_smoothing_q2_exact_for_quadratics(orc):
+    """x^2 + x y - y^2 lies in Q2: every sample is the exact gradient."""
+    el, dn = orc.mesh(2, 2, 3)
+    x, y = dn[:, 0], dn[:, 1]
+    G = orc.grad_local_smoothing(2, 2, 1, dn, el, x * x + x * y - y * y)
+    assert np.abs(G[:, 0] - (2 * x + y)).max() < 1e-12
+    assert np.abs(G[:, 1] - (x - 2 * y)).max() < 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_zz_recovers_linear_fields_exactly(orc, dim):
+    el, dn = orc.mesh(dim, 1, 4)
+    dn = _graded(dn, dim)
+    rng = np.random.default_rng(7 + dim)
+    a, b = rng.standard_normal(), rng.standard_normal(dim)
+    G = orc.grad_zz(dim, dn, el, a + dn[:, :dim] @ b)
+    assert np.abs(G - b[None, :]).max() < 1e-11
+
+
+def test_zz_graded_patch_least_squares_closed_form(orc):
+    """u = x^2 on a 2-D Q1 grid graded in x only: at a node with cells of
+    widths h1 (left) and h2 (right), the samples are the per-column constant
+    slopes 2 x_n - h1 and 2 x_n + h2 at the Gauss abscissae x_n - h1 (1 -+ 1/sqrt 3)/2
+    and x_n + h2 (1 +- 1/sqrt 3)/2 (two y points per cell, two cells in y in the
+    interior).  The fit of that step with a line in x (y carries no signal)
+    is a weighted least-squares problem solved here by numpy's SVD-based
+    lstsq from those closed-form samples; ZZ evaluates it at x_n."""
+    n = 5
+    el, dn = orc.mesh(2, 1, n)
+    dn = _graded(dn, 1)
+    xs = np.unique(np.round(dn[:, 0], 14))
+    G = orc.grad_zz(2, dn, el, dn[:, 0] ** 2)
+    g = 1 / math.sqrt(3)
+    for k, xn in enumerate(xs):
+        samples = []
+        if k > 0:
+            h1 = xn - xs[k - 1]
+            samples += [(xn - h1 * (1 - s) / 2, 2 * xn - h1) for s in (g, -g)]
+        if k < len(xs) - 1:
+            h2 = xs[k + 1] - xn
+            samples += [(xn + h2 * (1 + s) / 2, 2 * xn + h2) for s in (-g, g)]
+        P = np.array([[1.0, sx - xn] for sx, _ in samples])
+        b = np.array([v for _, v in samples])
+        coef = np.linalg.lstsq(P, b, rcond=None)[0]
+        # every node row with this x: interior rows in y double every sample,
+        # which leaves the fit unchanged
+        rows = np.abs(dn[:, 0] - xn) < 1e-12
+        assert np.abs(G[rows, 0] - coef[0]).max() < 1e-12, (k, xn)
+        assert np.abs(G[rows, 1]).max() < 1e-12

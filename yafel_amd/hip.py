@@ -109,6 +109,7 @@ for _n, _a in [
     ("yafel_hip_element_matrices_fp32", [_P, C.POINTER(Physics), _P]),
     ("yafel_hip_dirichlet_apply", [_P, _P, _P, _P]),
     ("yafel_hip_bc_mask_plane", [_P, _I, _D, _D, _I, _P]),
+    ("yafel_hip_gradient_local_smoothing", [_P, _P, _P]), ("yafel_hip_gradient_zz", [_P, _P, _P]),
     ("yafel_hip_structured_mesh", [_I, _I, _I, _I, _I, _P, _P]),
     ("yafel_hip_normal_fill", [C.c_uint64, _I64, _D, _D, _P]),
     ("yafel_hip_dg_create", [_P, _I, _I64, _P, _I64, _P, C.POINTER(_P)]),
@@ -502,6 +503,20 @@ class Mesh:
 
     def bc_mask_plane(self, mask_dev, axis, value, tol=1e-6, comp=-1):
         check(lib.yafel_hip_bc_mask_plane(self.h, axis, value, tol, comp, mask_dev.ptr), self.ctx.h)
+
+    def gradient_local_smoothing(self, u_dev, grad_dev=None):
+        """LocalSmoothingGradient: (ndofs, dim) gradient per dof into grad_dev
+        (allocated when None); returns the device array."""
+        G = grad_dev if grad_dev is not None else self.ctx.alloc(self.ndofs * self.dim)
+        check(lib.yafel_hip_gradient_local_smoothing(self.h, u_dev.ptr, G.ptr), self.ctx.h)
+        return G
+
+    def gradient_zz(self, u_dev, grad_dev=None):
+        """ZZGradientRecovery (p = 1, scalar field): (n_nodes, dim) recovered
+        nodal gradient into grad_dev (allocated when None)."""
+        G = grad_dev if grad_dev is not None else self.ctx.alloc(self.n_nodes * self.dim)
+        check(lib.yafel_hip_gradient_zz(self.h, u_dev.ptr, G.ptr), self.ctx.h)
+        return G
 
 
 BC_BOX, BC_XMIN = 0, 1
