@@ -380,6 +380,51 @@ def dg_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def recovery_bench(args):
+    """Gradient recovery after a solve, at the sizes of the apps that call it:
+    LocalSmoothingGradient on 3-D elasticity Q1 64^3 (apps/BasicLinearElasticity,
+    823,875 dofs) and ZZGradientRecovery on 2-D Poisson Q1 1024^2
+    (apps/BasicPoisson, 1,050,625 nodes), seeded N(0,1) fields.  The oracle's
+    serial restatement of each is timed on the same inputs beside them."""
+    from yafel_amd import hip
+    ctx = hip.Context(0)
+    out = {"workload": "gradient recovery: local smoothing, elasticity Q1 64^3 (dpn 3); ZZ, Poisson Q1 1024^2"}
+    reps = 20
+    cases = (("local_smoothing", 3, 64, 3, "gradient_local_smoothing"), ("zz", 2, 1024, 1, "gradient_zz"))
+    for name, dim, n, dpn, fn in cases:
+        m = hip.Mesh.structured(ctx, dim, 1, dpn, n)
+        u = ctx.upload(hip.normal(7000 + dim, m.ndofs))
+        rows = m.ndofs if name == "local_smoothing" else m.n_nodes
+        G = ctx.alloc(rows * dim)
+        getattr(m, fn)(u, G)
+        ctx.timer_begin()
+        for _ in range(reps):
+            getattr(m, fn)(u, G)
+        ms = ctx.timer_end() / reps
+        # unique bytes: connectivity, coordinates, the field, the gradient written
+        uniq = m.nel * m.nen * 4 + m.n_nodes * 24 + m.ndofs * 8 + rows * dim * 8
+        # the staged per-element values written once and read once
+        stage = (m.nel * m.nen * (1 + dpn * dim) if name == "local_smoothing" else m.nel * m.nen * 2 * dim) * 8
+        rec = {"dofs": m.ndofs, "elements": m.nel, "ms": ms, "unique_bytes": uniq, "staged_bytes": stage,
+               "gbs_unique": uniq / (ms * 1e-3) / 1e9, "gbs_with_staging": (uniq + 2 * stage) / (ms * 1e-3) / 1e9}
+        if not args.no_cpu:
+            from oracle.oracle import Oracle
+            orc = Oracle(fast=True)
+            el, dn = m.download()
+            uh = u.download()
+            t0 = time.perf_counter()
+            if name == "local_smoothing":
+                orc.grad_local_smoothing(dim, 1, dpn, dn, el, uh)
+            else:
+                orc.grad_zz(dim, dn, el, uh)
+            t_cpu = time.perf_counter() - t0
+            rec["cpu_baseline"] = {"ms": t_cpu * 1e3, "kind": "port", "cores": 1,
+                                   "sample": "the oracle's serial restatement on the same mesh and field"}
+        log(f"{name}: {rec}")
+        out[name] = rec
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -388,7 +433,7 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="cells per axis (64 = the benchmark config)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "serve", "c3", "spmv", "dg", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "serve", "c3", "spmv", "dg", "c5", "recovery"])
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--sweep-requests", type=int, default=300)
     ap.add_argument("--rates", default="50,200,500,1000,2000")
@@ -411,6 +456,9 @@ def main():
         return
     if args.workload == "c5":
         c5_bench(args)
+        return
+    if args.workload == "recovery":
+        recovery_bench(args)
         return
     if args.workload == "dg":
         dg_bench(args)

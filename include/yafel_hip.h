@@ -283,12 +283,14 @@ int yafel_hip_bc_mask_plane(yafel_hip_mesh m, int axis, double value, double tol
  * reference's RowMajor SolutionGradient) gets, for each dof, the volume-
  * weighted average over its node's elements and quadrature points of the
  * gradient of that dof's component: sum N_a jxw grad u_c / sum N_a jxw.
- * u_dev is the solution (n_dofs). */
+ * u_dev is the solution (n_dofs).  Stream-ordered on the mesh's context
+ * stream; the per-element staging is kept with the mesh. */
 int yafel_hip_gradient_local_smoothing(yafel_hip_mesh m, const double *u_dev, double *grad_dev);
 /* ZZGradientRecovery<Physics>(FESystem&) (include/assembly/ZZGradientRecovery.hpp:
  * 33-141), p = 1 and one dof per node: at every node, the least-squares
  * linear fit of the gradient samples at the quadrature points of the adjacent
- * elements, evaluated at the node; grad_dev is n_nodes x dim, row-major. */
+ * elements, evaluated at the node; grad_dev is n_nodes x dim, row-major.
+ * Stream-ordered, like the smoothing. */
 int yafel_hip_gradient_zz(yafel_hip_mesh m, const double *u_dev, double *grad_dev);
 
 /* -------------------------------------------------------------------- DG */

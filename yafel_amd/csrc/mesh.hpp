@@ -35,6 +35,7 @@ struct yafel_hip_mesh_s {
   std::unique_ptr<YhPlan> plan_csr, plan_bcsr;
   yh::DBuf stageK, stageR;  // element matrix / vector staging
   yh::DBuf stageK32;        // FP32 element matrix staging (matrix-core path)
+  yh::DBuf stageG;          // gradient recovery staging (per element values)
   ~yafel_hip_mesh_s() { yh::ctx_release(ctx); }
 };
 

@@ -11,9 +11,9 @@
 // per-element values (per (element, local node) weighted sums, or per
 // (element, qp) samples); the node pass walks the node's incidence list
 // (element order, the residual assembly's plan) -- deterministic, no atomics.
-// ZZ fits in coordinates centred on the node and scaled by the patch size:
-// the same least-squares solution, evaluated at the node as its constant
-// term, from 4 x 4 (3-D) normal equations that are well conditioned.
+// ZZ fits in coordinates centred on the node: the same least-squares
+// solution, evaluated at the node as its constant term, from at most 4 x 4
+// normal equations whose conditioning no longer depends on where the node is.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -227,9 +227,10 @@ __global__ __launch_bounds__(NT) void k_zz_samples(int64_t nel, const int32_t *_
 }
 
 // Node pass: the least-squares fit of the patch's samples with basis
-// (1, (x - x_n)/h) -- h the largest sample offset -- by its normal equations,
-// solved by Gaussian elimination with partial pivoting; the recovered
-// gradient is the fit's constant term (ZZGradientRecovery.hpp:112-134).
+// (1, x - x_n) by its normal equations, solved by Gaussian elimination with
+// partial pivoting (which a scaling of the offset columns would not change
+// beyond rounding); the recovered gradient is the fit's constant term
+// (ZZGradientRecovery.hpp:112-134).
 template <int DIM>
 __global__ __launch_bounds__(NT) void k_zz_node(int64_t nnodes, const double *__restrict__ X,
                                                 const int32_t *__restrict__ ptr, const uint32_t *__restrict__ inc,
@@ -240,16 +241,6 @@ __global__ __launch_bounds__(NT) void k_zz_node(int64_t nnodes, const double *__
   double xn[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) xn[d] = X[3 * A + d];
-  double h = 0;
-  for (int k = ptr[A]; k < ptr[A + 1]; ++k) {
-    const int64_t e = inc[k] / NEN;
-    for (int q = 0; q < NQP; ++q) {
-      const double *s = S + (e * NQP + q) * 2 * DIM;
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) h = fmax(h, fabs(s[d] - xn[d]));
-    }
-  }
-  const double ih = h > 0 ? 1.0 / h : 1.0;
   double M[NB][NB + DIM];  // [P^T P | P^T b]
 #pragma unroll
   for (int r = 0; r < NB; ++r)
@@ -262,7 +253,7 @@ __global__ __launch_bounds__(NT) void k_zz_node(int64_t nnodes, const double *__
       double pr[NB];
       pr[0] = 1.0;
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) pr[1 + d] = (s[d] - xn[d]) * ih;
+      for (int d = 0; d < DIM; ++d) pr[1 + d] = s[d] - xn[d];
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
 #pragma unroll
@@ -343,9 +334,9 @@ int yafel_hip_gradient_local_smoothing(yafel_hip_mesh m, const double *u_dev, do
   yafel_hip_ctx_s *ctx = m->ctx;
   HIPCHK(hipSetDevice(ctx->device));
   mesh_build_incidence(m);
-  DBuf S;
-  S.alloc((size_t)m->nel * m->nen * (1 + m->dpn * m->dim) * 8 + 8);
-  double *sp = S.as<double>();
+  const size_t need = (size_t)m->nel * m->nen * (1 + m->dpn * m->dim) * 8 + 8;
+  if (m->stageG.bytes < need) m->stageG.alloc(need);
+  double *sp = m->stageG.as<double>();
   if (m->dim == 2) {
     switch (m->p) {
       case 1: launch_smooth_p<2, 1>(m, u_dev, sp, grad_dev); break;
@@ -361,7 +352,6 @@ int yafel_hip_gradient_local_smoothing(yafel_hip_mesh m, const double *u_dev, do
       default: fail(YAFEL_HIP_EINVAL, "gradient_local_smoothing: 3-D elements are supported for p = 1..2");
     }
   }
-  HIPCHK(hipStreamSynchronize(ctx->stream));
   return YAFEL_HIP_OK;
   YH_TRY_END(m ? m->ctx : nullptr)
 }
@@ -377,9 +367,9 @@ int yafel_hip_gradient_zz(yafel_hip_mesh m, const double *u_dev, double *grad_de
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   mesh_build_incidence(m);
-  DBuf S;
-  S.alloc((size_t)m->nel * m->nqp * 2 * m->dim * 8 + 8);
-  double *sp = S.as<double>();
+  const size_t need = (size_t)m->nel * m->nqp * 2 * m->dim * 8 + 8;
+  if (m->stageG.bytes < need) m->stageG.alloc(need);
+  double *sp = m->stageG.as<double>();
   const int64_t ns = m->nel * m->nqp;
   if (m->dim == 2) {
     k_zz_samples<2><<<ceil_div(ns, NT), NT, 0, s>>>(m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(),
@@ -395,7 +385,6 @@ int yafel_hip_gradient_zz(yafel_hip_mesh m, const double *u_dev, double *grad_de
                                                          m->inc.as<uint32_t>(), sp, grad_dev);
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(s));
   return YAFEL_HIP_OK;
   YH_TRY_END(m ? m->ctx : nullptr)
 }
