@@ -2,6 +2,7 @@
 // CGAssembly<Physics>, DGAssembly<Physics>, DGRK4 and DirichletBC with the
 // reference's names.
 //
+//   Mesh(fname)                           src/mesh/Mesh.cpp:32-36, src/mesh/parse_gmsh.cpp:80-151 (gmsh 2.2)
 //   DoFManager(M, CG|DG, p, dof_per_node) src/utils/DoFManager.cpp:11-36 (numbering on the device)
 //   FESystem(dofm)                        include/fe_system/FESystem.hpp:18-56
 //   CGAssembly<Physics>(fesys, reqs)      include/assembly/CGAssembly.hpp:27-186
@@ -18,6 +19,9 @@
 #pragma once
 
 #include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "yafel_amd/linalg.hpp"
@@ -38,6 +42,36 @@ class Mesh {
  public:
   Mesh(int dim, std::vector<double> nodes_xyz, std::vector<int32_t> cells)
       : dim_(dim), nodes_(std::move(nodes_xyz)), cells_(std::move(cells)) {}
+  // Mesh(const std::string &fname) (src/mesh/Mesh.cpp:32-36): a gmsh 2.2 file.
+  // The volume cells (Hex8 if the file has any, else Quad4) are kept in file
+  // order; lines, points and (in 3-D) faces are boundary tags that
+  // CGAssembly skips (topoDim != nsd).  Simplex volume cells are not covered
+  // by the device elements: std::invalid_argument.
+  explicit Mesh(const std::string &fname) {
+    yafel_hip_gmsh g = nullptr;
+    hipdetail::check(yafel_hip_gmsh_read(fname.c_str(), &g));
+    std::shared_ptr<yafel_hip_gmsh_s> hold(g, [](yafel_hip_gmsh_s *p) { yafel_hip_gmsh_destroy(p); });
+    int64_t sz[4];
+    hipdetail::check(yafel_hip_gmsh_sizes(g, sz));
+    std::vector<int32_t> cn((std::size_t)sz[2]), off((std::size_t)sz[1] + 1), ty((std::size_t)sz[1]);
+    nodes_.resize((std::size_t)sz[0] * 3);
+    hipdetail::check(yafel_hip_gmsh_arrays(g, nodes_.data(), cn.data(), off.data(), ty.data(), nullptr, nullptr));
+    bool hex = false, tri = false, tet = false;
+    for (int t : ty) {
+      hex = hex || t == YAFEL_HIP_CELL_HEX8;
+      tri = tri || t == YAFEL_HIP_CELL_TRI3;
+      tet = tet || t == YAFEL_HIP_CELL_TET4;
+    }
+    if (tet || (tri && !hex))
+      throw std::invalid_argument("Mesh: simplex volume cells are not covered by the device elements");
+    dim_ = hex ? 3 : 2;
+    const int want = hex ? YAFEL_HIP_CELL_HEX8 : YAFEL_HIP_CELL_QUAD4, nen = hex ? 8 : 4;
+    for (std::size_t c = 0; c < ty.size(); ++c) {
+      if (ty[c] != want) continue;
+      if (off[c + 1] - off[c] != nen) throw std::invalid_argument("Mesh: cell with a wrong node count");
+      cells_.insert(cells_.end(), cn.begin() + off[c], cn.begin() + off[c + 1]);
+    }
+  }
   // unit square / cube with nx*ny(*nz) cells, corners ordered for polynomial order p
   static Mesh structured(int dim, int nx, int ny, int nz, int p) {
     const std::size_t nn = (std::size_t)(nx + 1) * (ny + 1) * (dim == 3 ? nz + 1 : 1);

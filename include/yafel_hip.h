@@ -277,6 +277,30 @@ int yafel_hip_dirichlet_apply(yafel_hip_matrix A, const uint8_t *mask_dev, const
 int yafel_hip_bc_mask_plane(yafel_hip_mesh m, int axis, double value, double tol, int comp,
                             uint8_t *mask_dev);
 
+/* ------------------------------------------------------------ mesh input */
+/* Mesh(const std::string &fname) -> Mesh::parse_gmsh (src/mesh/Mesh.cpp:32-36,
+ * src/mesh/parse_gmsh.cpp:80-151): a gmsh 2.2 ASCII file.  Node id i is stored
+ * at i-1 (3 coordinates); cell id c at c-1 with its gmsh tags and 0-based
+ * nodes at cell_offsets[c-1] (running offset in file order, the last entry
+ * the total); cell types are the reference's CellType
+ * (include/mesh/mesh_typedefs.hpp:42-50), an unknown gmsh type giving NONE.
+ * Host only; no context needed. */
+#define YAFEL_HIP_CELL_LINE2 0
+#define YAFEL_HIP_CELL_TRI3 1
+#define YAFEL_HIP_CELL_TET4 2
+#define YAFEL_HIP_CELL_QUAD4 3
+#define YAFEL_HIP_CELL_HEX8 4
+#define YAFEL_HIP_CELL_POINT1 5
+#define YAFEL_HIP_CELL_NONE 6
+typedef struct yafel_hip_gmsh_s *yafel_hip_gmsh;
+int yafel_hip_gmsh_read(const char *path, yafel_hip_gmsh *out);
+/* sizes[0..3] = nodes, cells, cell node entries, tags */
+int yafel_hip_gmsh_sizes(yafel_hip_gmsh g, int64_t *sizes);
+/* any pointer may be NULL; cell_offsets and tag_offsets hold cells+1 entries */
+int yafel_hip_gmsh_arrays(yafel_hip_gmsh g, double *nodes, int32_t *cell_nodes, int32_t *cell_offsets,
+                          int32_t *cell_types, int32_t *tag_offsets, int32_t *tags);
+int yafel_hip_gmsh_destroy(yafel_hip_gmsh g);
+
 /* ------------------------------------------------------ gradient recovery */
 /* LocalSmoothingGradient<Physics>(FESystem&) (include/assembly/
  * LocalSmoothingGradient.hpp:20-94): grad_dev (n_dofs x dim, row-major, the

@@ -155,6 +155,52 @@ static bool poisson_flow() {
   return std::sqrt(rr / bb) < 1e-9;
 }
 
+// Mesh(fname): a gmsh 2.2 file of the 3 x 3 unit square in counter-clockwise
+// Quad4 corners, with boundary lines, gives the same p = 2 system as the
+// generated mesh; a triangle mesh (the reference's test/square.msh layout) is
+// refused, the device elements being tensor-product ones.
+static bool gmsh_flow() {
+  Mesh S = Mesh::structured(2, 3, 3, 0, 2);
+  const char *path = "/tmp/yafel_hot_path_quad.msh";
+  {
+    std::FILE *f = std::fopen(path, "w");
+    if (!f) return false;
+    std::fprintf(f, "$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%zu\n", S.nNodes());
+    for (std::size_t i = 0; i < S.nNodes(); ++i)
+      std::fprintf(f, "%zu %.17g %.17g %.17g\n", i + 1, S.nodes()[3 * i], S.nodes()[3 * i + 1], S.nodes()[3 * i + 2]);
+    std::fprintf(f, "$EndNodes\n$Elements\n%zu\n", S.nCells() + 2);
+    std::fprintf(f, "1 1 2 7 1 1 2\n");  // a boundary line first, as gmsh writes them
+    for (std::size_t c = 0; c < S.nCells(); ++c) {
+      std::fprintf(f, "%zu 3 2 9 1", c + 2);
+      for (int k = 0; k < 4; ++k) std::fprintf(f, " %d", S.cells()[4 * c + k] + 1);
+      std::fprintf(f, "\n");
+    }
+    std::fprintf(f, "%zu 15 2 8 1 1\n$EndElements\n", S.nCells() + 2);
+    std::fclose(f);
+  }
+  Mesh G(path);
+  if (G.dim() != 2 || G.cells() != S.cells() || G.nodes().size() != S.nodes().size()) return false;
+  DoFManager dg(G, DoFManager::ManagerType::CG, 2, 1), ds(S, DoFManager::ManagerType::CG, 2, 1);
+  FESystem fg(dg), fs(ds);
+  CGAssembly<Poisson>(fg);
+  CGAssembly<Poisson>(fs);
+  auto Kg = fg.getGlobalTangent(), Ks = fs.getGlobalTangent();
+  if (Kg.row_ptr != Ks.row_ptr || Kg.col_index != Ks.col_index || Kg._data != Ks._data) return false;
+  {
+    std::FILE *f = std::fopen(path, "w");
+    if (!f) return false;
+    std::fprintf(f, "$Nodes\n3\n1 0 0 0\n2 1 0 0\n3 0 1 0\n$EndNodes\n$Elements\n1\n1 2 2 1 10 1 2 3\n$EndElements\n");
+    std::fclose(f);
+  }
+  try {
+    Mesh T(path);
+    return false;
+  } catch (const std::invalid_argument &) {
+  }
+  std::remove(path);
+  return true;
+}
+
 // apps/BasicAdvection_DG/main.cpp with a constant source and no wind: RK4
 // advances every dof by s * t exactly; with wind, mass is only lost through
 // the outflow boundary.
@@ -222,6 +268,7 @@ int main() {
         }));
     run(8, "poisson_flow", poisson_flow());
     run(9, "dg_flow", dg_flow());
+    run(10, "gmsh_flow", gmsh_flow());
   } catch (const std::exception &e) {
     std::printf("exception: %s\n", e.what());
     return 255;

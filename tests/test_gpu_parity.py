@@ -413,3 +413,34 @@ def test_multi_rhs_alone_vs_together(H, ctx, orc, method, prec, k):
         x1, i1 = A.solve_host(B[:, j], method=getattr(H, method), precision=getattr(H, prec),
                               stop=H.STOP_RELRES, tol=1e-8)
         assert np.abs(X[:, j] - x1).max() <= 1e-6 * np.abs(x1).max()
+
+
+# ------------------------------------------------------------ gmsh input
+@pytest.mark.parametrize("dim,p", [(2, 1), (2, 2), (3, 1)])
+def test_mesh_from_gmsh_assembles_like_the_generated_mesh(H, ctx, tmp_path, dim, p):
+    """A gmsh 2.2 file of the generated mesh (with a boundary line and a point
+    cell first) gives the same numbering and a bit-identical tangent."""
+    n = 6 if dim == 2 else 3
+    nodes, cells = H.structured_mesh(dim, n, n, n if dim == 3 else 0, p)
+    path = tmp_path / "m.msh"
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(len(nodes))]
+    lines += [f"{i + 1} {float(x[0])!r} {float(x[1])!r} {float(x[2])!r}" for i, x in enumerate(nodes)]
+    lines += ["$EndNodes", "$Elements", str(len(cells) + 2), "1 1 2 7 1 1 2", "2 15 2 8 1 1"]
+    typ = 3 if dim == 2 else 5
+    lines += [" ".join(str(v) for v in [c + 3, typ, 2, 9, 1, *(cells[c] + 1)]) for c in range(len(cells))]
+    lines += ["$EndElements"]
+    path.write_text("\n".join(lines) + "\n")
+    g = H.GmshMesh(path)
+    ph = H.physics(H.POISSON, dim, f=1.0)
+    out = []
+    for X, C_ in ((nodes, cells), (g.nodes, g.volume_cells(dim))):
+        m = H.Mesh(ctx, dim, p, 1, X, C_)
+        A = m.pattern(H.CSR, H.FP64)
+        R = ctx.upload(np.zeros(m.ndofs))
+        m.assemble(ph, A, R)
+        out.append((m.download(), A.arrays(), R.download()))
+    (el0, dn0), (pt0, c0, v0), r0 = out[0]
+    (el1, dn1), (pt1, c1, v1), r1 = out[1]
+    assert np.array_equal(el0, el1) and np.array_equal(dn0, dn1)
+    assert np.array_equal(pt0, pt1) and np.array_equal(c0, c1) and np.array_equal(v0, v1)
+    assert np.array_equal(r0, r1)

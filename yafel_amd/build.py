@@ -27,7 +27,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["runtime.hip", "sparse.hip", "blas.hip", "krylov.hip", "sort.hip", "triplets.hip",
-           "mesh.hip", "assembly.hip", "multi.hip", "serve.hip", "dg.hip", "recovery.hip"]
+           "mesh.hip", "assembly.hip", "multi.hip", "serve.hip", "dg.hip", "recovery.hip", "io.hip"]
 
 
 def _run(cmd):

@@ -110,6 +110,8 @@ for _n, _a in [
     ("yafel_hip_dirichlet_apply", [_P, _P, _P, _P]),
     ("yafel_hip_bc_mask_plane", [_P, _I, _D, _D, _I, _P]),
     ("yafel_hip_gradient_local_smoothing", [_P, _P, _P]), ("yafel_hip_gradient_zz", [_P, _P, _P]),
+    ("yafel_hip_gmsh_read", [C.c_char_p, _P]), ("yafel_hip_gmsh_sizes", [_P, _P]),
+    ("yafel_hip_gmsh_arrays", [_P] * 7), ("yafel_hip_gmsh_destroy", [_P]),
     ("yafel_hip_structured_mesh", [_I, _I, _I, _I, _I, _P, _P]),
     ("yafel_hip_normal_fill", [C.c_uint64, _I64, _D, _D, _P]),
     ("yafel_hip_dg_create", [_P, _I, _I64, _P, _I64, _P, C.POINTER(_P)]),
@@ -370,6 +372,53 @@ def structured_mesh(dim, nx, ny, nz=0, p=1):
     cells = np.zeros((nc, 2 ** dim), np.int32)
     check(lib.yafel_hip_structured_mesh(dim, nx, ny, nz, p, _ptr(nodes), _ptr(cells)))
     return nodes, cells
+
+
+CELL_LINE2, CELL_TRI3, CELL_TET4, CELL_QUAD4, CELL_HEX8, CELL_POINT1, CELL_NONE = range(7)
+
+
+class GmshMesh:
+    """Mesh(const std::string &fname) (src/mesh/parse_gmsh.cpp:80-151): the
+    arrays of a gmsh 2.2 ASCII file -- nodes (n, 3); cell_nodes (0-based) with
+    cell_offsets (cells + 1); cell_types (CELL_*); tags with tag_offsets."""
+
+    def __init__(self, path):
+        h = _P()
+        check(lib.yafel_hip_gmsh_read(os.fsencode(path), C.byref(h)))
+        try:
+            sz = np.zeros(4, np.int64)
+            check(lib.yafel_hip_gmsh_sizes(h, _ptr(sz)))
+            nn, nc, ncn, nt = (int(v) for v in sz)
+            self.nodes = np.zeros((nn, 3))
+            self.cell_nodes = np.zeros(ncn, np.int32)
+            self.cell_offsets = np.zeros(nc + 1, np.int32)
+            self.cell_types = np.zeros(nc, np.int32)
+            self.tag_offsets = np.zeros(nc + 1, np.int32)
+            self.tags = np.zeros(nt, np.int32)
+            check(lib.yafel_hip_gmsh_arrays(h, _ptr(self.nodes), _ptr(self.cell_nodes), _ptr(self.cell_offsets),
+                                            _ptr(self.cell_types), _ptr(self.tag_offsets), _ptr(self.tags)))
+        finally:
+            lib.yafel_hip_gmsh_destroy(h)
+
+    def cell(self, c):
+        return self.cell_nodes[self.cell_offsets[c]:self.cell_offsets[c + 1]]
+
+    def cell_tags(self, c):
+        return self.tags[self.tag_offsets[c]:self.tag_offsets[c + 1]]
+
+    def volume_cells(self, dim):
+        """The Quad4 (dim 2) or Hex8 (dim 3) cells as an (n, 2^dim) array in file
+        order -- the cells CGAssembly assembles (topoDim == nsd); lines and
+        points are boundary tags only."""
+        want = CELL_QUAD4 if dim == 2 else CELL_HEX8
+        # lower-dimensional cells (boundary lines, faces, points) are skipped
+        lower = {CELL_LINE2, CELL_POINT1} | ({CELL_QUAD4, CELL_TRI3} if dim == 3 else set())
+        other = {int(t) for t in self.cell_types} - lower - {want}
+        if other:
+            raise YafelHipError(EINVAL, f"gmsh mesh has cells the device elements do not cover: {sorted(other)}")
+        sel = [c for c in range(len(self.cell_types)) if self.cell_types[c] == want]
+        nen = 2 ** dim
+        return np.array([self.cell(c) for c in sel], np.int32).reshape(len(sel), nen)
 
 
 def normal(seed, n, mean=0.0, sd=1.0):
