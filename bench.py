@@ -363,10 +363,13 @@ def dg_bench(args):
         dg.residual(ph, U, R)
     t_res = ctx.timer_end() / reps
     nel, nen, nf, n1 = cells.shape[0], dg.nen, dg.nfaces, p + 1
-    # unique bytes of one evaluation: U, X, element mass inverses, face records,
-    # node lists and geometry, the face flux written and read back, R written
-    byts = (dg.ndofs * 8 + dg.ndofs * 24 + nel * nen * nen * 8 + nf * (16 + 2 * n1 * 4 + n1 * 32)
-            + 2 * nf * n1 * 8 + nel * 16 + dg.ndofs * 8)
+    na = dg.naffine
+    # unique bytes of one evaluation: U; per cell, its 64-byte affine record and,
+    # for a non-parallelogram cell, its mass inverse and node coordinates; face
+    # records, node lists and geometry; the per-cell face-node flux sums written
+    # and read back; the cells' face lists; R written
+    byts = (dg.ndofs * 8 + nel * 64 + (nel - na) * (nen * nen * 8 + nen * 24)
+            + nf * (16 + 2 * n1 * 4 + n1 * 32) + 2 * nel * 4 * n1 * 8 + nel * 16 + dg.ndofs * 8)
     dt, steps = 1e-4, 20
     dg.rk4(ph, U, dt, 1)
     ctx.sync()
@@ -374,6 +377,7 @@ def dg_bench(args):
     dg.rk4(ph, U, dt, steps)
     t_rk = (time.perf_counter() - t0) / steps * 1e3
     out = {"workload": f"DG upwind advection, Quad4 {n}^2, p = {p} ({dg.ndofs} dofs, {nf} faces)",
+           "affine_cells": na,
            "residual_us": t_res * 1e3, "residual_gbs": byts / (t_res * 1e-3) / 1e9,
            "residual_bytes": byts, "elements_per_s": nel / (t_res * 1e-3),
            "rk4_step_ms": t_rk, "rk4_steps_per_s": 1e3 / t_rk}

@@ -329,6 +329,12 @@ int yafel_hip_dg_create(yafel_hip_ctx ctx, int p, int64_t n_corner, const double
 int yafel_hip_dg_destroy(yafel_hip_dg dg);
 /* info[0] dofs, info[1] faces, info[2] boundary faces, info[3] dofs per element */
 int yafel_hip_dg_info(yafel_hip_dg dg, int64_t *info);
+/* No reference counterpart: the number of parallelogram cells.  Their mass
+ * matrix is det J times the reference one, so the residual applies the shared
+ * reference inverse divided by det J (the same operator as the reference's
+ * per-cell PartialPivLU inverse, to round-off) and reads one 64-byte record
+ * instead of the cell's inverse and node coordinates. */
+int yafel_hip_dg_affine_cells(yafel_hip_dg dg, int64_t *count);
 /* dof node coordinates (dofs * 3 doubles, device) */
 int yafel_hip_dg_nodes(yafel_hip_dg dg, double *nodes_dev);
 /* the face list in the reference's order (host arrays of `faces` entries; -1 = none) */

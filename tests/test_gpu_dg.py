@@ -44,6 +44,41 @@ def test_residual_matches_the_oracle(ctx, orc, p):
     assert np.abs(r - R).max() <= 1e-12 * scale
 
 
+@pytest.mark.parametrize("p", [1, 3, 4])
+@pytest.mark.parametrize("shape", ["rectangles", "parallelograms", "mixed"])
+def test_affine_cells_match_the_oracle(ctx, orc, p, shape):
+    """Parallelogram cells take the shared reference mass inverse scaled by
+    1/det J; the residual still matches the oracle's per-cell inverse on
+    rectangles, on a sheared (parallelogram) mesh and on a mesh where only
+    some cells are parallelograms."""
+    from yafel_amd import hip as H
+    nx, ny = 9, 7
+    jit = 0.0 if shape != "mixed" else 0.2
+    nodes, cells = rect_mesh(orc, nx, ny, 1.25, 0.8, jitter=jit, seed=5)
+    if shape == "parallelograms":
+        nodes = nodes.copy()
+        nodes[:, 0] += 0.4 * nodes[:, 1]
+    if shape == "mixed":
+        # straighten the nodes of the left half: those cells are rectangles again
+        flat, _ = rect_mesh(orc, nx, ny, 1.25, 0.8, jitter=0.0, seed=5)
+        left = flat[:, 0] < 0.5 * 1.25 + 1e-9
+        nodes = nodes.copy()
+        nodes[left] = flat[left]
+    dg = H.DG(ctx, p, nodes, cells)
+    if shape == "mixed":
+        assert 0 < dg.naffine < len(cells)
+    else:
+        assert dg.naffine == len(cells)
+    ph = H.physics(H.DG_ADVECTION, 2, f=0.3, U=(-1.0, 0.35, 0.0))
+    oph = ophysics(DG, 2, f=0.3, U=(-1.0, 0.35, 0.0))
+    U = np.random.default_rng(p + 17).standard_normal(dg.ndofs)
+    Rd, Fd = ctx.alloc(dg.ndofs), ctx.alloc(dg.ndofs)
+    dg.residual(ph, ctx.upload(U), Rd, Fd)
+    R, Rf = orc.dg_residual(p, nodes, cells, oph, U, face=True)
+    assert np.abs(Fd.download() - Rf).max() <= 1e-12 * np.abs(Rf).max()
+    assert np.abs(Rd.download() - R).max() <= 1e-12 * np.abs(R).max()
+
+
 def test_closed_forms_on_the_device(ctx, orc):
     # constant source, no advection: R == s
     dg, *_ = make(ctx, orc, 3, 5, 4, c=(0.0, 0.0), f=-0.75)

@@ -53,7 +53,10 @@ struct yafel_hip_dg_s {
   DBuf geo;               // double [nfaces][n1][4]: unit normal (x, y), jxw left list, jxw right list
   DBuf ef;                // int32 [nel][4]: (face << 1) | side (0 left, 1 right)
   DBuf minv;              // double [nel][nen][nen], transposed: minv[e][j][i] = (M_e^-1)_ij
-  DBuf flux;              // double [nfaces][n1] scratch of one evaluation
+  DBuf ginfo;             // double [nel][8]: affine flag, det J, J^-T (4) of affine cells
+  DBuf mrefT;             // double [nen][nen]: reference mass inverse, transposed
+  int64_t naffine = 0;    // cells on the affine path
+  DBuf flux;              // double [nel][4][n1]: face-node flux sums of one evaluation (k_dg_flux)
   DBuf rk;                // U0, k1..k4, W (6 * ndofs)
   DBuf tdev;              // device time
   hipGraphExec_t graph = nullptr;
@@ -246,6 +249,32 @@ __device__ void vol_geom(int nen, int64_t e, int q, const double *X, const doubl
   jxw = det * w[q];
 }
 
+// Affine cells (parallelograms: x00 - x10 + x11 - x01 = 0 at the corners)
+// have a constant Jacobian, so M_e = det J * M_ref and M_e^-1 = M_ref^-1 / det J:
+// such a cell needs its det J and J^-T (one 64-byte record) instead of its
+// 16x16 inverse and node coordinates.  The record holds J from quadrature
+// point 0, as the element pass would form it.
+__global__ void k_dg_affine(int64_t nel, int n1, int nen, const double *__restrict__ X, const double *__restrict__ dN,
+                            const double *__restrict__ w, double *__restrict__ ginfo) {
+  const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (e >= nel) return;
+  const double *x00 = &X[3 * (e * nen)], *x10 = &X[3 * (e * nen + n1 - 1)];
+  const double *x11 = &X[3 * (e * nen + nen - 1)], *x01 = &X[3 * (e * nen + nen - n1)];
+  double h = 0, b = 0;
+  for (int d = 0; d < 2; ++d) {
+    h = fmax(h, fmax(fabs(x10[d] - x00[d]), fabs(x01[d] - x00[d])));
+    b = fmax(b, fabs(x00[d] - x10[d] + x11[d] - x01[d]));
+  }
+  double *g = &ginfo[8 * e];
+  const bool aff = b <= 1e-13 * h;
+  double jxw, Ji[4];
+  vol_geom(nen, e, 0, X, dN, w, jxw, Ji);
+  g[0] = aff ? 1.0 : 0.0;
+  g[1] = jxw / w[0];
+  for (int k = 0; k < 4; ++k) g[2 + k] = Ji[k];
+  g[6] = g[7] = 0.0;
+}
+
 // Inverse of the element mass matrix M = sum_q jxw N N^T (LocalMass,
 // main.cpp:73-78) by Gauss-Jordan with partial pivoting; one workgroup of
 // nen threads per element, thread i owns row i of [M | I] in LDS.
@@ -311,157 +340,255 @@ __global__ __launch_bounds__(64) void k_dg_minv(int64_t nel, int nqp, const doub
 
 // ------------------------------------------------------------ residual
 // AdvectionPhysics::Flux (upwind) on interior faces, BoundaryFlux (outflow
-// only) on boundary faces, at every face point.
-__global__ void k_dg_flux(int64_t nf, int n1, int nen, const DGFaceRec *__restrict__ F, const int32_t *__restrict__ L,
-                          const int32_t *__restrict__ R, const double *__restrict__ bN,
-                          const double *__restrict__ geo, double c0, double c1, const double *__restrict__ U,
-                          double *__restrict__ flux) {
-  const int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x;
-  if (i >= nf * n1) return;
-  const int64_t f = i / n1;
-  const int fq = (int)(i - f * n1);
-  const DGFaceRec r = F[f];
-  const double vn = geo[4 * i] * c0 + geo[4 * i + 1] * c1;  // dot(n, convection_velocity)
-  double fl;
-  if (r.left >= 0 && r.right >= 0) {
-    double ul = 0, ur = 0;
-    for (int k = 0; k < n1; ++k) {
-      const double sv = bN[fq * n1 + k];
-      ul += U[(int64_t)r.left * nen + L[f * n1 + k]] * sv;
-      ur += U[(int64_t)r.right * nen + R[f * n1 + k]] * sv;
+// only) on boundary faces, at every face point.  Each side of the face gets
+// its face-node sums
+//   slot[e][lf][k] = -+ sum_fq bN_i(fq) flux(fq) jxw_side(fq)
+// at the position k of face node i on its cell's local face lf (+ for the
+// right cell of an interior face, - otherwise, DGAssembly.hpp:106-189), so
+// that the element pass reads its face terms from its own slots.  A face is
+// handled by n1 adjacent lanes (64 / n1 faces per wavefront): lane k loads
+// face node k's values and computes face point k's flux, and the lanes
+// exchange them by shuffles for the interpolations and the sums.
+template <int P>
+__global__ __launch_bounds__(NT) void k_dg_flux(int64_t nf, const DGFaceRec *__restrict__ F,
+                                               const int32_t *__restrict__ L, const int32_t *__restrict__ R,
+                                               const int32_t *__restrict__ fn, const double *__restrict__ bN,
+                                               const double *__restrict__ geo, double c0, double c1,
+                                               const double *__restrict__ U, double *__restrict__ slot) {
+  constexpr int n1 = P + 1, nen = n1 * n1, FPW = 64 / n1;
+  const int lane = threadIdx.x & 63, grp = lane / n1, k = lane - grp * n1;
+  const int64_t f = ((int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6)) * FPW + grp;
+  const bool on = grp < FPW && f < nf;
+  const int base = grp * n1;  // first lane of this face's group
+  DGFaceRec r{-1, -1, 0, 0};
+  double ul = 0, ur = 0, gx = 0, gy = 0, jl = 0, jr = 0;
+  int nl = -1, nr = -1;
+  if (on) {
+    r = F[f];
+    if (r.left >= 0) {
+      nl = L[f * n1 + k];
+      ul = U[(int64_t)r.left * nen + nl];
     }
-    fl = 0.5 * (vn * (ul + ur) + fabs(vn) * (ul - ur));
+    if (r.right >= 0) {
+      nr = R[f * n1 + k];
+      ur = U[(int64_t)r.right * nen + nr];
+    }
+    const double *g = &geo[4 * (f * n1 + k)];
+    gx = g[0];
+    gy = g[1];
+    jl = g[2];
+    jr = g[3];
+  }
+  const bool interior = r.left >= 0 && r.right >= 0;
+  // u of each side at face point k (lane k), from the n1 node values
+  double a = 0, b = 0;
+#pragma unroll
+  for (int j = 0; j < n1; ++j) {
+    const double sv = bN[k * n1 + j];
+    a += __shfl(ul, base + j, 64) * sv;
+    b += __shfl(ur, base + j, 64) * sv;
+  }
+  const double vn = gx * c0 + gy * c1;  // dot(n, convection_velocity)
+  double fl;
+  if (interior) {
+    fl = 0.5 * (vn * (a + b) + fabs(vn) * (a - b));
   } else {
-    const int64_t el = r.left >= 0 ? r.left : r.right;
-    const int32_t *nodes = r.left >= 0 ? &L[f * n1] : &R[f * n1];
-    double u = 0;
-    for (int k = 0; k < n1; ++k) u += U[el * nen + nodes[k]] * bN[fq * n1 + k];
+    const double u = r.left >= 0 ? a : b;
     fl = vn > 0 ? vn * u : 0 * vn * u;
   }
-  flux[i] = fl;
+  const double fjl = fl * jl, fjr = fl * jr;
+  // lane i: the sums of face node i for each side
+  double cl = 0, cr = 0;
+#pragma unroll
+  for (int q = 0; q < n1; ++q) {
+    const double sv = bN[q * n1 + k];
+    cl += sv * __shfl(fjl, base + q, 64);
+    cr += sv * __shfl(fjr, base + q, 64);
+  }
+  if (!on) return;
+  if (r.left >= 0) {
+    // the left list is the left cell's own face order: position k
+    slot[((int64_t)r.left * 4 + r.lf) * n1 + k] = -cl;
+  }
+  if (r.right >= 0) {
+    int pos = 0;
+#pragma unroll
+    for (int j = 0; j < n1; ++j)
+      if (fn[r.rf * n1 + j] == nr) pos = j;
+    slot[((int64_t)r.right * 4 + r.rf) * n1 + pos] = interior ? cr : -cr;
+  }
 }
 
 // Element pass, a lane per (element, dof): the 64 / nen elements of a
-// wavefront are handled by nen lanes each (p = 3: 4 elements of 16 dofs).
-// Lane (element, q) first forms the geometry and u at volume point q into
-// LDS; lane (element, a) then adds the face terms of dof a in face order
-// (DGAssembly.hpp:106-189), the volume term (LocalResidual, main.cpp:53-71)
-// and row a of M^-1 (R_face + R_vol).
+// wavefront are handled by nen lanes each (p = 3: 4 elements of 16 dofs),
+// and a workgroup loops over element groups so that it loads the shape
+// tables into LDS once.  Lane (element, q) first forms u at volume point q
+// (and, for a cell that is not a parallelogram, the geometry there) into
+// LDS; lane (element, a) then adds the face terms of dof a from the cell's
+// face slots in face order (DGAssembly.hpp:106-189), the volume term
+// (LocalResidual, main.cpp:53-71) and row a of M^-1 (R_face + R_vol).
+// On a parallelogram J^-T and det J are constant, so the volume term is
+//   det J [ alpha sum_q w_q u_q dN_a/dxi(q) + beta sum_q w_q u_q dN_a/deta(q)
+//           + src sum_q w_q N_a(q) ],
+// alpha, beta = J^-T applied to the velocity, and M_e^-1 = M_ref^-1 / det J:
+// about a third of the LDS reads of the general form.
 template <int P>
-__global__ __launch_bounds__(NT) void k_dg_elem(int64_t nel, const DGFaceRec *__restrict__ F,
-                                               const int32_t *__restrict__ L, const int32_t *__restrict__ R,
-                                               const int32_t *__restrict__ ef, const double *__restrict__ geo,
-                                               const double *__restrict__ flux, const double *__restrict__ bN,
+__global__ __launch_bounds__(NT) void k_dg_elem(int64_t nel, const int32_t *__restrict__ fn,
+                                               const int32_t *__restrict__ ef, const double *__restrict__ slot,
                                                const double *__restrict__ X, const double *__restrict__ N,
                                                const double *__restrict__ dN, const double *__restrict__ w,
-                                               const double *__restrict__ minvT, double c0, double c1, double src,
+                                               const double *__restrict__ minvT, const double *__restrict__ ginfo,
+                                               const double *__restrict__ mrefT, double c0, double c1, double src,
                                                const double *__restrict__ U, double *__restrict__ Rout,
                                                double *__restrict__ Rface) {
-  constexpr int n1 = P + 1, nen = n1 * n1, nqp = nen, EPW = 64 / nen, WPB = NT / 64;
+  constexpr int n1 = P + 1, nen = n1 * n1, nqp = nen, EPW = 64 / nen, WPB = NT / 64, EPB = WPB * EPW;
   constexpr int LD = nen + 1;  // padded rows: lanes reading a row or a column hit distinct banks
   __shared__ double sN[nqp][LD];
   __shared__ double sdN[nqp][LD][2];
+  __shared__ double sMr[nen][LD];  // sMr[j][i] = (M_ref^-1)_ij
+  __shared__ double sNw[nen];      // sum_q w_q N_a(q)
+  __shared__ int sFn[4][n1];       // local face nodes
   __shared__ double sU[WPB][EPW][nen];
   __shared__ double sX[WPB][EPW][nen][2];
-  __shared__ double sG[WPB][EPW][nqp][6];  // jxw, J^-T (4), u at each volume point
+  __shared__ double sG[WPB][EPW][nqp][6];  // jxw, J^-T (4), u at each volume point (general cells)
+  __shared__ double sWU[WPB][EPW][nqp];    // w_q u_q (parallelograms)
   __shared__ double sL[WPB][EPW][nen];
-  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int el = l / nen, a = l - el * nen;
-  const int64_t e = ((int64_t)blockIdx.x * WPB + wv) * EPW + el;
-  const bool act = el < EPW && e < nel;
   for (int t = threadIdx.x; t < nqp * nen; t += NT) {
     const int q = t / nen, A = t - q * nen;
     sN[q][A] = N[t];
     sdN[q][A][0] = dN[2 * t];
     sdN[q][A][1] = dN[2 * t + 1];
+    sMr[q][A] = mrefT[t];
   }
-  // row a of M^-1, loaded early so its latency hides behind the face and
-  // volume work
-  double mrow[nen];
-  if (act) {
-    const double *Mt = &minvT[e * nen * nen];
-#pragma unroll
-    for (int j = 0; j < nen; ++j) mrow[j] = Mt[j * nen + a];
-    sU[wv][el][a] = U[e * nen + a];
-    sX[wv][el][a][0] = X[3 * (e * nen + a)];
-    sX[wv][el][a][1] = X[3 * (e * nen + a) + 1];
+  for (int t = threadIdx.x; t < 4 * n1; t += NT) sFn[t / n1][t % n1] = fn[t];
+  if (threadIdx.x < nen) {
+    double sw = 0;
+    for (int q = 0; q < nqp; ++q) sw += w[q] * N[q * nen + threadIdx.x];
+    sNw[threadIdx.x] = sw;
   }
   __syncthreads();
-  if (act) {
-    const int q = a;
-    double J[4] = {0, 0, 0, 0};
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int el = l / nen, a = l - el * nen;
+  // the local faces through node a and its position on each (loop-invariant)
+  int lfk[2] = {0, 0}, nlf = 0;
 #pragma unroll
-    for (int A = 0; A < nen; ++A) {
-      const double x0 = sX[wv][el][A][0], x1 = sX[wv][el][A][1];
-      const double g0 = sdN[q][A][0], g1 = sdN[q][A][1];
-      J[0] += x0 * g0;
-      J[1] += x0 * g1;
-      J[2] += x1 * g0;
-      J[3] += x1 * g1;
-    }
-    const double det = J[0] * J[3] - J[1] * J[2];
-    double u = 0;
+  for (int lf = 0; lf < 4; ++lf)
 #pragma unroll
-    for (int A = 0; A < nen; ++A) u += sU[wv][el][A] * sN[q][A];
-    double *g = sG[wv][el][q];
-    g[0] = det * w[q];
-    g[1] = J[3] / det;
-    g[2] = -J[1] / det;
-    g[3] = -J[2] / det;
-    g[4] = J[0] / det;
-    g[5] = u;
-  }
-  __syncthreads();
-  double loc = 0, rf = 0;
-  if (act) {
-    // face terms of dof a, faces in the reference's order
-    int fe[4];
+    for (int j = 0; j < n1; ++j)
+      if (sFn[lf][j] == a && nlf < 2) lfk[nlf++] = lf * n1 + j;
+  const int64_t ngroups = (nel + EPB - 1) / EPB;
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int64_t e = (grp * WPB + wv) * EPW + el;
+    const bool act = el < EPW && e < nel;
+    // parallelograms: det J and J^-T from the record; the others: row a of
+    // their own M^-1, loaded early so its latency hides behind the face and
+    // volume work, and the node coordinates
+    double gi[6] = {0, 0, 0, 0, 0, 0};
+    if (act)
 #pragma unroll
-    for (int k = 0; k < 4; ++k) fe[k] = ef[4 * e + k];
-    for (int k = 1; k < 4; ++k)
-      for (int j = k; j > 0 && (fe[j - 1] >> 1) > (fe[j] >> 1); --j) {
-        const int t = fe[j];
-        fe[j] = fe[j - 1];
-        fe[j - 1] = t;
+      for (int k = 0; k < 6; ++k) gi[k] = ginfo[8 * e + k];
+    const bool aff = gi[0] != 0.0;
+    double mrow[nen];
+    int efl[4] = {-1, -1, -1, -1};
+    if (act) {
+      if (!aff) {
+        const double *Mt = &minvT[e * nen * nen];
+#pragma unroll
+        for (int j = 0; j < nen; ++j) mrow[j] = Mt[j * nen + a];
+        sX[wv][el][a][0] = X[3 * (e * nen + a)];
+        sX[wv][el][a][1] = X[3 * (e * nen + a) + 1];
       }
-    for (int k = 0; k < 4; ++k) {
-      if (fe[k] < 0) continue;  // a face merged away by a non-conforming mesh
-      const int64_t f = fe[k] >> 1;
-      const int side = fe[k] & 1;
-      const int32_t *nodes = side == 0 ? &L[f * n1] : &R[f * n1];
-      int i = -1;
 #pragma unroll
-      for (int ii = 0; ii < n1; ++ii)
-        if (nodes[ii] == a) i = ii;
-      if (i < 0) continue;
-      const DGFaceRec r = F[f];
-      const bool plus = r.left >= 0 && r.right >= 0 && side == 1;
+      for (int k = 0; k < 4; ++k) efl[k] = ef[4 * e + k];
+      sU[wv][el][a] = U[e * nen + a];
+    }
+    __syncthreads();
+    if (act) {
+      const int q = a;
+      double u = 0;
 #pragma unroll
-      for (int fq = 0; fq < n1; ++fq) {
-        const int64_t gi = f * n1 + fq;
-        const double v = bN[fq * n1 + i] * flux[gi] * geo[4 * gi + 2 + side];
-        loc = plus ? loc + v : loc - v;
+      for (int A = 0; A < nen; ++A) u += sU[wv][el][A] * sN[q][A];
+      if (aff) {
+        sWU[wv][el][q] = w[q] * u;
+      } else {
+        double J[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int A = 0; A < nen; ++A) {
+          const double x0 = sX[wv][el][A][0], x1 = sX[wv][el][A][1];
+          const double g0 = sdN[q][A][0], g1 = sdN[q][A][1];
+          J[0] += x0 * g0;
+          J[1] += x0 * g1;
+          J[2] += x1 * g0;
+          J[3] += x1 * g1;
+        }
+        const double det = J[0] * J[3] - J[1] * J[2];
+        double *g = sG[wv][el][q];
+        g[0] = det * w[q];
+        g[1] = J[3] / det;
+        g[2] = -J[1] / det;
+        g[3] = -J[2] / det;
+        g[4] = J[0] / det;
+        g[5] = u;
       }
     }
-    rf = loc;
-    // volume term
+    __syncthreads();
+    double loc = 0, rf = 0;
+    if (act) {
+      // face terms of dof a from its cell's face slots, faces in the
+      // reference's (global face) order; a face merged away by a
+      // non-conforming mesh has no record (ef < 0)
+      int fk[2], fid[2], nfa = 0;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (k >= nlf) continue;
+        const int g = efl[lfk[k] / n1];
+        if (g >= 0) {
+          fk[nfa] = lfk[k];
+          fid[nfa] = g >> 1;
+          ++nfa;
+        }
+      }
+      if (nfa == 2 && fid[1] < fid[0]) {
+        const int t = fk[0];
+        fk[0] = fk[1];
+        fk[1] = t;
+      }
+      for (int k = 0; k < nfa; ++k) loc += slot[e * 4 * n1 + fk[k]];
+      rf = loc;
+      // volume term
+      if (aff) {
+        const double al = gi[2] * c0 + gi[3] * c1, be = gi[4] * c0 + gi[5] * c1;
+        double s0 = 0, s1 = 0;
+#pragma unroll
+        for (int q = 0; q < nqp; ++q) {
+          const double wu = sWU[wv][el][q];
+          s0 += wu * sdN[q][a][0];
+          s1 += wu * sdN[q][a][1];
+        }
+        loc += gi[1] * (al * s0 + be * s1 + src * sNw[a]);
+#pragma unroll
+        for (int j = 0; j < nen; ++j) mrow[j] = sMr[j][a];
+      } else {
 #pragma unroll 4
-    for (int q = 0; q < nqp; ++q) {
-      const double *g = sG[wv][el][q];
-      const double g0 = sdN[q][a][0], g1 = sdN[q][a][1];
-      const double gx = g[1] * g0 + g[3] * g1, gy = g[2] * g0 + g[4] * g1;
-      loc += (g[5] * (gx * c0 + gy * c1) + src * sN[q][a]) * g[0];
+        for (int q = 0; q < nqp; ++q) {
+          const double *g = sG[wv][el][q];
+          const double g0 = sdN[q][a][0], g1 = sdN[q][a][1];
+          const double gx = g[1] * g0 + g[3] * g1, gy = g[2] * g0 + g[4] * g1;
+          loc += (g[5] * (gx * c0 + gy * c1) + src * sN[q][a]) * g[0];
+        }
+      }
+      sL[wv][el][a] = loc;
     }
-    sL[wv][el][a] = loc;
-  }
-  __syncthreads();
-  if (act) {
-    // R = R_face + M^-1 (R_face + R_vol), row a
-    double s = 0;
+    __syncthreads();
+    if (act) {
+      // R = R_face + M^-1 (R_face + R_vol), row a
+      double s = 0;
 #pragma unroll
-    for (int j = 0; j < nen; ++j) s += mrow[j] * sL[wv][el][j];
-    Rout[e * nen + a] = rf + s;
-    if (Rface) Rface[e * nen + a] = rf;
+      for (int j = 0; j < nen; ++j) s += mrow[j] * sL[wv][el][j];
+      if (aff) s /= gi[1];
+      Rout[e * nen + a] = rf + s;
+      if (Rface) Rface[e * nen + a] = rf;
+    }
   }
 }
 
@@ -498,16 +625,18 @@ DGPhys dg_phys(const yafel_hip_physics *phys) {
 }
 
 void dg_eval(yafel_hip_dg_s *D, const DGPhys &ph, const double *U, double *R, double *Rface, hipStream_t s) {
-  const int64_t nfq = D->nfaces * D->n1;
-  k_dg_flux<<<ceil_div(nfq, NT), NT, 0, s>>>(D->nfaces, D->n1, D->nen, D->faces.as<DGFaceRec>(),
-                                             D->lnodes.as<int32_t>(), D->rnodes.as<int32_t>(), D->bN.as<double>(),
-                                             D->geo.as<double>(), ph.c0, ph.c1, U, D->flux.as<double>());
-  HIPCHK(hipGetLastError());
 #define DG_ELEM(P)                                                                                               \
-  k_dg_elem<P><<<ceil_div(D->nel, (NT / 64) * (64 / ((P + 1) * (P + 1)))), NT, 0, s>>>(D->nel, D->faces.as<DGFaceRec>(), D->lnodes.as<int32_t>(), D->rnodes.as<int32_t>(), \
-                                D->ef.as<int32_t>(), D->geo.as<double>(), D->flux.as<double>(), D->bN.as<double>(), \
+  k_dg_flux<P><<<ceil_div(D->nfaces, (NT / 64) * (64 / (P + 1))), NT, 0, s>>>(D->nfaces, D->faces.as<DGFaceRec>(), D->lnodes.as<int32_t>(), \
+                                                      D->rnodes.as<int32_t>(), D->fnodes.as<int32_t>(),           \
+                                                      D->bN.as<double>(), D->geo.as<double>(), ph.c0, ph.c1, U,   \
+                                                      D->flux.as<double>());                                      \
+  k_dg_elem<P><<<(unsigned)std::min<int64_t>(ceil_div(D->nel, (NT / 64) * (64 / ((P + 1) * (P + 1)))), \
+                                               (int64_t)D->ctx->num_cus * 4),                                   \
+                 NT, 0, s>>>(D->nel, D->fnodes.as<int32_t>(),                                                  \
+                                D->ef.as<int32_t>(), D->flux.as<double>(), \
                                 D->X.as<double>(), D->tabN.as<double>(), D->tabdN.as<double>(), D->tabw.as<double>(), \
-                                D->minv.as<double>(), ph.c0, ph.c1, ph.src, U, R, Rface)
+                                D->minv.as<double>(), D->ginfo.as<double>(), D->mrefT.as<double>(), ph.c0, ph.c1, \
+                                ph.src, U, R, Rface)
   switch (D->p) {
     case 1: DG_ELEM(1); break;
     case 2: DG_ELEM(2); break;
@@ -660,6 +789,41 @@ int yafel_hip_dg_create(yafel_hip_ctx ctx, int p, int64_t n_corner, const double
   }
 #undef DG_MINV
   HIPCHK(hipGetLastError());
+  // reference mass matrix sum_q w_q N_i N_j and its inverse (Gauss-Jordan with
+  // partial pivoting), for the affine cells
+  {
+    std::vector<double> A((size_t)nen * 2 * nen, 0.0), mt((size_t)nen * nen);
+    for (int i = 0; i < nen; ++i) {
+      for (int j = 0; j < nen; ++j) {
+        double m = 0;
+        for (int q = 0; q < D->nqp; ++q) m += w[q] * N[q * nen + i] * N[q * nen + j];
+        A[(size_t)i * 2 * nen + j] = m;
+      }
+      A[(size_t)i * 2 * nen + nen + i] = 1.0;
+    }
+    for (int k = 0; k < nen; ++k) {
+      int pr = k;
+      for (int i = k + 1; i < nen; ++i)
+        if (std::fabs(A[(size_t)i * 2 * nen + k]) > std::fabs(A[(size_t)pr * 2 * nen + k])) pr = i;
+      REQUIRE(A[(size_t)pr * 2 * nen + k] != 0.0, "dg_create: singular reference mass matrix");
+      if (pr != k)
+        for (int j = 0; j < 2 * nen; ++j) std::swap(A[(size_t)k * 2 * nen + j], A[(size_t)pr * 2 * nen + j]);
+      const double d = A[(size_t)k * 2 * nen + k];
+      for (int j = 0; j < 2 * nen; ++j) A[(size_t)k * 2 * nen + j] /= d;
+      for (int i = 0; i < nen; ++i) {
+        if (i == k) continue;
+        const double l = A[(size_t)i * 2 * nen + k];
+        for (int j = 0; j < 2 * nen; ++j) A[(size_t)i * 2 * nen + j] -= l * A[(size_t)k * 2 * nen + j];
+      }
+    }
+    for (int i = 0; i < nen; ++i)
+      for (int j = 0; j < nen; ++j) mt[(size_t)j * nen + i] = A[(size_t)i * 2 * nen + nen + j];
+    upload(D->mrefT, mt, s);
+  }
+  D->ginfo.alloc(ncell * 64);
+  k_dg_affine<<<ceil_div(ncell, NT), NT, 0, s>>>(ncell, n1, nen, D->X.as<double>(), D->tabdN.as<double>(),
+                                                 D->tabw.as<double>(), D->ginfo.as<double>());
+  HIPCHK(hipGetLastError());
   int hb[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(hb, bad.p, 8, hipMemcpyDeviceToHost, s));
   // boundary face count, for the info call
@@ -669,7 +833,13 @@ int yafel_hip_dg_create(yafel_hip_ctx ctx, int p, int64_t n_corner, const double
   REQUIRE(hb[0] == 0, "dg_create: faces whose nodes do not match across the two cells (mesh not conforming)");
   REQUIRE(hb[1] == 0, "dg_create: singular element mass matrix");
   for (const auto &f : hf) D->nbnd += (f.left < 0 || f.right < 0) ? 1 : 0;
-  D->flux.alloc((int64_t)nf * n1 * 8);
+  {
+    std::vector<double> gh((size_t)ncell * 8);
+    HIPCHK(hipMemcpyAsync(gh.data(), D->ginfo.p, gh.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int64_t e = 0; e < ncell; ++e) D->naffine += gh[(size_t)e * 8] != 0.0 ? 1 : 0;
+  }
+  D->flux.alloc(ncell * 4 * n1 * 8);
   D->rk.alloc(6 * nd * 8);
   D->tdev.alloc(8);
   HIPCHK(hipMemsetAsync(D->tdev.p, 0, 8, s));
@@ -693,6 +863,14 @@ int yafel_hip_dg_info(yafel_hip_dg D, int64_t *info) {
   info[1] = D->nfaces;
   info[2] = D->nbnd;
   info[3] = D->nen;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(nullptr)
+}
+
+int yafel_hip_dg_affine_cells(yafel_hip_dg D, int64_t *count) {
+  YH_TRY_BEGIN
+  REQUIRE(D && count, "dg_affine_cells: null argument");
+  *count = D->naffine;
   return YAFEL_HIP_OK;
   YH_TRY_END(nullptr)
 }

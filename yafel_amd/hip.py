@@ -115,7 +115,7 @@ for _n, _a in [
     ("yafel_hip_structured_mesh", [_I, _I, _I, _I, _I, _P, _P]),
     ("yafel_hip_normal_fill", [C.c_uint64, _I64, _D, _D, _P]),
     ("yafel_hip_dg_create", [_P, _I, _I64, _P, _I64, _P, C.POINTER(_P)]),
-    ("yafel_hip_dg_destroy", [_P]), ("yafel_hip_dg_info", [_P, _P]), ("yafel_hip_dg_nodes", [_P, _P]),
+    ("yafel_hip_dg_destroy", [_P]), ("yafel_hip_dg_info", [_P, _P]), ("yafel_hip_dg_affine_cells", [_P, _P]), ("yafel_hip_dg_nodes", [_P, _P]),
     ("yafel_hip_dg_faces", [_P, _P, _P, _P, _P]),
     ("yafel_hip_dg_residual", [_P, C.POINTER(Physics), _D, _P, _P, _P]),
     ("yafel_hip_dg_rk4", [_P, C.POINTER(Physics), _D, _I64, C.POINTER(_D), _P]),
@@ -459,6 +459,9 @@ class DG:
         info = np.zeros(4, np.int64)
         check(lib.yafel_hip_dg_info(h, _ptr(info)), ctx.h)
         self.ndofs, self.nfaces, self.nboundary, self.nen = (int(v) for v in info)
+        na = _I64(0)
+        check(lib.yafel_hip_dg_affine_cells(h, C.byref(na)), ctx.h)
+        self.naffine = int(na.value)
 
     def close(self):
         if self.h:
