@@ -301,6 +301,30 @@ int yafel_hip_gmsh_arrays(yafel_hip_gmsh g, double *nodes, int32_t *cell_nodes, 
                           int32_t *cell_types, int32_t *tag_offsets, int32_t *tags);
 int yafel_hip_gmsh_destroy(yafel_hip_gmsh g);
 
+/* ---------------------------------------------------------------- output */
+/* OutputData (include/output/OutputData.hpp:19-57): a named array of
+ * n_locations * dof_per_loc values, the bits of `mask` standing for the
+ * reference's dof_mask entries (the components written). */
+typedef struct {
+  const char *name;
+  const double *data;  /* host pointer, or a device pointer when on_device */
+  int on_device;
+  int dof_per_loc;     /* the dof_mask length, 1..32 */
+  uint32_t mask;       /* bit c set: component c written */
+  int data_type;       /* 0 scalar, 1 vector, 2 tensor (OutputData::DataType) */
+  int location;        /* 0 point (n_points locations), 1 cell (n_cells) */
+} yafel_hip_output_data;
+/* SimulationOutput(fname_base, BackendType::VTU).captureFrame: VTUBackend
+ * (src/output/VTUBackend.cpp:19-230) writes fname_base.vtu with the OutputMesh
+ * of the cells (src/output/OutputMesh.cpp:11-48): each element of order p,
+ * given by its (p+1)^dim nodes in the DoFManager's local order, is drawn as
+ * p^dim quads / hexes.  The text is the reference's, number formatting
+ * included.  Scalars take one component; vectors and tensors up to three,
+ * padded with zeros to three (as the reference writes both); cell data has
+ * one value per cell.  Host side; device data is copied back. */
+int yafel_hip_vtu_write(const char *fname_base, int dim, int p, int64_t n_points, const double *points,
+                        int64_t n_cells, const int32_t *cells, int n_data, const yafel_hip_output_data *data);
+
 /* ------------------------------------------------------ gradient recovery */
 /* LocalSmoothingGradient<Physics>(FESystem&) (include/assembly/
  * LocalSmoothingGradient.hpp:20-94): grad_dev (n_dofs x dim, row-major, the

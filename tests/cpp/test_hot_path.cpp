@@ -6,9 +6,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <iostream>
 
 #include "yafel_amd/fe.hpp"
+#include "yafel_amd/output.hpp"
 
 using namespace yafel;
 using trip = std::tuple<std::size_t, std::size_t, double>;
@@ -143,6 +145,25 @@ static bool poisson_flow() {
   LocalSmoothingGradient<Poisson>(fes);
   mid_edge_and_centre(fes.getSolutionGradient(), gx, gc);
   if (!(gx < -0.30 && gx > -0.36 && gc < 1e-6)) return false;
+  // apps/BasicPoisson/main.cpp:149-150: the solution to a VTU frame
+  {
+    SimulationOutput so("/tmp/yafel_hot_path_poisson", BackendType::VTU);
+    so.captureFrame(fes);
+    std::FILE *f = std::fopen("/tmp/yafel_hot_path_poisson.vtu", "r");
+    if (!f) return false;
+    char line[256];
+    int lines = 0;
+    bool piece = false, named = false;
+    while (std::fgets(line, sizeof line, f)) {
+      ++lines;
+      if (std::strstr(line, "<Piece NumberOfPoints=\"1089\" NumberOfCells=\"1024\">")) piece = true;
+      if (std::strstr(line, "Name=\"Solution_0\" NumberOfComponents=\"1\"")) named = true;
+    }
+    std::fclose(f);
+    std::remove("/tmp/yafel_hot_path_poisson.vtu");
+    // 1089 points, 1024 cells (connectivity, offsets, types), 1089 values
+    if (!piece || !named || lines != 1089 + 3 * 1024 + 1089 + 23) return false;
+  }
   // the host copies agree with a device product
   auto K = fes.getGlobalTangent();
   Vector<> r = K * fes.getSolution() - fes.getGlobalResidual();

@@ -210,3 +210,163 @@ int yafel_hip_gmsh_destroy(yafel_hip_gmsh g) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- output
+// VTUBackend (src/output/VTUBackend.cpp:19-230) writing one frame of an
+// OutputMesh (src/output/OutputMesh.cpp:11-48): every element of order p is
+// drawn as its p^dim local sub-cells (make_tensorproduct_element.cpp:51-95,
+// corners (i,j), (i,j+1), (i+1,j+1), (i+1,j) of the x-fastest local node
+// grid, and the hexahedral analogue), VTK type 9 (quad) or 12 (hex).  The
+// text is the reference's: ostream default formatting of doubles (%g, six
+// significant digits), one cell per connectivity line, point data before
+// cell data, a cell value repeated on each of its sub-cells.
+namespace {
+
+struct VtuOut {
+  std::FILE *f = nullptr;
+  ~VtuOut() {
+    if (f) std::fclose(f);
+  }
+};
+
+void put_num(std::FILE *f, double v) { std::fprintf(f, "%g", v); }
+
+void write_array(std::FILE *f, const yafel_hip_output_data &d, const std::vector<double> &V, int64_t n_locs) {
+  std::vector<int> comps;
+  for (int c = 0; c < d.dof_per_loc; ++c)
+    if ((d.mask >> c) & 1u) comps.push_back(c);
+  const int ncomps = d.data_type == 0 ? 1 : d.data_type == 1 ? 3 : 9;
+  std::fprintf(f, "<DataArray type=\"Float32\" format=\"ascii\" Name=\"%s\" NumberOfComponents=\"%d\">\n", d.name,
+               ncomps);
+  static const char v_sep[3] = {' ', ' ', '\n'};
+  for (int64_t n = 0; n < n_locs; ++n) {
+    if (d.data_type == 0) {
+      put_num(f, V[(size_t)(n * d.dof_per_loc + comps[0])]);
+      std::fputc('\n', f);
+    } else {
+      // VTUBackend.cpp:164-184: vectors (and tensors) are written with the
+      // vector separators, padded with zeros to three components
+      int vc = 0;
+      for (int c : comps) {
+        put_num(f, V[(size_t)(n * d.dof_per_loc + c)]);
+        std::fputc(v_sep[vc++], f);
+      }
+      for (int i = vc; i < 3; ++i) {
+        std::fputc('0', f);
+        std::fputc(v_sep[i], f);
+      }
+    }
+  }
+  std::fprintf(f, "</DataArray>\n");
+}
+
+}  // namespace
+
+extern "C" int yafel_hip_vtu_write(const char *fname_base, int dim, int p, int64_t n_points, const double *points,
+                                   int64_t n_cells, const int32_t *cells, int n_data,
+                                   const yafel_hip_output_data *data) {
+  YH_TRY_BEGIN
+  REQUIRE(fname_base && points && cells, "vtu_write: null argument");
+  REQUIRE((dim == 2 || dim == 3) && p >= 1 && p <= 8, "vtu_write: dim must be 2 or 3 and p 1..8");
+  REQUIRE(n_points >= 0 && n_cells >= 0 && n_data >= 0 && (n_data == 0 || data), "vtu_write: bad sizes");
+  const int n1 = p + 1, nen = dim == 2 ? n1 * n1 : n1 * n1 * n1;
+  const int nsub = dim == 2 ? p * p : p * p * p, nv = dim == 2 ? 4 : 8;
+  for (int64_t i = 0; i < n_cells * nen; ++i)
+    REQUIRE(cells[i] >= 0 && cells[i] < n_points, "vtu_write: cell node out of range");
+  // data checks (VTUBackend.cpp:104-131) and the values on the host
+  std::vector<std::vector<double>> vals((size_t)n_data);
+  for (int k = 0; k < n_data; ++k) {
+    const yafel_hip_output_data &d = data[k];
+    REQUIRE(d.name && d.data, "vtu_write: data without a name or values");
+    REQUIRE(d.dof_per_loc >= 1 && d.dof_per_loc <= 32, "vtu_write: dof_per_loc must be 1..32");
+    REQUIRE(d.data_type >= 0 && d.data_type <= 2 && (d.location == 0 || d.location == 1),
+            "vtu_write: bad data type or location");
+    int nc = 0;
+    for (int c = 0; c < d.dof_per_loc; ++c) nc += (d.mask >> c) & 1u;
+    REQUIRE(nc >= 1, std::string("vtu_write: empty dof mask: data = ") + d.name);
+    REQUIRE(!(nc > 1 && d.data_type == 0),
+            std::string("VTUBackend::write_data: Incompatible OutputData::dof_mask and OutputData::DataType: data = ") +
+                d.name);
+    REQUIRE(nc <= 3, std::string("vtu_write: at most three components per array (the reference pads vectors and "
+                                 "tensors to three): data = ") + d.name);
+    REQUIRE(d.location == 0 || d.dof_per_loc == 1, "vtu_write: cell data with one value per cell only");
+    const int64_t nloc = d.location == 0 ? n_points : n_cells;
+    const size_t len = (size_t)(nloc * d.dof_per_loc);
+    vals[k].resize(len);
+    if (len) {
+      if (d.on_device) HIPCHK(hipMemcpy(vals[k].data(), d.data, len * 8, hipMemcpyDeviceToHost));
+      else std::memcpy(vals[k].data(), d.data, len * 8);
+    }
+  }
+  const std::string fname = std::string(fname_base) + ".vtu";
+  VtuOut out;
+  out.f = std::fopen(fname.c_str(), "w");
+  REQUIRE(out.f, "vtu_write: cannot open " + fname);
+  std::FILE *f = out.f;
+  const int64_t n_out = n_cells * nsub;
+  std::fprintf(f, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\">\n");
+  std::fprintf(f, "<UnstructuredGrid>\n<Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n", (long long)n_points,
+               (long long)n_out);
+  std::fprintf(f, "<Points>\n<DataArray type=\"Float32\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+  for (int64_t i = 0; i < n_points; ++i) {
+    put_num(f, points[3 * i]);
+    std::fputc(' ', f);
+    put_num(f, points[3 * i + 1]);
+    std::fputc(' ', f);
+    put_num(f, points[3 * i + 2]);
+    std::fputc('\n', f);
+  }
+  std::fprintf(f, "</DataArray>\n</Points>\n<Cells>\n<DataArray type=\"UInt32\" Name=\"connectivity\" format=\"ascii\">\n");
+  int sub[8];
+  for (int64_t e = 0; e < n_cells; ++e) {
+    const int32_t *g = &cells[e * nen];
+    for (int s = 0; s < nsub; ++s) {
+      if (dim == 2) {
+        const int i = s / p, j = s % p;
+        sub[0] = i * n1 + j;
+        sub[1] = i * n1 + j + 1;
+        sub[2] = (i + 1) * n1 + j + 1;
+        sub[3] = (i + 1) * n1 + j;
+      } else {
+        const int i = s / (p * p), j = (s / p) % p, k = s % p;
+        const int c = i * n1 * n1 + j * n1 + k;
+        sub[0] = c;
+        sub[1] = c + 1;
+        sub[2] = c + 1 + n1;
+        sub[3] = c + n1;
+        for (int t = 0; t < 4; ++t) sub[4 + t] = sub[t] + n1 * n1;
+      }
+      for (int t = 0; t < nv; ++t) std::fprintf(f, t + 1 < nv ? "%d " : "%d\n", g[sub[t]]);
+    }
+  }
+  std::fprintf(f, "</DataArray>\n<DataArray type=\"UInt32\" Name=\"offsets\" format=\"ascii\">\n");
+  for (int64_t c = 1; c <= n_out; ++c) std::fprintf(f, "%lld\n", (long long)(c * nv));
+  std::fprintf(f, "</DataArray>\n<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
+  for (int64_t c = 0; c < n_out; ++c) std::fprintf(f, "%d\n", dim == 2 ? 9 : 12);
+  std::fprintf(f, "</DataArray>\n</Cells>\n");
+  bool any = false;
+  for (int k = 0; k < n_data; ++k) any = any || data[k].location == 0;
+  if (any) {
+    std::fprintf(f, "<PointData>\n");
+    for (int k = 0; k < n_data; ++k)
+      if (data[k].location == 0) write_array(f, data[k], vals[k], n_points);
+    std::fprintf(f, "</PointData>\n");
+  }
+  any = false;
+  for (int k = 0; k < n_data; ++k) any = any || data[k].location == 1;
+  if (any) {
+    std::fprintf(f, "<CellData>\n");
+    for (int k = 0; k < n_data; ++k) {
+      if (data[k].location != 1) continue;
+      std::vector<double> rep((size_t)n_out);
+      for (int64_t c = 0; c < n_cells; ++c)
+        for (int s = 0; s < nsub; ++s) rep[(size_t)(c * nsub + s)] = vals[k][(size_t)c];
+      write_array(f, data[k], rep, n_out);
+    }
+    std::fprintf(f, "</CellData>\n");
+  }
+  std::fprintf(f, "</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
+  REQUIRE(std::ferror(f) == 0, "vtu_write: write error on " + fname);
+  return YAFEL_HIP_OK;
+  YH_TRY_END(nullptr)
+}

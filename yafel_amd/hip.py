@@ -112,6 +112,7 @@ for _n, _a in [
     ("yafel_hip_gradient_local_smoothing", [_P, _P, _P]), ("yafel_hip_gradient_zz", [_P, _P, _P]),
     ("yafel_hip_gmsh_read", [C.c_char_p, _P]), ("yafel_hip_gmsh_sizes", [_P, _P]),
     ("yafel_hip_gmsh_arrays", [_P] * 7), ("yafel_hip_gmsh_destroy", [_P]),
+    ("yafel_hip_vtu_write", [C.c_char_p, _I, _I, _I64, _P, _I64, _P, _I, _P]),
     ("yafel_hip_structured_mesh", [_I, _I, _I, _I, _I, _P, _P]),
     ("yafel_hip_normal_fill", [C.c_uint64, _I64, _D, _D, _P]),
     ("yafel_hip_dg_create", [_P, _I, _I64, _P, _I64, _P, C.POINTER(_P)]),
@@ -375,6 +376,34 @@ def structured_mesh(dim, nx, ny, nz=0, p=1):
 
 
 CELL_LINE2, CELL_TRI3, CELL_TET4, CELL_QUAD4, CELL_HEX8, CELL_POINT1, CELL_NONE = range(7)
+SCALAR, VECTOR, TENSOR = 0, 1, 2
+POINT_DATA, CELL_DATA = 0, 1
+
+
+class OutputData(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("data", C.c_void_p), ("on_device", C.c_int), ("dof_per_loc", C.c_int),
+                ("mask", C.c_uint32), ("data_type", C.c_int), ("location", C.c_int)]
+
+
+def write_vtu(fname_base, dim, p, points, cells, data=()):
+    """SimulationOutput(fname_base, VTU) + captureFrame: writes fname_base.vtu.
+    points (n, 3); cells (n_cells, (p+1)^dim) in the DoFManager's local node
+    order; data: tuples (name, values, dof_mask, data_type, location) with values
+    a numpy array or a DeviceArray, dof_mask a list of 0/1 (OutputData)."""
+    points = np.ascontiguousarray(points, np.float64)
+    cells = np.ascontiguousarray(cells, np.int32)
+    recs, keep = (OutputData * max(len(data), 1))(), []
+    for k, (name, vals, mask, dtype, loc) in enumerate(data):
+        dev = not isinstance(vals, np.ndarray)
+        if not dev:
+            vals = np.ascontiguousarray(vals, np.float64)
+            keep.append(vals)
+        bits = sum(1 << c for c, m in enumerate(mask) if m)
+        nb = name.encode()
+        keep.append(nb)
+        recs[k] = OutputData(nb, vals.ptr if dev else vals.ctypes.data, 1 if dev else 0, len(mask), bits, dtype, loc)
+    check(lib.yafel_hip_vtu_write(os.fsencode(str(fname_base)), dim, p, points.shape[0], _ptr(points), cells.shape[0],
+                                  _ptr(cells), len(data), recs))
 
 
 class GmshMesh:
