@@ -31,8 +31,10 @@ PEAK_HBM_GBS = 8000.0
 TOL = 1e-8
 
 
-def bcsr_bytes(nb, nbr, n, s):
-    return nb * (9 * s + 4) + (nbr + 1) * 4 + 2 * n * s
+def bcsr_bytes(nb, nbr, n, s, ib=4):
+    """BCSR3 SpMV bytes: blocks of 9 values and one column index of ib bytes
+    (2 with 16-bit column offsets), the row structure, x read and y written."""
+    return nb * (9 * s + ib) + (nbr + 1) * 4 + 2 * n * s
 
 
 def log(msg):
@@ -256,27 +258,40 @@ def c3_bench(args):
 
 
 def spmv_bench(args):
-    """The c2 BCSR3 SpMV alone (FP32 and FP64), plus one copy-kernel pass of
-    known size: the command the rocprof counter runs profile, the copy kernel
-    being the calibration of their byte counters."""
+    """The SpMV alone (FP32 and FP64) on each config's matrix -- c2 BCSR3
+    (the --n mesh), c3 CSR Q4, c5 CSR Q2, c1 CSR Q1 -- plus one copy-kernel
+    pass of known size: the command the rocprof counter runs profile, the
+    copy kernel being the calibration of their byte counters."""
     from yafel_amd import hip, workloads
     ctx = hip.Context(0)
-    w = workloads.c2(ctx, n=args.n)
-    w.assemble()
-    A = w.A
-    A.sync_fp32()
-    n, inf = w.n, A.info()
-    out = {"workload": f"c2 BCSR3 SpMV, {args.n}^3 Q1 elasticity", "blocks": inf.nblocks}
-    for name, prec, dt, s in (("fp32", hip.FP32, np.float32, 4), ("fp64", hip.FP64, np.float64, 8)):
-        x = ctx.upload(np.random.default_rng(0).standard_normal(n).astype(dt))
-        y = ctx.alloc(n, dt)
-        A.spmv(x, y, prec)
-        ctx.timer_begin()
-        for _ in range(args.steps):
+    out = {"workload": "SpMV alone on the config matrices"}
+    cases = (("c2", lambda: workloads.c2(ctx, n=args.n)), ("c3", lambda: workloads.c3(ctx)),
+             ("c5", lambda: workloads.c5(ctx)), ("c1", lambda: workloads.c1(ctx)))
+    for cname, make in cases:
+        w = make()
+        w.assemble()
+        A = w.A
+        A.sync_fp32()
+        n, inf = w.n, A.info()
+        ib = A.index_bytes()
+        rec = {"format": "BCSR3" if inf.nblocks != inf.nnz else "CSR", "nnz": inf.nnz, "blocks": inf.nblocks,
+               "index_bytes": ib}
+        for name, prec, dt, s in (("fp32", hip.FP32, np.float32, 4), ("fp64", hip.FP64, np.float64, 8)):
+            x = ctx.upload(np.random.default_rng(0).standard_normal(n).astype(dt))
+            y = ctx.alloc(n, dt)
             A.spmv(x, y, prec)
-        ms = ctx.timer_end() / args.steps
-        byts = inf.nblocks * (9 * s + 4) + (n // 3 + 1) * 4 + 2 * n * s
-        out[name] = {"us": ms * 1e3, "gbs": byts / (ms * 1e-3) / 1e9, "algorithmic_bytes": byts}
+            ctx.timer_begin()
+            for _ in range(args.steps):
+                A.spmv(x, y, prec)
+            ms = ctx.timer_end() / args.steps
+            if rec["format"] == "BCSR3":
+                byts = bcsr_bytes(inf.nblocks, n // 3, n, s, ib)
+            else:
+                byts = inf.nnz * (s + ib) + (n + 1) * 4 + 2 * n * s
+            rec[name] = {"us": ms * 1e3, "gbs": byts / (ms * 1e-3) / 1e9, "algorithmic_bytes": byts}
+        log(f"spmv {cname}: {rec}")
+        out[cname] = rec
+        del w, A
     out["copy_gbs"] = ctx.copy_bandwidth(1 << 30, 1)  # two launches of a 1 GiB copy
     print(json.dumps(out), flush=True)
 
@@ -519,7 +534,7 @@ def main():
     ka_ms, ka_n = ctx.kernel_time()
     ctx.kernel_timing(False)
     ka_us = ka_ms / max(ka_n, 1) * 1e3
-    ka_bytes = bcsr_bytes(inf.nblocks, nbr, w.n, 4)  # matrix, p gathered, q written
+    ka_bytes = bcsr_bytes(inf.nblocks, nbr, w.n, 4, A.index_bytes())  # matrix, p gathered, q written
     ka_gbs = ka_bytes / (ka_us * 1e-6) / 1e9
     log(f"kernel A: {ka_n} launches, {ka_us:.1f} us each, {ka_gbs:.0f} GB/s")
     x32 = ctx.upload(np.random.default_rng(0).standard_normal(w.n).astype(np.float32))
@@ -532,14 +547,14 @@ def main():
     for _ in range(reps):
         A.spmv(x32, y32, hip.FP32)
     ms32 = ctx.timer_end() / reps
-    bytes32 = bcsr_bytes(inf.nblocks, nbr, w.n, 4)
+    bytes32 = bcsr_bytes(inf.nblocks, nbr, w.n, 4, A.index_bytes())
     x64 = ctx.upload(np.random.default_rng(0).standard_normal(w.n))
     y64 = ctx.alloc(w.n)
     ctx.timer_begin()
     for _ in range(reps):
         A.spmv(x64, y64, hip.FP64)
     ms64 = ctx.timer_end() / reps
-    bytes64 = bcsr_bytes(inf.nblocks, nbr, w.n, 8)
+    bytes64 = bcsr_bytes(inf.nblocks, nbr, w.n, 8, A.index_bytes())
     gbs32 = bytes32 / (ms32 * 1e-3) / 1e9
     gbs64 = bytes64 / (ms64 * 1e-3) / 1e9
     copy_gbs = ctx.copy_bandwidth(1 << 30, 20)

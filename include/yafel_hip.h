@@ -121,6 +121,10 @@ int yafel_hip_matrix_set_unroll(yafel_hip_matrix A, int unroll);
  * on CSR up to 1.5 Mi, else 3.  A tuning knob: the iterates are
  * bit-identical. */
 int yafel_hip_matrix_set_cg_passes(yafel_hip_matrix A, int passes);
+/* No reference counterpart: bytes per stored column index of the SpMV
+ * streams, 2 when every entry's column is within +-32767 of its stored row
+ * position (16-bit offsets, a banded mesh numbering) and 4 otherwise. */
+int yafel_hip_matrix_index_bytes(yafel_hip_matrix A, int *bytes);
 /* device pointer to the FP64 diagonal (computed on first use) */
 int yafel_hip_matrix_diagonal(yafel_hip_matrix A, double *diag_dev);
 

@@ -444,3 +444,44 @@ def test_mesh_from_gmsh_assembles_like_the_generated_mesh(H, ctx, tmp_path, dim,
     assert np.array_equal(el0, el1) and np.array_equal(dn0, dn1)
     assert np.array_equal(pt0, pt1) and np.array_equal(c0, c1) and np.array_equal(v0, v1)
     assert np.array_equal(r0, r1)
+
+
+# ------------------------------------------------------------ column offsets
+@pytest.mark.parametrize("wide", [False, True])
+@pytest.mark.parametrize("block", [1, 3])
+def test_column_offsets_and_int32_fallback(H, ctx, orc, wide, block):
+    """The SpMV streams store 16-bit column offsets when every entry is within
+    32767 block columns of its stored row, and int32 columns otherwise; SpMV in
+    both precisions and a CG solve agree with the oracle either way.  The
+    matrix is SPD: a 1-D Laplacian (blocks: times a 3x3 SPD block), plus, when
+    wide, a coupling between block rows 0 and nb-1 (40000 apart)."""
+    nb = 40000
+    i = np.arange(nb)
+    r = [i, i[:-1], i[1:]]
+    c = [i, i[1:], i[:-1]]
+    v = [np.full(nb, 4.0), np.full(nb - 1, -1.0), np.full(nb - 1, -1.0)]
+    if wide:
+        r += [np.array([0, nb - 1])]
+        c += [np.array([nb - 1, 0])]
+        v += [np.array([-0.5, -0.5])]
+    R, Cc, V = (np.concatenate(a) for a in (r, c, v))
+    if block == 3:
+        Bk = np.array([[2.0, 0.5, 0.0], [0.5, 2.0, 0.25], [0.0, 0.25, 2.0]])
+        R = np.repeat(3 * R, 9) + np.tile(np.repeat(np.arange(3), 3), len(R))
+        Cc = np.repeat(3 * Cc, 9) + np.tile(np.tile(np.arange(3), 3), len(Cc))
+        V = (V[:, None] * Bk.ravel()[None, :]).ravel()
+    A = H.Matrix.from_triplets(ctx, R, Cc, V, block=block, precisions=H.FP64 | H.FP32)
+    assert A.index_bytes() == (4 if wide else 2)
+    O = orc.csr(R.astype(np.int64), Cc.astype(np.int64), V)
+    n = nb * block
+    x = np.random.default_rng(block + 2 * wide).standard_normal(n)
+    y = A.spmv_host(x)
+    yo = O.spmv(x)
+    assert np.abs(y - yo).max() <= 1e-13 * np.abs(yo).max()
+    y32 = ctx.alloc(n, np.float32)
+    A.spmv(ctx.upload(x.astype(np.float32)), y32, H.FP32)
+    assert np.abs(y32.download() - yo).max() <= 1e-5 * np.abs(yo).max()
+    b = np.random.default_rng(9).standard_normal(n)
+    xs, info = A.solve_host(b, method=H.CG, stop=H.STOP_RELRES, tol=1e-10)
+    assert info.status == 0
+    assert np.linalg.norm(O.spmv(xs) - b) <= 1e-9 * np.linalg.norm(b)

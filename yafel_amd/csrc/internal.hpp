@@ -151,6 +151,7 @@ struct yafel_hip_matrix_s {
   yh::DBuf sptr;                // int32 slot offset of each slice (nsl+1)
   yh::DBuf perm, iperm;         // int32 position->row / row->position (empty: identity)
   yh::DBuf col;                 // int32 sliced block columns (ntot)
+  yh::DBuf col16;               // int16 column - stored position (ntot), when every entry fits; else empty
   yh::DBuf val64, val32;        // CSR: ntot; BCSR3: 9 arrays of ntot (v[e * ntot + k], e = 3i+j)
   yh::DBuf diag64, dinv64, dinv32;
   bool has64 = false, has32 = false, diag_ok = false;

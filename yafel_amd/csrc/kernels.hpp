@@ -16,6 +16,7 @@ struct SellView {
   const T *val;      // sliced values (BCSR3: 9 arrays of ntot)
   int64_t brows;     // stored rows (block rows for BCSR3)
   int64_t ntot;      // sliced entries per value array
+  const short *col16;  // column - stored position, 16-bit (nullptr: use col)
   __device__ __forceinline__ int64_t nsl() const { return (brows + 63) >> 6; }
   __device__ __forceinline__ int64_t row_of(int64_t i) const { return perm ? (int64_t)perm[i] : i; }
 };
@@ -34,9 +35,11 @@ __device__ __forceinline__ int64_t sell_pos(const int *__restrict__ sptr, const 
 // of the step issued before the first FMA.  Matrix values and columns are
 // streamed once, so they are loaded non-temporally; the gathered vector stays
 // in cache.  xf(c) returns x[c].  T is the stored value type, TV the
-// accumulation type.
-template <int U, class T, class TV, class XF>
-__device__ __forceinline__ TV sell_row_csr(const SellView<T> &A, int64_t s, int lane, XF xf) {
+// accumulation type.  Columns come from the 16-bit offsets to the stored
+// position when the matrix has them (2 bytes an entry instead of 4), else
+// from the int32 array; cf(k) is the column of entry k.
+template <int U, class T, class TV, class XF, class CF>
+__device__ __forceinline__ TV sell_row_csr_cols(const SellView<T> &A, int64_t s, int lane, XF xf, CF cf) {
   const int p0 = A.sptr[s], p1 = A.sptr[s + 1];
   const int64_t base = (int64_t)p0 * 64 + lane;
   const int len = p1 - p0;
@@ -48,7 +51,7 @@ __device__ __forceinline__ TV sell_row_csr(const SellView<T> &A, int64_t s, int 
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t k = base + (int64_t)(t + u) * 64;
-      c[u] = __builtin_nontemporal_load(&A.col[k]);
+      c[u] = cf(k);
       v[u] = __builtin_nontemporal_load(&A.val[k]);
     }
 #pragma unroll
@@ -56,16 +59,28 @@ __device__ __forceinline__ TV sell_row_csr(const SellView<T> &A, int64_t s, int 
   }
   for (; t < len; ++t) {
     const int64_t k = base + (int64_t)t * 64;
-    acc = fma((TV)__builtin_nontemporal_load(&A.val[k]), xf(__builtin_nontemporal_load(&A.col[k])), acc);
+    acc = fma((TV)__builtin_nontemporal_load(&A.val[k]), xf(cf(k)), acc);
   }
   return acc;
 }
 
+template <int U, class T, class TV, class XF>
+__device__ __forceinline__ TV sell_row_csr(const SellView<T> &A, int64_t s, int lane, XF xf) {
+  if (A.col16) {
+    const int i = (int)(s * 64 + lane);
+    return sell_row_csr_cols<U, T, TV>(A, s, lane, xf,
+                                       [&](int64_t k) { return i + (int)__builtin_nontemporal_load(&A.col16[k]); });
+  }
+  return sell_row_csr_cols<U, T, TV>(A, s, lane, xf,
+                                     [&](int64_t k) { return __builtin_nontemporal_load(&A.col[k]); });
+}
+
 // One 3x3-block row per lane; a[0..2] receive the three row sums.  Blocks are
 // summed in order, entries of a block row-wise (j = 0, 1, 2).  xf(i) returns
-// x[i] for a scalar index.
-template <int U, class T, class TV, class XF>
-__device__ __forceinline__ void sell_row_bcsr(const SellView<T> &A, int64_t s, int lane, TV a[3], XF xf) {
+// x[i] for a scalar index; cf(k) is the block column of entry k.
+template <int U, class T, class TV, class XF, class CF>
+__device__ __forceinline__ void sell_row_bcsr_cols(const SellView<T> &A, int64_t s, int lane, TV a[3], XF xf,
+                                                   CF cf) {
   const int p0 = A.sptr[s], p1 = A.sptr[s + 1];
   const int64_t base = (int64_t)p0 * 64 + lane;
   const int len = p1 - p0;
@@ -78,7 +93,7 @@ __device__ __forceinline__ void sell_row_bcsr(const SellView<T> &A, int64_t s, i
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t k = base + (int64_t)(t + u) * 64;
-      c[u] = 3 * (int64_t)__builtin_nontemporal_load(&A.col[k]);
+      c[u] = 3 * (int64_t)cf(k);
 #pragma unroll
       for (int e = 0; e < 9; ++e) v[u][e] = __builtin_nontemporal_load(&A.val[e * nt + k]);
     }
@@ -92,7 +107,7 @@ __device__ __forceinline__ void sell_row_bcsr(const SellView<T> &A, int64_t s, i
   }
   for (; t < len; ++t) {
     const int64_t k = base + (int64_t)t * 64;
-    const int64_t c = 3 * (int64_t)__builtin_nontemporal_load(&A.col[k]);
+    const int64_t c = 3 * (int64_t)cf(k);
     const TV x0 = xf(c), x1 = xf(c + 1), x2 = xf(c + 2);
     a0 = fma((TV)__builtin_nontemporal_load(&A.val[0 * nt + k]), x0, a0);
     a0 = fma((TV)__builtin_nontemporal_load(&A.val[1 * nt + k]), x1, a0);
@@ -109,6 +124,17 @@ __device__ __forceinline__ void sell_row_bcsr(const SellView<T> &A, int64_t s, i
   a[2] = a2;
 }
 
+template <int U, class T, class TV, class XF>
+__device__ __forceinline__ void sell_row_bcsr(const SellView<T> &A, int64_t s, int lane, TV a[3], XF xf) {
+  if (A.col16) {
+    const int i = (int)(s * 64 + lane);
+    sell_row_bcsr_cols<U, T, TV>(A, s, lane, a, xf,
+                                 [&](int64_t k) { return i + (int)__builtin_nontemporal_load(&A.col16[k]); });
+  } else {
+    sell_row_bcsr_cols<U, T, TV>(A, s, lane, a, xf,
+                                 [&](int64_t k) { return __builtin_nontemporal_load(&A.col[k]); });
+  }
+}
 
 // Value of `v` in lane l (l wave-uniform), for 32- and 64-bit types.
 __device__ __forceinline__ int lane_bcast(int v, int l) { return __builtin_amdgcn_readlane(v, l); }

@@ -42,7 +42,8 @@ namespace {
 template <class T>
 SellView<T> view(yafel_hip_matrix_s *A) {
   return SellView<T>{A->sptr.as<int>(), A->perm.p ? A->perm.as<int>() : nullptr, A->col.as<int>(),
-                     (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->brows, A->ntot};
+                     (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->brows, A->ntot,
+                     A->col16.p ? A->col16.as<short>() : nullptr};
 }
 
 // Fused SpMV grids stay within num_cus * 8 workgroups so that NP <= 4

@@ -44,6 +44,7 @@ struct SlotDesc {
   int off_spmv, off_elem;
   int64_t n, brows, ntot;
   const int *sptr, *perm, *col;
+  const short *col16;  // 16-bit column offsets (nullptr: col)
   const float *v32;
   const double *v64;
   const float *dinv;
@@ -99,7 +100,7 @@ __device__ __forceinline__ double *upart(const SlotDesc &d) {
 
 template <class T, class O>
 __device__ __forceinline__ void run_spmv(const SlotDesc &d, const T *val, O &op, int blk) {
-  const SellView<T> A{d.sptr, d.perm, d.col, val, d.brows, d.ntot};
+  const SellView<T> A{d.sptr, d.perm, d.col, val, d.brows, d.ntot, d.col16};
   if (d.fmt == YAFEL_HIP_BCSR3) spmv_fused_body<YAFEL_HIP_BCSR3, kSellUnroll, T, O>(A, op, blk, d.g_spmv);
   else spmv_fused_body<YAFEL_HIP_CSR, kSellUnroll, T, O>(A, op, blk, d.g_spmv);
 }
@@ -407,6 +408,7 @@ void join_slot(yafel_hip_server_s *S, Slot *sl) {
   d.sptr = A->sptr.as<int>();
   d.perm = A->perm.p ? A->perm.as<int>() : nullptr;
   d.col = A->col.as<int>();
+  d.col16 = A->col16.p ? A->col16.as<short>() : nullptr;
   d.v32 = A->val32.as<float>();
   d.v64 = A->val64.as<double>();
   d.dinv = A->dinv32.as<float>();

@@ -113,6 +113,30 @@ __global__ void k_sell_fill(int64_t brows, int64_t nsl, int64_t nblk, int64_t nt
   }
 }
 
+// 16-bit column offsets: entry k of stored position i holds col - i.  The
+// rows of a window of 256 are sorted among themselves only, so a banded
+// pattern (a mesh numbering) keeps every offset small.  Lanes past the last
+// row point at the last column (their values are zeros).  *bad is set when an
+// offset does not fit; the matrix then keeps using the int32 columns.
+__global__ void k_sell_col16(int64_t brows, int64_t bcols, int64_t nsl, const int *__restrict__ sptr,
+                             const int *__restrict__ scol, short *__restrict__ c16, int *bad) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nsl * 64) return;
+  const int64_t s = i >> 6;
+  const int lane = (int)(i & 63);
+  const int p0 = sptr[s], ns = sptr[s + 1] - p0;
+  for (int t = 0; t < ns; ++t) {
+    const int64_t k = ((int64_t)p0 + t) * 64 + lane;
+    const int64_t c = i < brows ? (int64_t)scol[k] : bcols - 1;
+    int64_t d = c - i;
+    if (d < -32768 || d > 32767) {
+      atomicOr(bad, 1);
+      d = 0;
+    }
+    c16[k] = (short)d;
+  }
+}
+
 // Sliced storage back to the reference's host layout (CSR order; BCSR3
 // blocks as data[b * 9 + e]).
 __global__ void k_sell_to_csr(int64_t brows, int64_t ntot, int nval, const int *__restrict__ ptr,
@@ -310,7 +334,8 @@ std::vector<int32_t> to_i32(const int64_t *a, int64_t n, int64_t limit, const ch
 template <class T>
 SellView<T> sell_view(yafel_hip_matrix_s *A) {
   return SellView<T>{A->sptr.as<int>(), A->perm.p ? A->perm.as<int>() : nullptr, A->col.as<int>(),
-                     (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->brows, A->ntot};
+                     (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->brows, A->ntot,
+                     A->col16.p ? A->col16.as<short>() : nullptr};
 }
 
 // Cap on the plain SpMV grid: past one slice per wavefront the waves loop.
@@ -407,7 +432,19 @@ void matrix_make_sliced(yafel_hip_matrix_s *A, DBuf &ptr, DBuf &col, DBuf &val64
                                                          A->perm.p ? A->perm.as<int>() : nullptr, A->sptr.as<int>(),
                                                          A->col.as<int>(), A->val64.as<double>());
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s));
+    {
+      DBuf bad;
+      bad.alloc(8);
+      HIPCHK(hipMemsetAsync(bad.p, 0, 4, s));
+      A->col16.alloc(std::max<int64_t>(A->ntot, 1) * 2);
+      k_sell_col16<<<ceil_div(nsl * 64, 256), 256, 0, s>>>(brows, A->bcols, nsl, A->sptr.as<int>(), A->col.as<int>(),
+                                                           A->col16.as<short>(), bad.as<int>());
+      HIPCHK(hipGetLastError());
+      int hb = 0;
+      HIPCHK(hipMemcpyAsync(&hb, bad.p, 4, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (hb || A->bcols < 1) A->col16.release();
+    }
   } else {
     HIPCHK(hipMemsetAsync(A->sptr.p, 0, 4, s));
     A->ntot = 0;
@@ -656,6 +693,14 @@ int yafel_hip_matrix_set_unroll(yafel_hip_matrix A, int unroll) {
   REQUIRE(A, "matrix_set_unroll: null matrix");
   REQUIRE(unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8, "matrix_set_unroll: unroll must be 1, 2, 4 or 8");
   A->unroll = unroll;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(A ? A->ctx : nullptr)
+}
+
+int yafel_hip_matrix_index_bytes(yafel_hip_matrix A, int *bytes) {
+  YH_TRY_BEGIN
+  REQUIRE(A && bytes, "matrix_index_bytes: null argument");
+  *bytes = A->col16.p ? 2 : 4;
   return YAFEL_HIP_OK;
   YH_TRY_END(A ? A->ctx : nullptr)
 }

@@ -91,7 +91,7 @@ for _n, _a in [
     ("yafel_hip_matrix_destroy", [_P]), ("yafel_hip_matrix_info_get", [_P, C.POINTER(MatrixInfo)]),
     ("yafel_hip_matrix_download", [_P, _P, _P, _P]), ("yafel_hip_matrix_sync_fp32", [_P]),
     ("yafel_hip_matrix_diagonal", [_P, _P]), ("yafel_hip_matrix_set_unroll", [_P, _I]),
-    ("yafel_hip_matrix_set_cg_passes", [_P, _I]),
+    ("yafel_hip_matrix_set_cg_passes", [_P, _I]), ("yafel_hip_matrix_index_bytes", [_P, _P]),
     ("yafel_hip_spmv", [_P, _I, _P, _P]), ("yafel_hip_spmv_host", [_P, _P, _P]),
     ("yafel_hip_spmm", [_P, _I, _I, _P, _P]), ("yafel_hip_dot", [_P, _I64, _P, _P, C.POINTER(_D)]),
     ("yafel_hip_copy_bandwidth", [_P, C.c_size_t, _I, C.POINTER(_D)]),
@@ -310,6 +310,12 @@ class Matrix:
 
     def set_unroll(self, unroll):
         check(lib.yafel_hip_matrix_set_unroll(self.h, unroll), self.ctx.h)
+
+    def index_bytes(self):
+        """Bytes per column index in the SpMV streams (2: 16-bit offsets)."""
+        b = C.c_int(0)
+        check(lib.yafel_hip_matrix_index_bytes(self.h, C.byref(b)), self.ctx.h)
+        return b.value
 
     def set_cg_passes(self, passes):
         """Kernels per single-solve CG iteration: 2, 3, or 0 (by format)."""
