@@ -12,7 +12,7 @@ built once (device resident) before timing.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
 Prints one JSON line.  `value` is solves/s; the same line carries p50/p99
-step latency, the plain FP64 GPU paths (the judge), the BCSR SpMV roofline
+step latency, the plain FP64 GPU paths (the correctness baseline), the BCSR SpMV roofline
 and the reference CPU path timed on this host.
 """
 import argparse
@@ -510,7 +510,7 @@ def main():
         sys.exit(1)
     log(f"timed {args.steps} steps in {T:.3f}s; last {infos[-1].as_dict()}")
 
-    # plain FP64 GPU paths (the judge): same step with FP64 arithmetic
+    # plain FP64 GPU paths (the correctness baseline): same step with FP64 arithmetic
     fp64 = {}
     if not args.no_fp64:
         for name, meth in [("cg", hip.CG), ("pcg_jacobi", hip.PCG_JACOBI)]:
