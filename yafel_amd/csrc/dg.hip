@@ -419,6 +419,21 @@ __global__ __launch_bounds__(NT) void k_dg_flux(int64_t nf, const DGFaceRec *__r
   }
 }
 
+// Optional epilogue of the element pass for the RK4 stages (DGRK4.hpp:33-69):
+// mode 1 also writes the next stage's input w = u0 + a r, mode 2 writes the
+// step's result u = u0 + (dt/6) k1 + (dt/3) k2 + (dt/3) k3 + (dt/6) r (summed
+// left to right) and advances the device time.  Each lane reads u0 and writes
+// w or u at its own dof only, so w and u may be updated in place.
+struct RkOut {
+  int mode;  // 0: R only
+  const double *u0;
+  double *w;
+  double a;
+  const double *k1, *k2, *k3;
+  double dt;
+  double *u, *t;
+};
+
 // Element pass, a lane per (element, dof): the 64 / nen elements of a
 // wavefront are handled by nen lanes each (p = 3: 4 elements of 16 dofs),
 // and a workgroup loops over element groups so that it loads the shape
@@ -440,7 +455,7 @@ __global__ __launch_bounds__(NT) void k_dg_elem(int64_t nel, const int32_t *__re
                                                const double *__restrict__ minvT, const double *__restrict__ ginfo,
                                                const double *__restrict__ mrefT, double c0, double c1, double src,
                                                const double *__restrict__ U, double *__restrict__ Rout,
-                                               double *__restrict__ Rface) {
+                                               double *__restrict__ Rface, RkOut rk) {
   constexpr int n1 = P + 1, nen = n1 * n1, nqp = nen, EPW = 64 / nen, WPB = NT / 64, EPB = WPB * EPW;
   constexpr int LD = nen + 1;  // padded rows: lanes reading a row or a column hit distinct banks
   __shared__ double sN[nqp][LD];
@@ -586,33 +601,21 @@ __global__ __launch_bounds__(NT) void k_dg_elem(int64_t nel, const int32_t *__re
 #pragma unroll
       for (int j = 0; j < nen; ++j) s += mrow[j] * sL[wv][el][j];
       if (aff) s /= gi[1];
-      Rout[e * nen + a] = rf + s;
-      if (Rface) Rface[e * nen + a] = rf;
+      const double r = rf + s;
+      const int64_t o = e * nen + a;
+      if (rk.mode == 2) {
+        rk.u[o] = rk.u0[o] + (rk.dt / 6) * rk.k1[o] + (rk.dt / 3) * rk.k2[o] + (rk.dt / 3) * rk.k3[o] + (rk.dt / 6) * r;
+        if (o == 0) *rk.t += rk.dt;
+      } else {
+        Rout[o] = r;
+        if (rk.mode == 1) rk.w[o] = rk.u0[o] + rk.a * r;
+      }
+      if (Rface) Rface[o] = rf;
     }
   }
 }
 
 // ------------------------------------------------------------------ RK4
-__global__ void k_dg_copy(int64_t n, const double *__restrict__ a, double *__restrict__ b) {
-  const int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x;
-  if (i < n) b[i] = a[i];
-}
-
-__global__ void k_dg_axpy(int64_t n, const double *__restrict__ u0, const double *__restrict__ k, double a,
-                          double *__restrict__ w) {
-  const int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x;
-  if (i < n) w[i] = u0[i] + a * k[i];
-}
-
-// U = U0 + (dt/6) k1 + (dt/3) k2 + (dt/3) k3 + (dt/6) k4, summed left to right
-__global__ void k_dg_rk4_combine(int64_t n, const double *__restrict__ u0, const double *__restrict__ k1,
-                                 const double *__restrict__ k2, const double *__restrict__ k3,
-                                 const double *__restrict__ k4, double dt, double *__restrict__ u, double *t) {
-  const int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x;
-  if (i < n) u[i] = u0[i] + (dt / 6) * k1[i] + (dt / 3) * k2[i] + (dt / 3) * k3[i] + (dt / 6) * k4[i];
-  if (i == 0) *t += dt;
-}
-
 struct DGPhys {
   double c0, c1, src;
 };
@@ -624,7 +627,8 @@ DGPhys dg_phys(const yafel_hip_physics *phys) {
   return DGPhys{phys->U[0], phys->U[1], phys->f};
 }
 
-void dg_eval(yafel_hip_dg_s *D, const DGPhys &ph, const double *U, double *R, double *Rface, hipStream_t s) {
+void dg_eval(yafel_hip_dg_s *D, const DGPhys &ph, const double *U, double *R, double *Rface, hipStream_t s,
+             RkOut rk = RkOut{}) {
 #define DG_ELEM(P)                                                                                               \
   k_dg_flux<P><<<ceil_div(D->nfaces, (NT / 64) * (64 / (P + 1))), NT, 0, s>>>(D->nfaces, D->faces.as<DGFaceRec>(), D->lnodes.as<int32_t>(), \
                                                       D->rnodes.as<int32_t>(), D->fnodes.as<int32_t>(),           \
@@ -636,7 +640,7 @@ void dg_eval(yafel_hip_dg_s *D, const DGPhys &ph, const double *U, double *R, do
                                 D->ef.as<int32_t>(), D->flux.as<double>(), \
                                 D->X.as<double>(), D->tabN.as<double>(), D->tabdN.as<double>(), D->tabw.as<double>(), \
                                 D->minv.as<double>(), D->ginfo.as<double>(), D->mrefT.as<double>(), ph.c0, ph.c1, \
-                                ph.src, U, R, Rface)
+                                ph.src, U, R, Rface, rk)
   switch (D->p) {
     case 1: DG_ELEM(1); break;
     case 2: DG_ELEM(2); break;
@@ -651,16 +655,17 @@ void dg_eval(yafel_hip_dg_s *D, const DGPhys &ph, const double *U, double *R, do
 void dg_rk4_step(yafel_hip_dg_s *D, const DGPhys &ph, double dt, double *U, hipStream_t s) {
   const int64_t n = D->nel * D->nen;
   double *U0 = D->rk.as<double>(), *k1 = U0 + n, *k2 = k1 + n, *k3 = k2 + n, *k4 = k3 + n, *W = k4 + n;
-  const int g = ceil_div(n, NT);
-  k_dg_copy<<<g, NT, 0, s>>>(n, U, U0);
-  dg_eval(D, ph, U0, k1, nullptr, s);
-  k_dg_axpy<<<g, NT, 0, s>>>(n, U0, k1, dt / 2, W);
-  dg_eval(D, ph, W, k2, nullptr, s);
-  k_dg_axpy<<<g, NT, 0, s>>>(n, U0, k2, dt / 2, W);
-  dg_eval(D, ph, W, k3, nullptr, s);
-  k_dg_axpy<<<g, NT, 0, s>>>(n, U0, k3, dt, W);
-  dg_eval(D, ph, W, k4, nullptr, s);
-  k_dg_rk4_combine<<<g, NT, 0, s>>>(n, U0, k1, k2, k3, k4, dt, U, D->tdev.as<double>());
+  (void)U0;
+  (void)k4;
+  // the stage inputs and the result come from the element passes' epilogues:
+  // W = U + (dt/2) k1, W = U + (dt/2) k2, W = U + dt k3, then U itself
+  RkOut st{1, U, W, dt / 2, nullptr, nullptr, nullptr, 0.0, nullptr, nullptr};
+  dg_eval(D, ph, U, k1, nullptr, s, st);
+  dg_eval(D, ph, W, k2, nullptr, s, st);
+  st.a = dt;
+  dg_eval(D, ph, W, k3, nullptr, s, st);
+  RkOut fin{2, U, nullptr, 0.0, k1, k2, k3, dt, U, D->tdev.as<double>()};
+  dg_eval(D, ph, W, nullptr, nullptr, s, fin);
   HIPCHK(hipGetLastError());
 }
 
