@@ -265,66 +265,97 @@ __global__ __launch_bounds__(256) void k_element_mfma(int64_t nel, const int32_t
   static_assert(NEN <= 32 && KD % 2 == 0, "one 32x32 tile per element");
   __shared__ float sG[4][KD][32];
   __shared__ double sX[4][NEN][2];
-  __shared__ double sJ[4][NQP][5];  // J^-1 (row-major) and jxw
+  __shared__ double sW[4][NQP];       // jxw
+  __shared__ float sS[4][NQP][4];     // sqrt(jxw) J^-1, the factor of G at each qp
+  __shared__ double sdN[NQP][NEN][2]; // gradient table (FP64, for J)
+  __shared__ float sdNf[NQP][32][2];  // FP32 copy, zero past NEN (for G)
+  // the tables once per workgroup; the workgroup then loops over groups of
+  // four elements, one per wavefront
+  for (int t = threadIdx.x; t < NQP * 32; t += 256) {
+    const int q = t >> 5, a = t & 31;
+    const double g0 = a < NEN ? tabdN[(q * NEN + a) * 2] : 0.0, g1 = a < NEN ? tabdN[(q * NEN + a) * 2 + 1] : 0.0;
+    if (a < NEN) {
+      sdN[q][a][0] = g0;
+      sdN[q][a][1] = g1;
+    }
+    sdNf[q][a][0] = (float)g0;
+    sdNf[q][a][1] = (float)g1;
+  }
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int64_t e = (int64_t)blockIdx.x * 4 + w;
-  const bool live = e < nel;  // uniform over the wavefront
-  if (live && l < NEN) {
-    const int64_t node = el[e * NEN + l];
-    sX[w][l][0] = X[3 * node];
-    sX[w][l][1] = X[3 * node + 1];
-  }
-  __syncthreads();
-  if (live && l < NQP) {
-    // Element::update (Element.hpp:116-130)
-    double J[4] = {0, 0, 0, 0};
-    for (int A = 0; A < NEN; ++A) {
-      const double g0 = tabdN[(l * NEN + A) * 2], g1 = tabdN[(l * NEN + A) * 2 + 1];
-      J[0] += sX[w][A][0] * g0;
-      J[1] += sX[w][A][0] * g1;
-      J[2] += sX[w][A][1] * g0;
-      J[3] += sX[w][A][1] * g1;
+  const int64_t ngroups = (nel + 3) / 4;
+  // node coordinates of the next group are loaded while this one computes
+  double nx0 = 0, nx1 = 0;
+  auto load_x = [&](int64_t grp) {
+    const int64_t e = grp * 4 + w;
+    if (grp < ngroups && e < nel && l < NEN) {
+      const int64_t node = el[e * NEN + l];
+      nx0 = X[3 * node];
+      nx1 = X[3 * node + 1];
     }
-    double det, Ji[4];
-    jac_inv<2>(J, det, Ji);
-    for (int k = 0; k < 4; ++k) sJ[w][l][k] = Ji[k];
-    sJ[w][l][4] = det * tabw[l];
-  }
-  __syncthreads();
-  if (live) {
-    for (int t = l; t < KD * 32; t += 64) {
-      const int k = t >> 5, a = t & 31, q = k >> 1, d = k & 1;
-      float v = 0.f;
-      if (a < NEN) {
-        const double g0 = tabdN[(q * NEN + a) * 2], g1 = tabdN[(q * NEN + a) * 2 + 1];
-        const double gd = sJ[w][q][d] * g0 + sJ[w][q][2 + d] * g1;  // (J^-T grad_xi N_a)_d
-        v = (float)(sqrt(sJ[w][q][4]) * gd);
-      }
-      sG[w][k][a] = v;
+  };
+  load_x(blockIdx.x);
+  for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int64_t e = grp * 4 + w;
+    const bool live = e < nel;  // uniform over the wavefront
+    if (live && l < NEN) {
+      sX[w][l][0] = nx0;
+      sX[w][l][1] = nx1;
     }
-  }
-  __syncthreads();
-  if (live && want_k) {
-    v16f acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    __syncthreads();
+    load_x(grp + gridDim.x);
+    if (live && l < NQP) {
+      // Element::update (Element.hpp:116-130), in FP64
+      double J[4] = {0, 0, 0, 0};
 #pragma unroll 5
-    for (int kk = 0; kk < KD / 2; ++kk) {
-      const float v = sG[w][2 * kk + (l >> 5)][l & 31];
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(v, v, acc, 0, 0, 0);
-    }
-    const float ke = (float)(coef ? coef[e] : 1.0);
-    float *Kout = Ke + e * (int64_t)NEN * NEN;
-    const int col = l & 31;
+      for (int A = 0; A < NEN; ++A) {
+        const double g0 = sdN[l][A][0], g1 = sdN[l][A][1];
+        J[0] += sX[w][A][0] * g0;
+        J[1] += sX[w][A][0] * g1;
+        J[2] += sX[w][A][1] * g0;
+        J[3] += sX[w][A][1] * g1;
+      }
+      double det, Ji[4];
+      jac_inv<2>(J, det, Ji);
+      const double jxw = det * tabw[l], sq = sqrt(jxw);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      // accumulator i of lane l holds C[8 (i/4) + 4 (l/32) + i%4][l%32]
-      const int row = 8 * (i >> 2) + 4 * (l >> 5) + (i & 3);
-      if (row < NEN && col < NEN) Kout[row * NEN + col] = -ke * acc[i];
+      for (int k = 0; k < 4; ++k) sS[w][l][k] = (float)(sq * Ji[k]);
+      sW[w][l] = jxw;
     }
-  }
-  if (live && want_r && l < NEN) {
-    double r = 0;
-    for (int q = 0; q < NQP; ++q) r += (f * sJ[w][q][4]) * tabN[q * NEN + l];
-    Re[e * NEN + l] = r;
+    __syncthreads();
+    if (live) {
+      // G[2q + d][a] = sqrt(jxw_q) (J^-T grad_xi N_a)_d, formed in FP32 (the
+      // products are FP32 anyway); a lane fills column a of every other row pair
+      const int a = l & 31, half = l >> 5;
+#pragma unroll 5
+      for (int q = half; q < NQP; q += 2) {
+        const float g0 = sdNf[q][a][0], g1 = sdNf[q][a][1];
+        sG[w][2 * q][a] = sS[w][q][0] * g0 + sS[w][q][2] * g1;
+        sG[w][2 * q + 1][a] = sS[w][q][1] * g0 + sS[w][q][3] * g1;
+      }
+    }
+    __syncthreads();
+    if (live && want_k) {
+      v16f acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 5
+      for (int kk = 0; kk < KD / 2; ++kk) {
+        const float v = sG[w][2 * kk + (l >> 5)][l & 31];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(v, v, acc, 0, 0, 0);
+      }
+      const float ke = (float)(coef ? coef[e] : 1.0);
+      float *Kout = Ke + e * (int64_t)NEN * NEN;
+      const int col = l & 31;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        // accumulator i of lane l holds C[8 (i/4) + 4 (l/32) + i%4][l%32]
+        const int row = 8 * (i >> 2) + 4 * (l >> 5) + (i & 3);
+        if (row < NEN && col < NEN) Kout[row * NEN + col] = -ke * acc[i];
+      }
+    }
+    if (live && want_r && l < NEN) {
+      double r = 0;
+      for (int q = 0; q < NQP; ++q) r += (f * sW[w][q]) * tabN[q * NEN + l];
+      Re[e * NEN + l] = r;
+    }
   }
 }
 
@@ -571,7 +602,7 @@ int yafel_hip_element_matrices_fp32(yafel_hip_mesh m, const yafel_hip_physics *p
           "element_matrices_fp32: matrix-core element matrices cover 2-D Poisson with p = 3 or 4");
   HIPCHK(hipSetDevice(m->ctx->device));
   REQUIRE(!phys->coef || phys->coef_on_device, "element_matrices_fp32: coefficients must be device data");
-  const unsigned g = (unsigned)((m->nel + 3) / 4);
+  const unsigned g = (unsigned)std::min<int64_t>((m->nel + 3) / 4, (int64_t)m->ctx->num_cus * 3);
   hipStream_t s = m->ctx->stream;
   if (m->p == 3)
     k_element_mfma<3><<<g, 256, 0, s>>>(m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(), m->tabN.as<double>(),
@@ -621,7 +652,7 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
   // symmetric tangents (the reference mirrors B < A) stage half the blocks
   const bool sym = !f32 && phys->kind != YAFEL_HIP_PHYS_SUPG;
   if (f32) {
-    const unsigned g = (unsigned)((m->nel + 3) / 4);
+    const unsigned g = (unsigned)std::min<int64_t>((m->nel + 3) / 4, (int64_t)m->ctx->num_cus * 3);
     float *k32 = wk ? m->stageK32.as<float>() : nullptr;
     double *r64 = wr ? m->stageR.as<double>() : nullptr;
     if (m->p == 3)

@@ -46,11 +46,15 @@ SellView<T> view(yafel_hip_matrix_s *A) {
                      A->col16.p ? A->col16.as<short>() : nullptr};
 }
 
-// Fused SpMV grids stay within num_cus * 8 workgroups so that NP <= 4
-// partial sums per workgroup fit the partials workspace.
+// Fused SpMV grids: one slice per wavefront up to num_cus * 16 workgroups
+// (a matrix of 16 k slices, c3's, runs one slice per wavefront instead of
+// two), so that NP <= 4 partial sums per workgroup fit the partials
+// workspace of 8 * fused_cap doubles.
+inline int fused_cap(const yafel_hip_ctx_s *ctx) { return ctx->num_cus * 16; }
+
 template <class T, class Op>
 void launch_spmv_op(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
-  const int grid = sell_grid(A->nsl, A->ctx->num_cus * 8);
+  const int grid = sell_grid(A->nsl, fused_cap(A->ctx));
   const SellView<T> v = view<T>(A);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
@@ -68,7 +72,7 @@ void launch_spmv_op(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
 // The CG iteration's SpMV: partials only, folded by the next kernel.
 template <class T, class Op>
 int launch_spmv_nofold(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
-  const int grid = sell_grid(A->nsl, A->ctx->num_cus * 8);
+  const int grid = sell_grid(A->nsl, fused_cap(A->ctx));
   const SellView<T> v = view<T>(A);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
@@ -87,7 +91,7 @@ int launch_spmv_nofold(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
 // the SpMV grid.
 template <class T, class Op>
 int launch_qp(yafel_hip_matrix_s *A, const Op &op, int64_t n, hipStream_t s) {
-  const int grid = sell_grid(A->nsl, A->ctx->num_cus * 8);
+  const int grid = sell_grid(A->nsl, fused_cap(A->ctx));
   const SellView<T> v = view<T>(A);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
@@ -137,7 +141,7 @@ void ensure_ws(yafel_hip_ctx_s *ctx, int64_t n) {
     for (auto &b : w.vec64) b.alloc(need * 8);
     w.n64 = need;
   }
-  if (!w.partials.p) w.partials.alloc((size_t)ctx->num_cus * 8 * 8 * sizeof(double) + 4096);
+  if (!w.partials.p) w.partials.alloc((size_t)fused_cap(ctx) * 8 * sizeof(double) + 4096);
   if (!w.state.p) {
     w.state.alloc(4096);
     HIPCHK(hipMemsetAsync(w.state.p, 0, 4096, ctx->stream));
@@ -160,7 +164,7 @@ struct Solver {
   bool two_pass = false;  // CG iterations as QP, U (k_spmv_qp)
   KState *cur() const { return st + (ver & 1); }
   // partial regions: the CG SpMV's at 0, the update pass's at 2 * cap
-  int cap() const { return ctx->num_cus * 8; }
+  int cap() const { return fused_cap(ctx); }
 
   template <class O>
   O base(O op, int counter) const {
