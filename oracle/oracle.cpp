@@ -1360,6 +1360,11 @@ void orc_grad_zz(int dim, int64_t n_nodes, const double *dof_nodes, int64_t nel,
   for (i64 nd = 0; nd < n_nodes; ++nd) {
     const auto &ae = adj[nd];
     const int m = (int)ae.size() * T.nqp;
+    if (m == 0) {
+      // a node of no element: Eigen's solve of the empty system is zero
+      for (int i = 0; i < nsd; ++i) grad[nd * nsd + i] = 0;
+      continue;
+    }
     std::vector<double> P(m * nb), b(m * nsd);
     int idx = 0;
     for (i64 e : ae) {

@@ -89,3 +89,20 @@ def test_recovery_rejects_what_the_reference_does_not_define(H, ctx):
     m3 = H.Mesh.structured(ctx, 3, 1, 3, 3)
     with pytest.raises(H.YafelHipError):
         m3.gradient_zz(ctx.upload(np.zeros(m3.ndofs)))  # scalar fields only
+
+
+def test_zz_node_of_no_element_is_zero(H, ctx, orc):
+    """A node no cell references has an empty patch: its ZZ gradient is zero
+    (the least-squares solve of an empty system) and the other nodes' values
+    are those of the mesh without it."""
+    nodes, cells = H.structured_mesh(2, 9, 9, 0, 1)
+    nodes = _graded(nodes, 2)
+    extra = np.vstack([nodes, [[2.0, 2.0, 0.0]]])
+    m0, m1 = H.Mesh(ctx, 2, 1, 1, nodes, cells), H.Mesh(ctx, 2, 1, 1, extra, cells)
+    u = np.random.default_rng(3).standard_normal(m1.ndofs)
+    G0 = m0.gradient_zz(ctx.upload(u[:-1])).download().reshape(-1, 2)
+    G1 = m1.gradient_zz(ctx.upload(u)).download().reshape(-1, 2)
+    assert np.array_equal(G1[-1], [0.0, 0.0])
+    assert np.array_equal(G1[:-1], G0)
+    el, dn = m1.download()
+    assert np.abs(G1 - orc.grad_zz(2, dn, el, u)).max() <= 1e-10 * np.abs(G0).max()

@@ -454,3 +454,13 @@ def test_zz_graded_patch_least_squares_closed_form(orc):
         rows = np.abs(dn[:, 0] - xn) < 1e-12
         assert np.abs(G[rows, 0] - coef[0]).max() < 1e-12, (k, xn)
         assert np.abs(G[rows, 1]).max() < 1e-12
+
+
+def test_zz_node_of_no_element(orc):
+    """An unreferenced node gets the empty fit's zero; the others are unchanged."""
+    el, dn = orc.mesh(2, 1, 4)
+    dn = _graded(dn, 2)
+    u = np.random.default_rng(1).standard_normal(dn.shape[0] + 1)
+    G0 = orc.grad_zz(2, dn, el, u[:-1])
+    G1 = orc.grad_zz(2, np.vstack([dn, [[3.0, 3.0, 0.0]]]), el, u)
+    assert np.array_equal(G1[-1], [0.0, 0.0]) and np.array_equal(G1[:-1], G0)

@@ -93,7 +93,7 @@ int yafel_hip_gmsh_read(const char *path, yafel_hip_gmsh *out) {
   std::string line;
   std::vector<std::string> w;
   long long lineno = 0, nnodes = 0, nelems = 0;
-  int32_t offset = 0;
+  int64_t offset = 0;
   std::vector<std::vector<int32_t>> cell_tags, cell_conn;
   std::vector<int32_t> cell_off_file;
   auto next_line = [&]() {
@@ -154,14 +154,15 @@ int yafel_hip_gmsh_read(const char *path, yafel_hip_gmsh *out) {
         REQUIRE(nd >= 1 && nd <= nnodes, "gmsh_read: element node out of range on line " + std::to_string(lineno));
         cell_conn[e].push_back((int32_t)(nd - 1));
       }
-      cell_off_file[e] = offset;
-      offset += (int32_t)cell_conn[e].size();
+      cell_off_file[e] = (int32_t)offset;
+      offset += (int64_t)cell_conn[e].size();
+      REQUIRE(offset < (1LL << 31), "gmsh_read: more than 2^31 cell node entries");
     }
   }
   // cells in id order; each id's nodes at its file-order offset (the reference
   // stores cellOffsets_[id-1] = running offset and appends nodes in file order)
   g->cell_nodes.assign((size_t)offset, 0);
-  g->cell_offsets.assign((size_t)nelems + 1, offset);
+  g->cell_offsets.assign((size_t)nelems + 1, (int32_t)offset);
   for (size_t e = 0; e < (size_t)nelems; ++e) {
     g->cell_offsets[e] = cell_off_file[e];
     for (size_t k = 0; k < cell_conn[e].size(); ++k) g->cell_nodes[(size_t)cell_off_file[e] + k] = cell_conn[e][k];

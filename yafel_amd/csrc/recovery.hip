@@ -238,6 +238,12 @@ __global__ __launch_bounds__(NT) void k_zz_node(int64_t nnodes, const double *__
   constexpr int NEN = DIM == 2 ? 4 : 8, NQP = NEN, NB = DIM + 1;
   const int64_t A = blockIdx.x * (int64_t)NT + threadIdx.x;
   if (A >= nnodes) return;
+  if (ptr[A] == ptr[A + 1]) {
+    // a node of no element: an empty fit, whose minimum-norm solution is 0
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) G[A * DIM + d] = 0.0;
+    return;
+  }
   double xn[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) xn[d] = X[3 * A + d];
