@@ -38,6 +38,10 @@ __device__ __forceinline__ int64_t sell_pos(const int *__restrict__ sptr, const 
 // accumulation type.  Columns come from the 16-bit offsets to the stored
 // position when the matrix has them (2 bytes an entry instead of 4), else
 // from the int32 array; cf(k) is the column of entry k.
+// The steps are software-pipelined: a step's gathers are issued, then the
+// next step's columns and values, and only then does the lane wait for the
+// gathers, so the gathers' cache latency overlaps the next step's HBM loads
+// (vector loads complete in issue order, so the order of issue matters).
 template <int U, class T, class TV, class XF, class CF>
 __device__ __forceinline__ TV sell_row_csr_cols(const SellView<T> &A, int64_t s, int lane, XF xf, CF cf) {
   const int p0 = A.sptr[s], p1 = A.sptr[s + 1];
@@ -45,17 +49,36 @@ __device__ __forceinline__ TV sell_row_csr_cols(const SellView<T> &A, int64_t s,
   const int len = p1 - p0;
   TV acc = 0;
   int t = 0;
-  for (; t + U <= len; t += U) {
+  if (U <= len) {
     int c[U];
     T v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int64_t k = base + (int64_t)(t + u) * 64;
+      const int64_t k = base + (int64_t)u * 64;
       c[u] = cf(k);
       v[u] = __builtin_nontemporal_load(&A.val[k]);
     }
+    for (;;) {
+      TV xg[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) acc = fma((TV)v[u], xf(c[u]), acc);
+      for (int u = 0; u < U; ++u) xg[u] = xf(c[u]);
+      const bool more = t + 2 * U <= len;  // uniform over the slice
+      T vn[U];
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int64_t k = base + (int64_t)(t + U + u) * 64;
+          c[u] = cf(k);
+          vn[u] = __builtin_nontemporal_load(&A.val[k]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc = fma((TV)v[u], xg[u], acc);
+      t += U;
+      if (!more) break;
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = vn[u];
+    }
   }
   for (; t < len; ++t) {
     const int64_t k = base + (int64_t)t * 64;
