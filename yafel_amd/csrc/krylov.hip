@@ -183,15 +183,18 @@ struct Solver {
     op.pin = pin;
     op.nin = nin;
     op.part = ppart;
-    const int g = vgrid();
+    const int g = vgrid<typename O::value_type>();
     k_velem<O><<<g, NT, 0, s>>>(n, op);
     HIPCHK(hipGetLastError());
     ++ver;
     return g;
   }
-  // VE elements per thread; a multiple of 8 workgroups keeps the XCD-aware
-  // block order, and rounding up keeps each workgroup's chunk <= VE * NT
+  // ve_of<T>() elements per thread; a multiple of 8 workgroups keeps the
+  // XCD-aware block order, and rounding up keeps each workgroup's chunk
+  // <= that many times NT
+  template <class T>
   int vgrid() const {
+    constexpr int VE = ve_of<T>();
     int64_t g = (n + (int64_t)NT * VE - 1) / ((int64_t)NT * VE);
     if (g >= 8) g = (g + 7) / 8 * 8;
     if (g > cap()) g = cap();
@@ -258,7 +261,7 @@ struct Solver {
     a.in = cur();
     a.out = st + ((ver + 1) & 1);
     a.pin = pu;
-    a.nin = vgrid();
+    a.nin = vgrid<T>();
     a.part = part;
     a.p0 = (T *)v.p0;
     a.p1 = (T *)v.p1;

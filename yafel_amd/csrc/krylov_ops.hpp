@@ -533,12 +533,18 @@ __device__ __forceinline__ void refold_reduce(const double (&v)[NPIN], double (&
 // Versioned elementwise pass: fold the producer's NPIN partials, let the op
 // update the state (decide), publish it from workgroup 0, then finish the
 // elementwise body with up to NP partial sums of its own.  Each thread loads
-// its first VE elements' inputs before the fold, so their latency overlaps
-// the fold's; the grid is sized (velem_grid) so that VE elements per thread
-// cover the vector.  Thread 0 of each workgroup keeps the state in LDS and
+// its first VE = ve_of<value_type>() elements' inputs before the fold, so
+// their latency overlaps the fold's; the host sizes the grid so that VE
+// elements per thread cover the vector.  Thread 0 of each workgroup keeps the state in LDS and
 // runs decide(); the other threads read the outcome from there (no
 // per-thread copy of the state, which would land in scratch memory).
-constexpr int VE = 4;
+// Elements per thread, by vector type: on the 64^3 elasticity mixed solve
+// (FP32 vectors) 8 beat 4 and 16 (62.0 ms per step against 62.9 and 63.6);
+// on the Q4 256^2 Poisson FP64 solve 4 beat 8 (435 against 439 ms).
+template <class T>
+constexpr int ve_of() {
+  return sizeof(T) == 4 ? 8 : 4;
+}
 
 // the state moves between global memory and LDS as 8-byte words, a word per
 // thread
@@ -550,6 +556,7 @@ static_assert(sizeof(KState) % 8 == 0 && KSW <= NT, "KState words");
 // range publishes the state.
 template <class Op>
 __device__ __forceinline__ void velem_body(int64_t n, Op &op, int blk, int nblk) {
+  constexpr int VE = ve_of<typename Op::value_type>();
   __shared__ KState ss;
   __shared__ int sgo;
   const int t = threadIdx.x;
@@ -632,6 +639,7 @@ struct OpCgQ : OpBase {
 // U: alpha = rho / p.q; r -= alpha q, z = D^-1 r; r.z, r.r.
 template <class T, bool PREC>
 struct OpCgU : VOpBase {
+  using value_type = T;
   static constexpr int NPIN = 1, NP = 2;
   T *r, *z;
   const T *q, *dinv;
@@ -677,6 +685,7 @@ struct OpCgU : VOpBase {
 // p = z + beta p.
 template <class T>
 struct OpCgP : VOpBase {
+  using value_type = T;
   static constexpr int NPIN = 2, NP = 0;
   static constexpr bool kOnStop = true;
   T *p, *x;
@@ -708,6 +717,7 @@ struct OpCgP : VOpBase {
 // p = z + beta p after the initial or a refinement residual (KState::pupd).
 template <class T>
 struct OpCgP0 : VOpBase {
+  using value_type = T;
   static constexpr int NPIN = 0, NP = 0;
   T *p;
   const T *z;

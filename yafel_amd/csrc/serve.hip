@@ -386,10 +386,10 @@ void join_slot(yafel_hip_server_s *S, Slot *sl) {
   const int64_t n = A->rows;
   Req *r = sl->req;
   const int g_spmv = sell_grid(A->nsl, GCAP);
-  // elementwise passes: VE elements per thread, as the single solve sizes
-  // them -- every workgroup of a pass refolds the producer's partials, so
-  // fewer, fuller workgroups read far fewer partials
-  int64_t ge = ceil_div(n, (int64_t)NT * VE);
+  // elementwise passes: as many elements per thread as the single solve's
+  // FP32 passes take -- every workgroup of a pass refolds the producer's
+  // partials, so fewer, fuller workgroups read far fewer partials
+  int64_t ge = ceil_div(n, (int64_t)NT * ve_of<float>());
   if (ge >= 8) ge = (ge + 7) / 8 * 8;
   const int g_elem = (int)std::max<int64_t>(1, std::min<int64_t>(GCAP, ge));
   const size_t nbytes = 5 * (((size_t)n * 4 + 255) & ~(size_t)255) + (((size_t)n * 8 + 255) & ~(size_t)255) +
