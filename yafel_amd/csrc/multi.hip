@@ -135,6 +135,7 @@ __global__ __launch_bounds__(NT) void k_mspmm(CsrView<T> A, Op op0) {
   const int lane = threadIdx.x & 63;
   const int k = op.st->k;
   const bool on = lane < k;
+  op.bind1(on ? lane : 0);
   const int64_t units = ((A.rows + 63) >> 6) * H;
   const int64_t nw = (int64_t)gridDim.x * (NT / 64);
   for (int64_t u = (int64_t)dev::xcd_block() * (NT / 64) + (threadIdx.x >> 6); u < units; u += nw) {
@@ -245,6 +246,7 @@ struct MBase {
   __device__ const void *cscale() const { return nullptr; }
   // the four right-hand sides 4g .. 4g+3 of one lane (FP32 products)
   __device__ void bind4(int) {}
+  __device__ void bind1(int) {}  // the lane's right-hand side in k_mspmm
   __device__ void act4(int j0, bool (&a)[4]) const {
 #pragma unroll
     for (int c = 0; c < 4; ++c) a[c] = st->active[j0 + c] != 0;
@@ -470,14 +472,19 @@ struct MCgA : MBase {
     pnew = cur ? p0 : p1;
   }
   __device__ T bet(int j) const { return st->first ? T(0) : (T)st->beta[j]; }
+  // beta of this lane's right-hand side, read from the state once (bind1)
+  // instead of at every gather, where the loads of the state would stay in
+  // flight beside the gathers
+  T bl;
+  __device__ void bind1(int j) { bl = bet(j); }
   __device__ T x(int c, int j) const {
     const int64_t o = (int64_t)c * k + j;
-    return fma(bet(j), pold[o], z[o]);
+    return fma(bl, pold[o], z[o]);
   }
   __device__ void row(int64_t i, int j, T y, double *acc) const {
     if (!st->active[j]) return;
     const int64_t o = i * k + j;
-    const T pn = fma(bet(j), pold[o], z[o]);
+    const T pn = fma(bl, pold[o], z[o]);
     pnew[o] = pn;
     q[o] = y;
     acc[0] += (double)pn * (double)y;
