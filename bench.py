@@ -9,11 +9,16 @@ pattern + Dirichlet + Jacobi-PCG to ||b - A x|| / ||b|| <= 1e-8 with FP32
 inner iterations and FP64 refinement.  Mesh, DoF numbering and the pattern are
 built once (device resident) before timing.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serve] [--no-traffic]
 
 Prints one JSON line.  `value` is solves/s; the same line carries p50/p99
-step latency, the plain FP64 GPU paths (the correctness baseline), the BCSR SpMV roofline
-and the reference CPU path timed on this host.
+step latency, the plain FP64 GPU paths (the correctness baseline), the
+roofline of the CG SpMV (its HBM traffic measured by two rocprofv3 counter
+runs of this script's traffic probe, started first as child processes), a
+bounded served stream of the mixed Poisson / elasticity requests (solves/s,
+p50/p99 at three arrival rates, against the FP64 path one request at a
+time), and the reference CPU path timed on this host with the GPU
+solutions' error against its solve.  --workload selects the other modes.
 """
 import argparse
 import json
@@ -41,79 +46,216 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, fp64_cg_iters, hip):
-    """The reference's own CPU path on this host: its bcsr_spmv + cg_solve loop
-    body (compiled from the reference sources with its Release flags, serial
-    as in the reference), timed for a bounded number of iterations and scaled
-    to a full solve by the FP64 CG iteration count; plus the oracle's
-    element loop + the reference's sparse_bcsr triplet constructor on a 1/16
-    slab of the mesh, scaled to the full mesh."""
+def host_cpu():
+    """CPU model (from /proc/cpuinfo) and the threads this process may use:
+    its affinity set, capped by OMP_NUM_THREADS when that is set."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return model or platform.processor() or platform.machine(), threads
+
+
+def bcsr_to_csr(ptr, col, val):
+    """The scalar CSR of a BCSR3 matrix in the reference layout (block values
+    row-major): row 3I+i holds, block by block, columns 3J..3J+2."""
+    nbr = len(ptr) - 1
+    blen = np.diff(ptr)
+    brk = np.repeat(np.arange(nbr, dtype=np.int64), blen)
+    koff = np.arange(len(col), dtype=np.int64) - ptr[brk]
+    base = 9 * ptr[brk] + 3 * koff
+    i = np.arange(3, dtype=np.int64)
+    pos = (base[:, None, None] + (3 * i[None, :] * blen[brk][:, None])[:, :, None]
+           + i[None, None, :]).ravel()
+    ccol = np.empty(9 * len(col), np.int64)
+    cval = np.empty(9 * len(col))
+    ccol[pos] = np.broadcast_to((3 * col)[:, None, None] + i[None, None, :], (len(col), 3, 3)).ravel()
+    cval[pos] = val
+    rptr = np.zeros(3 * nbr + 1, np.int64)
+    rptr[1:] = np.cumsum(np.repeat(3 * blen, 3))
+    return rptr, ccol, cval
+
+
+def cpu_baseline(w, xs, fp64_cg_iters):
+    """The reference's own CPU path on this host, beside the GPU run.
+
+    Solve: the reference's cg_solve (cg_solve.hpp:33-81) with its serial
+    bcsr_spmv (bcsr_spmv.hpp:23-62), compiled from the reference sources with
+    its Release flags, run once in full on the assembled system to
+    ||r||^2/||b||^2 < 1e-16, i.e. ||r||/||b|| < 1e-8 (the reference's BCSR
+    path has no threads).  Its solution is the one the GPU solutions in `xs`
+    are compared with.  Beside it, the reference's threaded path: cg_solve's
+    loop body with csr_spmv (OpenMP schedule(static,128), csr_spmv.hpp:36-38)
+    on the same matrix in CSR, timed for a bounded number of iterations.
+    Assembly: CGAssembly's OpenMP element loop (the oracle's restatement) and
+    the reference's sparse_bcsr triplet constructor on a 64 x 64 x 4 slab
+    (1/16 of the elements), scaled by 16, on all threads and on one."""
     from oracle.oracle import Reference, Oracle, physics
+    from yafel_amd import workloads
     if not Reference.available(fast=True):
         return None
+    model, nthreads = host_cpu()
     ref = Reference(fast=True)
     ptr, col, val = w.A.arrays()
     b = w.b.download()
     M = ref.matrix_from_arrays(3, w.n, ptr, col, val)
     t = time.perf_counter()
-    M.cg_iterations(b, 2)
-    t2 = (time.perf_counter() - t) / 2
-    k = int(max(2, min(200, 12.0 / max(t2, 1e-6))))
+    x_ref = M.cg(b, tol=TOL * TOL)
+    solve_s = time.perf_counter() - t
+    relres = float(np.linalg.norm(b - M.spmv(x_ref)) / np.linalg.norm(b))
+    nref = float(np.linalg.norm(x_ref))
+    errs = {k: float(np.linalg.norm(x - x_ref) / nref) for k, x in xs.items()}
+    del M
+    log(f"cpu: reference cg_solve (BCSR, serial) {solve_s:.2f} s, rel residual {relres:.2e}; errors {errs}")
+    # the threaded solve path: csr_spmv with OpenMP
+    rptr, ccol, cval = bcsr_to_csr(ptr, col, val)
+    del val
+    Mc = ref.matrix_from_arrays(1, w.n, rptr, ccol, cval)
+    del rptr, ccol, cval
     t = time.perf_counter()
-    M.cg_iterations(b, k)
-    t_iter = (time.perf_counter() - t) / k
-    del M, val
-    # assembly sample: 64 x 64 x 4 slab (1/16 of the elements)
+    Mc.cg_iterations(b, 2)
+    t2 = (time.perf_counter() - t) / 2
+    k = int(max(3, min(100, 6.0 / max(t2, 1e-6))))
+    t = time.perf_counter()
+    Mc.cg_iterations(b, k)
+    csr_iter = (time.perf_counter() - t) / k
+    del Mc
+    try:
+        import ctypes
+        omp_threads = ctypes.CDLL("libgomp.so.1").omp_get_max_threads()
+    except OSError:
+        omp_threads = None
+    # assembly sample
     orc = Oracle(fast=True)
     nodes, cells = orc.structured_mesh(3, 64, 64, 4, 0)
     el, dn = orc.dofmanager(3, 1, nodes, cells)
-    lam, mu = 0.3 / (1.3 * 0.4), 1 / 2.6
-    t = time.perf_counter()
-    rows, cols, vals, R = orc.assemble(3, 1, 3, dn, el, physics(1, 3, lam=lam, mu=mu, body=(0, 0, -1)))
-    B = ref.matrix(rows, cols, vals, block=3)
-    t_asm = (time.perf_counter() - t) * 16
-    del B, rows, cols, vals
-    solve_s = t_iter * fp64_cg_iters
+    ph = physics(1, 3, lam=workloads.LAMBDA, mu=workloads.MU, body=(0, 0, -1))
+    asm = {}
+    for nt in sorted({1, nthreads}):
+        t = time.perf_counter()
+        rows, cols, vals, _ = orc.assemble_threads(nt, 3, 1, 3, dn, el, ph)
+        t_el = time.perf_counter() - t
+        t = time.perf_counter()
+        B = ref.matrix(rows, cols, vals, block=3)
+        t_tr = time.perf_counter() - t
+        del B, rows, cols, vals
+        asm[nt] = {"element_loop_s": 16 * t_el, "triplets_to_bcsr_s": 16 * t_tr, "total_s": 16 * (t_el + t_tr)}
+    csr_solve_s = csr_iter * fp64_cg_iters
+    best_solve = min(solve_s, csr_solve_s)
     return {
-        "value": 1.0 / (solve_s + t_asm), "unit": "solves/s", "cores": 1, "kind": "reference",
-        "sample": (f"{k} iterations of the reference cg_solve loop body with its serial bcsr_spmv on "
-                   f"the assembled 64^3 system ({t_iter * 1e3:.1f} ms/iter, scaled by the {fp64_cg_iters} "
-                   f"FP64 CG iterations to 1e-8) + oracle element loop and reference sparse_bcsr(triplets) "
-                   f"on a 64x64x4 slab scaled x16 ({t_asm:.1f} s); -O3 -ffast-math, 1 thread"),
-        "ms_per_iter": t_iter * 1e3, "assembly_s": t_asm, "solve_s": solve_s,
-        "cpu": platform.processor() or platform.machine(),
+        "value": 1.0 / (best_solve + asm[nthreads]["total_s"]), "unit": "solves/s", "cores": nthreads,
+        "kind": "reference",
+        "sample": (f"reference cg_solve + serial bcsr_spmv run once in full on the assembled 64^3 system "
+                   f"({solve_s:.1f} s, rel residual {relres:.1e}); reference cg_solve loop body with OpenMP "
+                   f"csr_spmv on the same matrix in CSR, {k} iterations ({csr_iter * 1e3:.1f} ms/iter) scaled by "
+                   f"the {fp64_cg_iters} FP64 CG iterations; CGAssembly element loop (OpenMP, oracle restatement) "
+                   f"+ reference sparse_bcsr(triplets) on a 64x64x4 slab scaled x16; value = best solve + "
+                   f"assembly on {nthreads} threads; -O3 -ffast-math"),
+        "cpu": model, "threads": nthreads, "omp_threads": omp_threads,
+        "solve_bcsr_serial_s": solve_s, "solve_bcsr_serial_rel_residual": relres,
+        "solve_csr_openmp_ms_per_iter": csr_iter * 1e3, "solve_csr_openmp_s": csr_solve_s,
+        "assembly": {str(k_): v for k_, v in asm.items()},
+        "value_1_thread": 1.0 / (solve_s + asm[1]["total_s"]),
+        "gpu_solution_rel_error_vs_reference": errs,
     }
 
 
 KERNEL_A = "k_spmv_fused<3, 4, float, OpCgQ<float>>"
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                            "c2_cg_kernel_a_hbm_traffic.md")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprof counter
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on the copy kernel), or None
-    when the summary covers another kernel."""
-    found, value = None, None
-    try:
-        with open(TRAFFIC_FILE) as f:
-            for line in f:
-                if line.startswith("traffic_kernel:"):
-                    found = line.split(":", 1)[1].strip()
-                if line.startswith("traffic_bytes_per_launch:"):
-                    value = float(line.split(":", 1)[1])
-    except OSError:
-        return None
-    return value if found == kernel else None
+def _counter_rows(d):
+    """(kernel name, per-dispatch counter value) rows of the counter CSVs under d."""
+    import csv
+    import glob
+    per = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                key = (f, row["Dispatch_Id"])
+                name, v = row["Kernel_Name"], float(row["Counter_Value"])
+                per[key] = (name, per.get(key, (name, 0.0))[1] + v)
+    return list(per.values())
+
+
+def measured_traffic(n):
+    """HBM bytes per launch of the CG SpMV kernel, measured now: two child runs
+    of this script's traffic probe under rocprofv3 --pmc, FETCH_SIZE then
+    WRITE_SIZE, each scaled by what the same counter reads for the copy kernel
+    moving a known 1 GiB (read and written).  Only launches that ran the
+    product count (at least half the largest launch's fetch).  Runs before
+    this process touches the GPU; returns (bytes or None, detail)."""
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, {"error": "rocprofv3 not found"}
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="yafel_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", counter, "--kernel-include-regex", "k_spmv_fused|k_copy", "--output-format", "csv",
+               "-d", d, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--workload",
+               "traffic-probe", "--n", str(n)]
+        t = time.perf_counter()
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True, timeout=240)
+            out, rc = r.stdout, r.returncode
+        except subprocess.TimeoutExpired as e:
+            out, rc = (e.stdout or b"").decode(errors="replace") if isinstance(e.stdout, bytes) else (e.stdout or ""), -9
+        rows = _counter_rows(d)
+        shutil.rmtree(d, ignore_errors=True)
+        log(f"traffic {counter}: rc {rc}, {len(rows)} dispatches, {time.perf_counter() - t:.1f} s")
+        # the profiler may end with a fault of its own after the program's last
+        # line; the probe's completion line and whole results are what count
+        if "traffic-probe done" not in out or not rows:
+            return None, {"error": f"{counter} run rc {rc}: {out[-400:]}"}
+        cg = [v for name, v in rows if "k_spmv_fused" in name and "OpCgQ<float>" in name]
+        cp = [v for name, v in rows if "k_copy" in name]
+        if not cg or not cp:
+            return None, {"error": f"{counter}: kernels not found in the counter output"}
+        full = [v for v in cg if v >= 0.5 * max(cg)]
+        res[counter] = {"kib_per_launch": float(np.mean(full)), "launches": len(full), "skipped": len(cg) - len(full),
+                        "copy_kib": float(np.median(cp)), "factor": (1 << 30) / (float(np.median(cp)) * 1024)}
+    fetch = res["FETCH_SIZE"]["kib_per_launch"] * 1024 * res["FETCH_SIZE"]["factor"]
+    write = res["WRITE_SIZE"]["kib_per_launch"] * 1024 * res["WRITE_SIZE"]["factor"]
+    return fetch + write, {"read_bytes": fetch, "write_bytes": write, **{k.lower(): v for k, v in res.items()}}
+
+
+def traffic_probe(args):
+    """The command the counter runs profile: one mixed c2 step (the CG SpMV
+    launches) and three launches of the 1 GiB copy kernel (the calibration)."""
+    from yafel_amd import hip, workloads
+    ctx = hip.Context(0)
+    w = workloads.c2(ctx, n=args.n)
+    info = w.step(method=hip.PCG_JACOBI, precision=hip.MIXED, stop=hip.STOP_RELRES, tol=TOL)
+    ctx.copy_bandwidth(1 << 30, 2)
+    ctx.sync()
+    print(f"traffic-probe done: {info.iterations} iterations", flush=True)
+
+
+def req_class(m):
+    return "3d_q1_elasticity" if m["dim"] == 3 else f"2d_q{m['p']}_poisson"
 
 
 def run_stream(hip, ctx, mats, arrivals, budget, slots, setup_threads=0):
     """Feed requests at their arrival times into a server; returns per-request
-    latency (arrival -> finished), makespan and the results."""
+    latency (arrival -> found finished), makespan, the results, the server
+    statistics and each request's (admission wait, setup, wait to join,
+    solve) times in ms."""
     srv = hip.Server(ctx, budget, slots, setup_threads)
     n = len(mats)
     t0 = time.perf_counter()
-    i, done_t, rid_of = 0, {}, {}
+    i, done_t, rid_of, times = 0, {}, {}, {}
     while len(done_t) < n:
         now = time.perf_counter() - t0
         while i < n and arrivals[i] <= now:
@@ -126,6 +268,7 @@ def run_stream(hip, ctx, mats, arrivals, budget, slots, setup_threads=0):
         tnow = time.perf_counter() - t0
         for rid in fin:
             done_t[rid_of[rid]] = tnow
+            times[rid_of[rid]] = srv.times(rid)
         if not fin and srv.stats()["active"] == 0 and i < n and arrivals[i] > tnow:
             time.sleep(min(arrivals[i] - tnow, 0.002))
     makespan = max(done_t.values())
@@ -135,44 +278,28 @@ def run_stream(hip, ctx, mats, arrivals, budget, slots, setup_threads=0):
     st = srv.stats()
     srv.close()
     lat = np.array([done_t[j] - arrivals[j] for j in range(n)])
-    return lat, makespan, results, st
+    return lat, makespan, results, st, times
 
 
-def serve_bench(args):
-    """c4: a served stream of Poisson / elasticity solves with Poisson arrivals."""
-    from yafel_amd import hip, workloads
-    ctx = hip.Context(0)
-    budget = 200 << 30
-    reqs = workloads.c4_requests(args.requests, seed=4000)
-    mats = [workloads.c4_materialise(r) for r in reqs]
-    log(f"{len(mats)} requests materialised; dofs total {sum(r['ndofs'] for r in reqs)}")
-    # warm-up (kernel code objects, allocator)
-    run_stream(hip, ctx, mats[:8], np.zeros(8), budget, args.slots, args.setup_threads)
-    sweep = []
-    rates = [float(r) for r in args.rates.split(",")]
-    for rate in rates:
-        nr = args.requests if rate == max(rates) else min(args.requests, args.sweep_requests)
-        arr = workloads.poisson_arrivals(nr, rate, seed=int(rate))
-        lat, span, res, st = run_stream(hip, ctx, mats[:nr], arr, budget, args.slots, args.setup_threads)
-        bad = sum(1 for x, inf in res.values() if inf.status != 0 or inf.rel_residual > TOL)
-        rec = {"rate": rate, "requests": nr, "solves_per_s": nr / span, "p50_ms": float(np.percentile(lat, 50) * 1e3),
-               "p99_ms": float(np.percentile(lat, 99) * 1e3), "failed": bad, "hbm_peak_gb": st["hbm_peak"] / 2**30,
-               "batched_launches": st["batched_launches"]}
-        log(f"rate {rate}: {rec}")
-        sweep.append(rec)
-        if rate == max(rates):
-            full = (lat, span, res, st, nr)
-    lat, span, res, st, nr = full
-    # alone vs in the mix: bit-identical
-    diffs = []
-    for j in range(min(6, nr)):
-        _, _, r1, _ = run_stream(hip, ctx, [mats[j]], np.zeros(1), budget, args.slots, args.setup_threads)
-        diffs.append(float(np.abs(r1[0][0] - res[j][0]).max()))
-    # the plain FP64 GPU path on a sample of the same requests, one at a time
-    ns = min(args.fp64_sample, nr)
+def latency_breakdown(mats, lat, times):
+    """Median (and p99 of the total) per request class: admission wait, setup,
+    wait to join the batch, solve, and arrival -> result."""
+    out = {}
+    for cls in sorted({req_class(m) for m in mats}):
+        idx = [j for j, m in enumerate(mats) if req_class(m) == cls]
+        t = np.array([times[j] for j in idx])
+        out[cls] = {"requests": len(idx), "wait_ms_p50": float(np.median(t[:, 0])),
+                    "setup_ms_p50": float(np.median(t[:, 1])), "join_ms_p50": float(np.median(t[:, 2])),
+                    "solve_ms_p50": float(np.median(t[:, 3])), "total_ms_p50": float(np.median(lat[idx]) * 1e3),
+                    "total_ms_p99": float(np.percentile(lat[idx], 99) * 1e3)}
+    return out
+
+
+def fp64_one_at_a_time(hip, ctx, mats):
+    """The plain FP64 GPU path (FP64 assembly, Jacobi-PCG in FP64 to the same
+    relative residual), one request at a time; returns solves/s."""
     t = time.perf_counter()
-    for j in range(ns):
-        m = mats[j]
+    for m in mats:
         mesh = hip.Mesh(ctx, m["dim"], m["p"], m["dpn"], m["nodes"], m["cells"])
         A = mesh.pattern(hip.BCSR3 if m["dpn"] == 3 else hip.CSR, hip.FP64)
         mesh.assemble(m["ph"], A, None, hip.TANGENT)
@@ -189,21 +316,63 @@ def serve_bench(args):
         x.upload(np.zeros(mesh.ndofs))
         info = A.solve(b, x, method=hip.PCG_JACOBI, precision=hip.FP64, stop=hip.STOP_RELRES, tol=TOL)
         assert info.status == 0, info.as_dict()
-    fp64_rate = ns / (time.perf_counter() - t)
-    value = nr / span
+    return len(mats) / (time.perf_counter() - t)
+
+
+def served_stream(hip, ctx, nreq, rates, fp64_sample, n_alone, slots=128, setup_threads=0, full_at_max=None):
+    """c4: a served stream of Poisson / elasticity solves with Poisson
+    arrivals at each rate; the record of every rate, the alone-vs-mix check,
+    and the plain FP64 GPU path on a sample of the same requests."""
+    from yafel_amd import workloads
+    budget = 200 << 30
+    nmax = max(nreq, full_at_max or 0)
+    reqs = workloads.c4_requests(nmax, seed=4000)
+    mats = [workloads.c4_materialise(r) for r in reqs]
+    log(f"{len(mats)} requests materialised; dofs total {sum(r['ndofs'] for r in reqs)}")
+    run_stream(hip, ctx, mats[:8], np.zeros(8), budget, slots, setup_threads)  # warm-up
+    sweep, top = [], None
+    for rate in rates:
+        nr = full_at_max if (full_at_max and rate == max(rates)) else nreq
+        arr = workloads.poisson_arrivals(nr, rate, seed=int(rate))
+        lat, span, res, st, times = run_stream(hip, ctx, mats[:nr], arr, budget, slots, setup_threads)
+        bad = sum(1 for x, inf in res.values() if inf.status != 0 or inf.rel_residual > TOL)
+        rec = {"rate": rate, "requests": nr, "solves_per_s": nr / span, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+               "p99_ms": float(np.percentile(lat, 99) * 1e3), "failed": bad, "hbm_peak_gb": st["hbm_peak"] / 2**30,
+               "batched_launches": st["batched_launches"], "by_class": latency_breakdown(mats[:nr], lat, times)}
+        log(f"rate {rate}: { {k: v for k, v in rec.items() if k != 'by_class'} }")
+        sweep.append(rec)
+        if rate == max(rates):
+            top = (rec, res)
+    rec, res = top
+    diffs = []
+    for j in range(min(n_alone, rec["requests"])):
+        _, _, r1, _, _ = run_stream(hip, ctx, [mats[j]], np.zeros(1), budget, slots, setup_threads)
+        diffs.append(float(np.abs(r1[0][0] - res[j][0]).max()))
+    fp64_rate = fp64_one_at_a_time(hip, ctx, mats[:min(fp64_sample, rec["requests"])])
+    return {"solves_per_s": rec["solves_per_s"], "p50_ms": rec["p50_ms"], "p99_ms": rec["p99_ms"],
+            "rate_sweep": sweep, "alone_vs_mix_max_abs_diff": max(diffs) if diffs else None,
+            "alone_vs_mix_requests": len(diffs),
+            "fp64_gpu_one_at_a_time_solves_per_s": fp64_rate, "fp64_sample": min(fp64_sample, rec["requests"]),
+            "speedup_vs_fp64_gpu": rec["solves_per_s"] / fp64_rate}
+
+
+def serve_bench(args):
+    """c4: a served stream of Poisson / elasticity solves with Poisson arrivals."""
+    from yafel_amd import hip
+    ctx = hip.Context(0)
+    rates = [float(r) for r in args.rates.split(",")]
+    rec = served_stream(hip, ctx, args.sweep_requests, rates, args.fp64_sample, 6, args.slots, args.setup_threads,
+                        full_at_max=args.requests)
+    nr = rec["rate_sweep"][-1]["requests"]
     out = {
         "metric": "solves/s + p99 latency at 1e-8 rel residual vs plain FP64 CG; SpMV HBM GB/s",
-        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": nr, "warmup": 8,
-        "ms_per_step": span / nr * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": rec["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": nr, "warmup": 8,
+        "ms_per_step": 1e3 / rec["solves_per_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32-inner/fp64-refinement", "data": "synthetic structured meshes (seeded)",
         "config": {"workload": f"served stream of {nr} solves, Poisson arrivals at {max(rates):g}/s: 2-D Poisson Q1/Q2 "
                                "(16-256 cells/axis) or 3-D Q1 elasticity (8-48), lognormal coefficients, N(0,1) loads, "
                                "200 GB HBM budget, batched Jacobi-PCG FP32 inner + FP64 refinement to 1e-8"},
-        "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-        "rate_sweep": sweep, "alone_vs_mix_max_abs_diff": max(diffs) if diffs else None,
-        "fp64_gpu_one_at_a_time_solves_per_s": fp64_rate, "fp64_sample": ns,
-        "speedup_vs_fp64_gpu": value / fp64_rate,
-        "server": {k: st[k] for k in ("admitted", "completed", "failed", "chunks", "batched_launches", "setup_ms")},
+        **{k: v for k, v in rec.items() if k != "solves_per_s"},
     }
     print(json.dumps(out), flush=True)
 
@@ -284,15 +453,19 @@ def spmv_bench(args):
             for _ in range(args.steps):
                 A.spmv(x, y, prec)
             ms = ctx.timer_end() / args.steps
+            # algorithmic bytes with int32 columns; the stored streams use ib bytes per column
             if rec["format"] == "BCSR3":
-                byts = bcsr_bytes(inf.nblocks, n // 3, n, s, ib)
+                byts, stored = bcsr_bytes(inf.nblocks, n // 3, n, s, 4), bcsr_bytes(inf.nblocks, n // 3, n, s, ib)
             else:
-                byts = inf.nnz * (s + ib) + (n + 1) * 4 + 2 * n * s
-            rec[name] = {"us": ms * 1e3, "gbs": byts / (ms * 1e-3) / 1e9, "algorithmic_bytes": byts}
+                byts = inf.nnz * (s + 4) + (n + 1) * 4 + 2 * n * s
+                stored = inf.nnz * (s + ib) + (n + 1) * 4 + 2 * n * s
+            rec[name] = {"us": ms * 1e3, "gbs": byts / (ms * 1e-3) / 1e9, "algorithmic_bytes": byts,
+                         "stored_bytes": stored, "gbs_stored": stored / (ms * 1e-3) / 1e9}
         log(f"spmv {cname}: {rec}")
         out[cname] = rec
         del w, A
     out["copy_gbs"] = ctx.copy_bandwidth(1 << 30, 1)  # two launches of a 1 GiB copy
+    out["read_gbs"] = ctx.read_bandwidth(1 << 30, 5)
     print(json.dumps(out), flush=True)
 
 
@@ -444,45 +617,45 @@ def recovery_bench(args):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=64, help="cells per axis (64 = the benchmark config)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "serve", "c3", "spmv", "dg", "c5", "recovery"])
-    ap.add_argument("--requests", type=int, default=2000)
-    ap.add_argument("--sweep-requests", type=int, default=300)
-    ap.add_argument("--rates", default="50,200,500,1000,2000")
-    ap.add_argument("--slots", type=int, default=128)
-    ap.add_argument("--fp64-sample", type=int, default=100)
-    ap.add_argument("--setup-threads", type=int, default=0, help="server setup workers (0: library default)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}): each solve of this workload "
-              "fits one MI355X, and multi-GPU sharding is not part of this build", file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "serve":
-        serve_bench(args)
-        return
-    if args.workload == "c3":
-        c3_bench(args)
-        return
-    if args.workload == "spmv":
-        spmv_bench(args)
-        return
-    if args.workload == "c5":
-        c5_bench(args)
-        return
-    if args.workload == "recovery":
-        recovery_bench(args)
-        return
-    if args.workload == "dg":
-        dg_bench(args)
-        return
+def multicg_bench(args):
+    """Multi-RHS Jacobi-PCG on 3-D elasticity Q1 32^3 in CSR (107,811 dofs),
+    seeded N(0,1) right-hand sides zero on the clamped face, to 1e-8: MIXED
+    with k = 64 (four right-hand sides per lane) and k = 63 (one per lane),
+    FP64 with k = 64 -- the batches behind the multi-RHS CG product figures."""
+    from yafel_amd import hip, workloads
+    ctx = hip.Context(0)
+    w = workloads.c2(ctx, n=32, fmt=hip.CSR)
+    w.assemble()
+    A, n = w.A, w.n
+    A.sync_fp32()
+    mask = w.mask.download().astype(bool)
+    out = {"workload": f"multi-RHS Jacobi-PCG, elasticity Q1 32^3 CSR ({n} dofs), to 1e-8"}
+    for name, prec, k in (("mixed_k64", hip.MIXED, 64), ("mixed_k63", hip.MIXED, 63), ("fp64_k64", hip.FP64, 64)):
+        B = np.stack([hip.normal(7100 + j, n) for j in range(k)], axis=1)
+        B[mask, :] = 0.0
+        Bd, Xd = ctx.upload(np.ascontiguousarray(B).ravel()), ctx.alloc(n * k)
+        kw = dict(method=hip.PCG_JACOBI, precision=prec, stop=hip.STOP_RELRES, tol=TOL)
+        Xd.upload(np.zeros(n * k))
+        A.solve_multi(Bd, Xd, k, **kw)
+        Xd.upload(np.zeros(n * k))
+        ctx.sync()
+        t0 = time.perf_counter()
+        infos = A.solve_multi(Bd, Xd, k, **kw)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        out[name] = {"ms": dt * 1e3, "iterations_mean": float(np.mean([i.iterations for i in infos])),
+                     "failed": sum(1 for i in infos if i.status != 0 or i.rel_residual >= TOL)}
+        log(f"{name}: {out[name]}")
+    print(json.dumps(out), flush=True)
 
+
+def c2_bench(args):
+    """The default line: c2 solves/s with the FP64 GPU paths, the roofline of
+    the CG SpMV, a bounded c4 served stream and the reference CPU path."""
+    # counters first, in child processes, before this process touches the GPU
+    traffic, traffic_detail = (None, {"error": "skipped (--no-traffic)"})
+    if not args.no_traffic:
+        traffic, traffic_detail = measured_traffic(args.n)
     from yafel_amd import hip, workloads
     ctx = hip.Context(0)
     t = time.perf_counter()
@@ -509,6 +682,7 @@ def main():
         log(f"NOT CONVERGED: {bad[:2]}")
         sys.exit(1)
     log(f"timed {args.steps} steps in {T:.3f}s; last {infos[-1].as_dict()}")
+    xs = {"mixed_pcg": w.x.download()}
 
     # plain FP64 GPU paths (the correctness baseline): same step with FP64 arithmetic
     fp64 = {}
@@ -520,12 +694,14 @@ def main():
             dt = time.perf_counter() - ts
             fp64[name] = {"solves_per_s": 1.0 / dt, "ms": dt * 1e3, "iterations": info.iterations,
                           "rel_residual": info.rel_residual, "status": hip.STATUS[info.status]}
+            xs["fp64_" + name] = w.x.download()
             log(f"fp64 {name}: {fp64[name]}")
         ctx.sync()
 
     # roofline of the dominant kernel: the CG iteration's FP32 SpMV (q = A p
     # and the p.q partials), every launch of one more step timed with its own
-    # HIP event pair on the context stream
+    # HIP event pair on the context stream; launches that returned at once
+    # (after convergence or at a refinement pause) are left out
     A = w.A
     inf = A.info()
     nbr = inf.rows // 3
@@ -534,9 +710,10 @@ def main():
     ka_ms, ka_n = ctx.kernel_time()
     ctx.kernel_timing(False)
     ka_us = ka_ms / max(ka_n, 1) * 1e3
-    ka_bytes = bcsr_bytes(inf.nblocks, nbr, w.n, 4, A.index_bytes())  # matrix, p gathered, q written
-    ka_gbs = ka_bytes / (ka_us * 1e-6) / 1e9
-    log(f"kernel A: {ka_n} launches, {ka_us:.1f} us each, {ka_gbs:.0f} GB/s")
+    alg_bytes = bcsr_bytes(inf.nblocks, nbr, w.n, 4, 4)  # matrix with int32 block columns, p read, q written
+    own_bytes = bcsr_bytes(inf.nblocks, nbr, w.n, 4, A.index_bytes())  # as stored (16-bit column offsets)
+    ka_gbs = alg_bytes / (ka_us * 1e-6) / 1e9
+    log(f"kernel A: {ka_n} launches that ran the product, {ka_us:.2f} us each, {ka_gbs:.0f} GB/s")
     x32 = ctx.upload(np.random.default_rng(0).standard_normal(w.n).astype(np.float32))
     y32 = ctx.alloc(w.n, np.float32)
     A.sync_fp32()
@@ -547,23 +724,31 @@ def main():
     for _ in range(reps):
         A.spmv(x32, y32, hip.FP32)
     ms32 = ctx.timer_end() / reps
-    bytes32 = bcsr_bytes(inf.nblocks, nbr, w.n, 4, A.index_bytes())
     x64 = ctx.upload(np.random.default_rng(0).standard_normal(w.n))
     y64 = ctx.alloc(w.n)
     ctx.timer_begin()
     for _ in range(reps):
         A.spmv(x64, y64, hip.FP64)
     ms64 = ctx.timer_end() / reps
-    bytes64 = bcsr_bytes(inf.nblocks, nbr, w.n, 8, A.index_bytes())
-    gbs32 = bytes32 / (ms32 * 1e-3) / 1e9
-    gbs64 = bytes64 / (ms64 * 1e-3) / 1e9
+    gbs32 = bcsr_bytes(inf.nblocks, nbr, w.n, 4, 4) / (ms32 * 1e-3) / 1e9
+    gbs64 = bcsr_bytes(inf.nblocks, nbr, w.n, 8, 4) / (ms64 * 1e-3) / 1e9
     copy_gbs = ctx.copy_bandwidth(1 << 30, 20)
-    log(f"spmv fp32 {ms32 * 1e3:.1f}us {gbs32:.0f} GB/s; fp64 {ms64 * 1e3:.1f}us {gbs64:.0f} GB/s; copy {copy_gbs:.0f} GB/s")
+    read_gbs = ctx.read_bandwidth(1 << 30, 20)
+    log(f"spmv fp32 {ms32 * 1e3:.1f}us {gbs32:.0f} GB/s; fp64 {ms64 * 1e3:.1f}us {gbs64:.0f} GB/s; "
+        f"copy {copy_gbs:.0f} GB/s; read {read_gbs:.0f} GB/s")
+    del x32, y32, x64, y64
+
+    served = None
+    if not args.no_serve:
+        rates = [float(r) for r in args.serve_rates.split(",")]
+        served = served_stream(hip, ctx, args.serve_requests, rates, args.serve_fp64_sample, 4, args.slots,
+                               args.setup_threads)
+        log(f"served: { {k: v for k, v in served.items() if k != 'rate_sweep'} }")
 
     cpu = None
     if not args.no_cpu:
         iters = fp64.get("cg", {}).get("iterations") or infos[-1].iterations
-        cpu = cpu_baseline(w, iters, hip)
+        cpu = cpu_baseline(w, xs, iters)
         log(f"cpu baseline: {cpu}")
 
     lat_ms = np.array(lat) * 1e3
@@ -582,14 +767,47 @@ def main():
         "rel_residual": infos[-1].rel_residual, "setup_s": setup_s,
         "fp64_gpu": fp64,
         "speedup_vs_fp64_cg": (value / fp64["cg"]["solves_per_s"]) if "cg" in fp64 else None,
-        "spmv_fp32_gbs": gbs32, "spmv_fp32_us": ms32 * 1e3, "spmv_fp64_gbs": gbs64, "copy_gbs": copy_gbs,
+        "spmv_fp32_gbs": gbs32, "spmv_fp32_us": ms32 * 1e3, "spmv_fp64_gbs": gbs64, "spmv_fp64_us": ms64 * 1e3,
+        "copy_gbs": copy_gbs, "read_gbs": read_gbs,
         "roofline": {"bound": "hbm", "achieved": ka_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": ka_gbs / PEAK_HBM_GBS, "traffic": measured_traffic(KERNEL_A),
-                     "kernel": KERNEL_A, "bytes_per_launch": ka_bytes, "us_per_launch": ka_us,
-                     "launches_timed": ka_n},
+                     "frac": ka_gbs / PEAK_HBM_GBS, "traffic": traffic,
+                     "kernel": KERNEL_A, "bytes_per_launch": alg_bytes, "stored_bytes_per_launch": own_bytes,
+                     "us_per_launch": ka_us, "launches_timed": ka_n, "traffic_detail": traffic_detail},
+        "served_stream": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=64, help="cells per axis (64 = the benchmark config)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
+    ap.add_argument("--no-serve", action="store_true", help="skip the bounded served stream")
+    ap.add_argument("--workload", default="c2",
+                    choices=["c2", "serve", "c3", "spmv", "dg", "c5", "recovery", "multicg", "traffic-probe"])
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--sweep-requests", type=int, default=300)
+    ap.add_argument("--rates", default="50,200,500,1000,2000")
+    ap.add_argument("--serve-requests", type=int, default=300, help="default line: requests per rate")
+    ap.add_argument("--serve-rates", default="50,200,2000", help="default line: arrival rates")
+    ap.add_argument("--serve-fp64-sample", type=int, default=40, help="default line: FP64 one-at-a-time sample")
+    ap.add_argument("--slots", type=int, default=128)
+    ap.add_argument("--fp64-sample", type=int, default=100)
+    ap.add_argument("--setup-threads", type=int, default=0, help="server setup workers (0: library default)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): each solve of this workload "
+              "fits one MI355X, and multi-GPU sharding is not part of this build", file=sys.stderr)
+        sys.exit(2)
+    modes = {"serve": serve_bench, "c3": c3_bench, "spmv": spmv_bench, "c5": c5_bench, "recovery": recovery_bench,
+             "dg": dg_bench, "multicg": multicg_bench, "traffic-probe": traffic_probe, "c2": c2_bench}
+    modes[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -149,13 +149,20 @@ int yafel_hip_dot(yafel_hip_ctx ctx, int64_t n, const double *a_dev, const doubl
 /* No reference counterpart: achievable HBM bandwidth of a plain copy kernel
  * (read + write bytes / time) over `bytes` per buffer, `reps` launches. */
 int yafel_hip_copy_bandwidth(yafel_hip_ctx ctx, size_t bytes, int reps, double *gbs);
+/* No reference counterpart: achievable HBM read bandwidth of a read-only
+ * stream (16-byte non-temporal loads) over `bytes`, `reps` launches; the
+ * ceiling for read-dominated kernels such as the SpMV. */
+int yafel_hip_read_bandwidth(yafel_hip_ctx ctx, size_t bytes, int reps, double *gbs);
 
 /* No reference counterpart: HIP-event timing of every launch of the CG
  * iteration's fused SpMV kernel (kernel A) on ctx, for roofline figures.
  * Each timed launch carries its start/stop event pair (hipExtLaunchKernel),
- * so an event pair spans that dispatch alone.  Enabling (or disabling)
- * resets the record; kernel_time returns the summed duration and the launch
- * count (stream-synchronising). */
+ * so an event pair spans that dispatch alone, and a work word that the launch
+ * sets when it runs the product.  Enabling (or disabling) resets the record;
+ * kernel_time returns the summed duration and the count of the launches that
+ * ran the product -- launches that returned at once, after convergence or at
+ * a refinement pause, are left out -- over the first 65,536 timed launches
+ * (stream-synchronising). */
 int yafel_hip_kernel_timing(yafel_hip_ctx ctx, int enable);
 int yafel_hip_kernel_time(yafel_hip_ctx ctx, double *ms_total, int64_t *launches);
 
@@ -424,7 +431,10 @@ typedef struct {
 
 int yafel_hip_server_create(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots, yafel_hip_server *out);
 /* as yafel_hip_server_create, with the number of setup worker threads (each
- * with its own setup stream; 0: the default of 3) */
+ * with its own setup stream; 0: the default of 3).  Admission charges each
+ * request's estimated bytes to hbm_budget; on top of that the workers keep
+ * released set-up buffers for reuse, together at most hbm_budget / 8 (and at
+ * most 4 GiB per worker), which the hbm_* statistics do not count. */
 int yafel_hip_server_create_ex(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots, int setup_threads,
                                yafel_hip_server *out);
 int yafel_hip_server_destroy(yafel_hip_server s);
@@ -436,8 +446,19 @@ int yafel_hip_server_submit(yafel_hip_server s, const yafel_hip_request *req, in
 int yafel_hip_server_progress(yafel_hip_server s, int64_t *done_ids, int max_ids, int *n_done);
 /* state (YAFEL_HIP_REQ_*) and, for finished requests, the dof count */
 int yafel_hip_server_query(yafel_hip_server s, int64_t id, int *state, int64_t *ndofs);
-/* copies a finished request's solution (DoFManager numbering) and releases it */
+/* copies a finished request's solution (DoFManager numbering) and releases it;
+ * for a failed or cancelled request returns YAFEL_HIP_ESTATE (the message says
+ * which) and releases it too, so its id is unknown afterwards */
 int yafel_hip_server_result(yafel_hip_server s, int64_t id, double *x, int64_t n, yafel_hip_solve_info *info);
+/* where a finished request's time went, in ms (host clock, before
+ * yafel_hip_server_result releases it): ms[0] queued until a setup worker
+ * took it (admission wait), ms[1] its setup (numbering, pattern, assembly,
+ * Dirichlet), ms[2] set up until its first batched chunk was launched,
+ * ms[3] from then until a progress call found it finished */
+int yafel_hip_server_times(yafel_hip_server s, int64_t id, double *ms);
+/* cancels a queued request: it leaves the queue and its inputs are freed at
+ * once; yafel_hip_server_query reports YAFEL_HIP_REQ_CANCELLED until
+ * yafel_hip_server_result releases it */
 int yafel_hip_server_cancel(yafel_hip_server s, int64_t id);
 int yafel_hip_server_stats_get(yafel_hip_server s, yafel_hip_server_stats *st);
 

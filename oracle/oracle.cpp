@@ -589,6 +589,59 @@ void orc_cg_assembly(int dim, int p, int dpn, int64_t n_nodes, const double *dof
   }
 }
 
+// CGAssembly.hpp:73-185 with its OpenMP structure, for the timed CPU baseline:
+// every thread builds its own element tables (one ElementFactory per thread,
+// :84), keeps its elements' triplets and a full-length residual private
+// (:80, :149-159), and appends them to the shared list under a critical
+// section (:175-181).  The append order follows thread timing, so the result
+// is orc_cg_assembly's up to the order of the triplets and the rounding of R.
+// Returns the number of triplets written (nel * nld^2).
+int64_t orc_cg_assembly_threads(int nthreads, int dim, int p, int dpn, int64_t n_nodes, const double *dof_nodes,
+                                int64_t nel, const int32_t *elements, const orc_physics *phys, int64_t *rows,
+                                int64_t *cols, double *vals, double *R) {
+  i64 filled = 0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+  {
+    ElementTables T = make_tables(dim, p);
+    const int nld = T.nen * dpn;
+    std::vector<double> K(nld * nld), Re(nld);
+    std::vector<i64> gd(nld);
+    std::vector<i64> tr, tc;
+    std::vector<double> tv, Rt(R ? (size_t)n_nodes * dpn : 0, 0.0);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (i64 e = 0; e < nel; ++e) {
+      const int32_t *conn = &elements[e * T.nen];
+      element_kernel(T, dpn, dof_nodes, conn, e, phys, K.data(), Re.data());
+      for (int A = 0; A < T.nen; ++A)
+        for (int c = 0; c < dpn; ++c) gd[A * dpn + c] = (i64)conn[A] * dpn + c;
+      for (int a = 0; a < nld; ++a) {
+        for (int b = 0; b < nld; ++b) {
+          tr.push_back(gd[a]);
+          tc.push_back(gd[b]);
+          tv.push_back(K[a * nld + b]);
+        }
+        if (R) Rt[gd[a]] += Re[a];
+      }
+    }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    {
+      std::copy(tr.begin(), tr.end(), rows + filled);
+      std::copy(tc.begin(), tc.end(), cols + filled);
+      std::copy(tv.begin(), tv.end(), vals + filled);
+      filled += (i64)tv.size();
+      if (R)
+        for (size_t i = 0; i < Rt.size(); ++i) R[i] += Rt[i];
+    }
+  }
+  return filled;
+}
+
 // sparse_csr<T>::sparse_csr(triplets), sparse_csr.hpp:90-149: sort on full
 // (row, col, val) tuples, sum duplicates in that order, row_ptr of max_row+2.
 orc_csr *orc_csr_from_triplets(int64_t nt, const int64_t *r, const int64_t *c, const double *v) {

@@ -63,6 +63,14 @@ typedef struct {
 void orc_cg_assembly(int dim, int p, int dpn, int64_t n_nodes, const double *dof_nodes,
                      int64_t nel, const int32_t *elements, const orc_physics *phys,
                      int64_t *rows, int64_t *cols, double *vals, double *R);
+/* CGAssembly's OpenMP structure (per-thread triplets and residual, appended
+ * under a critical section) on nthreads threads, for the timed CPU baseline;
+ * rows/cols/vals and R as above (all required), triplets in append order.
+ * Returns the triplet count. */
+int64_t orc_cg_assembly_threads(int nthreads, int dim, int p, int dpn, int64_t n_nodes,
+                                const double *dof_nodes, int64_t nel, const int32_t *elements,
+                                const orc_physics *phys, int64_t *rows, int64_t *cols, double *vals,
+                                double *R);
 /* local element matrix only (nen*dpn)^2 row-major, for element e */
 void orc_element_matrix(int dim, int p, int dpn, const double *dof_nodes,
                         const int32_t *elements, int64_t e, const orc_physics *phys,

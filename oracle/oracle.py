@@ -72,6 +72,8 @@ class Oracle:
         L.orc_dofmanager_cg.argtypes = [C.c_int, C.c_int, _I64, _P, _I64, _P, _P, _P]
         L.orc_cg_assembly.argtypes = [C.c_int, C.c_int, C.c_int, _I64, _P, _I64, _P, _P, _P, _P,
                                       _P, _P]
+        L.orc_cg_assembly_threads.restype = _I64
+        L.orc_cg_assembly_threads.argtypes = [C.c_int] * 4 + [_I64, _P, _I64, _P, _P, _P, _P, _P, _P]
         L.orc_element_matrix.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _I64, _P, _P, _P]
         L.orc_structured_mesh.argtypes = [C.c_int] * 5 + [_P, _P]
         L.orc_shape_tables.argtypes = [C.c_int, C.c_int, _P, _P, _P, _P]
@@ -154,6 +156,20 @@ class Oracle:
         self.lib.orc_cg_assembly(dim, p, dpn, dof_nodes.shape[0], _ptr(dof_nodes), nel,
                                  _ptr(elements), C.byref(ph), _ptr(rows), _ptr(cols), _ptr(vals),
                                  _ptr(R))
+        return rows, cols, vals, R
+
+    def assemble_threads(self, nthreads, dim, p, dpn, dof_nodes, elements, ph):
+        """CGAssembly with its OpenMP structure on nthreads threads (triplets in
+        append order); the timed CPU baseline's element loop."""
+        nel = elements.shape[0]
+        nld = elements.shape[1] * dpn
+        T = nel * nld * nld
+        rows, cols, vals = np.zeros(T, np.int64), np.zeros(T, np.int64), np.zeros(T)
+        R = np.zeros(dof_nodes.shape[0] * dpn)
+        nt = self.lib.orc_cg_assembly_threads(nthreads, dim, p, dpn, dof_nodes.shape[0], _ptr(dof_nodes), nel,
+                                              _ptr(elements), C.byref(ph), _ptr(rows), _ptr(cols), _ptr(vals),
+                                              _ptr(R))
+        assert nt == T
         return rows, cols, vals, R
 
     # -- gradient recovery -------------------------------------------------

@@ -114,4 +114,12 @@ def test_cancel_queued_request(ctx):
             pending.discard(rid)
     assert srv.query(ids[2])[0] == H.REQ_CANCELLED
     assert srv.stats()["cancelled"] == 1
+    assert srv.stats()["queued"] == 0
+    # result reports the cancellation once, then the request is forgotten
+    with pytest.raises(H.YafelHipError) as e:
+        lib_x = np.zeros(1)
+        H.check(H.lib.yafel_hip_server_result(srv.h, ids[2], H._ptr(lib_x), 1, None), ctx.h)
+    assert e.value.code == H.ESTATE and "cancelled" in str(e.value)
+    with pytest.raises(H.YafelHipError):
+        srv.query(ids[2])
     srv.close()

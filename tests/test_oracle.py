@@ -222,6 +222,19 @@ def test_supg_element_closed_form(orc):
 
 
 # ------------------------------------------------------------ sparse build
+@pytest.mark.parametrize("nthreads", [1, 3])
+def test_threaded_assembly_equals_the_element_order_loop(orc, nthreads):
+    """The CPU baseline's assembly (CGAssembly's OpenMP structure) emits the
+    serial loop's triplets, in another order, and the same residual."""
+    el, dn = orc.mesh(3, 1, 3)
+    ph = physics(1, 3, lam=LAM, mu=MU, body=(0, 0, -1))
+    r1, c1, v1, R1 = orc.assemble(3, 1, 3, dn, el, ph)
+    r2, c2, v2, R2 = orc.assemble_threads(nthreads, 3, 1, 3, dn, el, ph)
+    k1, k2 = np.lexsort((v1, c1, r1)), np.lexsort((v2, c2, r2))
+    assert np.array_equal(r1[k1], r2[k2]) and np.array_equal(c1[k1], c2[k2]) and np.array_equal(v1[k1], v2[k2])
+    assert np.allclose(R1, R2, rtol=0, atol=1e-15)
+
+
 @pytest.mark.parametrize("tag", ["q1", "q2", "q4", "hex"])
 def test_csr_from_triplets_matches_reference(orc, golden, tag):
     s = golden("sparse_build.npz")

@@ -34,7 +34,41 @@ __global__ __launch_bounds__(NT) void k_copy(int64_t n16, const v4i *__restrict_
   for (; i < n16; i += stride) b[i] = __builtin_nontemporal_load(&a[i]);
 }
 
+// Read-only stream: 16-byte non-temporal loads folded into one word per
+// thread (written once, so the loads cannot be dropped).  The ceiling for
+// kernels that read far more than they write, such as the SpMV.
+__global__ __launch_bounds__(NT) void k_read(int64_t n16, const v4i *__restrict__ a, int *__restrict__ out) {
+  int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  v4i acc = {0, 0, 0, 0};
+  for (; i < n16; i += stride) acc ^= __builtin_nontemporal_load(&a[i]);
+  out[blockIdx.x * NT + threadIdx.x] = acc.x ^ acc.y ^ acc.z ^ acc.w;
+}
+
 }  // namespace
+
+extern "C" int yafel_hip_read_bandwidth(yafel_hip_ctx ctx, size_t bytes, int reps, double *gbs) {
+  YH_TRY_BEGIN
+  REQUIRE(ctx && gbs && reps > 0 && bytes >= 16, "read_bandwidth: bad argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  const int64_t n16 = (int64_t)(bytes / 16);
+  const int grid = ctx->num_cus * 16;
+  DBuf a, o;
+  a.alloc(n16 * 16);
+  o.alloc((size_t)grid * NT * sizeof(int));
+  HIPCHK(hipMemsetAsync(a.p, 1, n16 * 16, ctx->stream));
+  k_read<<<grid, NT, 0, ctx->stream>>>(n16, a.as<v4i>(), o.as<int>());
+  HIPCHK(hipEventRecord(ctx->t0, ctx->stream));
+  for (int r = 0; r < reps; ++r) k_read<<<grid, NT, 0, ctx->stream>>>(n16, a.as<v4i>(), o.as<int>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->t1, ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->t1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+  *gbs = (double)n16 * 16 * reps / (ms * 1e-3) / 1e9;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(ctx)
+}
 
 extern "C" int yafel_hip_copy_bandwidth(yafel_hip_ctx ctx, size_t bytes, int reps, double *gbs) {
   YH_TRY_BEGIN

@@ -78,9 +78,9 @@ struct DBuf {
 
 // Stream for the stream-ordered allocations of the calling thread (nullptr:
 // plain hipMalloc/hipFree).  With a stream set, buffers the thread releases
-// itself are kept in a per-thread pool for its next allocations; resetting
-// frees the pool.
-void set_alloc_stream(hipStream_t s);
+// itself are kept in a per-thread pool of at most max_cached bytes for its
+// next allocations; resetting frees the pool.
+void set_alloc_stream(hipStream_t s, size_t max_cached = 0);
 
 }  // namespace yh
 
@@ -119,12 +119,17 @@ struct yafel_hip_ctx_s {
   bool ktime = false;
   std::vector<hipEvent_t> kev;  // start/stop pairs of the recorded launches
   size_t kused = 0;             // events in use
+  // one word per recorded launch, set to 1 by the launch when it ran the
+  // product (a launch after convergence or a refinement pause returns at once)
+  yh::DBuf kwork;
+  static constexpr size_t kKworkCap = (size_t)1 << 16;
 };
 
 namespace yh {
 // When kernel timing is on, returns a (start, stop) event pair to record
-// around one launch on ctx->stream; nullptr pair otherwise.
-void ktime_pair(yafel_hip_ctx_s *ctx, hipEvent_t *start, hipEvent_t *stop);
+// around one launch on ctx->stream and the launch's work word (nullptr past
+// kKworkCap launches); nullptrs otherwise.
+void ktime_pair(yafel_hip_ctx_s *ctx, hipEvent_t *start, hipEvent_t *stop, unsigned **work);
 }  // namespace yh
 
 struct YhPlan;  // assembly gather plan (per mesh & format)

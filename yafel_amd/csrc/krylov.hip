@@ -53,11 +53,12 @@ SellView<T> view(yafel_hip_matrix_s *A) {
 inline int fused_cap(const yafel_hip_ctx_s *ctx) { return ctx->num_cus * 16; }
 
 template <class T, class Op>
-void launch_spmv_op(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
+void launch_spmv_op(yafel_hip_matrix_s *A, const Op &op_in, hipStream_t s) {
   const int grid = sell_grid(A->nsl, fused_cap(A->ctx));
   const SellView<T> v = view<T>(A);
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
+  Op op = op_in;
+  if constexpr (Op::kTimed) ktime_pair(A->ctx, &e0, &e1, &op.work);
   auto kern = A->format == YAFEL_HIP_CSR ? k_spmv_fused<YAFEL_HIP_CSR, kSellUnroll, T, Op>
                                          : k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op>;
   if (e0) hipExtLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, s, e0, e1, 0, v, op);
@@ -71,11 +72,12 @@ void launch_spmv_op(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
 
 // The CG iteration's SpMV: partials only, folded by the next kernel.
 template <class T, class Op>
-int launch_spmv_nofold(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
+int launch_spmv_nofold(yafel_hip_matrix_s *A, const Op &op_in, hipStream_t s) {
   const int grid = sell_grid(A->nsl, fused_cap(A->ctx));
   const SellView<T> v = view<T>(A);
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
+  Op op = op_in;
+  if constexpr (Op::kTimed) ktime_pair(A->ctx, &e0, &e1, &op.work);
   auto kern = A->format == YAFEL_HIP_CSR && A->unroll == 8 ? k_spmv_fused<YAFEL_HIP_CSR, 8, T, Op>
               : A->format == YAFEL_HIP_CSR               ? k_spmv_fused<YAFEL_HIP_CSR, kSellUnroll, T, Op>
                                                          : k_spmv_fused<YAFEL_HIP_BCSR3, kSellUnroll, T, Op>;
@@ -90,11 +92,12 @@ int launch_spmv_nofold(yafel_hip_matrix_s *A, const Op &op, hipStream_t s) {
 // The two-pass CG's SpMV (k_spmv_qp): versioned like the update passes, with
 // the SpMV grid.
 template <class T, class Op>
-int launch_qp(yafel_hip_matrix_s *A, const Op &op, int64_t n, hipStream_t s) {
+int launch_qp(yafel_hip_matrix_s *A, const Op &op_in, int64_t n, hipStream_t s) {
   const int grid = sell_grid(A->nsl, fused_cap(A->ctx));
   const SellView<T> v = view<T>(A);
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (Op::kTimed) ktime_pair(A->ctx, &e0, &e1);
+  Op op = op_in;
+  if constexpr (Op::kTimed) ktime_pair(A->ctx, &e0, &e1, &op.work);
   auto kern = A->format == YAFEL_HIP_CSR && A->unroll == 8 ? k_spmv_qp<YAFEL_HIP_CSR, 8, T, Op>
               : A->format == YAFEL_HIP_CSR               ? k_spmv_qp<YAFEL_HIP_CSR, kSellUnroll, T, Op>
                                                          : k_spmv_qp<YAFEL_HIP_BCSR3, kSellUnroll, T, Op>;

@@ -626,8 +626,11 @@ struct OpCgQ : OpBase {
   static constexpr int NP = 1;
   const T *p;
   T *q;
+  unsigned *work = nullptr;  // timed launches: set once the product runs
   __device__ bool skip() const { return !running(); }
-  __device__ void begin() {}
+  __device__ void begin() {
+    if (work && blockIdx.x == 0 && threadIdx.x == 0) *work = 1u;
+  }
   __device__ T x(int64_t c) const { return p[c]; }
   __device__ void row(int64_t i, T y, double *acc) const {
     q[i] = y;
@@ -819,6 +822,7 @@ struct OpCgQP : VOpBase {
   T *pnew;
   T alpha, beta;
   int xupd;
+  unsigned *work = nullptr;  // timed launches: set when the launch runs the product
   __device__ bool active(const KState &s) const { return running(s); }
   // flags: kGo (form p_new, q = A p_new), kXupd (x += alpha p first)
   __device__ int decide(KState &s, const double *f) const {
@@ -839,6 +843,7 @@ struct OpCgQP : VOpBase {
     alpha = (T)s.alpha;
     beta = (T)s.beta;
     xupd = fl & kXupd;
+    if (work && (fl & kGo) && blockIdx.x == 0 && threadIdx.x == 0) *work = 1u;
     // s is the published state: with kGo its pidx already names p_new
     const int pi = (fl & kGo) ? (s.pidx ^ 1) : s.pidx;
     pold = pi ? p1 : p0;

@@ -40,7 +40,7 @@ struct TempPool {
   hipStream_t s = nullptr;
   std::unordered_map<size_t, std::vector<void *>> free;
   size_t cached = 0;
-  static constexpr size_t kMaxCached = (size_t)4 << 30;
+  size_t max_cached = 0;  // bytes the pool may hold
   void *take(size_t sc) {
     auto it = free.find(sc);
     if (it == free.end() || it->second.empty()) return nullptr;
@@ -50,7 +50,7 @@ struct TempPool {
     return p;
   }
   bool put(void *p, size_t sc) {
-    if (cached + sc > kMaxCached) return false;
+    if (cached + sc > max_cached) return false;
     free[sc].push_back(p);
     cached += sc;
     return true;
@@ -68,13 +68,14 @@ static size_t size_class(size_t b) {
   return sc;
 }
 
-void set_alloc_stream(hipStream_t s) {
+void set_alloc_stream(hipStream_t s, size_t max_cached) {
   delete g_pool;
   g_pool = nullptr;
   g_alloc_stream = s;
   if (s) {
     g_pool = new TempPool();
     g_pool->s = s;
+    g_pool->max_cached = max_cached;
   }
 }
 
@@ -123,9 +124,12 @@ int grid_for(yafel_hip_ctx_s *ctx, int64_t work_items, int block) {
 
 namespace yh {
 
-void ktime_pair(yafel_hip_ctx_s *ctx, hipEvent_t *start, hipEvent_t *stop) {
+void ktime_pair(yafel_hip_ctx_s *ctx, hipEvent_t *start, hipEvent_t *stop, unsigned **work) {
   *start = *stop = nullptr;
+  *work = nullptr;
   if (!ctx->ktime) return;
+  const size_t idx = ctx->kused / 2;
+  if (idx < yafel_hip_ctx_s::kKworkCap) *work = ctx->kwork.as<unsigned>() + idx;
   while (ctx->kev.size() < ctx->kused + 2) {
     hipEvent_t e;
     HIPCHK(hipEventCreate(&e));
@@ -292,6 +296,12 @@ int yafel_hip_kernel_timing(yafel_hip_ctx ctx, int enable) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->ktime = enable != 0;
   ctx->kused = 0;
+  if (ctx->ktime) {
+    const size_t b = yafel_hip_ctx_s::kKworkCap * sizeof(unsigned);
+    if (ctx->kwork.bytes < b) ctx->kwork.alloc(b);
+    HIPCHK(hipMemsetAsync(ctx->kwork.p, 0, b, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
   return YAFEL_HIP_OK;
   YH_TRY_END(ctx)
 }
@@ -300,14 +310,21 @@ int yafel_hip_kernel_time(yafel_hip_ctx ctx, double *ms_total, int64_t *launches
   YH_TRY_BEGIN
   REQUIRE(ctx && ms_total && launches, "kernel_time: null argument");
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  const size_t nl = ctx->kused / 2;
+  std::vector<unsigned> work(std::min(nl, yafel_hip_ctx_s::kKworkCap));
+  if (!work.empty())
+    HIPCHK(hipMemcpy(work.data(), ctx->kwork.p, work.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
   double t = 0;
-  for (size_t i = 0; i + 1 < ctx->kused; i += 2) {
+  int64_t nw = 0;
+  for (size_t i = 0; i < work.size(); ++i) {
+    if (!work[i]) continue;  // returned at once: not a product
     float f = 0;
-    HIPCHK(hipEventElapsedTime(&f, ctx->kev[i], ctx->kev[i + 1]));
+    HIPCHK(hipEventElapsedTime(&f, ctx->kev[2 * i], ctx->kev[2 * i + 1]));
     t += f;
+    ++nw;
   }
   *ms_total = t;
-  *launches = (int64_t)(ctx->kused / 2);
+  *launches = nw;
   return YAFEL_HIP_OK;
   YH_TRY_END(ctx)
 }

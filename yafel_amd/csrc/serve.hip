@@ -224,6 +224,9 @@ struct Req {
   std::vector<double> x;
   yafel_hip_solve_info info{};
   std::string error;
+  // host time stamps: submitted, taken by a setup worker, set up, joined the
+  // batch (its first chunk launched), found finished
+  std::chrono::steady_clock::time_point t_submit, t_admit, t_ready, t_join, t_done;
 };
 
 struct Slot {
@@ -234,6 +237,7 @@ struct Slot {
   size_t buf_bytes = 0;
   SlotDesc d{};
   int64_t ndofs = 0;
+  int64_t last_chunk = 0;  // sequence number of the last chunk whose table lists the slot
   bool joined = false, retired = false;
 };
 
@@ -255,7 +259,12 @@ struct yafel_hip_server_s {
   std::map<int64_t, std::unique_ptr<Req>> reqs;
   std::deque<Req *> queue;
   std::vector<Slot *> active;
-  std::vector<Slot *> grave;  // retired, buffers freed stream-ordered
+  // Retired slots whose matrix, load and mask may still be read by a chunk in
+  // flight: a slot found finished when chunk N is drained was already listed
+  // in chunk N + 1.  They are freed once the last chunk listing them is drained.
+  std::vector<Slot *> grave;
+  int64_t chunk_seq = 0;    // chunks launched
+  int64_t drained_seq = 0;  // the last chunk drained (chunks drain in launch order)
   yafel_hip_server_stats stats{};
   // descriptor / status rings: each progress call uses at most two entries and
   // at most two calls' work is in flight, so an entry is never rewritten while
@@ -266,6 +275,7 @@ struct yafel_hip_server_s {
   int ring = 0;
   struct Inflight {
     hipEvent_t ev;
+    int64_t seq;
     int ring;
     std::vector<Slot *> slots;
     char *pin;
@@ -429,11 +439,19 @@ void join_slot(yafel_hip_server_s *S, Slot *sl) {
   HIPCHK(hipMemsetAsync(d.x, 0, n * 8, s));
 }
 
-void release_slot(yafel_hip_server_s *S, Slot *sl) {
+// The slot's solve vectors and state: freed on the solve stream, so the free is
+// ordered after every chunk already launched with the slot in its table.
+void release_solve_buffers(yafel_hip_server_s *S, Slot *sl) {
   if (sl->buf) (void)hipFreeAsync(sl->buf, S->ctx->stream);
   sl->buf = nullptr;
-  // the solve stream is done with the slot (its last chunk was drained) and
-  // the setup stream finished building it: free stream-ordered, no sync
+}
+
+// The slot's matrix, load and mask were allocated on a setup stream, which is
+// not ordered after the solve stream.  Called only when no chunk that lists
+// the slot can still run on the device (its last chunk was drained on the
+// host, or the solve stream was synchronized), so freeing them on the setup
+// stream cannot race a read.  Returns the slot's bytes to the HBM budget.
+void release_matrix(yafel_hip_server_s *S, Slot *sl) {
   delete sl->A;
   sl->A = nullptr;
   sl->b.release();
@@ -450,7 +468,10 @@ void release_slot(yafel_hip_server_s *S, Slot *sl) {
 // builds on its own setup context.
 void worker_main(yafel_hip_server_s *S, yafel_hip_ctx_s *c) {
   (void)hipSetDevice(S->ctx->device);
-  set_alloc_stream(c->stream);
+  // the workers' caches of released set-up buffers are not charged to the
+  // budget, so together they are held to an eighth of it (4 GiB per worker at most)
+  const size_t cache = std::min<size_t>((size_t)4 << 30, S->budget / (8 * std::max<size_t>(1, S->sctx.size())));
+  set_alloc_stream(c->stream, cache);
   std::unique_lock<std::mutex> lk(S->mu);
   while (true) {
     S->cv.wait(lk, [S] {
@@ -462,6 +483,7 @@ void worker_main(yafel_hip_server_s *S, yafel_hip_ctx_s *c) {
     Req *r = S->queue.front();
     S->queue.pop_front();
     r->state = YAFEL_HIP_REQ_RUNNING;
+    r->t_admit = std::chrono::steady_clock::now();
     S->in_use += r->est_bytes;
     S->peak = std::max(S->peak, S->in_use);
     S->held += 1;
@@ -481,6 +503,7 @@ void worker_main(yafel_hip_server_s *S, yafel_hip_ctx_s *c) {
     S->stats.setup_ms += ms;
     if (sl) {
       r->ndofs = sl->ndofs;
+      r->t_ready = std::chrono::steady_clock::now();
       S->ready.push_back(sl);
       S->stats.admitted++;
       S->cv_ready.notify_one();
@@ -544,6 +567,7 @@ void drain_one(yafel_hip_server_s *S, std::vector<int64_t> &done_ids) {
   auto f = S->inflight.front();
   S->inflight.pop_front();
   HIPCHK(hipEventSynchronize(f.ev));
+  S->drained_seq = f.seq;
   const SlotStatus *st = (const SlotStatus *)f.pin;
   for (size_t i = 0; i < f.slots.size(); ++i) {
     Slot *sl = f.slots[i];
@@ -561,6 +585,7 @@ void drain_one(yafel_hip_server_s *S, std::vector<int64_t> &done_ids) {
     {
       std::lock_guard<std::mutex> lk(S->mu);
       r->state = YAFEL_HIP_REQ_DONE;
+      r->t_done = std::chrono::steady_clock::now();
       S->stats.completed++;
     }
     done_ids.push_back(r->id);
@@ -620,18 +645,16 @@ int yafel_hip_server_destroy(yafel_hip_server S) {
   S->cv.notify_all();
   for (auto &w : S->workers)
     if (w.joinable()) w.join();
+  // nothing that lists a slot runs after this
   (void)hipStreamSynchronize(S->ctx->stream);
   for (Slot *sl : S->ready) S->joining.push_back(sl);
   S->ready.clear();
-  for (Slot *sl : S->active) {
-    release_slot(S, sl);
-    delete sl;
-  }
-  for (Slot *sl : S->joining) {
-    release_slot(S, sl);
-    delete sl;
-  }
-  for (Slot *sl : S->grave) delete sl;
+  for (auto *v : {&S->active, &S->joining, &S->grave})
+    for (Slot *sl : *v) {
+      release_solve_buffers(S, sl);
+      release_matrix(S, sl);
+      delete sl;
+    }
   (void)hipStreamSynchronize(S->ctx->stream);
   for (auto &e : S->evs) (void)hipEventDestroy(e);
   if (S->pin) (void)hipHostFree(S->pin);
@@ -667,6 +690,7 @@ int yafel_hip_server_submit(yafel_hip_server S, const yafel_hip_request *q, int6
   }
   r->phys.coef = nullptr;
   r->est_bytes = estimate_bytes(*r);
+  r->t_submit = std::chrono::steady_clock::now();
   if (r->est_bytes > S->budget) {
     std::lock_guard<std::mutex> lk(S->mu);
     S->stats.failed++;
@@ -710,11 +734,12 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
   }
   // 2. drain the oldest chunk when two are in flight
   while (S->inflight.size() >= 2) drain_one(S, done);
-  // 3. retire finished slots (their x is on the host); buffers freed stream-ordered
+  // 3. retire finished slots (their x is on the host): the solve buffers are
+  //    freed on the solve stream now, the matrix once its last chunk drains
   std::vector<Slot *> keep;
   for (Slot *sl : S->active) {
     if (sl->retired) {
-      release_slot(S, sl);
+      release_solve_buffers(S, sl);
       S->grave.push_back(sl);
     } else {
       keep.push_back(sl);
@@ -730,7 +755,11 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
                                                                      (int)S->joining.size(), (int)(S->ver & 1));
     HIPCHK(hipGetLastError());
     launch_phase<PH_INIT>(S, e, (int)S->joining.size(), ts);
-    for (Slot *sl : S->joining) S->active.push_back(sl);
+    const auto now = std::chrono::steady_clock::now();
+    for (Slot *sl : S->joining) {
+      sl->req->t_join = now;
+      S->active.push_back(sl);
+    }
     S->joining.clear();
   }
   // 5. one chunk for every active slot
@@ -752,6 +781,8 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
     HIPCHK(hipGetLastError());
     yafel_hip_server_s::Inflight f;
     f.ev = S->evs[e];
+    f.seq = ++S->chunk_seq;
+    for (Slot *sl : S->active) sl->last_chunk = f.seq;
     f.ring = e;
     f.slots = S->active;
     f.pin = S->pin + e * yafel_hip_server_s::STAGE;
@@ -762,10 +793,19 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
   } else {
     while (!S->inflight.empty()) drain_one(S, done);
   }
-  // graveyard entries are safe to delete once nothing in flight references them
-  if (S->inflight.empty()) {
-    for (Slot *sl : S->grave) delete sl;
-    S->grave.clear();
+  // retired slots whose last chunk has drained: nothing on the device reads
+  // their matrix any more
+  {
+    std::vector<Slot *> wait;
+    for (Slot *sl : S->grave) {
+      if (sl->last_chunk <= S->drained_seq) {
+        release_matrix(S, sl);
+        delete sl;
+      } else {
+        wait.push_back(sl);
+      }
+    }
+    S->grave.swap(wait);
   }
   {
     std::lock_guard<std::mutex> lk(S->mu);
@@ -802,12 +842,41 @@ int yafel_hip_server_result(yafel_hip_server S, int64_t id, double *x, int64_t n
   auto it = S->reqs.find(id);
   REQUIRE(it != S->reqs.end(), "server_result: unknown request id");
   Req *r = it->second.get();
-  if (r->state == YAFEL_HIP_REQ_FAILED) fail(YAFEL_HIP_ESTATE, "server_result: request failed: " + r->error);
+  // a failed or cancelled request is reported once and then forgotten, as a
+  // finished one is once its result has been read
+  if (r->state == YAFEL_HIP_REQ_FAILED) {
+    const std::string msg = "server_result: request failed: " + r->error;
+    S->reqs.erase(it);
+    fail(YAFEL_HIP_ESTATE, msg);
+  }
+  if (r->state == YAFEL_HIP_REQ_CANCELLED) {
+    S->reqs.erase(it);
+    fail(YAFEL_HIP_ESTATE, "server_result: request was cancelled");
+  }
   REQUIRE(r->state == YAFEL_HIP_REQ_DONE, "server_result: request not finished");
   REQUIRE(n >= (int64_t)r->x.size(), "server_result: output too short");
   if (x) std::memcpy(x, r->x.data(), r->x.size() * 8);
   if (info) *info = r->info;
   S->reqs.erase(it);
+  return YAFEL_HIP_OK;
+  YH_TRY_END(S ? S->ctx : nullptr)
+}
+
+int yafel_hip_server_times(yafel_hip_server S, int64_t id, double *ms) {
+  YH_TRY_BEGIN
+  REQUIRE(S && ms, "server_times: null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  REQUIRE(it != S->reqs.end(), "server_times: unknown request id");
+  const Req *r = it->second.get();
+  REQUIRE(r->state == YAFEL_HIP_REQ_DONE, "server_times: request not finished");
+  auto d = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
+  ms[0] = d(r->t_submit, r->t_admit);
+  ms[1] = d(r->t_admit, r->t_ready);
+  ms[2] = d(r->t_ready, r->t_join);
+  ms[3] = d(r->t_join, r->t_done);
   return YAFEL_HIP_OK;
   YH_TRY_END(S ? S->ctx : nullptr)
 }
@@ -822,6 +891,14 @@ int yafel_hip_server_cancel(yafel_hip_server S, int64_t id) {
   if (r->state == YAFEL_HIP_REQ_QUEUED) {
     r->state = YAFEL_HIP_REQ_CANCELLED;
     S->stats.cancelled++;
+    // out of the queue, and its mesh and load freed; the small record stays
+    // for server_query until server_result reports the cancellation
+    S->queue.erase(std::remove(S->queue.begin(), S->queue.end(), r), S->queue.end());
+    S->stats.queued = (int64_t)S->queue.size();
+    std::vector<double>().swap(r->nodes);
+    std::vector<double>().swap(r->coef);
+    std::vector<double>().swap(r->load);
+    std::vector<int32_t>().swap(r->cells);
   } else {
     fail(YAFEL_HIP_ESTATE, "server_cancel: only queued requests can be cancelled");
   }

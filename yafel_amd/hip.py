@@ -95,6 +95,7 @@ for _n, _a in [
     ("yafel_hip_spmv", [_P, _I, _P, _P]), ("yafel_hip_spmv_host", [_P, _P, _P]),
     ("yafel_hip_spmm", [_P, _I, _I, _P, _P]), ("yafel_hip_dot", [_P, _I64, _P, _P, C.POINTER(_D)]),
     ("yafel_hip_copy_bandwidth", [_P, C.c_size_t, _I, C.POINTER(_D)]),
+    ("yafel_hip_read_bandwidth", [_P, C.c_size_t, _I, C.POINTER(_D)]),
     ("yafel_hip_kernel_timing", [_P, _I]), ("yafel_hip_kernel_time", [_P, C.POINTER(_D), C.POINTER(_I64)]),
     ("yafel_hip_solve_opts_default", [C.POINTER(SolveOpts)]),
     ("yafel_hip_solve", [_P, C.POINTER(SolveOpts), _P, _P, C.POINTER(SolveInfo)]),
@@ -128,6 +129,7 @@ for _n, _a in [
     ("yafel_hip_server_query", [_P, _I64, C.POINTER(_I), C.POINTER(_I64)]),
     ("yafel_hip_server_result", [_P, _I64, _P, _I64, C.POINTER(SolveInfo)]),
     ("yafel_hip_server_cancel", [_P, _I64]),
+    ("yafel_hip_server_times", [_P, _I64, _P]),
     ("yafel_hip_server_stats_get", [_P, C.POINTER(ServerStats)]),
 ]:
     _sig(_n, _I, _a)
@@ -192,7 +194,8 @@ class Context:
         check(lib.yafel_hip_kernel_timing(self.h, 1 if enable else 0), self.h)
 
     def kernel_time(self):
-        """(summed ms, launches) of the CG kernel-A launches since timing was enabled."""
+        """(summed ms, launches) of the CG kernel-A launches that ran the product
+        since timing was enabled (launches that returned at once are left out)."""
         ms, n = C.c_double(), _I64()
         check(lib.yafel_hip_kernel_time(self.h, C.byref(ms), C.byref(n)), self.h)
         return ms.value, n.value
@@ -200,6 +203,11 @@ class Context:
     def copy_bandwidth(self, nbytes=1 << 30, reps=20):
         g = _D()
         check(lib.yafel_hip_copy_bandwidth(self.h, nbytes, reps, C.byref(g)), self.h)
+        return g.value
+
+    def read_bandwidth(self, nbytes=1 << 30, reps=20):
+        g = _D()
+        check(lib.yafel_hip_read_bandwidth(self.h, nbytes, reps, C.byref(g)), self.h)
         return g.value
 
     def dot(self, a, b):
@@ -665,6 +673,12 @@ class Server:
         info = SolveInfo()
         check(lib.yafel_hip_server_result(self.h, rid, _ptr(x), x.size, C.byref(info)), self.ctx.h)
         return x[:nd], info
+
+    def times(self, rid):
+        """(admission wait, setup, wait to join, solve) in ms of a finished request."""
+        ms = (C.c_double * 4)()
+        check(lib.yafel_hip_server_times(self.h, rid, ms), self.ctx.h)
+        return tuple(ms)
 
     def cancel(self, rid):
         check(lib.yafel_hip_server_cancel(self.h, rid), self.ctx.h)
