@@ -121,6 +121,13 @@ int yafel_hip_matrix_set_unroll(yafel_hip_matrix A, int unroll);
  * on CSR up to 1.5 Mi, else 3.  A tuning knob: the iterates are
  * bit-identical. */
 int yafel_hip_matrix_set_cg_passes(yafel_hip_matrix A, int passes);
+/* bytes of each of the matrix's streams (the values of one precision with
+ * their column indices) that the SpMV kernels read with the caching policy;
+ * the rest is read non-temporally and does not allocate in the 256 MB
+ * Infinity Cache.  -1 (the default) is 224 MiB: the leading part of a stream
+ * larger than the cache stays resident across the products of a solve.  A
+ * tuning knob: results are identical for every value. */
+int yafel_hip_matrix_set_cache_bytes(yafel_hip_matrix A, int64_t bytes);
 /* No reference counterpart: bytes per stored column index of the SpMV
  * streams, 2 when every entry's column is within +-32767 of its stored row
  * position (16-bit offsets, a banded mesh numbering) and 4 otherwise. */

@@ -163,6 +163,7 @@ struct yafel_hip_matrix_s {
   void *mesh = nullptr;         // mesh this pattern came from (assembly)
   YhPlan *plan = nullptr;
   int unroll = 4;               // slots per step of the plain SpMV kernel
+  int64_t cache_bytes = -1;     // bytes of each value precision's stream read with the caching policy (-1: default)
   int cg_passes = 0;            // kernels per single-solve CG iteration (2, 3; 0: by size)
   // reference-layout extras: the host row_ptr may be longer than brows+1
   std::vector<int64_t> ptr_tail;
@@ -170,6 +171,20 @@ struct yafel_hip_matrix_s {
 };
 
 namespace yh {
+
+// Default share of a matrix stream (values and columns of one precision)
+// read with the caching policy, so that it stays in the 256 MB Infinity Cache
+// across the products of a solve while the vectors keep room beside it.
+constexpr int64_t kCacheBytesDefault = (int64_t)224 << 20;
+
+// Slots of A's sliced streams for values of vb bytes that fall within
+// `budget` bytes (the matrix's own budget when budget < 0).
+inline int keep_slots(const yafel_hip_matrix_s *A, int vb, int64_t budget = -1) {
+  if (budget < 0) budget = A->cache_bytes < 0 ? kCacheBytesDefault : A->cache_bytes;
+  const int64_t per = 64LL * ((A->format == YAFEL_HIP_BCSR3 ? 9 : 1) * vb + (A->col16.p ? 2 : 4));
+  const int64_t slots = A->ntot / 64, k = budget / per;
+  return (int)(k < slots ? k : slots);
+}
 
 // ------------------------------------------------------- launch helpers
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }

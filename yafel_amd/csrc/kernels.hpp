@@ -17,11 +17,36 @@ struct SellView {
   int64_t brows;     // stored rows (block rows for BCSR3)
   int64_t ntot;      // sliced entries per value array
   const short *col16;  // column - stored position, 16-bit (nullptr: use col)
+  int keep;            // slices starting below this slot read with the caching policy
   __device__ __forceinline__ int64_t nsl() const { return (brows + 63) >> 6; }
   __device__ __forceinline__ int64_t row_of(int64_t i) const { return perm ? (int64_t)perm[i] : i; }
 };
 
 namespace dev {
+
+// A load the compiler issues where it is written: a relaxed wavefront-scope
+// atomic load is a plain global load on the device, but an ordered memory
+// access is not sunk past a later branch (a load issued before a test keeps
+// its latency overlapped with the test's own load).
+template <class T>
+__device__ __forceinline__ T load_in_place(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// Matrix streams.  Past a byte budget (SellView::keep, in slots) they are
+// loaded non-temporally: such loads do not allocate in the Infinity Cache,
+// so a stream larger than the cache does not evict itself and everything else
+// on every pass.  The leading slots within the budget are loaded with the
+// default policy and stay cache-resident across the products of a solve.
+template <bool KEEP, class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+  if constexpr (KEEP) return *p;
+  else return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ bool keep_slice(const SellView<T> &A, int p0) {
+  return p0 < A.keep;
+}
 
 // Sliced position of entry t of row r (iperm may be nullptr: identity).
 __device__ __forceinline__ int64_t sell_pos(const int *__restrict__ sptr, const int *__restrict__ iperm,
@@ -42,9 +67,9 @@ __device__ __forceinline__ int64_t sell_pos(const int *__restrict__ sptr, const 
 // next step's columns and values, and only then does the lane wait for the
 // gathers, so the gathers' cache latency overlaps the next step's HBM loads
 // (vector loads complete in issue order, so the order of issue matters).
-template <int U, class T, class TV, class XF, class CF>
-__device__ __forceinline__ TV sell_row_csr_cols(const SellView<T> &A, int64_t s, int lane, XF xf, CF cf) {
-  const int p0 = A.sptr[s], p1 = A.sptr[s + 1];
+// p0, p1: the slice's slot range (sptr[s], sptr[s + 1]).
+template <int U, class T, class TV, bool KEEP, class XF, class CF>
+__device__ __forceinline__ TV sell_row_csr_cols(const SellView<T> &A, int p0, int p1, int lane, XF xf, CF cf) {
   const int64_t base = (int64_t)p0 * 64 + lane;
   const int len = p1 - p0;
   TV acc = 0;
@@ -56,7 +81,7 @@ __device__ __forceinline__ TV sell_row_csr_cols(const SellView<T> &A, int64_t s,
     for (int u = 0; u < U; ++u) {
       const int64_t k = base + (int64_t)u * 64;
       c[u] = cf(k);
-      v[u] = __builtin_nontemporal_load(&A.val[k]);
+      v[u] = ld_stream<KEEP>(&A.val[k]);
     }
     for (;;) {
       TV xg[U];
@@ -69,7 +94,7 @@ __device__ __forceinline__ TV sell_row_csr_cols(const SellView<T> &A, int64_t s,
         for (int u = 0; u < U; ++u) {
           const int64_t k = base + (int64_t)(t + U + u) * 64;
           c[u] = cf(k);
-          vn[u] = __builtin_nontemporal_load(&A.val[k]);
+          vn[u] = ld_stream<KEEP>(&A.val[k]);
         }
       }
 #pragma unroll
@@ -82,29 +107,40 @@ __device__ __forceinline__ TV sell_row_csr_cols(const SellView<T> &A, int64_t s,
   }
   for (; t < len; ++t) {
     const int64_t k = base + (int64_t)t * 64;
-    acc = fma((TV)__builtin_nontemporal_load(&A.val[k]), xf(cf(k)), acc);
+    acc = fma((TV)ld_stream<KEEP>(&A.val[k]), xf(cf(k)), acc);
   }
   return acc;
 }
 
-template <int U, class T, class TV, class XF>
-__device__ __forceinline__ TV sell_row_csr(const SellView<T> &A, int64_t s, int lane, XF xf) {
+// Row sum of slice s's row at `lane`, its slot range already loaded.
+template <int U, class T, class TV, bool KEEP, class XF>
+__device__ __forceinline__ TV sell_row_csr_keep(const SellView<T> &A, int64_t s, int p0, int p1, int lane, XF xf) {
   if (A.col16) {
     const int i = (int)(s * 64 + lane);
-    return sell_row_csr_cols<U, T, TV>(A, s, lane, xf,
-                                       [&](int64_t k) { return i + (int)__builtin_nontemporal_load(&A.col16[k]); });
+    return sell_row_csr_cols<U, T, TV, KEEP>(A, p0, p1, lane, xf,
+                                             [&](int64_t k) { return i + (int)ld_stream<KEEP>(&A.col16[k]); });
   }
-  return sell_row_csr_cols<U, T, TV>(A, s, lane, xf,
-                                     [&](int64_t k) { return __builtin_nontemporal_load(&A.col[k]); });
+  return sell_row_csr_cols<U, T, TV, KEEP>(A, p0, p1, lane, xf,
+                                           [&](int64_t k) { return ld_stream<KEEP>(&A.col[k]); });
+}
+
+template <int U, class T, class TV, class XF>
+__device__ __forceinline__ TV sell_row_csr_at(const SellView<T> &A, int64_t s, int p0, int p1, int lane, XF xf) {
+  if (keep_slice(A, p0)) return sell_row_csr_keep<U, T, TV, true>(A, s, p0, p1, lane, xf);
+  return sell_row_csr_keep<U, T, TV, false>(A, s, p0, p1, lane, xf);
+}
+
+template <int U, class T, class TV, class XF>
+__device__ __forceinline__ TV sell_row_csr(const SellView<T> &A, int64_t s, int lane, XF xf) {
+  return sell_row_csr_at<U, T, TV>(A, s, A.sptr[s], A.sptr[s + 1], lane, xf);
 }
 
 // One 3x3-block row per lane; a[0..2] receive the three row sums.  Blocks are
 // summed in order, entries of a block row-wise (j = 0, 1, 2).  xf(i) returns
 // x[i] for a scalar index; cf(k) is the block column of entry k.
-template <int U, class T, class TV, class XF, class CF>
-__device__ __forceinline__ void sell_row_bcsr_cols(const SellView<T> &A, int64_t s, int lane, TV a[3], XF xf,
+template <int U, class T, class TV, bool KEEP, class XF, class CF>
+__device__ __forceinline__ void sell_row_bcsr_cols(const SellView<T> &A, int p0, int p1, int lane, TV a[3], XF xf,
                                                    CF cf) {
-  const int p0 = A.sptr[s], p1 = A.sptr[s + 1];
   const int64_t base = (int64_t)p0 * 64 + lane;
   const int len = p1 - p0;
   const int64_t nt = A.ntot;
@@ -118,7 +154,7 @@ __device__ __forceinline__ void sell_row_bcsr_cols(const SellView<T> &A, int64_t
       const int64_t k = base + (int64_t)(t + u) * 64;
       c[u] = 3 * (int64_t)cf(k);
 #pragma unroll
-      for (int e = 0; e < 9; ++e) v[u][e] = __builtin_nontemporal_load(&A.val[e * nt + k]);
+      for (int e = 0; e < 9; ++e) v[u][e] = ld_stream<KEEP>(&A.val[e * nt + k]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -132,31 +168,44 @@ __device__ __forceinline__ void sell_row_bcsr_cols(const SellView<T> &A, int64_t
     const int64_t k = base + (int64_t)t * 64;
     const int64_t c = 3 * (int64_t)cf(k);
     const TV x0 = xf(c), x1 = xf(c + 1), x2 = xf(c + 2);
-    a0 = fma((TV)__builtin_nontemporal_load(&A.val[0 * nt + k]), x0, a0);
-    a0 = fma((TV)__builtin_nontemporal_load(&A.val[1 * nt + k]), x1, a0);
-    a0 = fma((TV)__builtin_nontemporal_load(&A.val[2 * nt + k]), x2, a0);
-    a1 = fma((TV)__builtin_nontemporal_load(&A.val[3 * nt + k]), x0, a1);
-    a1 = fma((TV)__builtin_nontemporal_load(&A.val[4 * nt + k]), x1, a1);
-    a1 = fma((TV)__builtin_nontemporal_load(&A.val[5 * nt + k]), x2, a1);
-    a2 = fma((TV)__builtin_nontemporal_load(&A.val[6 * nt + k]), x0, a2);
-    a2 = fma((TV)__builtin_nontemporal_load(&A.val[7 * nt + k]), x1, a2);
-    a2 = fma((TV)__builtin_nontemporal_load(&A.val[8 * nt + k]), x2, a2);
+    a0 = fma((TV)ld_stream<KEEP>(&A.val[0 * nt + k]), x0, a0);
+    a0 = fma((TV)ld_stream<KEEP>(&A.val[1 * nt + k]), x1, a0);
+    a0 = fma((TV)ld_stream<KEEP>(&A.val[2 * nt + k]), x2, a0);
+    a1 = fma((TV)ld_stream<KEEP>(&A.val[3 * nt + k]), x0, a1);
+    a1 = fma((TV)ld_stream<KEEP>(&A.val[4 * nt + k]), x1, a1);
+    a1 = fma((TV)ld_stream<KEEP>(&A.val[5 * nt + k]), x2, a1);
+    a2 = fma((TV)ld_stream<KEEP>(&A.val[6 * nt + k]), x0, a2);
+    a2 = fma((TV)ld_stream<KEEP>(&A.val[7 * nt + k]), x1, a2);
+    a2 = fma((TV)ld_stream<KEEP>(&A.val[8 * nt + k]), x2, a2);
   }
   a[0] = a0;
   a[1] = a1;
   a[2] = a2;
 }
 
-template <int U, class T, class TV, class XF>
-__device__ __forceinline__ void sell_row_bcsr(const SellView<T> &A, int64_t s, int lane, TV a[3], XF xf) {
+template <int U, class T, class TV, bool KEEP, class XF>
+__device__ __forceinline__ void sell_row_bcsr_keep(const SellView<T> &A, int64_t s, int p0, int p1, int lane,
+                                                   TV a[3], XF xf) {
   if (A.col16) {
     const int i = (int)(s * 64 + lane);
-    sell_row_bcsr_cols<U, T, TV>(A, s, lane, a, xf,
-                                 [&](int64_t k) { return i + (int)__builtin_nontemporal_load(&A.col16[k]); });
+    sell_row_bcsr_cols<U, T, TV, KEEP>(A, p0, p1, lane, a, xf,
+                                       [&](int64_t k) { return i + (int)ld_stream<KEEP>(&A.col16[k]); });
   } else {
-    sell_row_bcsr_cols<U, T, TV>(A, s, lane, a, xf,
-                                 [&](int64_t k) { return __builtin_nontemporal_load(&A.col[k]); });
+    sell_row_bcsr_cols<U, T, TV, KEEP>(A, p0, p1, lane, a, xf,
+                                       [&](int64_t k) { return ld_stream<KEEP>(&A.col[k]); });
   }
+}
+
+template <int U, class T, class TV, class XF>
+__device__ __forceinline__ void sell_row_bcsr_at(const SellView<T> &A, int64_t s, int p0, int p1, int lane, TV a[3],
+                                                 XF xf) {
+  if (keep_slice(A, p0)) sell_row_bcsr_keep<U, T, TV, true>(A, s, p0, p1, lane, a, xf);
+  else sell_row_bcsr_keep<U, T, TV, false>(A, s, p0, p1, lane, a, xf);
+}
+
+template <int U, class T, class TV, class XF>
+__device__ __forceinline__ void sell_row_bcsr(const SellView<T> &A, int64_t s, int lane, TV a[3], XF xf) {
+  sell_row_bcsr_at<U, T, TV>(A, s, A.sptr[s], A.sptr[s + 1], lane, a, xf);
 }
 
 // Value of `v` in lane l (l wave-uniform), for 32- and 64-bit types.

@@ -43,7 +43,7 @@ template <class T>
 SellView<T> view(yafel_hip_matrix_s *A) {
   return SellView<T>{A->sptr.as<int>(), A->perm.p ? A->perm.as<int>() : nullptr, A->col.as<int>(),
                      (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->brows, A->ntot,
-                     A->col16.p ? A->col16.as<short>() : nullptr};
+                     A->col16.p ? A->col16.as<short>() : nullptr, keep_slots(A, (int)sizeof(T))};
 }
 
 // Fused SpMV grids: one slice per wavefront up to num_cus * 16 workgroups

@@ -108,39 +108,53 @@ __global__ __launch_bounds__(NTF) void k_finish(Op op0, int nblk) {
 // range; the row -> block mapping depends on nothing else, so a solve's
 // results do not depend on what runs beside it.
 // T is the stored matrix type, Op::value_type the vector type.
+// The wave's first slice's slot range is loaded before the state test, so
+// that its latency overlaps the state read instead of following it.
 template <int FMT, int U, class T, class Op>
 __device__ __forceinline__ void spmv_fused_body(const SellView<T> &A, Op &op, int blk, int nblk) {
   using TV = typename Op::value_type;
+  constexpr int RPL = FMT == YAFEL_HIP_CSR ? 1 : 3;  // rows per lane
+  const int lane = threadIdx.x & 63;
+  const int64_t nsl = A.nsl(), nw = (int64_t)nblk * (NT / 64);
+  int64_t s = (int64_t)blk * (NT / 64) + (threadIdx.x >> 6);
+  // unconditional loads at a clamped, in-bounds slice (sptr has nsl + 1
+  // entries): no branch, so nothing waits on them before the state test
+  const int64_t sc = s < nsl ? s : (nsl > 0 ? nsl - 1 : 0);
+  int p0 = dev::load_in_place(&A.sptr[sc]), p1 = dev::load_in_place(&A.sptr[sc + (nsl > 0 ? 1 : 0)]);
   if (op.skip()) return;
   op.begin();
-  const int lane = threadIdx.x & 63;
   constexpr int NP = Op::NP;
   double acc[NP > 0 ? NP : 1];
 #pragma unroll
   for (int j = 0; j < (NP > 0 ? NP : 1); ++j) acc[j] = 0;
-  const int64_t nsl = A.nsl(), nw = (int64_t)nblk * (NT / 64);
-  for (int64_t s = (int64_t)blk * (NT / 64) + (threadIdx.x >> 6); s < nsl; s += nw) {
+  bool first = true;
+  for (; s < nsl; s += nw) {
+    if (!first) {
+      p0 = A.sptr[s];
+      p1 = A.sptr[s + 1];
+    }
     const int64_t i = s * 64 + lane;
     // row sums accumulate in FP64 whatever the storage and vector precision
-    if (FMT == YAFEL_HIP_CSR) {
-      const double y = dev::sell_row_csr<U, T, double>(A, s, lane, [&](int c) { return (double)op.x(c); });
-      if (i < A.brows) op.row(A.row_of(i), (TV)y, acc);
-    } else {
-      double a[3];
-      dev::sell_row_bcsr<U, T, double>(A, s, lane, a, [&](int64_t c) { return (double)op.x(c); });
-      if (i < A.brows) {
-        const int64_t r = A.row_of(i);
-        op.row(3 * r, (TV)a[0], acc);
-        op.row(3 * r + 1, (TV)a[1], acc);
-        op.row(3 * r + 2, (TV)a[2], acc);
-      }
+    double a[RPL];
+    if (FMT == YAFEL_HIP_CSR)
+      a[0] = dev::sell_row_csr_at<U, T, double>(A, s, p0, p1, lane, [&](int c) { return (double)op.x(c); });
+    else
+      dev::sell_row_bcsr_at<U, T, double>(A, s, p0, p1, lane, a, [&](int64_t c) { return (double)op.x(c); });
+    if (i < A.brows) {
+      const int64_t r = A.row_of(i);
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) op.row(RPL * r + k, (TV)a[k], acc);
     }
+    first = false;
   }
+  if constexpr (Op::kTimed) op.mark_work();
   finish_partials<NP>(op, acc, blk, nblk);
 }
 
+// at least four waves per SIMD (at most 128 VGPRs): a c2-sized grid of about
+// a thousand workgroups then runs in one round of residency
 template <int FMT, int U, class T, class Op>
-__global__ __launch_bounds__(NT) void k_spmv_fused(SellView<T> A, Op op0) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_spmv_fused(SellView<T> A, Op op0) {
   Op op = op0;
   spmv_fused_body<FMT, U, T, Op>(A, op, dev::xcd_block(), gridDim.x);
 }
@@ -173,7 +187,8 @@ struct OpBase {
   KState *st;
   double *part;
   unsigned int *counter;
-  __device__ bool running() const { return !st->done && !st->paused; }
+  // both flags read unconditionally (one scalar round trip, not two)
+  __device__ bool running() const { return (st->done | st->paused) == 0; }
 };
 
 // Stopping test after an iteration (FP64 path) or pause request (MIXED).
@@ -626,9 +641,10 @@ struct OpCgQ : OpBase {
   static constexpr int NP = 1;
   const T *p;
   T *q;
-  unsigned *work = nullptr;  // timed launches: set once the product runs
+  unsigned *work = nullptr;  // timed launches: set once the product ran
   __device__ bool skip() const { return !running(); }
-  __device__ void begin() {
+  __device__ void begin() {}
+  __device__ void mark_work() const {
     if (work && blockIdx.x == 0 && threadIdx.x == 0) *work = 1u;
   }
   __device__ T x(int64_t c) const { return p[c]; }

@@ -45,6 +45,8 @@ struct SlotDesc {
   int64_t n, brows, ntot;
   const int *sptr, *perm, *col;
   const short *col16;  // 16-bit column offsets (nullptr: col)
+  int keep32;          // leading slots of the FP32 stream read with the caching policy
+  int pad1;
   const float *v32;
   const double *v64;
   const float *dinv;
@@ -98,9 +100,10 @@ __device__ __forceinline__ double *upart(const SlotDesc &d) {
   return d.part + 2 * (size_t)(d.g_spmv > d.g_elem ? d.g_spmv : d.g_elem);
 }
 
+// The FP64 stream is read once per refinement: non-temporal throughout.
 template <class T, class O>
 __device__ __forceinline__ void run_spmv(const SlotDesc &d, const T *val, O &op, int blk) {
-  const SellView<T> A{d.sptr, d.perm, d.col, val, d.brows, d.ntot, d.col16};
+  const SellView<T> A{d.sptr, d.perm, d.col, val, d.brows, d.ntot, d.col16, sizeof(T) == 4 ? d.keep32 : 0};
   if (d.fmt == YAFEL_HIP_BCSR3) spmv_fused_body<YAFEL_HIP_BCSR3, kSellUnroll, T, O>(A, op, blk, d.g_spmv);
   else spmv_fused_body<YAFEL_HIP_CSR, kSellUnroll, T, O>(A, op, blk, d.g_spmv);
 }
@@ -524,8 +527,14 @@ void worker_main(yafel_hip_server_s *S, yafel_hip_ctx_s *c) {
 int upload_table(yafel_hip_server_s *S, const std::vector<Slot *> &slots, int *tot_spmv, int *tot_elem) {
   std::vector<SlotDesc> h(slots.size());
   int os = 0, oe = 0;
+  // the cache budget shared by the table's FP32 streams in proportion to their size
+  double total = 0;
+  for (Slot *sl : slots) total += (double)sl->A->ntot * (sl->A->format == YAFEL_HIP_BCSR3 ? 38 : 6);
   for (size_t i = 0; i < slots.size(); ++i) {
     h[i] = slots[i]->d;
+    const yafel_hip_matrix_s *A = slots[i]->A;
+    const double mine = (double)A->ntot * (A->format == YAFEL_HIP_BCSR3 ? 38 : 6);
+    h[i].keep32 = keep_slots(A, 4, total > 0 ? (int64_t)(kCacheBytesDefault * (mine / total)) : 0);
     h[i].off_spmv = os;
     h[i].off_elem = oe;
     os += h[i].g_spmv;

@@ -335,7 +335,7 @@ template <class T>
 SellView<T> sell_view(yafel_hip_matrix_s *A) {
   return SellView<T>{A->sptr.as<int>(), A->perm.p ? A->perm.as<int>() : nullptr, A->col.as<int>(),
                      (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->brows, A->ntot,
-                     A->col16.p ? A->col16.as<short>() : nullptr};
+                     A->col16.p ? A->col16.as<short>() : nullptr, keep_slots(A, (int)sizeof(T))};
 }
 
 // Cap on the plain SpMV grid: past one slice per wavefront the waves loop.
@@ -693,6 +693,15 @@ int yafel_hip_matrix_set_unroll(yafel_hip_matrix A, int unroll) {
   REQUIRE(A, "matrix_set_unroll: null matrix");
   REQUIRE(unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8, "matrix_set_unroll: unroll must be 1, 2, 4 or 8");
   A->unroll = unroll;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(A ? A->ctx : nullptr)
+}
+
+int yafel_hip_matrix_set_cache_bytes(yafel_hip_matrix A, int64_t bytes) {
+  YH_TRY_BEGIN
+  REQUIRE(A, "matrix_set_cache_bytes: null matrix");
+  REQUIRE(bytes >= -1, "matrix_set_cache_bytes: bytes must be -1 (default) or >= 0");
+  A->cache_bytes = bytes;
   return YAFEL_HIP_OK;
   YH_TRY_END(A ? A->ctx : nullptr)
 }

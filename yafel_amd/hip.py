@@ -92,6 +92,7 @@ for _n, _a in [
     ("yafel_hip_matrix_download", [_P, _P, _P, _P]), ("yafel_hip_matrix_sync_fp32", [_P]),
     ("yafel_hip_matrix_diagonal", [_P, _P]), ("yafel_hip_matrix_set_unroll", [_P, _I]),
     ("yafel_hip_matrix_set_cg_passes", [_P, _I]), ("yafel_hip_matrix_index_bytes", [_P, _P]),
+    ("yafel_hip_matrix_set_cache_bytes", [_P, _I64]),
     ("yafel_hip_spmv", [_P, _I, _P, _P]), ("yafel_hip_spmv_host", [_P, _P, _P]),
     ("yafel_hip_spmm", [_P, _I, _I, _P, _P]), ("yafel_hip_dot", [_P, _I64, _P, _P, C.POINTER(_D)]),
     ("yafel_hip_copy_bandwidth", [_P, C.c_size_t, _I, C.POINTER(_D)]),
@@ -318,6 +319,10 @@ class Matrix:
 
     def set_unroll(self, unroll):
         check(lib.yafel_hip_matrix_set_unroll(self.h, unroll), self.ctx.h)
+
+    def set_cache_bytes(self, nbytes):
+        """Bytes of each matrix stream read with the caching policy (-1: default)."""
+        check(lib.yafel_hip_matrix_set_cache_bytes(self.h, int(nbytes)), self.ctx.h)
 
     def index_bytes(self):
         """Bytes per column index in the SpMV streams (2: 16-bit offsets)."""
