@@ -212,6 +212,21 @@ int yafel_hip_solve_opts_default(yafel_hip_solve_opts *opts);
 /* Solve A x = b; x_dev holds x0 on entry.  b_dev/x_dev are FP64 device vectors. */
 int yafel_hip_solve(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const double *b_dev,
                     double *x_dev, yafel_hip_solve_info *info);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (a tight serving loop, a harness that forbids hipMalloc): the same
+ * solve as yafel_hip_solve, with the same kernels and the same results, whose
+ * workspace (the Krylov vectors, the partial sums and the solver state) is a
+ * file-scope device array of the library instead of memory it allocates.
+ * The array holds YAFEL_HIP_NOALLOC_MAX_ROWS rows (about 85 MiB on each GPU
+ * the library runs on, held until the process ends); a larger matrix is
+ * refused with YAFEL_HIP_EINVAL.  The call allocates nothing: the matrix's
+ * diagonal and, for the MIXED precisions, its FP32 copy must exist already
+ * (yafel_hip_matrix_diagonal and yafel_hip_matrix_sync_fp32 once beforehand;
+ * after an assembly they are refreshed in place).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+#define YAFEL_HIP_NOALLOC_MAX_ROWS 1114112
+int yafel_hip_solve_noalloc(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const double *b_dev,
+                            double *x_dev, yafel_hip_solve_info *info);
 /* host-vector form of cg_solve / pcg_solve / bicgstab_solve: x0 may be NULL (zeros) */
 int yafel_hip_solve_host(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const double *b,
                          const double *x0, double *x, yafel_hip_solve_info *info);

@@ -109,6 +109,7 @@ struct yafel_hip_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipEvent_t sev[2] = {nullptr, nullptr};  // the solvers' chunk events (created with the context)
   std::string err;
   int num_cus = 256;
   YhWorkspace ws;
@@ -126,6 +127,9 @@ struct yafel_hip_ctx_s {
 };
 
 namespace yh {
+// Looks up, once per device and with that device current, the device address
+// of the allocation-free solve's scratch (yafel_hip_solve_noalloc).
+void noalloc_scratch_init(int device);
 // When kernel timing is on, returns a (start, stop) event pair to record
 // around one launch on ctx->stream and the launch's work word (nullptr past
 // kKworkCap launches); nullptrs otherwise.

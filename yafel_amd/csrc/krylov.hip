@@ -24,6 +24,8 @@
 // formed with the FP64 matrix (a residual replacement that keeps the search
 // direction), and the FP64 true residual decides convergence.
 #include <chrono>
+#include <mutex>
+#include <string>
 #include <cmath>
 #include <hip/hip_ext.h>
 
@@ -160,6 +162,7 @@ struct Solver {
   KState *st;        // two versions: st[0], st[1]
   int64_t ver = 0;   // st[ver & 1] is current in stream order
   double *part;
+  void *wv[10];      // ten n-vector slots of 8 bytes an element (FP32 vectors use half)
   const double *b;
   double *x;
   hipStream_t s;
@@ -351,12 +354,9 @@ struct Solver {
 
   template <class TM, class T>
   void run(yafel_hip_solve_info *info) {
-    YhWorkspace &w = ctx->ws;
-    const size_t es = sizeof(T);
-    // inner vectors carved from the FP64 workspace (FP32 uses half of each)
-    auto vec = [&](int i) { return w.vec64[i].p; };
+    // inner vectors in the workspace slots (FP32 uses half of each)
+    auto vec = [&](int i) { return wv[i]; };
     Vecs v{vec(0), prec ? vec(1) : vec(0), vec(2), vec(3), vec(4), vec(5), vec(6), vec(7), vec(8), vec(9), vec(1), vec(3)};
-    (void)es;
     if (!cgfam) {
       // BiCGSTAB: v.z unused; ph uses vec(3) (p1 slot), q unused
       v.z = vec(0);
@@ -388,9 +388,7 @@ struct Solver {
     char *pin = (char *)ctx->pinned;
     const size_t slot = 256;
     static_assert(sizeof(KState) <= slot && 2 * sizeof(KState) <= 4096, "KState slots");
-    hipEvent_t ev[2];
-    HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    hipEvent_t *ev = ctx->sev;
     auto chunk = [&](int slot_i) {
       for (int k = 0; k < K; ++k) {
         if (cgfam && two_pass) {
@@ -424,8 +422,6 @@ struct Solver {
         fail(YAFEL_HIP_EDEVICE, "solver: exceeded 900 s");
     }
     HIPCHK(hipStreamSynchronize(s));
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
     KState fs;
     HIPCHK(hipMemcpy(&fs, cur(), sizeof(KState), hipMemcpyDeviceToHost));
     if (fs.special == 1) HIPCHK(hipMemcpyAsync(x, b, n * 8, hipMemcpyDeviceToDevice, s));
@@ -464,25 +460,40 @@ int yafel_hip_solve_opts_default(yafel_hip_solve_opts *o) {
   return YAFEL_HIP_OK;
 }
 
-int yafel_hip_solve(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const double *b_dev,
-                    double *x_dev, yafel_hip_solve_info *info) {
-  YH_TRY_BEGIN
-  REQUIRE(A && opts && info, "solve: null argument");
-  REQUIRE(A->rows == A->cols, "solve: matrix must be square");
-  REQUIRE(A->rows == 0 || (b_dev && x_dev), "solve: null vector");
-  REQUIRE(opts->method >= 0 && opts->method <= 3, "solve: unknown method");
+}  // extern "C"
+
+namespace {
+
+// Workspace of one solve: ten n-vector slots of 8-byte elements, the
+// partials area (8 doubles per fused-SpMV workgroup plus 4 KiB) and two
+// solver states (4 KiB).
+struct SolveWs {
+  void *vec[10];
+  double *part;
+  KState *st;
+};
+
+size_t partials_bytes(const yafel_hip_ctx_s *ctx) { return (size_t)fused_cap(ctx) * 8 * sizeof(double) + 4096; }
+
+// Checks the arguments and returns the solver, or nullptr when there is
+// nothing to solve (n == 0).
+bool solve_setup(Solver &S, yafel_hip_matrix_s *A, const yafel_hip_solve_opts *opts, const double *b_dev,
+                 double *x_dev, yafel_hip_solve_info *info, const char *who) {
+  const std::string w(who);
+  REQUIRE(A && opts && info, w + ": null argument");
+  REQUIRE(A->rows == A->cols, w + ": matrix must be square");
+  REQUIRE(A->rows == 0 || (b_dev && x_dev), w + ": null vector");
+  REQUIRE(opts->method >= 0 && opts->method <= 3, w + ": unknown method");
   REQUIRE(opts->precision == YAFEL_HIP_FP64 || opts->precision == YAFEL_HIP_MIXED ||
               opts->precision == YAFEL_HIP_MIXED_MATRIX,
-          "solve: precision must be FP64, MIXED or MIXED_MATRIX");
-  REQUIRE(opts->stop == 0 || opts->stop == 1, "solve: unknown stopping rule");
-  yafel_hip_ctx_s *ctx = A->ctx;
-  HIPCHK(hipSetDevice(ctx->device));
+          w + ": precision must be FP64, MIXED or MIXED_MATRIX");
+  REQUIRE(opts->stop == 0 || opts->stop == 1, w + ": unknown stopping rule");
+  HIPCHK(hipSetDevice(A->ctx->device));
   std::memset(info, 0, sizeof(*info));
   const int64_t n = A->rows;
-  if (n == 0) return YAFEL_HIP_OK;
-  REQUIRE(A->has64, "solve: matrix has no FP64 values");
-  Solver S{};
-  S.ctx = ctx;
+  if (n == 0) return false;
+  REQUIRE(A->has64, w + ": matrix has no FP64 values");
+  S.ctx = A->ctx;
   S.A = A;
   S.o = *opts;
   S.n = n;
@@ -493,25 +504,118 @@ int yafel_hip_solve(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const 
   if (S.o.refine_delta <= 0) S.o.refine_delta = 0.1;
   if (S.o.check_every <= 0) S.o.check_every = 16;
   S.two_pass = S.cgfam && (A->cg_passes == 2 || (A->cg_passes == 0 && two_pass_pays(A, opts->precision)));
-  S.s = ctx->stream;
-  if (S.prec || true) {
-    if (!A->diag_ok) matrix_build_diag(A);
-  }
-  if (S.mixed && !A->has32) matrix_build_fp32(A);
-  ensure_ws(ctx, n);
-  S.st = ctx->ws.state.as<KState>();
-  S.part = ctx->ws.partials.as<double>();
+  S.s = S.ctx->stream;
   S.b = b_dev;
   S.x = x_dev;
+  return true;
+}
+
+void solve_run(Solver &S, const SolveWs &ws, yafel_hip_solve_info *info) {
+  yafel_hip_ctx_s *ctx = S.ctx;
+  for (int i = 0; i < 10; ++i) S.wv[i] = ws.vec[i];
+  S.st = ws.st;
+  S.part = ws.part;
   HIPCHK(hipEventRecord(ctx->t0, S.s));
-  if (opts->precision == YAFEL_HIP_MIXED) S.run<float, float>(info);
-  else if (opts->precision == YAFEL_HIP_MIXED_MATRIX) S.run<float, double>(info);
+  if (S.o.precision == YAFEL_HIP_MIXED) S.run<float, float>(info);
+  else if (S.o.precision == YAFEL_HIP_MIXED_MATRIX) S.run<float, double>(info);
   else S.run<double, double>(info);
   HIPCHK(hipEventRecord(ctx->t1, S.s));
   HIPCHK(hipEventSynchronize(ctx->t1));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
   info->ms = ms;
+}
+
+// ------------------------------------------------- allocation-free solve
+// The scratch of yafel_hip_solve_noalloc: a file-scope device array, one
+// copy per GPU, held until the process ends.  In host code `scratch` names
+// a host-side stand-in, not device memory: its device address is looked up
+// once per device (noalloc_scratch_init, from yafel_hip_ctx_create) and only
+// that address is handed to kernels and copies.
+constexpr size_t kNoallocVec = (((size_t)YAFEL_HIP_NOALLOC_MAX_ROWS * 8) + 255) & ~(size_t)255;
+constexpr int kNoallocMaxCus = 512;  // partials sized for up to 512 compute units
+constexpr size_t kNoallocPart = (size_t)kNoallocMaxCus * 16 * 8 * sizeof(double) + 4096;
+constexpr size_t kNoallocState = 4096;
+constexpr size_t kNoallocBytes = 10 * kNoallocVec + kNoallocPart + kNoallocState;
+static_assert(2 * sizeof(KState) <= kNoallocState, "two solver states fit the state area");
+constexpr int kNoallocDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch[kNoallocDevices];  // device address of `scratch` per device (nullptr: not looked up)
+
+}  // namespace
+
+namespace yh {
+void noalloc_scratch_init(int device) {
+  REQUIRE(device >= 0 && device < kNoallocDevices, "noalloc scratch: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return;
+  int cur = -1;
+  HIPCHK(hipGetDevice(&cur));
+  REQUIRE(cur == device, "noalloc scratch: the device must be current");
+  void *p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  REQUIRE(p != nullptr, "noalloc scratch: no device address");
+  g_scratch[device] = p;
+}
+}  // namespace yh
+
+extern "C" {
+
+int yafel_hip_solve(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const double *b_dev,
+                    double *x_dev, yafel_hip_solve_info *info) {
+  YH_TRY_BEGIN
+  Solver S{};
+  if (!solve_setup(S, A, opts, b_dev, x_dev, info, "solve")) return YAFEL_HIP_OK;
+  yafel_hip_ctx_s *ctx = A->ctx;
+  if (!A->diag_ok) matrix_build_diag(A);
+  if (S.mixed && !A->has32) matrix_build_fp32(A);
+  ensure_ws(ctx, S.n);
+  SolveWs ws;
+  for (int i = 0; i < 10; ++i) ws.vec[i] = ctx->ws.vec64[i].p;
+  ws.part = ctx->ws.partials.as<double>();
+  ws.st = ctx->ws.state.as<KState>();
+  solve_run(S, ws, info);
+  return YAFEL_HIP_OK;
+  YH_TRY_END(A ? A->ctx : nullptr)
+}
+
+int yafel_hip_solve_noalloc(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, const double *b_dev,
+                            double *x_dev, yafel_hip_solve_info *info) {
+  YH_TRY_BEGIN
+  Solver S{};
+  if (!solve_setup(S, A, opts, b_dev, x_dev, info, "solve_noalloc")) return YAFEL_HIP_OK;
+  yafel_hip_ctx_s *ctx = A->ctx;
+  REQUIRE(S.n <= YAFEL_HIP_NOALLOC_MAX_ROWS,
+          "solve_noalloc: " + std::to_string(S.n) + " rows, more than YAFEL_HIP_NOALLOC_MAX_ROWS (" +
+              std::to_string((long long)YAFEL_HIP_NOALLOC_MAX_ROWS) + ")");
+  REQUIRE(partials_bytes(ctx) <= kNoallocPart, "solve_noalloc: device has more compute units than the scratch serves");
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = ctx->device < kNoallocDevices ? g_scratch[ctx->device] : nullptr;
+  }
+  REQUIRE(base, "solve_noalloc: scratch not set up on this device");
+  // the diagonal and the FP32 copy are refreshed in place; their buffers
+  // must exist already
+  if (!A->diag_ok) {
+    REQUIRE(A->diag64.p, "solve_noalloc: the matrix has no diagonal yet (call yafel_hip_matrix_diagonal once)");
+    matrix_build_diag(A);
+  }
+  if (S.mixed && !A->has32) {
+    const size_t need = (size_t)std::max<int64_t>(A->ntot * (A->format == YAFEL_HIP_CSR ? 1 : 9), 1) * 4;
+    REQUIRE(A->val32.p && A->val32.bytes >= need,
+            "solve_noalloc: the matrix has no FP32 copy yet (call yafel_hip_matrix_sync_fp32 once)");
+    matrix_build_fp32(A);
+  }
+  unsigned char *p = (unsigned char *)base;
+  SolveWs ws;
+  for (int i = 0; i < 10; ++i) ws.vec[i] = p + i * kNoallocVec;
+  ws.part = (double *)(p + 10 * kNoallocVec);
+  ws.st = (KState *)(p + 10 * kNoallocVec + kNoallocPart);
+  solve_run(S, ws, info);
   return YAFEL_HIP_OK;
   YH_TRY_END(A ? A->ctx : nullptr)
 }

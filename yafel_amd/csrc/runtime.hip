@@ -151,6 +151,8 @@ static void ctx_free(yafel_hip_ctx_s *ctx) {
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   (void)hipEventDestroy(ctx->t0);
   (void)hipEventDestroy(ctx->t1);
+  for (hipEvent_t e : ctx->sev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -184,8 +186,11 @@ int yafel_hip_ctx_create(int device, yafel_hip_ctx *out) {
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&c->t0));
   HIPCHK(hipEventCreate(&c->t1));
+  HIPCHK(hipEventCreateWithFlags(&c->sev[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c->sev[1], hipEventDisableTiming));
   c->pinned_bytes = 65536;
   HIPCHK(hipHostMalloc(&c->pinned, c->pinned_bytes, hipHostMallocDefault));
+  noalloc_scratch_init(device);
   *out = c;
   return YAFEL_HIP_OK;
   YH_TRY_END(nullptr)

@@ -100,6 +100,7 @@ for _n, _a in [
     ("yafel_hip_kernel_timing", [_P, _I]), ("yafel_hip_kernel_time", [_P, C.POINTER(_D), C.POINTER(_I64)]),
     ("yafel_hip_solve_opts_default", [C.POINTER(SolveOpts)]),
     ("yafel_hip_solve", [_P, C.POINTER(SolveOpts), _P, _P, C.POINTER(SolveInfo)]),
+    ("yafel_hip_solve_noalloc", [_P, C.POINTER(SolveOpts), _P, _P, C.POINTER(SolveInfo)]),
     ("yafel_hip_solve_host", [_P, C.POINTER(SolveOpts), _P, _P, _P, C.POINTER(SolveInfo)]),
     ("yafel_hip_solve_multi", [_P, C.POINTER(SolveOpts), _I, _P, _P, C.POINTER(SolveInfo)]),
     ("yafel_hip_mesh_create_cg", [_P, _I, _I, _I, _I64, _P, _I64, _P, C.POINTER(_P)]),
@@ -372,6 +373,13 @@ class Matrix:
         o = self.opts(**kw)
         info = SolveInfo()
         check(lib.yafel_hip_solve(self.h, C.byref(o), b_dev.ptr, x_dev.ptr, C.byref(info)), self.ctx.h)
+        return info
+
+    def solve_noalloc(self, b_dev, x_dev, **kw):
+        """yafel_hip_solve with the library's own device scratch: allocates nothing."""
+        o = self.opts(**kw)
+        info = SolveInfo()
+        check(lib.yafel_hip_solve_noalloc(self.h, C.byref(o), b_dev.ptr, x_dev.ptr, C.byref(info)), self.ctx.h)
         return info
 
     def solve_multi(self, B_dev, X_dev, k, **kw):
