@@ -204,6 +204,61 @@ def test_assembly_pattern_and_values(H, ctx, orc, tag, dim, p, dpn, n, fmt):
         assert np.abs(Ke[e] - Ko).max() <= 1e-12 * np.abs(Ko).max()
 
 
+# cells where J is not diagonal: jittered and sheared meshes
+DISTORTED = [
+    # tag, dim, p, dpn, n, fmt
+    ("poisson_q1", 2, 1, 1, 6, "CSR"),
+    ("poisson_q2", 2, 2, 1, 4, "CSR"),
+    ("poisson_q4", 2, 4, 1, 3, "CSR"),
+    ("elast_quad", 2, 1, 2, 5, "CSR"),
+    ("elast_hex", 3, 1, 3, 3, "BCSR3"),
+    ("elast_hex2", 3, 2, 3, 2, "BCSR3"),
+    ("supg_q1", 2, 1, 1, 6, "CSR"),
+    ("supg_q2", 2, 2, 1, 4, "CSR"),
+]
+
+
+@pytest.mark.parametrize("kind", ["jitter", "shear"])
+@pytest.mark.parametrize("tag,dim,p,dpn,n,fmt", DISTORTED)
+def test_assembly_on_distorted_cells(H, ctx, orc, kind, tag, dim, p, dpn, n, fmt):
+    """Device tangent, residual and element matrices against the oracle on
+    cells whose J is not diagonal (the oracle is pinned there by the patch
+    test, the parallelogram closed forms and the load integral); the device
+    tangent also passes the patch test itself (Poisson, elasticity)."""
+    from meshes import corner_mesh, interior
+    nodes, cells = corner_mesh(orc, dim, n, p, kind, seed=11)
+    m = H.Mesh(ctx, dim, p, dpn, nodes, cells)
+    A = m.pattern(getattr(H, fmt))
+    ph, oph = case_physics(H, tag, dim, m.nel)
+    R = ctx.upload(np.zeros(m.ndofs))
+    m.assemble(ph, A, R)
+    el, dn = orc.dofmanager(dim, p, nodes, cells)
+    rows, cols, vals, Ro = orc.assemble(dim, p, dpn, dn, el, oph)
+    O = orc.csr(rows, cols, vals, block=3 if fmt == "BCSR3" else 1)
+    ptr, col, val = A.arrays()
+    optr, ocol, oval = O.arrays()
+    assert np.array_equal(ptr, optr) and np.array_equal(col, ocol)
+    assert np.abs(val - oval).max() <= 1e-12 * np.abs(oval).max()
+    assert np.abs(R.download() - Ro).max() <= 1e-12 * max(np.abs(Ro).max(), 1e-300)
+    Ke = m.element_matrices(ph)
+    for e in (0, m.nel // 2, m.nel - 1):
+        Ko, _ = orc.element_matrix(dim, p, dpn, dn, el, e, oph)
+        assert np.abs(Ke[e] - Ko).max() <= 1e-12 * np.abs(Ko).max()
+    if not tag.startswith("supg"):
+        # constant coefficient: K (a + B x) = 0 at interior rows
+        ph1, _ = case_physics(H, tag, dim, m.nel)
+        ph1.coef = None
+        A1 = m.pattern(getattr(H, fmt))
+        m.assemble(ph1, A1, None, H.TANGENT)
+        rng = np.random.default_rng(5)
+        a, B = rng.standard_normal(dpn), rng.standard_normal((dpn, dim))
+        u = (a[None, :] + dn[:, :dim] @ B.T).ravel()
+        r = A1.spmv_host(u)
+        inner = np.repeat(interior(dn, dim, kind), dpn)
+        _, _, v1 = A1.arrays()
+        assert np.abs(r[inner]).max() < 1e-12 * np.abs(v1).max() * np.abs(u).max()
+
+
 @pytest.mark.parametrize("fmt", ["CSR", "BCSR3"])
 def test_dirichlet_matches_oracle(H, ctx, orc, fmt):
     dim, p, dpn, n = 3, 1, 3, 3

@@ -55,6 +55,25 @@ def test_zz_matches_oracle(H, ctx, orc, dim, n):
     assert np.abs(G - Go).max() <= 1e-10 * scale
 
 
+@pytest.mark.parametrize("kind", ["jitter", "shear"])
+@pytest.mark.parametrize("dim,p,dpn,n", [(2, 1, 1, 12), (2, 3, 1, 5), (3, 1, 3, 5), (3, 2, 1, 3)])
+def test_local_smoothing_on_distorted_cells(H, ctx, orc, kind, dim, p, dpn, n):
+    """On cells whose J is not diagonal (the oracle is pinned there by the
+    exact recovery of linear fields)."""
+    from meshes import corner_mesh
+    nodes, cells = corner_mesh(orc, dim, n, p, kind, seed=21)
+    m = H.Mesh(ctx, dim, p, dpn, nodes, cells)
+    el, dn = m.download()
+    u = np.random.default_rng(300 + dim + p).standard_normal(m.ndofs)
+    G = m.gradient_local_smoothing(ctx.upload(u)).download().reshape(m.ndofs, dim)
+    Go = orc.grad_local_smoothing(dim, p, dpn, dn, el, u)
+    assert np.abs(G - Go).max() <= 1e-12 * np.abs(Go).max()
+    if p == 1 and dpn == 1:
+        G = m.gradient_zz(ctx.upload(u)).download().reshape(m.n_nodes, dim)
+        Go = orc.grad_zz(dim, dn, el, u)
+        assert np.abs(G - Go).max() <= 1e-10 * np.abs(Go).max()
+
+
 def test_recovery_of_a_solved_poisson_field(H, ctx, orc):
     """Solve -lap u = 1 (Q1 64^2, homogeneous Dirichlet), recover the gradient
     both ways: each agrees with the oracle's recovery of the same solution, and

@@ -221,6 +221,116 @@ def test_supg_element_closed_form(orc):
     assert np.abs(K - Kc).max() < 1e-12 * np.abs(Kc).max()
 
 
+# ------------------------------------------- cells where J is not diagonal
+# A wrong transformation of the shape gradients (J^-1 for J^-T), a wrong
+# Jacobian index order or a wrong det J cannot be seen on axis-aligned cells,
+# where J is diagonal.  These pins use jittered and sheared meshes.
+from meshes import SHEAR, corner_mesh, interior  # noqa: E402
+
+
+def linear_field(dn, dim, dpn, seed=0):
+    """u = a + B x at the dof nodes (dof = node * dpn + comp); linear fields
+    lie in the isoparametric space of every cell."""
+    rng = np.random.default_rng(seed)
+    a, B = rng.standard_normal(dpn), rng.standard_normal((dpn, dim))
+    u = a[None, :] + dn[:, :dim] @ B.T
+    return u.ravel(), B
+
+
+@pytest.mark.parametrize("dim,p", [(2, 1), (2, 2), (2, 4), (3, 1), (3, 2)])
+@pytest.mark.parametrize("kind", ["jitter", "shear"])
+@pytest.mark.parametrize("phys", ["poisson", "elasticity"])
+def test_patch_test_on_distorted_cells(orc, dim, p, kind, phys):
+    """K (a + B x) vanishes at every interior row: the element gradients of a
+    linear field are exact on any cell, and the fluxes of neighbouring cells
+    cancel.  With J^-1 in place of J^-T the computed gradient is J^-1 J^T B,
+    which is not B where J is not symmetric."""
+    n = 4 if dim == 2 else 3
+    nodes, cells = corner_mesh(orc, dim, n, p, kind)
+    el, dn = orc.dofmanager(dim, p, nodes, cells)
+    dpn = 1 if phys == "poisson" else dim
+    ph = physics(0, dim, f=0.0) if phys == "poisson" else physics(1, dim, lam=LAM, mu=MU)
+    rows, cols, vals, _ = orc.assemble(dim, p, dpn, dn, el, ph)
+    A = orc.csr(rows, cols, vals)
+    u, _ = linear_field(dn, dim, dpn)
+    r = A.spmv(u)
+    inner = np.repeat(interior(dn, dim, kind), dpn)
+    scale = np.abs(vals).max() * np.abs(u).max()
+    assert inner.sum() > 0 and np.abs(r[~inner]).max() > 1e-3 * scale   # boundary rows carry the flux
+    assert np.abs(r[inner]).max() < 1e-12 * scale
+
+
+@pytest.mark.parametrize("dim,p", [(2, 1), (2, 3), (3, 1), (3, 2)])
+def test_load_integrates_the_domain_on_jittered_cells(orc, dim, p):
+    """R_A = f int N_A: on the jittered unit square / cube the loads add up to
+    f |Omega| = f (det J, its sign and the weights), and a body force b gives
+    b_i |Omega| in each component."""
+    n = 4 if dim == 2 else 3
+    nodes, cells = corner_mesh(orc, dim, n, p, "jitter", seed=3)
+    el, dn = orc.dofmanager(dim, p, nodes, cells)
+    _, _, _, R = orc.assemble(dim, p, 1, dn, el, physics(0, dim, f=2.5), want=("R",))
+    assert abs(R.sum() - 2.5) < 1e-13
+    body = (0.3, -0.7, 1.1)[:dim] + (0.0,) * (3 - dim)
+    _, _, _, R = orc.assemble(dim, p, dim, dn, el, physics(1, dim, lam=LAM, mu=MU, body=body), want=("R",))
+    for i in range(dim):
+        assert abs(R[i::dim].sum() - body[i]) < 1e-13
+
+
+def parallelogram_grad_integrals(orc, dim, p, J):
+    """X[j][l][A, B] = int d_xj N_A d_xl N_B over a cell with constant J
+    (J_ij = d x_i / d xi_j): det J sum_mn Jinv_mj Jinv_nl Iref[m][n], Iref the
+    reference-cell (side 2) integrals of the 1-D GLL matrices."""
+    Iref = grad_integrals(orc, dim, p, 2.0)
+    Ji, det = np.linalg.inv(J), np.linalg.det(J)
+    X = [[sum(Ji[m, j] * Ji[k, l] * Iref[m][k] for m in range(dim) for k in range(dim)) * det
+          for l in range(dim)] for j in range(dim)]
+    return X
+
+
+@pytest.mark.parametrize("dim,p", [(2, 1), (2, 2), (2, 3), (3, 1), (3, 2)])
+def test_parallelogram_element_closed_form(orc, dim, p):
+    """On a sheared mesh every cell has J = (h/2) S: the Poisson and elasticity
+    element matrices equal the closed forms built from the 1-D GLL matrices
+    through J^-1 J^-T (a transposed J, J^-1 for J^-T or a lost det J fails)."""
+    n = 3 if dim == 2 else 2
+    nodes, cells = corner_mesh(orc, dim, n, p, "shear")
+    el, dn = orc.dofmanager(dim, p, nodes, cells)
+    J = SHEAR[dim] * (0.5 / n)
+    X = parallelogram_grad_integrals(orc, dim, p, J)
+    coef = np.linspace(0.5, 2.0, el.shape[0])
+    e = el.shape[0] // 2
+    K, _ = orc.element_matrix(dim, p, 1, dn, el, e, physics(0, dim, coef=coef, f=0.0))
+    Kc = -coef[e] * sum(X[j][j] for j in range(dim))
+    assert np.abs(K - Kc).max() < 1e-12 * np.abs(Kc).max()
+    K, _ = orc.element_matrix(dim, p, dim, dn, el, e, physics(1, dim, coef=coef, lam=LAM, mu=MU))
+    lam, mu = coef[e] * LAM, coef[e] * MU
+    tr = sum(X[j][j] for j in range(dim))
+    Kc = np.zeros_like(K)
+    for i in range(dim):
+        for k in range(dim):
+            Kc[i::dim, k::dim] = -(lam * X[i][k] + mu * X[k][i] + (mu * tr if i == k else 0))
+    assert np.abs(K - Kc).max() < 1e-12 * np.abs(Kc).max()
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_supg_linear_field_on_jittered_cells(orc, p):
+    """SUPG K (a + b.x) at an interior node is NSD (U.b) int N_A (the
+    advection term, added once per component; the diffusion fluxes cancel),
+    and int N_A is the Poisson load with f = 1 on the same mesh."""
+    n = 5
+    nodes, cells = corner_mesh(orc, 2, n, p, "jitter", seed=7)
+    el, dn = orc.dofmanager(2, p, nodes, cells)
+    U = (math.cos(0.4), math.sin(0.4), 0.0)
+    rows, cols, vals, _ = orc.assemble(2, p, 1, dn, el, physics(2, 2, U=U, kdiff=0.01, h=1 / n, alpha=0.8))
+    A = orc.csr(rows, cols, vals)
+    u, B = linear_field(dn, 2, 1, seed=5)
+    _, _, _, mass = orc.assemble(2, p, 1, dn, el, physics(0, 2, f=1.0), want=("R",))
+    inner = interior(dn, 2, "jitter")
+    expect = 2 * (U[0] * B[0, 0] + U[1] * B[0, 1]) * mass
+    r = A.spmv(u)
+    assert np.abs(r[inner] - expect[inner]).max() < 1e-12 * np.abs(expect).max()
+
+
 # ------------------------------------------------------------ sparse build
 @pytest.mark.parametrize("nthreads", [1, 3])
 def test_threaded_assembly_equals_the_element_order_loop(orc, nthreads):
@@ -400,6 +510,31 @@ def test_local_smoothing_recovers_linear_fields_exactly(orc, dim, p, dpn):
     G = orc.grad_local_smoothing(dim, p, dpn, dn, el, u)
     expect = np.tile(B, (dn.shape[0], 1))
     assert np.abs(G - expect).max() < 1e-12
+
+
+@pytest.mark.parametrize("dim,p,dpn", [(2, 1, 1), (2, 2, 2), (2, 3, 1), (3, 1, 3), (3, 2, 1)])
+@pytest.mark.parametrize("kind", ["jitter", "shear"])
+def test_local_smoothing_linear_fields_on_distorted_cells(orc, dim, p, dpn, kind):
+    """As above on cells whose J is not diagonal: the element gradients of a
+    linear field are exact only through J^-T."""
+    nodes, cells = corner_mesh(orc, dim, 3, p, kind, seed=dim + p)
+    el, dn = orc.dofmanager(dim, p, nodes, cells)
+    rng = np.random.default_rng(dim * 10 + p + dpn)
+    a, B = rng.standard_normal(dpn), rng.standard_normal((dpn, dim))
+    u = (a[None, :] + dn[:, :dim] @ B.T).ravel()
+    G = orc.grad_local_smoothing(dim, p, dpn, dn, el, u)
+    assert np.abs(G - np.tile(B, (dn.shape[0], 1))).max() < 1e-11
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kind", ["jitter", "shear"])
+def test_zz_linear_fields_on_distorted_cells(orc, dim, kind):
+    nodes, cells = corner_mesh(orc, dim, 4 if dim == 2 else 3, 1, kind, seed=2 * dim)
+    el, dn = orc.dofmanager(dim, 1, nodes, cells)
+    rng = np.random.default_rng(17 + dim)
+    a, b = rng.standard_normal(), rng.standard_normal(dim)
+    G = orc.grad_zz(dim, dn, el, a + dn[:, :dim] @ b)
+    assert np.abs(G - b[None, :]).max() < 1e-10
 
 
 @pytest.mark.parametrize("dim", [2, 3])

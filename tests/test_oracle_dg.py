@@ -104,6 +104,32 @@ def test_linear_field_element_part_is_minus_c_dot_grad(orc, p):
         np.testing.assert_allclose(d[e], ex, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_linear_field_on_parallelograms(orc, p):
+    # as above on a sheared mesh (x -> x + 0.35 y): affine cells whose J is not
+    # diagonal, so the volume term goes through J^-T and the face terms through
+    # slanted normals; the inflow faces are still the left column and top row
+    nx, ny = 5, 4
+    nodes, cells = rect_mesh(orc, nx, ny, 1.3, 0.9)
+    nodes[:, 0] += 0.35 * nodes[:, 1]
+    c = (0.7, -0.3)
+    a = np.array([1.3, -0.4])
+    ph = physics(DG, 2, U=(c[0], c[1], 0))
+    X = orc.dg_nodes(p, nodes, cells)
+    U = X[:, 0] * a[0] + X[:, 1] * a[1] + 0.25
+    R, Rf = orc.dg_residual(p, nodes, cells, ph, U, face=True)
+    d = (R - Rf).reshape(-1, (p + 1) ** 2)
+    ex = -(c[0] * a[0] + c[1] * a[1])
+    checked = 0
+    for e in range(cells.shape[0]):
+        i, j = e % nx, e // nx
+        if i == 0 or j == ny - 1:
+            continue
+        np.testing.assert_allclose(d[e], ex, rtol=0, atol=5e-12)   # M^-1 of sheared cells: a few 1e-12
+        checked += 1
+    assert checked == (nx - 1) * (ny - 1)
+
+
 @pytest.mark.parametrize("c", [(1.0, 0.0), (-1.0, 0.5), (0.3, 0.8)])
 def test_constant_field_face_terms_are_the_outflow(orc, c):
     # U = 1: interior fluxes cancel between the two cells; the sum of the face
