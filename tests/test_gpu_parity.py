@@ -117,6 +117,28 @@ def test_spmm_matches_columnwise_spmv(H, ctx, golden):
             assert np.abs(Y[:, j] - yj).max() <= 1e-13 * np.abs(yj).max()
 
 
+@pytest.mark.parametrize("k", [3, 16, 64])
+def test_spmm_fp32_scattered_columns(H, ctx, k):
+    """FP32 products on a pattern whose slices touch more distinct columns
+    than the staged product holds in LDS (random columns: about 600 per
+    slice), so those slices gather directly, beside slices that stage."""
+    rng = np.random.default_rng(31 + k)
+    n = 3000
+    rows = np.repeat(np.arange(n), 10)
+    cols = rng.integers(0, n, rows.size)
+    cols[: 10 * 640] = (rows[: 10 * 640] + rng.integers(-20, 21, 10 * 640)) % n   # banded leading slices
+    vals = rng.standard_normal(rows.size)
+    A = H.Matrix.from_triplets(ctx, rows, cols, vals, precisions=H.FP64 | H.FP32)
+    A.sync_fp32()
+    X = rng.standard_normal((n, k))
+    Y32d = ctx.alloc(n * k, np.float32)
+    A.spmm(ctx.upload(X.astype(np.float32).ravel()), Y32d, k, H.FP32)
+    Y32 = Y32d.download().reshape(n, k)
+    for j in range(k):
+        yj = A.spmv_host(X[:, j])
+        assert np.abs(Y32[:, j] - yj).max() <= 1e-5 * np.abs(yj).max()
+
+
 @pytest.mark.parametrize("k", [1, 4, 37, 64, 68, 70])
 def test_spmm_sorted_rows_matches_columnwise_spmv(H, ctx, k):
     """A Q2 pattern (rows sorted by length inside the slices, so stored

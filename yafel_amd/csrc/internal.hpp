@@ -161,6 +161,19 @@ struct yafel_hip_matrix_s {
   yh::DBuf perm, iperm;         // int32 position->row / row->position (empty: identity)
   yh::DBuf col;                 // int32 sliced block columns (ntot)
   yh::DBuf col16;               // int16 column - stored position (ntot), when every entry fits; else empty
+  // CSR-order copy for the FP32 multi-RHS products that stage rows of X in
+  // LDS (built on first use): rows in groups of 64 consecutive rows, each
+  // group's distinct columns, and every entry's position in its group's list
+  yh::DBuf mx_ptr;              // int32 (rows+1): row pointer, every row padded to a multiple of 4 entries
+  yh::DBuf mx_src;              // int32: sliced position of each entry in CSR order (-1: padding)
+  yh::DBuf mx_lidx;             // uint16: position of the entry's column in its group's list (0: padding)
+  yh::DBuf mx_dptr;             // int32 (groups+1): start of each group's list in mx_dcol
+  yh::DBuf mx_dcol;             // int32: the groups' distinct columns, ascending within a group
+  yh::DBuf mx_val32;            // FP32 values in CSR order, zero padding (refreshed from val32)
+  int mx_dmax = -1;             // longest list (-1: not built; 0: not usable)
+  int64_t mx_emax = 0;          // most (padded) entries in a group
+  int64_t v32_gen = 0;          // bumped whenever val32 is rebuilt
+  int64_t mx_gen = -1;          // the val32 generation mx_val32 holds
   yh::DBuf val64, val32;        // CSR: ntot; BCSR3: 9 arrays of ntot (v[e * ntot + k], e = 3i+j)
   yh::DBuf diag64, dinv64, dinv32;
   bool has64 = false, has32 = false, diag_ok = false;
@@ -206,6 +219,9 @@ inline int sell_grid(int64_t nsl, int cap) {
 void spmv_launch(yafel_hip_matrix_s *A, int precision, const void *x, void *y, hipStream_t s);
 void spmm_launch(yafel_hip_matrix_s *A, int precision, int k, const void *X, void *Y,
                  hipStream_t s);
+// FP32 CSR Y = A X with the slices' X rows staged in LDS (k <= 64); false when
+// the matrix does not qualify and the caller must use another kernel.
+bool spmm_lds_launch(yafel_hip_matrix_s *A, int k, const float *X, float *Y, hipStream_t s);
 void matrix_build_fp32(yafel_hip_matrix_s *A);
 void matrix_build_diag(yafel_hip_matrix_s *A);
 // Converts CSR-ordered device arrays (int32 ptr[brows+1], col[nblk], FP64

@@ -30,6 +30,12 @@ namespace {
 constexpr int NT = 256;
 constexpr int KMAX = 64;
 
+// what a product op's row epilogue reads of its own row (pre4), loaded before
+// the row's sum so that the loads overlap it
+struct Pre4 {
+  float4 a, b;
+};
+
 struct MState {
   int done, paused, first, k;
   int method, stop, mixed, pidx;
@@ -216,6 +222,368 @@ __global__ __launch_bounds__(NT) void k_mspmm4(CsrView<float> A, Op op0) {
   finish4<NP>(op, acc);
 }
 
+// FP32 products with rows of X staged in LDS (k a multiple of 4).  Rows go
+// in groups of 64 consecutive rows (CSR order, whatever the sliced storage's
+// row order); a group touches far fewer distinct columns than it has entries
+// (about 340 against 1,000 on a Q2 mesh), so a workgroup takes one group and
+// one pass of 32 right-hand sides at a time:
+//  1. stage: the X rows of the group's distinct columns (mx_dcol), 128 bytes
+//     each, transformed as the op transforms a gathered row (op.stage4: the
+//     column's Jacobi scale, p = z + beta p, ...), every load of a thread
+//     issued before its first LDS store;
+//  2. rows: lane (rho = lane >> 3, q = lane & 7) takes right-hand sides
+//     4q .. 4q+3 of the pass for row rho of a set of 8 rows two apart (the
+//     same node type on a structured mesh, so rows of one length); the lane
+//     loads its row's values and local indices (mx_lidx) in CSR order and
+//     reads 16 bytes of staged x per entry, so one LDS read feeds four FMAs.
+//     A row is summed in column order.
+// A group with more distinct columns than the LDS holds reads x directly.
+constexpr int kStageRows = 384;  // distinct columns staged per group: 48 KiB at 32 right-hand sides
+constexpr int KC = 32;           // right-hand sides per pass
+constexpr int kStageIter = kStageRows / (NT / 8);
+
+struct StagedCsr {
+  const int *ptr;  // CSR row pointer
+  const float *val;
+  const unsigned short *lidx;
+  const int *dptr, *dcol;
+  int64_t rows;
+  int dcap;  // rows of X the launch's LDS holds
+};
+
+// Per-RHS partials of the staged products.  Lane (rho, q) holds right-hand
+// sides 4q .. 4q+3 of each pass (a0: first pass, a1: second); the 8 rows of
+// a wave are folded by shuffles (fixed order), then the waves through LDS.
+template <int NP, class Op>
+__device__ __forceinline__ void finish_staged(Op &op, double (&a0)[NP][4], double (&a1)[NP][4]) {
+  __shared__ double lds[NP][NT / 64][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane & 7;
+  const int blk = dev::xcd_block();
+#pragma unroll
+  for (int j = 0; j < NP; ++j)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double s0 = a0[j][c], s1 = a1[j][c];
+#pragma unroll
+      for (int o = 8; o < 64; o <<= 1) {
+        s0 += __shfl_xor(s0, o, 64);
+        s1 += __shfl_xor(s1, o, 64);
+      }
+      if (lane < 8) {
+        lds[j][w][4 * q + c] = s0;
+        lds[j][w][32 + 4 * q + c] = s1;
+      }
+    }
+  __syncthreads();
+  if (w == 0) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      double s = 0;
+#pragma unroll
+      for (int ww = 0; ww < NT / 64; ++ww) s += lds[j][ww][lane];
+      op.part[((size_t)j * gridDim.x + blk) * 64 + lane] = s;
+    }
+  }
+}
+
+constexpr int kStageEnt = 2048;  // entries of a group staged in LDS
+
+template <class Op>
+__global__ __launch_bounds__(NT) void k_mspmm_lds(StagedCsr A, Op op0) {
+  extern __shared__ float4 xs4[];  // A.dcap rows of KC / 4
+  __shared__ __align__(16) float ev[kStageEnt];
+  __shared__ __align__(16) unsigned short el[kStageEnt];
+  __shared__ int ep[65];
+  constexpr int NP = Op::NP;
+  Op op = op0;
+  if (op.skip()) return;
+  op.begin();
+  constexpr int NPA = NP > 0 ? NP : 1;
+  double a0[NPA][4], a1[NPA][4];
+#pragma unroll
+  for (int j = 0; j < NPA; ++j)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a0[j][c] = a1[j][c] = 0;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 7, rho = lane >> 3;
+  const int k = op.k;
+  const int nh = (k + KC - 1) / KC;
+  const int64_t ngr = (A.rows + 63) >> 6;
+  for (int64_t g = dev::xcd_block(); g < ngr; g += gridDim.x) {
+    const int d0 = A.dptr[g], d = A.dptr[g + 1] - d0;
+    const int64_t r0 = g * 64, r1 = r0 + 64 < A.rows ? r0 + 64 : A.rows;
+    const int e0 = A.ptr[r0], ne = A.ptr[r1] - e0;
+    const bool staged = d <= A.dcap && ne <= kStageEnt;
+    if (staged) {
+      __syncthreads();  // every wave is done with the previous group
+      if (tid < 65) ep[tid] = A.ptr[r0 + tid < r1 ? r0 + tid : r1] - e0;
+      float vv[kStageEnt / NT];
+      unsigned short ll[kStageEnt / NT];
+#pragma unroll
+      for (int m = 0; m < kStageEnt / NT; ++m) {
+        const int i = tid + m * NT;
+        vv[m] = i < ne ? __builtin_nontemporal_load(&A.val[e0 + i]) : 0.f;
+        ll[m] = i < ne ? __builtin_nontemporal_load(&A.lidx[e0 + i]) : (unsigned short)0;
+      }
+#pragma unroll
+      for (int m = 0; m < kStageEnt / NT; ++m) {
+        const int i = tid + m * NT;
+        if (i < ne) {
+          ev[i] = vv[m];
+          el[i] = ll[m];
+        }
+      }
+    }
+    for (int h = 0; h < nh; ++h) {
+      const int j0 = h * KC, jq = j0 / 4 + q;  // this lane's quad of right-hand sides
+      const bool on = 4 * jq < k;
+      op.bind4(on ? jq : j0 / 4);
+      op.bind4s(on ? jq : j0 / 4);
+      const int sigma0 = 2 * w;  // the wave's row sets: rows a, a + 2, ..., a + 14
+      if (staged) {
+        if (h > 0) __syncthreads();  // every wave is done with the previous pass's x
+        if (on) {
+          // thread (i0 = tid >> 3, q) stages quad q of rows i0, i0 + 32, ...
+          const int i0 = tid >> 3;
+          int c[kStageIter];
+          float4 v[kStageIter];
+#pragma unroll
+          for (int m = 0; m < kStageIter; ++m) {
+            const int i = i0 + m * (NT / 8);
+            c[m] = i < d ? A.dcol[d0 + i] : -1;
+          }
+#pragma unroll
+          for (int m = 0; m < kStageIter; ++m)
+            if (c[m] >= 0) v[m] = op.stage4(c[m], jq);
+#pragma unroll
+          for (int m = 0; m < kStageIter; ++m)
+            if (c[m] >= 0) xs4[(i0 + m * (NT / 8)) * (KC / 4) + q] = v[m];
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int m = 0; m < 2; ++m) {
+          const int sigma = sigma0 + m;
+          const int rl = (sigma >> 1) * 16 + (sigma & 1) + 2 * rho;  // row within the group
+          const int64_t r = r0 + rl;
+          // rows are padded to a multiple of 4 entries (zero values), so a
+          // lane reads its row four entries at a time: 16 bytes of values
+          // and 8 bytes of positions
+          const int b = ep[rl], n4 = r < A.rows ? (ep[rl + 1] - b) >> 2 : 0;
+          const Pre4 pr = op.pre4(r < A.rows ? r : A.rows - 1, 4 * (on ? jq : j0 / 4));
+          float4 y = make_float4(0.f, 0.f, 0.f, 0.f);
+          const float4 *ev4 = reinterpret_cast<const float4 *>(ev + b);
+          const uint2 *el4 = reinterpret_cast<const uint2 *>(el + b);
+          for (int t = 0; t < n4; ++t) {
+            const float4 v4 = ev4[t];
+            const uint2 l4 = el4[t];
+            const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+            const int li[4] = {(int)(l4.x & 0xffffu), (int)(l4.x >> 16), (int)(l4.y & 0xffffu), (int)(l4.y >> 16)};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const float4 x = xs4[li[e] * (KC / 4) + q];
+              y.x = fmaf(vv[e], x.x, y.x);
+              y.y = fmaf(vv[e], x.y, y.y);
+              y.z = fmaf(vv[e], x.z, y.z);
+              y.w = fmaf(vv[e], x.w, y.w);
+            }
+          }
+          if (r < A.rows && on) {
+            if (h == 0) op.row4p(r, 4 * jq, y, pr, a0);
+            else op.row4p(r, 4 * jq, y, pr, a1);
+          }
+        }
+      } else {
+        // direct reads of x for groups too large to stage
+#pragma unroll 1
+        for (int m = 0; m < 2; ++m) {
+          const int sigma = sigma0 + m;
+          const int64_t r = r0 + (sigma >> 1) * 16 + (sigma & 1) + 2 * rho;
+          float4 y = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (r < A.rows) {
+            const int b = A.ptr[r], n = A.ptr[r + 1] - b;
+            for (int t = 0; t < n; ++t) {
+              const float vv = A.val[b + t];
+              const float4 x = op.stage4(A.dcol[d0 + A.lidx[b + t]], on ? jq : j0 / 4);
+              y.x = fmaf(vv, x.x, y.x);
+              y.y = fmaf(vv, x.y, y.y);
+              y.z = fmaf(vv, x.z, y.z);
+              y.w = fmaf(vv, x.w, y.w);
+            }
+          }
+          if (r < A.rows && on) {
+            if (h == 0) op.row4p(r, 4 * jq, y, op.pre4(r, 4 * jq), a0);
+            else op.row4p(r, 4 * jq, y, op.pre4(r, 4 * jq), a1);
+          }
+        }
+      }
+    }
+  }
+  if constexpr (NP > 0) finish_staged<NP>(op, a0, a1);
+}
+
+// The staged product software-pipelined, for matrices whose every group fits
+// the LDS budgets: while the waves compute one unit (a group and a pass of 32
+// right-hand sides) from LDS, every thread's loads of the next unit's rows of
+// X -- and at a group's second pass, of the next group's entries and list --
+// are already in flight into registers, and are stored to LDS after the next
+// barrier.  Same rows, sums and order as k_mspmm_lds.
+template <class Op>
+__global__ __launch_bounds__(NT) void k_mspmm_pipe(StagedCsr A, Op op0) {
+  extern __shared__ float4 xs4[];  // A.dcap rows of KC / 4
+  __shared__ __align__(16) float ev[kStageEnt];
+  __shared__ __align__(16) unsigned short el[kStageEnt];
+  __shared__ int ep[65];
+  constexpr int NP = Op::NP, SE = kStageEnt / NT;
+  Op op = op0;
+  if (op.skip()) return;
+  op.begin();
+  constexpr int NPA = NP > 0 ? NP : 1;
+  double a0[NPA][4], a1[NPA][4];
+#pragma unroll
+  for (int j = 0; j < NPA; ++j)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a0[j][c] = a1[j][c] = 0;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 7, rho = lane >> 3, i0 = tid >> 3;
+  const int k = op.k;
+  const int nh = (k + KC - 1) / KC;
+  const int64_t ngr = (A.rows + 63) >> 6;
+  int64_t g = dev::xcd_block();
+  if (g >= ngr) {
+    if constexpr (NP > 0) finish_staged<NP>(op, a0, a1);
+    return;
+  }
+  // registers of the next unit.  Every load is unconditional, at a clamped
+  // in-bounds position, and nothing computes on its result before the store
+  // to LDS: a conditional load, or arithmetic on its result, makes the
+  // compiler wait for it at once instead of during the rows.
+  using Raw = typename Op::Raw;
+  int pc[kStageIter];
+  Raw px[kStageIter];
+  float pv[SE];
+  unsigned short pl[SE];
+  int pp = 0, d0 = 0, d = 0, ne = 0;
+  auto load_meta = [&](int64_t gg) {  // the group's list and entries
+    d0 = A.dptr[gg];
+    d = A.dptr[gg + 1] - d0;
+    const int64_t r0 = gg * 64, r1 = r0 + 64 < A.rows ? r0 + 64 : A.rows;
+    const int e0 = A.ptr[r0];
+    ne = A.ptr[r1] - e0;
+    {
+      const int64_t rp = r0 + (tid < 64 ? tid : 64);
+      pp = A.ptr[rp < r1 ? rp : r1] - e0;  // used by threads 0..64
+    }
+#pragma unroll
+    for (int m = 0; m < SE; ++m) {
+      const int i = tid + m * NT;
+      const int64_t e = ne > 0 ? e0 + (i < ne ? i : ne - 1) : 0;
+      pv[m] = __builtin_nontemporal_load(&A.val[e]);
+      pl[m] = __builtin_nontemporal_load(&A.lidx[e]);
+    }
+#pragma unroll
+    for (int m = 0; m < kStageIter; ++m) {
+      const int i = i0 + m * (NT / 8);
+      pc[m] = A.dcol[d0 + (i < d ? i : 0)];
+    }
+  };
+  auto load_x = [&](int h) {  // quad q of the pass's rows i0, i0 + 32, ... (raw)
+    const int jq = h * (KC / 4) + q;
+    const int jl = 4 * jq < k ? jq : h * (KC / 4);
+#pragma unroll
+    for (int m = 0; m < kStageIter; ++m) px[m] = op.load4(pc[m], jl);
+  };
+  load_meta(g);
+  load_x(0);
+  // the rows' op for each pass, bound once: the active flags and betas of a
+  // lane's four right-hand sides do not change during the launch
+  Op opb[2] = {op, op};
+#pragma unroll
+  for (int hh = 0; hh < 2; ++hh) {
+    const int jq = hh * (KC / 4) + q;
+    opb[hh].bind4(4 * jq < k ? jq : 0);
+  }
+  int h = 0;
+  while (true) {
+    __syncthreads();  // every wave is done with the LDS of the previous unit
+    // every prefetched value is stored below: wait for all of them here, in
+    // one place, so that no load is still counted as pending after the
+    // (partly conditional) stores -- a later wait for the epilogue operands
+    // would otherwise wait again (vmcnt(0); expcnt and lgkmcnt not waited)
+    __builtin_amdgcn_s_waitcnt(0x0f70);
+    if (h == 0) {
+      if (tid < 65) ep[tid] = pp;
+#pragma unroll
+      for (int m = 0; m < SE; ++m) {
+        const bool in = tid + m * NT < ne;
+        ev[tid + m * NT] = in ? pv[m] : 0.f;
+        el[tid + m * NT] = in ? pl[m] : (unsigned short)0;
+      }
+    }
+    {
+      const int jq = h * (KC / 4) + q;
+      const bool on = 4 * jq < k;
+      op.bind4s(on ? jq : h * (KC / 4));
+#pragma unroll
+      for (int m = 0; m < kStageIter; ++m)
+        if (i0 + m * (NT / 8) < d && on) xs4[(i0 + m * (NT / 8)) * (KC / 4) + q] = op.make4(px[m]);
+    }
+    __syncthreads();
+    const int j0 = h * KC, jq = j0 / 4 + q;
+    const bool on = 4 * jq < k;
+    // the two row sets of the wave (rows a, a + 2, ..., a + 14) and their
+    // epilogue operands, loaded before the next unit's loads: vector loads
+    // complete in issue order, so a wait for these must not wait for those
+    int rls[2];
+    Pre4 prs[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int sigma = 2 * w + m;
+      rls[m] = (sigma >> 1) * 16 + (sigma & 1) + 2 * rho;
+      const int64_t r = g * 64 + rls[m];
+      prs[m] = op.pre4(r < A.rows ? r : A.rows - 1, 4 * (on ? jq : j0 / 4));
+    }
+    // the next unit's loads, in flight during this unit's rows
+    const int64_t gn = g + gridDim.x;
+    const bool second = h + 1 < nh;
+    if (second) load_x(h + 1);
+    else if (gn < ngr) load_meta(gn);
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int rl = rls[m];
+      const int64_t r = g * 64 + rl;
+      const int b = ep[rl], n4 = r < A.rows ? (ep[rl + 1] - b) >> 2 : 0;
+      float4 y = make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 *ev4 = reinterpret_cast<const float4 *>(ev + b);
+      const uint2 *el4 = reinterpret_cast<const uint2 *>(el + b);
+      for (int t = 0; t < n4; ++t) {
+        const float4 v4 = ev4[t];
+        const uint2 l4 = el4[t];
+        const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+        const int li[4] = {(int)(l4.x & 0xffffu), (int)(l4.x >> 16), (int)(l4.y & 0xffffu), (int)(l4.y >> 16)};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float4 x = xs4[li[e] * (KC / 4) + q];
+          y.x = fmaf(vv[e], x.x, y.x);
+          y.y = fmaf(vv[e], x.y, y.y);
+          y.z = fmaf(vv[e], x.z, y.z);
+          y.w = fmaf(vv[e], x.w, y.w);
+        }
+      }
+      if (r < A.rows && on) {
+        if (h == 0) opb[0].row4p(r, 4 * jq, y, prs[m], a0);
+        else opb[1].row4p(r, 4 * jq, y, prs[m], a1);
+      }
+    }
+    if (second) {
+      ++h;
+    } else {
+      if (gn >= ngr) break;
+      g = gn;
+      h = 0;
+      load_x(0);  // its list arrived during the rows above
+    }
+  }
+  if constexpr (NP > 0) finish_staged<NP>(op, a0, a1);
+}
+
 template <class Op>
 __global__ __launch_bounds__(NT) void k_melem(int64_t n, Op op0) {
   Op op = op0;
@@ -244,8 +612,15 @@ struct MBase {
   __device__ void last_all() const {}
   // product ops: no column scale unless the op supplies one
   __device__ const void *cscale() const { return nullptr; }
-  // the four right-hand sides 4g .. 4g+3 of one lane (FP32 products)
-  __device__ void bind4(int) {}
+  // the four right-hand sides 4g .. 4g+3 of one lane (FP32 products): their
+  // active flags, read once (they change only in last_lane, after every row
+  // of the launch); and of the rows one thread stages (staged products)
+  bool a4c[4];
+  __device__ void bind4(int g) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a4c[c] = st->active[4 * g + c] != 0;
+  }
+  __device__ void bind4s(int) {}
   // the lane's right-hand side in k_mspmm: its active flag read once (it
   // changes only in last_lane, after every row of the launch).  Only the FP32
   // CG product uses it: there the compiler then keeps 64 rows' gathers in
@@ -253,10 +628,6 @@ struct MBase {
   // while the FP64 products it was tried on came out 15-40 % slower.
   bool act1 = true;
   __device__ void bind1(int j) { act1 = st->active[j] != 0; }
-  __device__ void act4(int j0, bool (&a)[4]) const {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) a[c] = st->active[j0 + c] != 0;
-  }
   template <class T>
   __device__ static void store4(T *dst, int64_t o, const float4 &v, const bool (&a)[4]) {
     if (a[0] && a[1] && a[2] && a[3]) {
@@ -272,6 +643,8 @@ struct MBase {
 
 __device__ __forceinline__ float4 ld4(const float *p, int64_t o) { return *reinterpret_cast<const float4 *>(p + o); }
 __device__ __forceinline__ float comp(const float4 &v, int c) { return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w; }
+__device__ __forceinline__ float4 scale4(const float4 &v, float d) { return make_float4(v.x * d, v.y * d, v.z * d, v.w * d); }
+
 
 __device__ void request_pause_or_advance(MState *st, int j, double rho_new, double rr, bool cgfam) {
   st->it[j] += 1;
@@ -343,21 +716,29 @@ struct MV : MBase {
   __device__ void begin() {}
   __device__ const T *cscale() const { return dinv; }
   __device__ T x(int c, int j) const { return p[(int64_t)c * k + j]; }
+  // staged rows carry the column scale: A D^-1 p = A (D^-1 p)
+  struct Raw {
+    float4 x;
+    float d;
+  };
+  __device__ Raw load4(int c, int g) const { return Raw{ld4((const float *)p, (int64_t)c * k + 4 * g), dinv[c]}; }
+  __device__ static float4 make4(const Raw &r) { return scale4(r.x, r.d); }
+  __device__ float4 stage4(int c, int g) const { return make4(load4(c, g)); }
   __device__ void row(int64_t i, int j, T y, double *acc) const {
     if (!st->active[j]) return;
     v[i * k + j] = y;
     acc[0] += (double)rhat[i * k + j] * (double)y;
   }
   __device__ float4 x4(int c, int g) const { return ld4((const float *)p, (int64_t)c * k + 4 * g); }
-  __device__ void row4(int64_t i, int j0, const float4 &y, double (&acc)[NP][4]) const {
-    bool a[4];
-    act4(j0, a);
-    const int64_t o = i * k + j0;
-    store4(v, o, y, a);
-    const float4 rh = ld4((const float *)rhat, o);
+  __device__ Pre4 pre4(int64_t i, int j0) const { return Pre4{ld4((const float *)rhat, i * k + j0), {}}; }
+  __device__ void row4p(int64_t i, int j0, const float4 &y, const Pre4 &pr, double (&acc)[NP][4]) const {
+    store4(v, i * k + j0, y, a4c);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      if (a[c]) acc[0][c] += (double)comp(rh, c) * (double)comp(y, c);
+      if (a4c[c]) acc[0][c] += (double)comp(pr.a, c) * (double)comp(y, c);
+  }
+  __device__ void row4(int64_t i, int j0, const float4 &y, double (&acc)[NP][4]) const {
+    row4p(i, j0, y, pre4(i, j0), acc);
   }
   __device__ void last_lane(int j, const double *s) const {
     if (!st->active[j]) return;
@@ -400,6 +781,13 @@ struct MT : MBase {
   __device__ void begin() {}
   __device__ const T *cscale() const { return dinv; }
   __device__ T x(int c, int j) const { return s[(int64_t)c * k + j]; }
+  struct Raw {
+    float4 x;
+    float d;
+  };
+  __device__ Raw load4(int c, int g) const { return Raw{ld4((const float *)s, (int64_t)c * k + 4 * g), dinv[c]}; }
+  __device__ static float4 make4(const Raw &r) { return scale4(r.x, r.d); }
+  __device__ float4 stage4(int c, int g) const { return make4(load4(c, g)); }
   __device__ void row(int64_t i, int j, T y, double *acc) const {
     if (!st->active[j]) return;
     t[i * k + j] = y;
@@ -407,19 +795,19 @@ struct MT : MBase {
     acc[1] += (double)y * (double)s[i * k + j];
   }
   __device__ float4 x4(int c, int g) const { return ld4((const float *)s, (int64_t)c * k + 4 * g); }
-  __device__ void row4(int64_t i, int j0, const float4 &y, double (&acc)[NP][4]) const {
-    bool a[4];
-    act4(j0, a);
-    const int64_t o = i * k + j0;
-    store4(t, o, y, a);
-    const float4 sv = ld4((const float *)s, o);
+  __device__ Pre4 pre4(int64_t i, int j0) const { return Pre4{ld4((const float *)s, i * k + j0), {}}; }
+  __device__ void row4p(int64_t i, int j0, const float4 &y, const Pre4 &pr, double (&acc)[NP][4]) const {
+    store4(t, i * k + j0, y, a4c);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      if (a[c]) {
+      if (a4c[c]) {
         const double yc = comp(y, c);
         acc[0][c] += yc * yc;
-        acc[1][c] += yc * (double)comp(sv, c);
+        acc[1][c] += yc * (double)comp(pr.a, c);
       }
+  }
+  __device__ void row4(int64_t i, int j0, const float4 &y, double (&acc)[NP][4]) const {
+    row4p(i, j0, y, pre4(i, j0), acc);
   }
   __device__ void last_lane(int j, const double *sm) const {
     if (!st->active[j]) return;
@@ -500,6 +888,7 @@ struct MCgA : MBase {
   }
   float b4[4];  // beta of this lane's four right-hand sides (bind4)
   __device__ void bind4(int g) {
+    MBase::bind4(g);
 #pragma unroll
     for (int c = 0; c < 4; ++c) b4[c] = (float)bet(4 * g + c);
   }
@@ -509,16 +898,40 @@ struct MCgA : MBase {
                        fmaf(b4[3], po.w, zz.w));
   }
   __device__ float4 x4(int c, int g) const { return pn4((int64_t)c * k + 4 * g); }
-  __device__ void row4(int64_t i, int j0, const float4 &y, double (&acc)[NP][4]) const {
-    bool a[4];
-    act4(j0, a);
+  // staged rows: p_new = z + beta p_old of the column (betas of bind4s)
+  float b4s[4];  // beta of the four right-hand sides this thread stages (bind4s)
+  __device__ void bind4s(int g) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) b4s[c] = (float)bet(4 * g + c);
+  }
+  struct Raw {
+    float4 po, z;
+  };
+  __device__ Raw load4(int c, int g) const {
+    const int64_t o = (int64_t)c * k + 4 * g;
+    return Raw{ld4((const float *)pold, o), ld4((const float *)z, o)};
+  }
+  __device__ float4 make4(const Raw &r) const {
+    return make_float4(fmaf(b4s[0], r.po.x, r.z.x), fmaf(b4s[1], r.po.y, r.z.y), fmaf(b4s[2], r.po.z, r.z.z),
+                       fmaf(b4s[3], r.po.w, r.z.w));
+  }
+  __device__ float4 stage4(int c, int g) const { return make4(load4(c, g)); }
+  __device__ Pre4 pre4(int64_t i, int j0) const {
     const int64_t o = i * k + j0;
-    const float4 pn = pn4(o);
-    store4(pnew, o, pn, a);
-    store4(q, o, y, a);
+    return Pre4{ld4((const float *)pold, o), ld4((const float *)z, o)};
+  }
+  __device__ void row4p(int64_t i, int j0, const float4 &y, const Pre4 &pr, double (&acc)[NP][4]) const {
+    const int64_t o = i * k + j0;
+    const float4 pn = make_float4(fmaf(b4[0], pr.a.x, pr.b.x), fmaf(b4[1], pr.a.y, pr.b.y),
+                                  fmaf(b4[2], pr.a.z, pr.b.z), fmaf(b4[3], pr.a.w, pr.b.w));
+    store4(pnew, o, pn, a4c);
+    store4(q, o, y, a4c);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      if (a[c]) acc[0][c] += (double)comp(pn, c) * (double)comp(y, c);
+      if (a4c[c]) acc[0][c] += (double)comp(pn, c) * (double)comp(y, c);
+  }
+  __device__ void row4(int64_t i, int j0, const float4 &y, double (&acc)[NP][4]) const {
+    row4p(i, j0, y, pre4(i, j0), acc);
   }
   __device__ void last_lane(int j, const double *s) const {
     if (!st->active[j]) return;
@@ -702,6 +1115,136 @@ struct MReport : MBase {
   }
 };
 
+// Y = A X for the plain product (yafel_hip_spmm): no state, no sums.
+struct MPlain {
+  static constexpr int NP = 0;
+  const float *X;
+  float *Y;
+  int k;
+  double *part;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() {}
+  __device__ void bind4(int) {}
+  __device__ void bind4s(int) {}
+  struct Raw {
+    float4 x;
+  };
+  __device__ Raw load4(int c, int g) const { return Raw{ld4(X, (int64_t)c * k + 4 * g)}; }
+  __device__ static float4 make4(const Raw &r) { return r.x; }
+  __device__ float4 stage4(int c, int g) const { return ld4(X, (int64_t)c * k + 4 * g); }
+  __device__ Pre4 pre4(int64_t, int) const { return Pre4{}; }
+  __device__ void row4p(int64_t i, int j0, const float4 &y, const Pre4 &, double (&)[1][4]) const {
+    *reinterpret_cast<float4 *>(Y + i * k + j0) = y;
+  }
+};
+
+// The CSR-order copy of the staged products (pattern part, once per
+// pattern): each entry's sliced position, the distinct columns of each group
+// of 64 consecutive rows and each entry's position in its group's list.
+// mx_dmax = 0 when a list is too long for 16-bit positions (never staged).
+__global__ void k_take(int64_t n, const int *__restrict__ src, const float *__restrict__ v, float *__restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = src[i] >= 0 ? v[src[i]] : 0.f;
+}
+
+void matrix_build_staged(yafel_hip_matrix_s *A) {
+  if (A->mx_dmax >= 0) return;
+  const int64_t rows = A->brows, nsl = A->nsl, ntot = A->ntot, nnz = A->nblk;
+  hipStream_t st = A->ctx->stream;
+  std::vector<int> ptr(rows + 1), sptr(nsl + 1), col(std::max<int64_t>(ntot, 1)), ip;
+  HIPCHK(hipMemcpyAsync(ptr.data(), A->ptr.p, (rows + 1) * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(sptr.data(), A->sptr.p, (nsl + 1) * 4, hipMemcpyDeviceToHost, st));
+  if (ntot) HIPCHK(hipMemcpyAsync(col.data(), A->col.p, ntot * 4, hipMemcpyDeviceToHost, st));
+  if (A->iperm.p) {
+    ip.resize(rows);
+    HIPCHK(hipMemcpyAsync(ip.data(), A->iperm.p, rows * 4, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  const int64_t ngr = (rows + 63) / 64;
+  // rows padded to a multiple of 4 entries (zero values, position 0)
+  std::vector<int> pp(rows + 1, 0);
+  for (int64_t r = 0; r < rows; ++r) pp[r + 1] = pp[r] + ((ptr[r + 1] - ptr[r] + 3) & ~3);
+  const int64_t npad = pp[rows];
+  REQUIRE(npad < INT32_MAX, "staged products: too many entries");
+  std::vector<int> src(std::max<int64_t>(npad, 1), -1), dptr(ngr + 1), dcol, tmp, ecol(std::max<int64_t>(npad, 1), 0);
+  std::vector<unsigned short> lidx(std::max<int64_t>(npad, 1), 0);
+  dcol.reserve(nnz / 3 + 1);
+  int dmax = 0;
+  int64_t emax = 0;
+  bool fits = true;
+  for (int64_t g = 0; g < ngr; ++g) {
+    const int64_t r0 = 64 * g, r1 = std::min<int64_t>(r0 + 64, rows);
+    tmp.clear();
+    for (int64_t r = r0; r < r1; ++r) {
+      const int64_t i = ip.empty() ? r : ip[r];
+      const int64_t base = (int64_t)sptr[i >> 6] * 64 + (i & 63);
+      for (int t = 0; t < ptr[r + 1] - ptr[r]; ++t) {
+        const int64_t k = base + (int64_t)t * 64;
+        src[pp[r] + t] = (int)k;
+        ecol[pp[r] + t] = col[k];
+        tmp.push_back(col[k]);
+      }
+    }
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    dptr[g] = (int)dcol.size();
+    dcol.insert(dcol.end(), tmp.begin(), tmp.end());
+    dmax = std::max(dmax, (int)tmp.size());
+    emax = std::max<int64_t>(emax, pp[r1] - pp[r0]);
+    if (tmp.size() > 65535) {
+      fits = false;
+      continue;
+    }
+    for (int64_t e = pp[r0]; e < pp[r1]; ++e)
+      if (src[e] >= 0) lidx[e] = (unsigned short)(std::lower_bound(tmp.begin(), tmp.end(), ecol[e]) - tmp.begin());
+  }
+  dptr[ngr] = (int)dcol.size();
+  A->mx_ptr.alloc((rows + 1) * 4);
+  HIPCHK(hipMemcpy(A->mx_ptr.p, pp.data(), (rows + 1) * 4, hipMemcpyHostToDevice));
+  A->mx_src.alloc(src.size() * 4);
+  A->mx_lidx.alloc(lidx.size() * 2);
+  A->mx_dptr.alloc((ngr + 1) * 4);
+  A->mx_dcol.alloc(std::max<size_t>(dcol.size(), 1) * 4);
+  HIPCHK(hipMemcpy(A->mx_src.p, src.data(), src.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(A->mx_lidx.p, lidx.data(), lidx.size() * 2, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(A->mx_dptr.p, dptr.data(), (ngr + 1) * 4, hipMemcpyHostToDevice));
+  if (!dcol.empty()) HIPCHK(hipMemcpy(A->mx_dcol.p, dcol.data(), dcol.size() * 4, hipMemcpyHostToDevice));
+  A->mx_dmax = fits ? dmax : 0;
+  A->mx_emax = emax;
+}
+
+// The staged products' FP32 values, refreshed when the FP32 copy changed.
+void matrix_refresh_staged(yafel_hip_matrix_s *A) {
+  if (A->mx_gen == A->v32_gen) return;
+  const int64_t nnz = (int64_t)(A->mx_src.bytes / 4);
+  if (A->mx_val32.bytes < (size_t)std::max<int64_t>(nnz, 1) * 4) A->mx_val32.alloc(std::max<int64_t>(nnz, 1) * 4);
+  if (nnz) {
+    k_take<<<ceil_div(nnz, 256), 256, 0, A->ctx->stream>>>(nnz, A->mx_src.as<int>(), A->val32.as<float>(),
+                                                          A->mx_val32.as<float>());
+    HIPCHK(hipGetLastError());
+  }
+  A->mx_gen = A->v32_gen;
+}
+
+// True when the staged products run on A (CSR, lists with 16-bit positions).
+bool staged_ok(yafel_hip_matrix_s *A) {
+  if (A->format != YAFEL_HIP_CSR || A->brows == 0) return false;
+  matrix_build_staged(A);
+  return A->mx_dmax > 0;
+}
+
+template <class Op>
+void launch_mspmm_lds(yafel_hip_matrix_s *A, const Op &op, int grid, hipStream_t s) {
+  matrix_refresh_staged(A);
+  const StagedCsr v{A->mx_ptr.as<int>(), A->mx_val32.as<float>(), A->mx_lidx.as<unsigned short>(),
+                    A->mx_dptr.as<int>(), A->mx_dcol.as<int>(), A->brows, std::min(A->mx_dmax, kStageRows)};
+  if (A->mx_dmax <= kStageRows && A->mx_emax <= kStageEnt)
+    k_mspmm_pipe<Op><<<grid, NT, (size_t)v.dcap * KC * sizeof(float), s>>>(v, op);
+  else
+    k_mspmm_lds<Op><<<grid, NT, (size_t)v.dcap * KC * sizeof(float), s>>>(v, op);
+  HIPCHK(hipGetLastError());
+}
+
 template <class T>
 CsrView<T> cview(yafel_hip_matrix_s *A) {
   return CsrView<T>{A->sptr.as<int>(), A->perm.p ? A->perm.as<int>() : nullptr, A->col.as<int>(),
@@ -721,6 +1264,7 @@ struct Multi {
   int grid;
   DBuf vecs[9];
   DBuf partbuf, stbuf;
+  bool staged = false;  // FP32 products stage X in LDS
 
   template <class O> O base(O op, int c) const {
     op.st = st;
@@ -734,10 +1278,17 @@ struct Multi {
     // grid
     const bool v4 = sizeof(T) == 4 && k % 4 == 0;
     const int64_t units = A->nsl * (sizeof(T) == 4 && !v4 ? 1 : 2);
-    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((units + 3) / 4, grid));
+    int g = (int)std::max<int64_t>(1, std::min<int64_t>((units + 3) / 4, grid));
     if constexpr (sizeof(T) == 4) {
-      if (v4) k_mspmm4<O><<<g, NT, 0, s>>>(cview<T>(A), op);
-      else k_mspmm<T, O><<<g, NT, 0, s>>>(cview<T>(A), op);
+      if (staged && v4) {
+        // one group of 64 rows and one pass of 32 right-hand sides per workgroup at a time
+        g = (int)std::max<int64_t>(1, std::min<int64_t>((A->brows + 63) / 64, grid));
+        launch_mspmm_lds(A, op, g, s);
+      } else if (v4) {
+        k_mspmm4<O><<<g, NT, 0, s>>>(cview<T>(A), op);
+      } else {
+        k_mspmm<T, O><<<g, NT, 0, s>>>(cview<T>(A), op);
+      }
     } else {
       k_mspmm<T, O><<<g, NT, 0, s>>>(cview<T>(A), op);
     }
@@ -875,6 +1426,17 @@ struct Multi {
 
 }  // namespace
 
+namespace yh {
+bool spmm_lds_launch(yafel_hip_matrix_s *A, int k, const float *X, float *Y, hipStream_t s) {
+  // float4 staging reads need 16-byte aligned rows of X
+  if (k < 1 || k > KMAX || k % 4 != 0 || ((uintptr_t)X % 16) != 0 || ((uintptr_t)Y % 16) != 0 || !staged_ok(A))
+    return false;
+  MPlain op{X, Y, k, nullptr};
+  launch_mspmm_lds(A, op, (int)std::min<int64_t>((A->brows + 63) / 64, (int64_t)A->ctx->num_cus * 8), s);
+  return true;
+}
+}  // namespace yh
+
 extern "C" int yafel_hip_solve_multi(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, int k,
                                      const double *B_dev, double *X_dev, yafel_hip_solve_info *info) {
   YH_TRY_BEGIN
@@ -895,6 +1457,7 @@ extern "C" int yafel_hip_solve_multi(yafel_hip_matrix A, const yafel_hip_solve_o
   if (!A->diag_ok) matrix_build_diag(A);
   const bool mixed = opts->precision == YAFEL_HIP_MIXED;
   if (mixed && !A->has32) matrix_build_fp32(A);
+  const bool staged = mixed && staged_ok(A);  // the FP32 products stage X in LDS
   HIPCHK(hipEventRecord(ctx->t0, ctx->stream));
   // batches of up to 64 right-hand sides, each a contiguous column block;
   // the batch is copied into a packed n x kb buffer and back
@@ -907,6 +1470,7 @@ extern "C" int yafel_hip_solve_multi(yafel_hip_matrix A, const yafel_hip_solve_o
     M.n = n;
     M.k = kb;
     M.mixed = mixed;
+    M.staged = staged;
     M.cgfam = opts->method == YAFEL_HIP_CG || opts->method == YAFEL_HIP_PCG_JACOBI;
     M.prec = opts->method == YAFEL_HIP_PCG_JACOBI || opts->method == YAFEL_HIP_BICGSTAB_JACOBI;
     if (!M.prec && !M.cgfam) fail(YAFEL_HIP_EINVAL, "solve_multi: BiCGSTAB runs with right Jacobi preconditioning only");

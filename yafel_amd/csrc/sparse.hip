@@ -483,6 +483,7 @@ void spmm_launch(yafel_hip_matrix_s *A, int precision, int k, const void *X, voi
   const int gcsr = (int)std::min<int64_t>((units + 3) / 4, (int64_t)A->ctx->num_cus * 8);
   if (precision == YAFEL_HIP_FP32) {
     REQUIRE(A->has32, "spmm: matrix has no FP32 copy");
+    if (spmm_lds_launch(A, k, (const float *)X, (float *)Y, s)) return;
     if (A->format == YAFEL_HIP_CSR && k % 4 == 0 && ((uintptr_t)X % 16) == 0 && ((uintptr_t)Y % 16) == 0)
       k_csr_spmm4<<<(int)std::min<int64_t>((A->nsl * 2 + 3) / 4, (int64_t)A->ctx->num_cus * 8), 256, 0, s>>>(
           A->rows, k, sp, pm, c, A->val32.as<float>(), (const float *)X, (float *)Y);
@@ -511,6 +512,7 @@ void matrix_build_fp32(yafel_hip_matrix_s *A) {
     HIPCHK(hipGetLastError());
   }
   A->has32 = true;
+  A->v32_gen++;
 }
 
 void matrix_build_diag(yafel_hip_matrix_s *A) {
