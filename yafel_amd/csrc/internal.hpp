@@ -162,9 +162,11 @@ struct yafel_hip_matrix_s {
   yh::DBuf col;                 // int32 sliced block columns (ntot)
   yh::DBuf col16;               // int16 column - stored position (ntot), when every entry fits; else empty
   // CSR-order copy for the FP32 multi-RHS products that stage rows of X in
-  // LDS (built on first use): rows in groups of 64 consecutive rows, each
-  // group's distinct columns, and every entry's position in its group's list
-  yh::DBuf mx_ptr;              // int32 (rows+1): row pointer, every row padded to a multiple of 4 entries
+  // LDS (built on first use): rows in groups of up to 64 grown breadth-first
+  // over the pattern, 64 positions per group (rows longest first, then
+  // empty), each group's distinct columns, and every entry's position in its
+  // group's list
+  yh::DBuf mx_ptr;              // int32 (positions+1): entry pointer, every row padded to a multiple of 4 entries
   yh::DBuf mx_src;              // int32: sliced position of each entry in CSR order (-1: padding)
   yh::DBuf mx_lidx;             // uint16: position of the entry's column in its group's list (0: padding)
   yh::DBuf mx_dptr;             // int32 (groups+1): start of each group's list in mx_dcol
@@ -174,6 +176,11 @@ struct yafel_hip_matrix_s {
   int64_t mx_emax = 0;          // most (padded) entries in a group
   int64_t v32_gen = 0;          // bumped whenever val32 is rebuilt
   int64_t mx_gen = -1;          // the val32 generation mx_val32 holds
+  yh::DBuf mx_grow;             // row of each staged position (64 per group), -1 when empty
+  int64_t mx_ngr = 0;           // staged groups
+  yh::DBuf mx_val32s;           // staged values times their column's D^-1
+  int64_t mx_sgen = -1, mx_dgen = -1;  // the val32 and diagonal generations mx_val32s holds
+  int64_t diag_gen = 0;         // bumped whenever the diagonal is rebuilt
   yh::DBuf val64, val32;        // CSR: ntot; BCSR3: 9 arrays of ntot (v[e * ntot + k], e = 3i+j)
   yh::DBuf diag64, dinv64, dinv32;
   bool has64 = false, has32 = false, diag_ok = false;

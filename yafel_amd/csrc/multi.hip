@@ -95,12 +95,22 @@ __global__ __launch_bounds__(NTM) void k_mfinish(Op op0, int nblk) {
   __shared__ double lds[NP][NW][64];
   __shared__ double sums[NP][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  // thread (w, lane) sums blocks w, w+16, ... ; then the 16 parts in order
+  // thread (w, lane) sums blocks w, w+16, ... in order; then the 16 parts in
+  // order.  Up to 32 of a thread's loads are issued before the first add, so
+  // a fold over 512 blocks is one round trip per sum instead of four.
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     double s = 0;
-#pragma unroll 8
-    for (int b = w; b < nblk; b += NW) s += op.part[((size_t)j * nblk + b) * 64 + lane];
+    for (int b0 = w; b0 < nblk; b0 += 32 * NW) {
+      double v[32];
+#pragma unroll
+      for (int m = 0; m < 32; ++m) {
+        const int b = b0 + m * NW;
+        v[m] = b < nblk ? op.part[((size_t)j * nblk + b) * 64 + lane] : 0.0;
+      }
+#pragma unroll
+      for (int m = 0; m < 32; ++m) s += v[m];
+    }
     lds[j][w][lane] = s;
   }
   __syncthreads();
@@ -223,33 +233,41 @@ __global__ __launch_bounds__(NT) void k_mspmm4(CsrView<float> A, Op op0) {
 }
 
 // FP32 products with rows of X staged in LDS (k a multiple of 4).  Rows go
-// in groups of 64 consecutive rows (CSR order, whatever the sliced storage's
-// row order); a group touches far fewer distinct columns than it has entries
-// (about 340 against 1,000 on a Q2 mesh), so a workgroup takes one group and
-// one pass of 32 right-hand sides at a time:
+// in groups of up to 64, each grown breadth-first over the pattern from the
+// lowest unassigned row, so that a group is a compact patch of the mesh: it
+// touches far fewer distinct columns than it has entries (about 130 against
+// 1,000 on a Q2 mesh, where 64 consecutive rows of a 1025-node-wide grid
+// touch 340), so a workgroup takes one group and one pass of 32 right-hand
+// sides at a time.  A group's 64 positions hold its rows longest first (empty
+// positions: row -1, no entries):
 //  1. stage: the X rows of the group's distinct columns (mx_dcol), 128 bytes
 //     each, transformed as the op transforms a gathered row (op.stage4: the
 //     column's Jacobi scale, p = z + beta p, ...), every load of a thread
 //     issued before its first LDS store;
 //  2. rows: lane (rho = lane >> 3, q = lane & 7) takes right-hand sides
-//     4q .. 4q+3 of the pass for row rho of a set of 8 rows two apart (the
-//     same node type on a structured mesh, so rows of one length); the lane
-//     loads its row's values and local indices (mx_lidx) in CSR order and
-//     reads 16 bytes of staged x per entry, so one LDS read feeds four FMAs.
-//     A row is summed in column order.
+//     4q .. 4q+3 of the pass for position rho of a set of 8 consecutive
+//     positions (rows of about one length); wave w takes sets w and 7 - w, a
+//     long and a short one.  The lane reads its row's values and local
+//     indices (mx_lidx) from the group's copy in CSR order and 16 bytes of
+//     staged x per entry, so one LDS read feeds four FMAs.  A row is summed
+//     in column order.
 // A group with more distinct columns than the LDS holds reads x directly.
 constexpr int kStageRows = 384;  // distinct columns staged per group: 48 KiB at 32 right-hand sides
 constexpr int KC = 32;           // right-hand sides per pass
 constexpr int kStageIter = kStageRows / (NT / 8);
 
 struct StagedCsr {
-  const int *ptr;  // CSR row pointer
+  const int *ptr;  // entry pointer per position (64 per group)
   const float *val;
   const unsigned short *lidx;
   const int *dptr, *dcol;
-  int64_t rows;
-  int dcap;  // rows of X the launch's LDS holds
+  const int *grow;  // row of each position, -1 when empty
+  int64_t ngr;      // groups
+  int dcap;         // rows of X the launch's LDS holds
 };
+
+// the row set of sigma = 2w + m: sets w and 7 - w of wave w
+__device__ __forceinline__ int staged_set(int w, int m) { return m == 0 ? w : 7 - w; }
 
 // Per-RHS partials of the staged products.  Lane (rho, q) holds right-hand
 // sides 4q .. 4q+3 of each pass (a0: first pass, a1: second); the 8 rows of
@@ -307,15 +325,14 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds(StagedCsr A, Op op0) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 7, rho = lane >> 3;
   const int k = op.k;
   const int nh = (k + KC - 1) / KC;
-  const int64_t ngr = (A.rows + 63) >> 6;
-  for (int64_t g = dev::xcd_block(); g < ngr; g += gridDim.x) {
+  for (int64_t g = dev::xcd_block(); g < A.ngr; g += gridDim.x) {
     const int d0 = A.dptr[g], d = A.dptr[g + 1] - d0;
-    const int64_t r0 = g * 64, r1 = r0 + 64 < A.rows ? r0 + 64 : A.rows;
-    const int e0 = A.ptr[r0], ne = A.ptr[r1] - e0;
+    const int64_t r0 = g * 64;  // first position of the group
+    const int e0 = A.ptr[r0], ne = A.ptr[r0 + 64] - e0;
     const bool staged = d <= A.dcap && ne <= kStageEnt;
     if (staged) {
       __syncthreads();  // every wave is done with the previous group
-      if (tid < 65) ep[tid] = A.ptr[r0 + tid < r1 ? r0 + tid : r1] - e0;
+      if (tid < 65) ep[tid] = A.ptr[r0 + tid] - e0;
       float vv[kStageEnt / NT];
       unsigned short ll[kStageEnt / NT];
 #pragma unroll
@@ -338,7 +355,6 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds(StagedCsr A, Op op0) {
       const bool on = 4 * jq < k;
       op.bind4(on ? jq : j0 / 4);
       op.bind4s(on ? jq : j0 / 4);
-      const int sigma0 = 2 * w;  // the wave's row sets: rows a, a + 2, ..., a + 14
       if (staged) {
         if (h > 0) __syncthreads();  // every wave is done with the previous pass's x
         if (on) {
@@ -361,14 +377,13 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds(StagedCsr A, Op op0) {
         __syncthreads();
 #pragma unroll 1
         for (int m = 0; m < 2; ++m) {
-          const int sigma = sigma0 + m;
-          const int rl = (sigma >> 1) * 16 + (sigma & 1) + 2 * rho;  // row within the group
-          const int64_t r = r0 + rl;
+          const int rl = staged_set(w, m) * 8 + rho;  // position within the group
+          const int64_t r = A.grow[r0 + rl];
           // rows are padded to a multiple of 4 entries (zero values), so a
           // lane reads its row four entries at a time: 16 bytes of values
           // and 8 bytes of positions
-          const int b = ep[rl], n4 = r < A.rows ? (ep[rl + 1] - b) >> 2 : 0;
-          const Pre4 pr = op.pre4(r < A.rows ? r : A.rows - 1, 4 * (on ? jq : j0 / 4));
+          const int b = ep[rl], n4 = r >= 0 ? (ep[rl + 1] - b) >> 2 : 0;
+          const Pre4 pr = op.pre4(r >= 0 ? r : 0, 4 * (on ? jq : j0 / 4));
           float4 y = make_float4(0.f, 0.f, 0.f, 0.f);
           const float4 *ev4 = reinterpret_cast<const float4 *>(ev + b);
           const uint2 *el4 = reinterpret_cast<const uint2 *>(el + b);
@@ -386,7 +401,7 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds(StagedCsr A, Op op0) {
               y.w = fmaf(vv[e], x.w, y.w);
             }
           }
-          if (r < A.rows && on) {
+          if (r >= 0 && on) {
             if (h == 0) op.row4p(r, 4 * jq, y, pr, a0);
             else op.row4p(r, 4 * jq, y, pr, a1);
           }
@@ -395,11 +410,11 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds(StagedCsr A, Op op0) {
         // direct reads of x for groups too large to stage
 #pragma unroll 1
         for (int m = 0; m < 2; ++m) {
-          const int sigma = sigma0 + m;
-          const int64_t r = r0 + (sigma >> 1) * 16 + (sigma & 1) + 2 * rho;
+          const int64_t pos = r0 + staged_set(w, m) * 8 + rho;
+          const int64_t r = A.grow[pos];
           float4 y = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (r < A.rows) {
-            const int b = A.ptr[r], n = A.ptr[r + 1] - b;
+          if (r >= 0) {
+            const int b = A.ptr[pos], n = A.ptr[pos + 1] - b;
             for (int t = 0; t < n; ++t) {
               const float vv = A.val[b + t];
               const float4 x = op.stage4(A.dcol[d0 + A.lidx[b + t]], on ? jq : j0 / 4);
@@ -409,7 +424,7 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds(StagedCsr A, Op op0) {
               y.w = fmaf(vv, x.w, y.w);
             }
           }
-          if (r < A.rows && on) {
+          if (r >= 0 && on) {
             if (h == 0) op.row4p(r, 4 * jq, y, op.pre4(r, 4 * jq), a0);
             else op.row4p(r, 4 * jq, y, op.pre4(r, 4 * jq), a1);
           }
@@ -426,12 +441,15 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds(StagedCsr A, Op op0) {
 // X -- and at a group's second pass, of the next group's entries and list --
 // are already in flight into registers, and are stored to LDS after the next
 // barrier.  Same rows, sums and order as k_mspmm_lds.
+// Two waves per SIMD: the LDS of two workgroups per CU is what the staged
+// lists were sized for, and the scheduler may then keep more loads in flight.
 template <class Op>
-__global__ __launch_bounds__(NT) void k_mspmm_pipe(StagedCsr A, Op op0) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mspmm_pipe(StagedCsr A, Op op0) {
   extern __shared__ float4 xs4[];  // A.dcap rows of KC / 4
   __shared__ __align__(16) float ev[kStageEnt];
   __shared__ __align__(16) unsigned short el[kStageEnt];
   __shared__ int ep[65];
+  __shared__ int grw[64];  // row of each position of the group
   constexpr int NP = Op::NP, SE = kStageEnt / NT;
   Op op = op0;
   if (op.skip()) return;
@@ -445,7 +463,7 @@ __global__ __launch_bounds__(NT) void k_mspmm_pipe(StagedCsr A, Op op0) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 7, rho = lane >> 3, i0 = tid >> 3;
   const int k = op.k;
   const int nh = (k + KC - 1) / KC;
-  const int64_t ngr = (A.rows + 63) >> 6;
+  const int64_t ngr = A.ngr;
   int64_t g = dev::xcd_block();
   if (g >= ngr) {
     if constexpr (NP > 0) finish_staged<NP>(op, a0, a1);
@@ -460,17 +478,15 @@ __global__ __launch_bounds__(NT) void k_mspmm_pipe(StagedCsr A, Op op0) {
   Raw px[kStageIter];
   float pv[SE];
   unsigned short pl[SE];
-  int pp = 0, d0 = 0, d = 0, ne = 0;
-  auto load_meta = [&](int64_t gg) {  // the group's list and entries
+  int pp = 0, prow = 0, d0 = 0, d = 0, ne = 0;
+  auto load_meta = [&](int64_t gg) {  // the group's list, entries and rows
     d0 = A.dptr[gg];
     d = A.dptr[gg + 1] - d0;
-    const int64_t r0 = gg * 64, r1 = r0 + 64 < A.rows ? r0 + 64 : A.rows;
+    const int64_t r0 = gg * 64;
     const int e0 = A.ptr[r0];
-    ne = A.ptr[r1] - e0;
-    {
-      const int64_t rp = r0 + (tid < 64 ? tid : 64);
-      pp = A.ptr[rp < r1 ? rp : r1] - e0;  // used by threads 0..64
-    }
+    ne = A.ptr[r0 + 64] - e0;
+    pp = A.ptr[r0 + (tid < 64 ? tid : 64)] - e0;  // used by threads 0..64
+    prow = A.grow[r0 + (tid & 63)];                // used by threads 0..63
 #pragma unroll
     for (int m = 0; m < SE; ++m) {
       const int i = tid + m * NT;
@@ -481,7 +497,7 @@ __global__ __launch_bounds__(NT) void k_mspmm_pipe(StagedCsr A, Op op0) {
 #pragma unroll
     for (int m = 0; m < kStageIter; ++m) {
       const int i = i0 + m * (NT / 8);
-      pc[m] = A.dcol[d0 + (i < d ? i : 0)];
+      pc[m] = A.dcol[d > 0 ? d0 + (i < d ? i : 0) : 0];
     }
   };
   auto load_x = [&](int h) {  // quad q of the pass's rows i0, i0 + 32, ... (raw)
@@ -510,6 +526,7 @@ __global__ __launch_bounds__(NT) void k_mspmm_pipe(StagedCsr A, Op op0) {
     __builtin_amdgcn_s_waitcnt(0x0f70);
     if (h == 0) {
       if (tid < 65) ep[tid] = pp;
+      if (tid < 64) grw[tid] = prow;
 #pragma unroll
       for (int m = 0; m < SE; ++m) {
         const bool in = tid + m * NT < ne;
@@ -531,14 +548,13 @@ __global__ __launch_bounds__(NT) void k_mspmm_pipe(StagedCsr A, Op op0) {
     // the two row sets of the wave (rows a, a + 2, ..., a + 14) and their
     // epilogue operands, loaded before the next unit's loads: vector loads
     // complete in issue order, so a wait for these must not wait for those
-    int rls[2];
+    int rls[2], rws[2];
     Pre4 prs[2];
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-      const int sigma = 2 * w + m;
-      rls[m] = (sigma >> 1) * 16 + (sigma & 1) + 2 * rho;
-      const int64_t r = g * 64 + rls[m];
-      prs[m] = op.pre4(r < A.rows ? r : A.rows - 1, 4 * (on ? jq : j0 / 4));
+      rls[m] = staged_set(w, m) * 8 + rho;
+      rws[m] = grw[rls[m]];
+      prs[m] = op.pre4(rws[m] >= 0 ? rws[m] : 0, 4 * (on ? jq : j0 / 4));
     }
     // the next unit's loads, in flight during this unit's rows
     const int64_t gn = g + gridDim.x;
@@ -548,8 +564,8 @@ __global__ __launch_bounds__(NT) void k_mspmm_pipe(StagedCsr A, Op op0) {
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
       const int rl = rls[m];
-      const int64_t r = g * 64 + rl;
-      const int b = ep[rl], n4 = r < A.rows ? (ep[rl + 1] - b) >> 2 : 0;
+      const int64_t r = rws[m];
+      const int b = ep[rl], n4 = r >= 0 ? (ep[rl + 1] - b) >> 2 : 0;
       float4 y = make_float4(0.f, 0.f, 0.f, 0.f);
       const float4 *ev4 = reinterpret_cast<const float4 *>(ev + b);
       const uint2 *el4 = reinterpret_cast<const uint2 *>(el + b);
@@ -558,16 +574,24 @@ __global__ __launch_bounds__(NT) void k_mspmm_pipe(StagedCsr A, Op op0) {
         const uint2 l4 = el4[t];
         const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
         const int li[4] = {(int)(l4.x & 0xffffu), (int)(l4.x >> 16), (int)(l4.y & 0xffffu), (int)(l4.y >> 16)};
+        float4 xq[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xq[e] = xs4[li[e] * (KC / 4) + q];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const float4 x = xs4[li[e] * (KC / 4) + q];
-          y.x = fmaf(vv[e], x.x, y.x);
-          y.y = fmaf(vv[e], x.y, y.y);
-          y.z = fmaf(vv[e], x.z, y.z);
-          y.w = fmaf(vv[e], x.w, y.w);
+          y.x = fmaf(vv[e], xq[e].x, y.x);
+          y.y = fmaf(vv[e], xq[e].y, y.y);
+          y.z = fmaf(vv[e], xq[e].z, y.z);
+          y.w = fmaf(vv[e], xq[e].w, y.w);
         }
+        // the four staged x reads issued together, then the FMAs: one LDS
+        // round trip per four entries instead of one per entry
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // entries (LDS read)
+        __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);  // positions -> addresses
+        __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);  // staged x (LDS read)
+        __builtin_amdgcn_sched_group_barrier(0x002, 16, 0); // FMAs
       }
-      if (r < A.rows && on) {
+      if (r >= 0 && on) {
         if (h == 0) opb[0].row4p(r, 4 * jq, y, prs[m], a0);
         else opb[1].row4p(r, 4 * jq, y, prs[m], a1);
       }
@@ -603,7 +627,81 @@ __global__ __launch_bounds__(NT) void k_melem(int64_t n, Op op0) {
   finish<NP>(op, acc);
 }
 
+// FP32 vector passes with k a multiple of 4.  Lane (rs = lane >> 4,
+// g = lane & 15) takes right-hand sides 4g .. 4g+3 of rows 4i + rs, so every
+// load and store is 16 bytes per lane.  A workgroup owns `per` consecutive
+// units of 64 rows and walks them ME4 rows per wave at a time: every load of
+// those rows (op.load4) is issued before the first store (op.apply4), which
+// keeps 16 rows' operands in flight per wave where a row-at-a-time loop had
+// one.  The lane's active flags and step scalars are read once (op.bind4):
+// they change only in the fold after the pass.  Workgroups of 1024 threads,
+// two per CU, so that the fold after the pass reads a quarter of the
+// partials that 256-thread workgroups would leave.
+constexpr int NT4 = 1024;   // threads of k_melem4
+
+template <class Op>
+__global__ __launch_bounds__(NT4) void k_melem4(int64_t n, int64_t per, Op op0) {
+  // ME4 row sets of 64 rows (16 waves x 4 rows) per step: 4, or 2 for the
+  // passes with five operands per row, within 128 registers
+  constexpr int NP = Op::NP, NPA = NP > 0 ? NP : 1, RS = (NT4 / 64) * 4, ME4 = Op::ME4;
+  Op op = op0;
+  if (op.skip()) return;
+  op.begin();
+  double acc[NPA][4];
+#pragma unroll
+  for (int j = 0; j < NPA; ++j)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[j][c] = 0;
+  const int lane = threadIdx.x & 63, g = lane & 15, rs = lane >> 4, w = threadIdx.x >> 6;
+  const bool on = g < (op.k >> 2);
+  const int gl = on ? g : 0;
+  op.bind4(gl);
+  const int64_t r0 = (int64_t)dev::xcd_block() * per * 64;
+  const int64_t r1 = r0 + per * 64 < n ? r0 + per * 64 : n;
+  for (int64_t rb = r0 + w * 4 + rs; rb < r1; rb += RS * ME4) {
+    typename Op::In in[ME4];
+#pragma unroll
+    for (int u = 0; u < ME4; ++u) {
+      const int64_t r = rb + u * RS < r1 ? rb + u * RS : rb;  // in bounds; not applied
+      in[u] = op.load4(r, r * op.k + 4 * gl);
+    }
+#pragma unroll
+    for (int u = 0; u < ME4; ++u) {
+      const int64_t r = rb + u * RS;
+      if (r < r1 && on) op.apply4(r, r * op.k + 4 * gl, in[u], acc);
+    }
+  }
+  if constexpr (NP > 0) {
+    // the four rows of a wave folded by shuffles (fixed order), then the
+    // waves through LDS, into the same per-block partials as finish()
+    __shared__ double lds[NP][NT4 / 64][64];
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        double v = acc[j][c];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (rs == 0) lds[j][w][4 * g + c] = v;
+      }
+    __syncthreads();
+    if (w == 0) {
+      const int blk = dev::xcd_block();
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        double v = 0;
+#pragma unroll
+        for (int q = 0; q < NT4 / 64; ++q) v += lds[j][q][lane];
+        op.part[((size_t)j * gridDim.x + blk) * 64 + lane] = v;
+      }
+    }
+  }
+}
+
 struct MBase {
+  static constexpr bool V4 = false;           // has load4 / apply4 for k_melem4
+  static constexpr int ME4 = 4;               // k_melem4 row sets per step
+  static constexpr bool kScaledVals = false;  // staged products read A D^-1
   MState *st;
   double *part;
   unsigned int *counter;
@@ -703,6 +801,27 @@ struct MP : MBase {
     const T beta = st->first ? T(0) : (T)st->beta[j], om = (T)st->omega[j];
     p[o] = r[o] + beta * (p[o] - om * v[o]);
   }
+  static constexpr bool V4 = sizeof(T) == 4;
+  float be4[4], om4[4];
+  __device__ void bind4(int g) {
+    MBase::bind4(g);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      be4[c] = st->first ? 0.f : (float)st->beta[4 * g + c];
+      om4[c] = (float)st->omega[4 * g + c];
+    }
+  }
+  struct In {
+    float4 r, p, v;
+  };
+  __device__ In load4(int64_t, int64_t o) const {
+    return In{ld4((const float *)r, o), ld4((const float *)p, o), ld4((const float *)v, o)};
+  }
+  __device__ void apply4(int64_t, int64_t o, const In &a, double (&)[1][4]) const {
+    const float4 pn = make_float4(a.r.x + be4[0] * (a.p.x - om4[0] * a.v.x), a.r.y + be4[1] * (a.p.y - om4[1] * a.v.y),
+                                  a.r.z + be4[2] * (a.p.z - om4[2] * a.v.z), a.r.w + be4[3] * (a.p.w - om4[3] * a.v.w));
+    store4((float *)p, o, pn, a4c);
+  }
   __device__ void last_lane(int, const double *) const {}
 };
 
@@ -716,14 +835,13 @@ struct MV : MBase {
   __device__ void begin() {}
   __device__ const T *cscale() const { return dinv; }
   __device__ T x(int c, int j) const { return p[(int64_t)c * k + j]; }
-  // staged rows carry the column scale: A D^-1 p = A (D^-1 p)
-  struct Raw {
-    float4 x;
-    float d;
-  };
-  __device__ Raw load4(int c, int g) const { return Raw{ld4((const float *)p, (int64_t)c * k + 4 * g), dinv[c]}; }
-  __device__ static float4 make4(const Raw &r) { return scale4(r.x, r.d); }
-  __device__ float4 stage4(int c, int g) const { return make4(load4(c, g)); }
+  // the staged products read A D^-1 (the values scaled by their column's
+  // D^-1 at set-up), so the staged rows of p are p itself
+  static constexpr bool kScaledVals = true;
+  using Raw = float4;
+  __device__ Raw load4(int c, int g) const { return ld4((const float *)p, (int64_t)c * k + 4 * g); }
+  __device__ static float4 make4(const Raw &r) { return r; }
+  __device__ float4 stage4(int c, int g) const { return load4(c, g); }
   __device__ void row(int64_t i, int j, T y, double *acc) const {
     if (!st->active[j]) return;
     v[i * k + j] = y;
@@ -768,6 +886,22 @@ struct MS : MBase {
     const int64_t o = i * k + j;
     s[o] = r[o] - v[o] * (T)st->alpha[j];
   }
+  static constexpr bool V4 = sizeof(T) == 4;
+  float al4[4];
+  __device__ void bind4(int g) {
+    MBase::bind4(g);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) al4[c] = (float)st->alpha[4 * g + c];
+  }
+  struct In {
+    float4 r, v;
+  };
+  __device__ In load4(int64_t, int64_t o) const { return In{ld4((const float *)r, o), ld4((const float *)v, o)}; }
+  __device__ void apply4(int64_t, int64_t o, const In &a, double (&)[1][4]) const {
+    const float4 sn = make_float4(a.r.x - a.v.x * al4[0], a.r.y - a.v.y * al4[1], a.r.z - a.v.z * al4[2],
+                                  a.r.w - a.v.w * al4[3]);
+    store4((float *)s, o, sn, a4c);
+  }
   __device__ void last_lane(int, const double *) const {}
 };
 
@@ -781,13 +915,12 @@ struct MT : MBase {
   __device__ void begin() {}
   __device__ const T *cscale() const { return dinv; }
   __device__ T x(int c, int j) const { return s[(int64_t)c * k + j]; }
-  struct Raw {
-    float4 x;
-    float d;
-  };
-  __device__ Raw load4(int c, int g) const { return Raw{ld4((const float *)s, (int64_t)c * k + 4 * g), dinv[c]}; }
-  __device__ static float4 make4(const Raw &r) { return scale4(r.x, r.d); }
-  __device__ float4 stage4(int c, int g) const { return make4(load4(c, g)); }
+  // staged products: values of A D^-1, rows of s as they are
+  static constexpr bool kScaledVals = true;
+  using Raw = float4;
+  __device__ Raw load4(int c, int g) const { return ld4((const float *)s, (int64_t)c * k + 4 * g); }
+  __device__ static float4 make4(const Raw &r) { return r; }
+  __device__ float4 stage4(int c, int g) const { return load4(c, g); }
   __device__ void row(int64_t i, int j, T y, double *acc) const {
     if (!st->active[j]) return;
     t[i * k + j] = y;
@@ -838,6 +971,42 @@ struct MX : MBase {
     r[o] = ri;
     acc[0] += (double)rhat[o] * (double)ri;
     acc[1] += (double)ri * (double)ri;
+  }
+  static constexpr bool V4 = sizeof(T) == 4;
+  static constexpr int ME4 = 2;
+  float al4[4], om4[4];
+  __device__ void bind4(int g) {
+    MBase::bind4(g);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      al4[c] = (float)st->alpha[4 * g + c];
+      om4[c] = (float)st->omega[4 * g + c];
+    }
+  }
+  struct In {
+    float4 x, p, s, t, rh;
+    float d;
+  };
+  __device__ In load4(int64_t i, int64_t o) const {
+    return In{ld4((const float *)x, o), ld4((const float *)p, o), ld4((const float *)s, o), ld4((const float *)t, o),
+              ld4((const float *)rhat, o), (float)dinv[i]};
+  }
+  __device__ void apply4(int64_t, int64_t o, const In &a, double (&acc)[2][4]) const {
+    const float xv[4] = {a.x.x, a.x.y, a.x.z, a.x.w}, pv[4] = {a.p.x, a.p.y, a.p.z, a.p.w};
+    const float sv[4] = {a.s.x, a.s.y, a.s.z, a.s.w}, tv[4] = {a.t.x, a.t.y, a.t.z, a.t.w};
+    const float hv[4] = {a.rh.x, a.rh.y, a.rh.z, a.rh.w};
+    float xn[4], rn[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      xn[c] = xv[c] + ((a.d * pv[c]) * al4[c] + (a.d * sv[c]) * om4[c]);
+      rn[c] = sv[c] - tv[c] * om4[c];
+      if (a4c[c]) {
+        acc[0][c] += (double)hv[c] * (double)rn[c];
+        acc[1][c] += (double)rn[c] * (double)rn[c];
+      }
+    }
+    store4((float *)x, o, make_float4(xn[0], xn[1], xn[2], xn[3]), a4c);
+    store4((float *)r, o, make_float4(rn[0], rn[1], rn[2], rn[3]), a4c);
   }
   __device__ void last_lane(int j, const double *sm) const {
     if (!st->active[j]) return;
@@ -973,6 +1142,40 @@ struct MCgB : MBase {
     }
     acc[0] += (double)ri * (double)zi;
     acc[1] += (double)ri * (double)ri;
+  }
+  static constexpr bool V4 = sizeof(T) == 4;
+  static constexpr int ME4 = 2;
+  float al4[4];
+  __device__ void bind4(int g) {
+    MBase::bind4(g);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) al4[c] = (float)st->alpha[4 * g + c];
+  }
+  struct In {
+    float4 x, p, q, r;
+    float d;
+  };
+  __device__ In load4(int64_t i, int64_t o) const {
+    return In{ld4((const float *)x, o), ld4((const float *)p, o), ld4((const float *)q, o), ld4((const float *)r, o),
+              prec ? (float)dinv[i] : 1.f};
+  }
+  __device__ void apply4(int64_t, int64_t o, const In &a, double (&acc)[2][4]) const {
+    const float xv[4] = {a.x.x, a.x.y, a.x.z, a.x.w}, pv[4] = {a.p.x, a.p.y, a.p.z, a.p.w};
+    const float qv[4] = {a.q.x, a.q.y, a.q.z, a.q.w}, rv[4] = {a.r.x, a.r.y, a.r.z, a.r.w};
+    float xn[4], rn[4], zn[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      xn[c] = fmaf(al4[c], pv[c], xv[c]);
+      rn[c] = fmaf(-al4[c], qv[c], rv[c]);
+      zn[c] = prec ? a.d * rn[c] : rn[c];
+      if (a4c[c]) {
+        acc[0][c] += (double)rn[c] * (double)zn[c];
+        acc[1][c] += (double)rn[c] * (double)rn[c];
+      }
+    }
+    store4((float *)x, o, make_float4(xn[0], xn[1], xn[2], xn[3]), a4c);
+    store4((float *)r, o, make_float4(rn[0], rn[1], rn[2], rn[3]), a4c);
+    if (prec) store4((float *)z, o, make_float4(zn[0], zn[1], zn[2], zn[3]), a4c);
   }
   __device__ void last_lane(int j, const double *sm) const {
     if (!st->active[j]) return;
@@ -1118,6 +1321,7 @@ struct MReport : MBase {
 // Y = A X for the plain product (yafel_hip_spmm): no state, no sums.
 struct MPlain {
   static constexpr int NP = 0;
+  static constexpr bool kScaledVals = false;
   const float *X;
   float *Y;
   int k;
@@ -1147,6 +1351,17 @@ __global__ void k_take(int64_t n, const int *__restrict__ src, const float *__re
   if (i < n) out[i] = src[i] >= 0 ? v[src[i]] : 0.f;
 }
 
+// the same, each value times its column's D^-1 (the product the FP32
+// sliced kernels form as they load a value and its column's scale)
+__global__ void k_take_scaled(int64_t n, const int *__restrict__ src, const int *__restrict__ col,
+                              const float *__restrict__ v, const float *__restrict__ dinv, float *__restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int e = src[i];
+    out[i] = e >= 0 ? v[e] * dinv[col[e]] : 0.f;
+  }
+}
+
 void matrix_build_staged(yafel_hip_matrix_s *A) {
   if (A->mx_dmax >= 0) return;
   const int64_t rows = A->brows, nsl = A->nsl, ntot = A->ntot, nnz = A->nblk;
@@ -1160,11 +1375,62 @@ void matrix_build_staged(yafel_hip_matrix_s *A) {
     HIPCHK(hipMemcpyAsync(ip.data(), A->iperm.p, rows * 4, hipMemcpyDeviceToHost, st));
   }
   HIPCHK(hipStreamSynchronize(st));
-  const int64_t ngr = (rows + 63) / 64;
+  // each row's columns and their positions in the sliced storage, CSR order
+  std::vector<int> rcol(std::max<int64_t>(nnz, 1)), rsrc(std::max<int64_t>(nnz, 1));
+  for (int64_t r = 0; r < rows; ++r) {
+    const int64_t i = ip.empty() ? r : ip[r];
+    const int64_t base = (int64_t)sptr[i >> 6] * 64 + (i & 63);
+    for (int t = 0; t < ptr[r + 1] - ptr[r]; ++t) {
+      const int64_t k = base + (int64_t)t * 64;
+      rsrc[ptr[r] + t] = (int)k;
+      rcol[ptr[r] + t] = col[k];
+    }
+  }
+  col.clear();
+  col.shrink_to_fit();
+  // groups of up to 64 rows, each grown breadth-first over the pattern from
+  // the lowest row not yet in a group; `order` is the queue of every group
+  std::vector<int> grp(rows, -1), order, gstart;
+  order.reserve(rows);
+  for (int64_t seed = 0; seed < rows; ++seed) {
+    if (grp[seed] >= 0) continue;
+    const int g = (int)gstart.size();
+    const size_t b = order.size();
+    gstart.push_back((int)b);
+    grp[seed] = g;
+    order.push_back((int)seed);
+    for (size_t head = b; head < order.size() && order.size() - b < 64; ++head) {
+      const int u = order[head];
+      for (int e = ptr[u]; e < ptr[u + 1] && order.size() - b < 64; ++e) {
+        const int v = rcol[e];
+        if (v >= 0 && v < rows && grp[v] < 0) {
+          grp[v] = g;
+          order.push_back(v);
+        }
+      }
+    }
+  }
+  const int64_t ngr = (int64_t)gstart.size();
+  gstart.push_back((int)rows);
+  // 64 positions per group: its rows longest first (ties in queue order),
+  // then empty positions
+  const int64_t npos = ngr * 64;
+  REQUIRE(npos < INT32_MAX, "staged products: too many rows");
+  std::vector<int> grow(std::max<int64_t>(npos, 1), -1);
+  for (int64_t g = 0; g < ngr; ++g) {
+    auto first = order.begin() + gstart[g], last = order.begin() + gstart[g + 1];
+    std::stable_sort(first, last, [&](int x, int y) { return ptr[x + 1] - ptr[x] > ptr[y + 1] - ptr[y]; });
+    std::copy(first, last, grow.begin() + 64 * g);
+  }
+  order.clear();
+  order.shrink_to_fit();
   // rows padded to a multiple of 4 entries (zero values, position 0)
-  std::vector<int> pp(rows + 1, 0);
-  for (int64_t r = 0; r < rows; ++r) pp[r + 1] = pp[r] + ((ptr[r + 1] - ptr[r] + 3) & ~3);
-  const int64_t npad = pp[rows];
+  std::vector<int> pp(npos + 1, 0);
+  for (int64_t p = 0; p < npos; ++p) {
+    const int r = grow[p];
+    pp[p + 1] = pp[p] + (r >= 0 ? ((ptr[r + 1] - ptr[r] + 3) & ~3) : 0);
+  }
+  const int64_t npad = pp[npos];
   REQUIRE(npad < INT32_MAX, "staged products: too many entries");
   std::vector<int> src(std::max<int64_t>(npad, 1), -1), dptr(ngr + 1), dcol, tmp, ecol(std::max<int64_t>(npad, 1), 0);
   std::vector<unsigned short> lidx(std::max<int64_t>(npad, 1), 0);
@@ -1173,16 +1439,15 @@ void matrix_build_staged(yafel_hip_matrix_s *A) {
   int64_t emax = 0;
   bool fits = true;
   for (int64_t g = 0; g < ngr; ++g) {
-    const int64_t r0 = 64 * g, r1 = std::min<int64_t>(r0 + 64, rows);
+    const int64_t p0 = 64 * g, p1 = p0 + 64;
     tmp.clear();
-    for (int64_t r = r0; r < r1; ++r) {
-      const int64_t i = ip.empty() ? r : ip[r];
-      const int64_t base = (int64_t)sptr[i >> 6] * 64 + (i & 63);
+    for (int64_t p = p0; p < p1; ++p) {
+      const int r = grow[p];
+      if (r < 0) continue;
       for (int t = 0; t < ptr[r + 1] - ptr[r]; ++t) {
-        const int64_t k = base + (int64_t)t * 64;
-        src[pp[r] + t] = (int)k;
-        ecol[pp[r] + t] = col[k];
-        tmp.push_back(col[k]);
+        src[pp[p] + t] = rsrc[ptr[r] + t];
+        ecol[pp[p] + t] = rcol[ptr[r] + t];
+        tmp.push_back(rcol[ptr[r] + t]);
       }
     }
     std::sort(tmp.begin(), tmp.end());
@@ -1190,17 +1455,19 @@ void matrix_build_staged(yafel_hip_matrix_s *A) {
     dptr[g] = (int)dcol.size();
     dcol.insert(dcol.end(), tmp.begin(), tmp.end());
     dmax = std::max(dmax, (int)tmp.size());
-    emax = std::max<int64_t>(emax, pp[r1] - pp[r0]);
+    emax = std::max<int64_t>(emax, pp[p1] - pp[p0]);
     if (tmp.size() > 65535) {
       fits = false;
       continue;
     }
-    for (int64_t e = pp[r0]; e < pp[r1]; ++e)
+    for (int64_t e = pp[p0]; e < pp[p1]; ++e)
       if (src[e] >= 0) lidx[e] = (unsigned short)(std::lower_bound(tmp.begin(), tmp.end(), ecol[e]) - tmp.begin());
   }
   dptr[ngr] = (int)dcol.size();
-  A->mx_ptr.alloc((rows + 1) * 4);
-  HIPCHK(hipMemcpy(A->mx_ptr.p, pp.data(), (rows + 1) * 4, hipMemcpyHostToDevice));
+  A->mx_ptr.alloc((npos + 1) * 4);
+  HIPCHK(hipMemcpy(A->mx_ptr.p, pp.data(), (npos + 1) * 4, hipMemcpyHostToDevice));
+  A->mx_grow.alloc(std::max<int64_t>(npos, 1) * 4);
+  HIPCHK(hipMemcpy(A->mx_grow.p, grow.data(), std::max<int64_t>(npos, 1) * 4, hipMemcpyHostToDevice));
   A->mx_src.alloc(src.size() * 4);
   A->mx_lidx.alloc(lidx.size() * 2);
   A->mx_dptr.alloc((ngr + 1) * 4);
@@ -1209,6 +1476,7 @@ void matrix_build_staged(yafel_hip_matrix_s *A) {
   HIPCHK(hipMemcpy(A->mx_lidx.p, lidx.data(), lidx.size() * 2, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(A->mx_dptr.p, dptr.data(), (ngr + 1) * 4, hipMemcpyHostToDevice));
   if (!dcol.empty()) HIPCHK(hipMemcpy(A->mx_dcol.p, dcol.data(), dcol.size() * 4, hipMemcpyHostToDevice));
+  A->mx_ngr = ngr;
   A->mx_dmax = fits ? dmax : 0;
   A->mx_emax = emax;
 }
@@ -1226,6 +1494,24 @@ void matrix_refresh_staged(yafel_hip_matrix_s *A) {
   A->mx_gen = A->v32_gen;
 }
 
+// The staged values of A D^-1, refreshed when the FP32 copy or the diagonal
+// changed.
+const float *matrix_staged_scaled(yafel_hip_matrix_s *A) {
+  if (A->mx_sgen == A->v32_gen && A->mx_dgen == A->diag_gen) return A->mx_val32s.as<float>();
+  REQUIRE(A->diag_ok && A->dinv32.p, "staged products: no diagonal");
+  const int64_t nnz = (int64_t)(A->mx_src.bytes / 4);
+  if (A->mx_val32s.bytes < (size_t)std::max<int64_t>(nnz, 1) * 4) A->mx_val32s.alloc(std::max<int64_t>(nnz, 1) * 4);
+  if (nnz) {
+    k_take_scaled<<<ceil_div(nnz, 256), 256, 0, A->ctx->stream>>>(nnz, A->mx_src.as<int>(), A->col.as<int>(),
+                                                                 A->val32.as<float>(), A->dinv32.as<float>(),
+                                                                 A->mx_val32s.as<float>());
+    HIPCHK(hipGetLastError());
+  }
+  A->mx_sgen = A->v32_gen;
+  A->mx_dgen = A->diag_gen;
+  return A->mx_val32s.as<float>();
+}
+
 // True when the staged products run on A (CSR, lists with 16-bit positions).
 bool staged_ok(yafel_hip_matrix_s *A) {
   if (A->format != YAFEL_HIP_CSR || A->brows == 0) return false;
@@ -1236,8 +1522,9 @@ bool staged_ok(yafel_hip_matrix_s *A) {
 template <class Op>
 void launch_mspmm_lds(yafel_hip_matrix_s *A, const Op &op, int grid, hipStream_t s) {
   matrix_refresh_staged(A);
-  const StagedCsr v{A->mx_ptr.as<int>(), A->mx_val32.as<float>(), A->mx_lidx.as<unsigned short>(),
-                    A->mx_dptr.as<int>(), A->mx_dcol.as<int>(), A->brows, std::min(A->mx_dmax, kStageRows)};
+  const float *val = Op::kScaledVals ? matrix_staged_scaled(A) : A->mx_val32.as<float>();
+  const StagedCsr v{A->mx_ptr.as<int>(), val, A->mx_lidx.as<unsigned short>(), A->mx_dptr.as<int>(),
+                    A->mx_dcol.as<int>(), A->mx_grow.as<int>(), A->mx_ngr, std::min(A->mx_dmax, kStageRows)};
   if (A->mx_dmax <= kStageRows && A->mx_emax <= kStageEnt)
     k_mspmm_pipe<Op><<<grid, NT, (size_t)v.dcap * KC * sizeof(float), s>>>(v, op);
   else
@@ -1281,8 +1568,10 @@ struct Multi {
     int g = (int)std::max<int64_t>(1, std::min<int64_t>((units + 3) / 4, grid));
     if constexpr (sizeof(T) == 4) {
       if (staged && v4) {
-        // one group of 64 rows and one pass of 32 right-hand sides per workgroup at a time
-        g = (int)std::max<int64_t>(1, std::min<int64_t>((A->brows + 63) / 64, grid));
+        // one group of 64 rows and one pass of 32 right-hand sides per
+        // workgroup at a time; the LDS holds two workgroups per CU, so a
+        // larger grid would only add partials for the fold to read
+        g = (int)std::max<int64_t>(1, std::min<int64_t>(A->mx_ngr, std::min<int64_t>(grid, 2 * (int64_t)ctx->num_cus)));
         launch_mspmm_lds(A, op, g, s);
       } else if (v4) {
         k_mspmm4<O><<<g, NT, 0, s>>>(cview<T>(A), op);
@@ -1299,10 +1588,19 @@ struct Multi {
     }
   }
   template <class O> void elem(const O &op) {
-    k_melem<O><<<grid, NT, 0, s>>>(n, op);
+    int g = grid;
+    if (O::V4 && k % 4 == 0) {
+      // units of 64 rows, spread evenly over two workgroups per CU
+      const int64_t cap = std::max<int64_t>(1, std::min<int64_t>(grid, 2 * (int64_t)ctx->num_cus));
+      const int64_t units = (n + 63) / 64, per = std::max<int64_t>(1, (units + cap - 1) / cap);
+      g = (int)((units + per - 1) / per);
+      if constexpr (O::V4) k_melem4<O><<<g, NT4, 0, s>>>(n, per, op);
+    } else {
+      k_melem<O><<<g, NT, 0, s>>>(n, op);
+    }
     HIPCHK(hipGetLastError());
     if constexpr (O::NP > 0) {
-      k_mfinish<O><<<1, NTM, 0, s>>>(op, grid);
+      k_mfinish<O><<<1, NTM, 0, s>>>(op, g);
       HIPCHK(hipGetLastError());
     }
   }
@@ -1432,7 +1730,7 @@ bool spmm_lds_launch(yafel_hip_matrix_s *A, int k, const float *X, float *Y, hip
   if (k < 1 || k > KMAX || k % 4 != 0 || ((uintptr_t)X % 16) != 0 || ((uintptr_t)Y % 16) != 0 || !staged_ok(A))
     return false;
   MPlain op{X, Y, k, nullptr};
-  launch_mspmm_lds(A, op, (int)std::min<int64_t>((A->brows + 63) / 64, (int64_t)A->ctx->num_cus * 8), s);
+  launch_mspmm_lds(A, op, (int)std::max<int64_t>(1, std::min<int64_t>(A->mx_ngr, (int64_t)A->ctx->num_cus * 2)), s);
   return true;
 }
 }  // namespace yh

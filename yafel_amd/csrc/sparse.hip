@@ -535,6 +535,7 @@ void matrix_build_diag(yafel_hip_matrix_s *A) {
     HIPCHK(hipGetLastError());
   }
   A->diag_ok = true;
+  A->diag_gen++;
 }
 
 // Shared by the triplet and pattern constructors: takes the CSR-ordered device
