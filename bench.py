@@ -247,12 +247,14 @@ def req_class(m):
     return "3d_q1_elasticity" if m["dim"] == 3 else f"2d_q{m['p']}_poisson"
 
 
-def run_stream(hip, ctx, mats, arrivals, budget, slots, setup_threads=0):
+def run_stream(hip, ctx, mats, arrivals, budget, slots, setup_threads=0, graphs=False):
     """Feed requests at their arrival times into a server; returns per-request
     latency (arrival -> found finished), makespan, the results, the server
     statistics and each request's (admission wait, setup, wait to join,
-    solve) times in ms."""
+    solve) times in ms.  graphs: replay chunks from per-epoch HIP graphs."""
     srv = hip.Server(ctx, budget, slots, setup_threads)
+    if graphs:
+        srv.set_graphs(True)
     n = len(mats)
     t0 = time.perf_counter()
     i, done_t, rid_of, times = 0, {}, {}, {}
@@ -319,10 +321,12 @@ def fp64_one_at_a_time(hip, ctx, mats):
     return len(mats) / (time.perf_counter() - t)
 
 
-def served_stream(hip, ctx, nreq, rates, fp64_sample, n_alone, slots=128, setup_threads=0, full_at_max=None):
+def served_stream(hip, ctx, nreq, rates, fp64_sample, n_alone, slots=128, setup_threads=0, full_at_max=None,
+                  graph_rates=(50.0, 200.0)):
     """c4: a served stream of Poisson / elasticity solves with Poisson
     arrivals at each rate; the record of every rate, the alone-vs-mix check,
-    and the plain FP64 GPU path on a sample of the same requests."""
+    the plain FP64 GPU path on a sample of the same requests, and the same
+    streams at graph_rates with the chunks replayed from per-epoch graphs."""
     from yafel_amd import workloads
     budget = 200 << 30
     nmax = max(nreq, full_at_max or 0)
@@ -343,6 +347,18 @@ def served_stream(hip, ctx, nreq, rates, fp64_sample, n_alone, slots=128, setup_
         sweep.append(rec)
         if rate == max(rates):
             top = (rec, res)
+    graph_sweep = []
+    for rate in graph_rates:
+        arr = workloads.poisson_arrivals(nreq, rate, seed=int(rate))
+        lat, span, res_g, st, _ = run_stream(hip, ctx, mats[:nreq], arr, budget, slots, setup_threads, graphs=True)
+        direct = next((r for r in sweep if r["rate"] == rate and r["requests"] == nreq), None)
+        g = {"rate": rate, "requests": nreq, "solves_per_s": nreq / span, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+             "p99_ms": float(np.percentile(lat, 99) * 1e3),
+             "failed": sum(1 for x, inf in res_g.values() if inf.status != 0 or inf.rel_residual > TOL),
+             "graph_captures": st["graph_captures"], "graph_replays": st["graph_replays"], "chunks": st["chunks"],
+             "direct_p50_ms": direct["p50_ms"] if direct else None, "direct_p99_ms": direct["p99_ms"] if direct else None}
+        log(f"graph replay, rate {rate}: {g}")
+        graph_sweep.append(g)
     rec, res = top
     diffs = []
     for j in range(min(n_alone, rec["requests"])):
@@ -350,7 +366,8 @@ def served_stream(hip, ctx, nreq, rates, fp64_sample, n_alone, slots=128, setup_
         diffs.append(float(np.abs(r1[0][0] - res[j][0]).max()))
     fp64_rate = fp64_one_at_a_time(hip, ctx, mats[:min(fp64_sample, rec["requests"])])
     return {"solves_per_s": rec["solves_per_s"], "p50_ms": rec["p50_ms"], "p99_ms": rec["p99_ms"],
-            "rate_sweep": sweep, "alone_vs_mix_max_abs_diff": max(diffs) if diffs else None,
+            "rate_sweep": sweep, "graph_replay": graph_sweep,
+            "alone_vs_mix_max_abs_diff": max(diffs) if diffs else None,
             "alone_vs_mix_requests": len(diffs),
             "fp64_gpu_one_at_a_time_solves_per_s": fp64_rate, "fp64_sample": min(fp64_sample, rec["requests"]),
             "speedup_vs_fp64_gpu": rec["solves_per_s"] / fp64_rate}

@@ -449,6 +449,8 @@ typedef struct {
   int64_t chunks, batched_launches;
   size_t hbm_in_use, hbm_peak, hbm_budget;
   double setup_ms;              /* host time the setup worker spent building requests */
+  int64_t graph_captures;       /* slot-set epochs captured into graphs (graph replay on) */
+  int64_t graph_replays;        /* chunks replayed from a graph */
 } yafel_hip_server_stats;
 
 int yafel_hip_server_create(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots, yafel_hip_server *out);
@@ -483,6 +485,12 @@ int yafel_hip_server_times(yafel_hip_server s, int64_t id, double *ms);
  * yafel_hip_server_result releases it */
 int yafel_hip_server_cancel(yafel_hip_server s, int64_t id);
 int yafel_hip_server_stats_get(yafel_hip_server s, yafel_hip_server_stats *st);
+/* on != 0: replay each chunk from a HIP graph while the active slot set stays
+ * the same (captured at the set's first chunk, one graph per state-version
+ * parity; a join or a retirement starts a new capture); 0 (the default):
+ * submit every launch directly.  Results are bit-identical either way.  No
+ * reference counterpart (the reference has no request server). */
+int yafel_hip_server_set_graphs(yafel_hip_server s, int on);
 
 /* ----------------------------------------------------- synthetic inputs */
 /* Structured unit-square/cube corner mesh (cells ordered as

@@ -18,8 +18,10 @@ def materialised(i, r):
     return workloads.c4_materialise(r)
 
 
-def run_all(H, ctx, reqs, budget=8 << 30, slots=16, stagger=False):
+def run_all(H, ctx, reqs, budget=8 << 30, slots=16, stagger=False, graphs=False):
     srv = H.Server(ctx, budget, slots)
+    if graphs:
+        srv.set_graphs(True)
     ids, out = [], {}
     for i, m in enumerate(reqs):
         ids.append(srv.submit(m["dim"], m["p"], m["dpn"], m["nodes"], m["cells"], m["ph"], load=m["load"],
@@ -52,6 +54,23 @@ def test_alone_and_in_the_mix_are_identical(ctx):
         assert info_mix.status == 0 and info_mix.rel_residual < 1e-8
         assert np.array_equal(x_alone, x_mix)          # bit-identical alone vs in the mix
         assert info_alone.iterations == info_mix.iterations
+
+
+def test_graph_replay_matches_direct_submission(ctx):
+    """Chunks replayed from per-epoch graphs (captured again at every join and
+    retirement) give the same bits as directly submitted launches, over
+    epochs that last several chunks."""
+    from yafel_amd import hip as H
+    reqs = [materialised(i, r) for i, r in enumerate(MIX)]
+    direct, st0 = run_all(H, ctx, reqs, stagger=True)
+    replay, st1 = run_all(H, ctx, reqs, stagger=True, graphs=True)
+    assert st0["graph_captures"] == 0 and st0["graph_replays"] == 0
+    assert st1["graph_captures"] >= 2 and st1["graph_replays"] > st1["graph_captures"]
+    assert st1["completed"] == len(reqs) and st1["failed"] == 0
+    for (x0, i0), (x1, i1) in zip(direct, replay):
+        assert i1.status == 0 and i1.rel_residual < 1e-8
+        assert np.array_equal(x0, x1)
+        assert i0.iterations == i1.iterations
 
 
 def test_served_result_matches_direct_solve(ctx, orc):

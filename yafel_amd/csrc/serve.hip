@@ -285,6 +285,23 @@ struct yafel_hip_server_s {
   };
   std::deque<Inflight> inflight;
   hipEvent_t evs[RING];
+  // Graph replay (off unless yafel_hip_server_set_graphs turns it on): while
+  // the active slot set stays the same, a chunk's launches are replayed from
+  // an executable graph, one per state-version parity (a chunk advances the
+  // version by an odd count, so consecutive chunks alternate), captured at the
+  // epoch's first chunk with its own descriptor table, status buffer and
+  // pinned status area.  A join or a retirement ends the epoch.
+  bool graphs = false;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  int64_t epoch = 0;           // bumped at every join and retirement
+  int64_t gepoch = -1;         // the epoch the graphs were captured in
+  // graphs of ended epochs with the last chunk launched while they were
+  // current: destroyed once that chunk has drained
+  std::vector<std::pair<int64_t, hipGraphExec_t>> gold;
+  DBuf gdesc, gstatus[2];
+  char *gpin = nullptr;        // two status areas
+  hipEvent_t gev[2] = {nullptr, nullptr};
+  int64_t captures = 0, replays = 0;
   char *pin = nullptr;  // RING status areas, then RING table staging areas
   int K = 8;
   int64_t ver = 0;  // state version parity of the next launch (every slot's)
@@ -524,7 +541,9 @@ void worker_main(yafel_hip_server_s *S, yafel_hip_ctx_s *c) {
 }
 
 // upload a descriptor table for `slots` (offsets recomputed) into ring entry
-int upload_table(yafel_hip_server_s *S, const std::vector<Slot *> &slots, int *tot_spmv, int *tot_elem) {
+// e (returned), or into `dst` when given (the graph epoch's table)
+int upload_table(yafel_hip_server_s *S, const std::vector<Slot *> &slots, int *tot_spmv, int *tot_elem,
+                 DBuf *dst = nullptr) {
   std::vector<SlotDesc> h(slots.size());
   int os = 0, oe = 0;
   // the cache budget shared by the table's FP32 streams in proportion to their size
@@ -551,7 +570,7 @@ int upload_table(yafel_hip_server_s *S, const std::vector<Slot *> &slots, int *t
   // stream-ordered copy from this entry's own pinned staging area
   char *stage = S->pin + (yafel_hip_server_s::RING + e) * yafel_hip_server_s::STAGE;
   std::memcpy(stage, h.data(), bytes);
-  HIPCHK(hipMemcpyAsync(S->desc[e].p, stage, bytes, hipMemcpyHostToDevice, S->ctx->stream));
+  HIPCHK(hipMemcpyAsync(dst ? dst->p : S->desc[e].p, stage, bytes, hipMemcpyHostToDevice, S->ctx->stream));
   return e;
 }
 
@@ -559,17 +578,91 @@ int upload_table(yafel_hip_server_s *S, const std::vector<Slot *> &slots, int *t
 // followed by their per-slot folds, the versioned passes advance the state
 // version of every slot in the table.
 template <int PH>
-void launch_phase(yafel_hip_server_s *S, int e, int ns, int blocks) {
+void launch_phase(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int blocks) {
   if (ns == 0 || blocks == 0) return;
   const int par = (int)(S->ver & 1);
-  kb_phase<PH><<<blocks, NT, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns, par);
+  kb_phase<PH><<<blocks, NT, 0, S->ctx->stream>>>(desc, ns, par);
   HIPCHK(hipGetLastError());
   if (PH == PH_INIT || PH == PH_REF) {
-    kb_finish<PH><<<ns, NTF, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns, par);
+    kb_finish<PH><<<ns, NTF, 0, S->ctx->stream>>>(desc, ns, par);
     HIPCHK(hipGetLastError());
   }
   if (PH == PH_U || PH == PH_P || PH == PH_P0) ++S->ver;
   S->stats.batched_launches++;
+}
+
+// One chunk for the slots of table `desc`: p = z + beta p after a slot's
+// initial or refinement residual, K iterations, the refinement, and every
+// slot's status into `status`.
+void chunk_launches(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int ts, int te, SlotStatus *status) {
+  launch_phase<PH_P0>(S, desc, ns, te);
+  for (int k = 0; k < S->K; ++k) {
+    launch_phase<PH_Q>(S, desc, ns, ts);
+    launch_phase<PH_U>(S, desc, ns, te);
+    launch_phase<PH_P>(S, desc, ns, te);
+  }
+  launch_phase<PH_FOLD>(S, desc, ns, te);
+  launch_phase<PH_REF>(S, desc, ns, ts);
+  kb_status<<<ceil_div(ns, 64), 64, 0, S->ctx->stream>>>(desc, ns, (int)(S->ver & 1), status);
+  HIPCHK(hipGetLastError());
+}
+
+// Retire the current graphs: a chunk replayed from them may still run, so
+// they are destroyed after the last chunk launched so far has drained.
+void drop_graphs(yafel_hip_server_s *S) {
+  for (auto &g : S->gexec)
+    if (g) {
+      S->gold.emplace_back(S->chunk_seq, g);
+      g = nullptr;
+    }
+  S->gepoch = -1;
+}
+
+void reap_graphs(yafel_hip_server_s *S, bool all) {
+  std::vector<std::pair<int64_t, hipGraphExec_t>> keep;
+  for (auto &o : S->gold) {
+    if (all || o.first <= S->drained_seq) (void)hipGraphExecDestroy(o.second);
+    else keep.push_back(o);
+  }
+  S->gold.swap(keep);
+}
+
+// Capture the chunk of the current active set twice, starting at each
+// version parity; the launches only record (nothing runs, no counter moves).
+void capture_graphs(yafel_hip_server_s *S) {
+  drop_graphs(S);
+  using SS = yafel_hip_server_s;
+  if (S->gdesc.bytes < SS::STAGE) S->gdesc.alloc(SS::STAGE);
+  for (auto &b : S->gstatus)
+    if (b.bytes < SS::STAGE) b.alloc(SS::STAGE);
+  int ts, te;
+  upload_table(S, S->active, &ts, &te, &S->gdesc);
+  const int ns = (int)S->active.size();
+  const int64_t ver = S->ver, launches = S->stats.batched_launches;
+  hipStream_t st = S->ctx->stream;
+  for (int p = 0; p < 2; ++p) {
+    S->ver = p;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    try {
+      chunk_launches(S, S->gdesc.as<SlotDesc>(), ns, ts, te, S->gstatus[p].as<SlotStatus>());
+      HIPCHK(hipMemcpyAsync(S->gpin + p * SS::STAGE, S->gstatus[p].p, ns * sizeof(SlotStatus), hipMemcpyDeviceToHost,
+                            st));
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      S->ver = ver;
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(st, &g));
+    const hipError_t rc = hipGraphInstantiate(&S->gexec[p], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(rc);
+  }
+  S->ver = ver;
+  S->stats.batched_launches = launches;
+  S->gepoch = S->epoch;
+  S->captures++;
 }
 
 void drain_one(yafel_hip_server_s *S, std::vector<int64_t> &done_ids) {
@@ -638,6 +731,8 @@ int yafel_hip_server_create_ex(yafel_hip_ctx ctx, size_t hbm_budget, int max_slo
   for (auto &e : S->evs) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipHostMalloc((void **)&S->pin, 2 * yafel_hip_server_s::RING * yafel_hip_server_s::STAGE,
                        hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&S->gpin, 2 * yafel_hip_server_s::STAGE, hipHostMallocDefault));
+  for (auto &e : S->gev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking));
   for (auto *c : S->sctx) S->workers.emplace_back(worker_main, S, c);
   *out = S;
@@ -665,8 +760,13 @@ int yafel_hip_server_destroy(yafel_hip_server S) {
       delete sl;
     }
   (void)hipStreamSynchronize(S->ctx->stream);
+  drop_graphs(S);
+  reap_graphs(S, true);
   for (auto &e : S->evs) (void)hipEventDestroy(e);
+  for (auto &e : S->gev)
+    if (e) (void)hipEventDestroy(e);
   if (S->pin) (void)hipHostFree(S->pin);
+  if (S->gpin) (void)hipHostFree(S->gpin);
   for (auto *c : S->sctx) (void)hipStreamSynchronize(c->stream);
   if (S->copy_stream) (void)hipStreamDestroy(S->copy_stream);
   for (auto *c : S->sctx) yafel_hip_ctx_destroy(c);
@@ -754,16 +854,18 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
       keep.push_back(sl);
     }
   }
+  if (keep.size() != S->active.size()) S->epoch++;
   S->active.swap(keep);
   // 4. joining slots: allocate, initialise (FP64 residual) and add to the batch
   if (!S->joining.empty()) {
+    S->epoch++;
     for (Slot *sl : S->joining) join_slot(S, sl);
     int ts, te;
     const int e = upload_table(S, S->joining, &ts, &te);
     kb_init_state<<<(int)S->joining.size(), 64, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(),
                                                                      (int)S->joining.size(), (int)(S->ver & 1));
     HIPCHK(hipGetLastError());
-    launch_phase<PH_INIT>(S, e, (int)S->joining.size(), ts);
+    launch_phase<PH_INIT>(S, S->desc[e].as<SlotDesc>(), (int)S->joining.size(), ts);
     const auto now = std::chrono::steady_clock::now();
     for (Slot *sl : S->joining) {
       sl->req->t_join = now;
@@ -773,35 +875,39 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
   }
   // 5. one chunk for every active slot
   if (!S->active.empty()) {
-    int ts, te;
-    const int e = upload_table(S, S->active, &ts, &te);
     const int ns = (int)S->active.size();
-    // p = z + beta p after a slot's initial or refinement residual
-    launch_phase<PH_P0>(S, e, ns, te);
-    for (int k = 0; k < S->K; ++k) {
-      launch_phase<PH_Q>(S, e, ns, ts);
-      launch_phase<PH_U>(S, e, ns, te);
-      launch_phase<PH_P>(S, e, ns, te);
-    }
-    launch_phase<PH_FOLD>(S, e, ns, te);
-    launch_phase<PH_REF>(S, e, ns, ts);
-    kb_status<<<ceil_div(ns, 64), 64, 0, S->ctx->stream>>>(S->desc[e].as<SlotDesc>(), ns, (int)(S->ver & 1),
-                                                            S->status[e].as<SlotStatus>());
-    HIPCHK(hipGetLastError());
     yafel_hip_server_s::Inflight f;
-    f.ev = S->evs[e];
+    if (S->graphs) {
+      if (S->gepoch != S->epoch) capture_graphs(S);
+      // the two chunks in flight have opposite parities, so each graph's
+      // status area and event are free again once the older one drained
+      const int p = (int)(S->ver & 1);
+      HIPCHK(hipGraphLaunch(S->gexec[p], S->ctx->stream));
+      S->ver += 2 * S->K + 1;
+      S->stats.batched_launches += 3 * S->K + 3;
+      S->replays++;
+      f.ev = S->gev[p];
+      f.ring = -1;
+      f.pin = S->gpin + p * yafel_hip_server_s::STAGE;
+    } else {
+      int ts, te;
+      const int e = upload_table(S, S->active, &ts, &te);
+      chunk_launches(S, S->desc[e].as<SlotDesc>(), ns, ts, te, S->status[e].as<SlotStatus>());
+      f.ev = S->evs[e];
+      f.ring = e;
+      f.pin = S->pin + e * yafel_hip_server_s::STAGE;
+      HIPCHK(hipMemcpyAsync(f.pin, S->status[e].p, ns * sizeof(SlotStatus), hipMemcpyDeviceToHost, S->ctx->stream));
+    }
     f.seq = ++S->chunk_seq;
     for (Slot *sl : S->active) sl->last_chunk = f.seq;
-    f.ring = e;
     f.slots = S->active;
-    f.pin = S->pin + e * yafel_hip_server_s::STAGE;
-    HIPCHK(hipMemcpyAsync(f.pin, S->status[e].p, ns * sizeof(SlotStatus), hipMemcpyDeviceToHost, S->ctx->stream));
     HIPCHK(hipEventRecord(f.ev, S->ctx->stream));
     S->inflight.push_back(f);
     S->stats.chunks++;
   } else {
     while (!S->inflight.empty()) drain_one(S, done);
   }
+  reap_graphs(S, false);
   // retired slots whose last chunk has drained: nothing on the device reads
   // their matrix any more
   {
@@ -920,6 +1026,18 @@ int yafel_hip_server_stats_get(yafel_hip_server S, yafel_hip_server_stats *st) {
   REQUIRE(S && st, "server_stats: null argument");
   std::lock_guard<std::mutex> lk(S->mu);
   *st = S->stats;
+  st->graph_captures = S->captures;
+  st->graph_replays = S->replays;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(S ? S->ctx : nullptr)
+}
+
+int yafel_hip_server_set_graphs(yafel_hip_server S, int on) {
+  YH_TRY_BEGIN
+  REQUIRE(S, "server_set_graphs: null server");
+  HIPCHK(hipSetDevice(S->ctx->device));
+  if (!on) drop_graphs(S);
+  S->graphs = on != 0;
   return YAFEL_HIP_OK;
   YH_TRY_END(S ? S->ctx : nullptr)
 }

@@ -65,7 +65,8 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", _I64), ("admitted", _I64), ("completed", _I64), ("failed", _I64),
                 ("cancelled", _I64), ("queued", _I64), ("active", _I64), ("chunks", _I64),
                 ("batched_launches", _I64), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t),
-                ("hbm_budget", C.c_size_t), ("setup_ms", C.c_double)]
+                ("hbm_budget", C.c_size_t), ("setup_ms", C.c_double), ("graph_captures", _I64),
+                ("graph_replays", _I64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -133,6 +134,7 @@ for _n, _a in [
     ("yafel_hip_server_cancel", [_P, _I64]),
     ("yafel_hip_server_times", [_P, _I64, _P]),
     ("yafel_hip_server_stats_get", [_P, C.POINTER(ServerStats)]),
+    ("yafel_hip_server_set_graphs", [_P, _I]),
 ]:
     _sig(_n, _I, _a)
 _sig("yafel_hip_last_error", C.c_char_p, [_P])
@@ -695,6 +697,10 @@ class Server:
 
     def cancel(self, rid):
         check(lib.yafel_hip_server_cancel(self.h, rid), self.ctx.h)
+
+    def set_graphs(self, on):
+        """Replay each chunk from a HIP graph while the slot set stays the same."""
+        check(lib.yafel_hip_server_set_graphs(self.h, 1 if on else 0), self.ctx.h)
 
     def stats(self):
         s = ServerStats()
