@@ -76,3 +76,34 @@ def test_fp32_element_matrices_against_fp64_elementwise(ctx, orc, p):
     K32 = mesh.element_matrices_fp32(H.physics(H.POISSON, 2, coef_dev=cd)).download().reshape(K64.shape)
     for e in range(K64.shape[0]):
         assert np.abs(K32[e] - K64[e]).max() <= 2e-6 * np.abs(K64[e]).max()
+
+
+@pytest.mark.parametrize("p", [3, 4])
+def test_fp32_assembly_writes_the_element_order_fp64_sums(ctx, orc, p):
+    """The matrix-core assembly writes entries with one contributing element
+    straight into the matrix and sums shared entries from a compact staging:
+    every stored value is bit for bit the FP64 sum, in element order, of the
+    FP32 element matrices."""
+    from yafel_amd import hip as H
+    n = 7
+    nodes, cells = rect_mesh(orc, n, n + 1, 1.1, 0.9, jitter=0.2, seed=41)
+    coef = np.exp(0.5 * np.random.default_rng(42).standard_normal(cells.shape[0]))
+    mesh = H.Mesh(ctx, 2, p, 1, nodes, cells)
+    cd = ctx.upload(coef)
+    ph = H.physics(H.POISSON, 2, coef_dev=cd)
+    A = mesh.pattern(H.CSR, H.FP64)
+    mesh.assemble(ph, A, None, H.TANGENT | H.FP32_ELEMENTS)
+    mesh.assemble(ph, A, None, H.TANGENT | H.FP32_ELEMENTS)   # the map is reused
+    ptr, col, val = A.arrays()
+    el, _ = mesh.download()
+    nen = (p + 1) ** 2
+    K32 = mesh.element_matrices_fp32(ph).download().reshape(-1, nen, nen)
+    rows = np.repeat(el, nen, axis=1).ravel()
+    cols = np.tile(el, (1, nen)).ravel()
+    nd = mesh.ndofs
+    acc = np.zeros(nd * nd)
+    np.add.at(acc, rows.astype(np.int64) * nd + cols, K32.astype(np.float64).ravel())
+    r = np.repeat(np.arange(nd), np.diff(ptr))
+    assert np.array_equal(val, acc[r * nd + col])
+    shared = np.bincount(rows.astype(np.int64) * nd + cols, minlength=nd * nd)[r * nd + col]
+    assert (shared > 1).any() and (shared == 1).any()

@@ -18,6 +18,7 @@
 #include "internal.hpp"
 #include "kernels.hpp"
 #include "mesh.hpp"
+#include "sortprim.hpp"
 
 using namespace yh;
 
@@ -260,7 +261,10 @@ __global__ __launch_bounds__(256) void k_element_mfma(int64_t nel, const int32_t
                                                       const double *__restrict__ tabdN,
                                                       const double *__restrict__ tabw, double f,
                                                       const double *__restrict__ coef, float *__restrict__ Ke,
-                                                      double *__restrict__ Re, int want_k, int want_r) {
+                                                      double *__restrict__ Re, int want_k, int want_r,
+                                                      const int32_t *__restrict__ dmap = nullptr,
+                                                      double *__restrict__ val = nullptr,
+                                                      float *__restrict__ mstage = nullptr) {
   constexpr int N1 = P + 1, NEN = N1 * N1, NQP = NEN, KD = 2 * NQP;
   static_assert(NEN <= 32 && KD % 2 == 0, "one 32x32 tile per element");
   __shared__ float sG[4][KD][32];
@@ -342,13 +346,36 @@ __global__ __launch_bounds__(256) void k_element_mfma(int64_t nel, const int32_t
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(v, v, acc, 0, 0, 0);
       }
       const float ke = (float)(coef ? coef[e] : 1.0);
-      float *Kout = Ke + e * (int64_t)NEN * NEN;
       const int col = l & 31;
+      if (dmap) {
+        // straight into the matrix: an entry with one contributing element
+        // goes to its sliced position as FP64 (what the gather's FP64 sum of
+        // one FP32 term gives), the others to their place in the compact
+        // staging of shared entries (code -1 - index)
+        const int32_t *dm = dmap + e * (int64_t)NEN * NEN;
+        int32_t d[16];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        // accumulator i of lane l holds C[8 (i/4) + 4 (l/32) + i%4][l%32]
-        const int row = 8 * (i >> 2) + 4 * (l >> 5) + (i & 3);
-        if (row < NEN && col < NEN) Kout[row * NEN + col] = -ke * acc[i];
+        for (int i = 0; i < 16; ++i) {
+          const int row = 8 * (i >> 2) + 4 * (l >> 5) + (i & 3);
+          d[i] = row < NEN && col < NEN ? dm[row * NEN + col] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int row = 8 * (i >> 2) + 4 * (l >> 5) + (i & 3);
+          if (row < NEN && col < NEN) {
+            const float v = -ke * acc[i];
+            if (d[i] >= 0) val[d[i]] = (double)v;
+            else mstage[-1 - (int64_t)d[i]] = v;
+          }
+        }
+      } else {
+        float *Kout = Ke + e * (int64_t)NEN * NEN;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          // accumulator i of lane l holds C[8 (i/4) + 4 (l/32) + i%4][l%32]
+          const int row = 8 * (i >> 2) + 4 * (l >> 5) + (i & 3);
+          if (row < NEN && col < NEN) Kout[row * NEN + col] = -ke * acc[i];
+        }
       }
     }
     if (live && want_r && l < NEN) {
@@ -422,6 +449,51 @@ __global__ __launch_bounds__(NT) void k_gather(int64_t nruns, int nen, const int
   }
 }
 
+// Direct-write map of the FP32 matrix-core assembly (CSR, one dof per node):
+// for every element entry t = e nen^2 + a nen + b, the sliced position of its
+// stored entry when t is that entry's only contribution, else -1 - (its index
+// in the compact staging of shared entries, which keeps each shared entry's
+// contributions together and in element order).  off: exclusive scan of
+// (1 << 32 | count) over the runs with more than one contribution.
+__global__ void k_dmap_count(int64_t nruns, const int32_t *__restrict__ cptr, int64_t *__restrict__ cnt) {
+  const int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (j > nruns) return;
+  const int c = j < nruns ? cptr[j + 1] - cptr[j] : 0;
+  cnt[j] = c > 1 ? ((int64_t)1 << 32) | c : 0;
+}
+
+__global__ void k_dmap_fill(int64_t nruns, const int32_t *__restrict__ cptr, const uint32_t *__restrict__ contrib,
+                            const int32_t *__restrict__ run_row, const int32_t *__restrict__ node_ptr,
+                            const int32_t *__restrict__ sptr, const int32_t *__restrict__ iperm,
+                            const int64_t *__restrict__ off, int32_t *__restrict__ dmap, int32_t *__restrict__ mpos,
+                            int32_t *__restrict__ mbeg) {
+  const int64_t j = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (j >= nruns) return;
+  const int c0 = cptr[j], c1 = cptr[j + 1];
+  const int A = run_row[j];
+  const int64_t pos = dev::sell_pos(sptr, iperm, A, j - node_ptr[A]);
+  if (c1 - c0 == 1) {
+    dmap[contrib[c0]] = (int32_t)pos;
+  } else {
+    const int64_t o = off[j];
+    const int m = (int)(o >> 32), b = (int)(o & 0xffffffffLL);
+    mpos[m] = (int32_t)pos;
+    mbeg[m] = b;
+    for (int c = c0; c < c1; ++c) dmap[contrib[c]] = -1 - (b + (c - c0));
+  }
+}
+
+// the shared entries: each one's contributions summed in element order, as
+// k_gather sums them
+__global__ void k_gather_shared(int64_t nm, const int32_t *__restrict__ mpos, const int32_t *__restrict__ mbeg,
+                                const float *__restrict__ mstage, double *__restrict__ val) {
+  const int64_t m = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (m >= nm) return;
+  double acc = 0;
+  for (int c = mbeg[m]; c < mbeg[m + 1]; ++c) acc += (double)mstage[c];
+  val[mpos[m]] = acc;
+}
+
 __global__ void k_gather_res(int64_t nnodes, int dpn, const int32_t *__restrict__ ptr,
                              const uint32_t *__restrict__ inc, const double *__restrict__ Re,
                              double *__restrict__ R) {
@@ -444,20 +516,33 @@ __global__ void k_sub(int64_t n, const double *a, double *rhs) {
   if (i < n) rhs[i] -= a[i];
 }
 
+// DirichletBC::apply on the sliced storage: a thread per (block) row walks its
+// entries DU at a time, every column and mask load of a step issued before the
+// first test, so that a row costs len / DU dependent round trips instead of
+// 2 len; entries past the row's length are never touched (padding repeats the
+// last column, which may be the diagonal).
+constexpr int DU = 8;
+
 __global__ void k_dirichlet_csr(int64_t rows, const int32_t *ptr, const int32_t *sptr, const int32_t *iperm,
                                 const int32_t *col, double *val, const uint8_t *mask, const double *g, double *rhs) {
   int64_t r = blockIdx.x * (int64_t)NT + threadIdx.x;
   if (r >= rows) return;
   const bool mr = mask[r] != 0;
   const int len = ptr[r + 1] - ptr[r];
-  for (int t = 0; t < len; ++t) {
-    const int64_t k = dev::sell_pos(sptr, iperm, r, t);
-    const int c = col[k];
-    if (mr || mask[c]) {
-      val[k] = 0;
-      if (r == c) {
-        rhs[r] = g ? g[c] : 0.0;
-        val[k] = 1;
+  const int64_t k0 = dev::sell_pos(sptr, iperm, r, 0);
+  for (int t0 = 0; t0 < len; t0 += DU) {
+    int c[DU];
+    uint8_t mc[DU];
+#pragma unroll
+    for (int u = 0; u < DU; ++u) c[u] = col[k0 + (int64_t)(t0 + u < len ? t0 + u : len - 1) * 64];
+#pragma unroll
+    for (int u = 0; u < DU; ++u) mc[u] = mask[c[u]];
+#pragma unroll
+    for (int u = 0; u < DU; ++u) {
+      if (t0 + u < len && (mr || mc[u])) {
+        const int64_t k = k0 + (int64_t)(t0 + u) * 64;
+        val[k] = r == c[u] ? 1.0 : 0.0;
+        if (r == c[u]) rhs[r] = g ? g[c[u]] : 0.0;
       }
     }
   }
@@ -468,20 +553,34 @@ __global__ void k_dirichlet_bcsr(int64_t brows, int64_t ntot, const int32_t *ptr
   int64_t br = blockIdx.x * (int64_t)NT + threadIdx.x;
   if (br >= brows) return;
   const int len = ptr[br + 1] - ptr[br];
-  for (int t = 0; t < len; ++t) {
-    const int64_t k = dev::sell_pos(sptr, iperm, br, t);
-    const int64_t bc = col[k];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        const int64_t r = 3 * br + i, c = 3 * bc + j;
-        if (mask[r] || mask[c]) {
-          val[(int64_t)(i * 3 + j) * ntot + k] = 0;
-          if (r == c) {
-            rhs[r] = g ? g[c] : 0.0;
-            val[(int64_t)(i * 3 + j) * ntot + k] = 1;
+  const int64_t k0 = dev::sell_pos(sptr, iperm, br, 0);
+  bool mr[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) mr[i] = mask[3 * br + i] != 0;
+  for (int t0 = 0; t0 < len; t0 += DU) {
+    int bcs[DU];
+    uint8_t mc[DU][3];
+#pragma unroll
+    for (int u = 0; u < DU; ++u) bcs[u] = col[k0 + (int64_t)(t0 + u < len ? t0 + u : len - 1) * 64];
+#pragma unroll
+    for (int u = 0; u < DU; ++u)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) mc[u][j] = mask[3 * (int64_t)bcs[u] + j];
+#pragma unroll
+    for (int u = 0; u < DU; ++u) {
+      if (t0 + u >= len) continue;
+      const int64_t k = k0 + (int64_t)(t0 + u) * 64;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const int64_t r = 3 * br + i, c = 3 * (int64_t)bcs[u] + j;
+          if (mr[i] || mc[u][j]) {
+            val[(int64_t)(i * 3 + j) * ntot + k] = r == c ? 1.0 : 0.0;
+            if (r == c) rhs[r] = g ? g[c] : 0.0;
           }
         }
-      }
+    }
   }
 }
 
@@ -617,6 +716,42 @@ int yafel_hip_element_matrices_fp32(yafel_hip_mesh m, const yafel_hip_physics *p
   YH_TRY_END(m ? m->ctx : nullptr)
 }
 
+// The direct-write map of A's pattern (CSR, one dof per node), see k_dmap_fill.
+void build_direct_map(yafel_hip_mesh_s *m, yafel_hip_matrix_s *A) {
+  if (A->asm_nshared >= 0) return;
+  YhPlan *pl = A->plan;
+  hipStream_t s = m->ctx->stream;
+  const int64_t nruns = pl->nruns, nent = m->nel * (int64_t)m->nen * m->nen;
+  REQUIRE(A->ntot < INT32_MAX && nent < INT32_MAX, "assemble: matrix too large for the direct-write map");
+  DBuf cnt, off;
+  cnt.alloc((nruns + 1) * 8);
+  off.alloc((nruns + 1) * 8);
+  k_dmap_count<<<ceil_div(nruns + 1, NT), NT, 0, s>>>(nruns, pl->cptr.as<int32_t>(), cnt.as<int64_t>());
+  HIPCHK(hipGetLastError());
+  exclusive_scan_i64(cnt.as<int64_t>(), off.as<int64_t>(), nruns + 1, s);
+  int64_t tot = 0;
+  HIPCHK(hipMemcpyAsync(&tot, off.as<int64_t>() + nruns, 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const int64_t nm = tot >> 32;
+  const int32_t ncon = (int32_t)(tot & 0xffffffffLL);
+  A->asm_dmap.alloc(std::max<int64_t>(nent, 1) * 4);
+  A->asm_mpos.alloc(std::max<int64_t>(nm, 1) * 4);
+  A->asm_mbeg.alloc((nm + 1) * 4);
+  A->asm_mstage.alloc(std::max<int64_t>(ncon, 1) * 4);
+  if (nruns > 0) {
+    k_dmap_fill<<<ceil_div(nruns, NT), NT, 0, s>>>(nruns, pl->cptr.as<int32_t>(), pl->contrib.as<uint32_t>(),
+                                                   pl->run_row.as<int32_t>(), pl->node_ptr.as<int32_t>(),
+                                                   A->sptr.as<int32_t>(),
+                                                   A->iperm.p ? A->iperm.as<int32_t>() : nullptr,
+                                                   off.as<int64_t>(), A->asm_dmap.as<int32_t>(),
+                                                   A->asm_mpos.as<int32_t>(), A->asm_mbeg.as<int32_t>());
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemcpyAsync(A->asm_mbeg.as<int32_t>() + nm, &ncon, 4, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  A->asm_nshared = nm;
+}
+
 int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requirements,
                        yafel_hip_matrix A, double *R_dev) {
   YH_TRY_BEGIN
@@ -645,7 +780,11 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
   REQUIRE(!f32 || (m->dim == 2 && (m->p == 3 || m->p == 4) && m->dpn == 1 && phys->kind == YAFEL_HIP_PHYS_POISSON),
           "assemble: FP32 matrix-core element matrices cover 2-D Poisson with p = 3 or 4");
   if (wk && !f32 && m->stageK.bytes < (size_t)m->nel * nld * nld * 8) m->stageK.alloc((size_t)m->nel * nld * nld * 8);
-  if (wk && f32 && m->stageK32.bytes < (size_t)m->nel * nld * nld * 4)
+  // the FP32 matrix-core tangent of a CSR pattern is written straight into
+  // the matrix (entries shared by several elements through a compact staging)
+  const bool direct = wk && f32 && A->format == YAFEL_HIP_CSR;
+  if (direct) build_direct_map(m, A);
+  if (wk && f32 && !direct && m->stageK32.bytes < (size_t)m->nel * nld * nld * 4)
     m->stageK32.alloc((size_t)m->nel * nld * nld * 4);
   if (wr && m->stageR.bytes < (size_t)m->nel * nld * 8) m->stageR.alloc((size_t)m->nel * nld * 8);
   Phys ph = make_phys(phys);
@@ -653,17 +792,26 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
   const bool sym = !f32 && phys->kind != YAFEL_HIP_PHYS_SUPG;
   if (f32) {
     const unsigned g = (unsigned)std::min<int64_t>((m->nel + 3) / 4, (int64_t)m->ctx->num_cus * 3);
-    float *k32 = wk ? m->stageK32.as<float>() : nullptr;
+    float *k32 = wk && !direct ? m->stageK32.as<float>() : nullptr;
     double *r64 = wr ? m->stageR.as<double>() : nullptr;
+    const int32_t *dmap = direct ? A->asm_dmap.as<int32_t>() : nullptr;
+    double *dval = direct ? A->val64.as<double>() : nullptr;
+    float *mst = direct ? A->asm_mstage.as<float>() : nullptr;
     if (m->p == 3)
       k_element_mfma<3><<<g, 256, 0, s>>>(m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(),
                                            m->tabN.as<double>(), m->tabdN.as<double>(), m->tabw.as<double>(), ph.f,
-                                           coef, k32, r64, wk, wr);
+                                           coef, k32, r64, wk, wr, dmap, dval, mst);
     else
       k_element_mfma<4><<<g, 256, 0, s>>>(m->nel, m->elements.as<int32_t>(), m->nodes.as<double>(),
                                            m->tabN.as<double>(), m->tabdN.as<double>(), m->tabw.as<double>(), ph.f,
-                                           coef, k32, r64, wk, wr);
+                                           coef, k32, r64, wk, wr, dmap, dval, mst);
     HIPCHK(hipGetLastError());
+    if (direct && A->asm_nshared > 0) {
+      k_gather_shared<<<ceil_div(A->asm_nshared, NT), NT, 0, s>>>(A->asm_nshared, A->asm_mpos.as<int32_t>(),
+                                                                  A->asm_mbeg.as<int32_t>(),
+                                                                  A->asm_mstage.as<float>(), A->val64.as<double>());
+      HIPCHK(hipGetLastError());
+    }
   } else {
     launch_element(m, ph, coef, wk ? m->stageK.as<double>() : nullptr, wr ? m->stageR.as<double>() : nullptr, wk,
                    wr, sym);
@@ -674,7 +822,7 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
     const int32_t *sp = A->sptr.as<int32_t>();
     const int32_t *ip = A->iperm.p ? A->iperm.as<int32_t>() : nullptr;
     double *val = A->val64.as<double>();
-    if (pl->nruns > 0) {
+    if (pl->nruns > 0 && !direct) {
 #define GATHER(DPN_, FMT_, KT_, SYM_, K_)                                                                        \
   k_gather<DPN_, FMT_, KT_, SYM_><<<g, NT, 0, s>>>(pl->nruns, m->nen, pl->cptr.as<int32_t>(),                     \
                                                    pl->contrib.as<uint32_t>(), K_, pl->run_row.as<int32_t>(),     \

@@ -176,6 +176,12 @@ struct yafel_hip_matrix_s {
   int64_t mx_emax = 0;          // most (padded) entries in a group
   int64_t v32_gen = 0;          // bumped whenever val32 is rebuilt
   int64_t mx_gen = -1;          // the val32 generation mx_val32 holds
+  // direct-write map of the FP32 matrix-core assembly (built on first use):
+  // per element entry its sliced position, or -1 - its index in the compact
+  // staging of entries shared by several elements; per shared entry its
+  // position and the start of its contributions
+  yh::DBuf asm_dmap, asm_mpos, asm_mbeg, asm_mstage;
+  int64_t asm_nshared = -1;     // shared entries (-1: map not built)
   yh::DBuf mx_grow;             // row of each staged position (64 per group), -1 when empty
   int64_t mx_ngr = 0;           // staged groups
   yh::DBuf mx_val32s;           // staged values times their column's D^-1
