@@ -452,9 +452,10 @@ def c3_bench(args):
 
 def spmv_bench(args):
     """The SpMV alone (FP32 and FP64) on each config's matrix -- c2 BCSR3
-    (the --n mesh), c3 CSR Q4, c5 CSR Q2, c1 CSR Q1 -- plus one copy-kernel
-    pass of known size: the command the rocprof counter runs profile, the
-    copy kernel being the calibration of their byte counters."""
+    (the --n mesh), c3 CSR Q4, c5 CSR Q2 (with the k = 64 multi-RHS product),
+    c1 CSR Q1 -- plus one copy-kernel pass of known size: the command the
+    rocprof counter runs profile, the copy kernel being the calibration of
+    their byte counters."""
     from yafel_amd import hip, workloads
     ctx = hip.Context(0)
     out = {"workload": "SpMV alone on the config matrices"}
@@ -485,6 +486,21 @@ def spmv_bench(args):
                 stored = inf.nnz * (s + ib) + (n + 1) * 4 + 2 * n * s
             rec[name] = {"us": ms * 1e3, "gbs": byts / (ms * 1e-3) / 1e9, "algorithmic_bytes": byts,
                          "stored_bytes": stored, "gbs_stored": stored / (ms * 1e-3) / 1e9}
+        if cname == "c5":
+            # the multi-RHS product on the same matrix, k = 64 right-hand sides
+            k = 64
+            for name, prec, dt, sz in (("spmm_fp32_k64", hip.FP32, np.float32, 4), ("spmm_fp64_k64", hip.FP64,
+                                                                                      np.float64, 8)):
+                X = ctx.upload(np.random.default_rng(1).standard_normal(n * k).astype(dt))
+                Y = ctx.alloc(n * k, dt)
+                A.spmm(X, Y, k, prec)
+                ctx.timer_begin()
+                for _ in range(max(1, args.steps // 5)):
+                    A.spmm(X, Y, k, prec)
+                ms = ctx.timer_end() / max(1, args.steps // 5)
+                byts = inf.nnz * (sz + 4) + (n + 1) * 4 + 2 * k * n * sz
+                rec[name] = {"us": ms * 1e3, "gbs": byts / (ms * 1e-3) / 1e9, "algorithmic_bytes": byts}
+                del X, Y
         log(f"spmv {cname}: {rec}")
         out[cname] = rec
         del w, A
