@@ -39,6 +39,7 @@ struct Pre4 {
 struct MState {
   int done, paused, first, k;
   int method, stop, mixed, pidx;
+  int nact;  // active right-hand sides after the last fold (within the 64-byte header the host polls)
   long long iter, max_iter, refinements;
   double tol, delta2;
   unsigned int counter[8];
@@ -782,9 +783,10 @@ __device__ void pause_when_half_want(MState *st) {
 }
 
 __device__ void all_inactive_done(MState *st) {
-  int any = 0;
-  for (int j = 0; j < st->k; ++j) any |= st->active[j];
-  if (!any) st->done = 1;
+  int na = 0;
+  for (int j = 0; j < st->k; ++j) na += st->active[j] != 0;
+  st->nact = na;
+  if (!na) st->done = 1;
 }
 
 // ---- BiCGSTAB (right Jacobi): p = r + beta (p - omega v)
@@ -1538,6 +1540,27 @@ CsrView<T> cview(yafel_hip_matrix_s *A) {
                     (const T *)(sizeof(T) == 8 ? A->val64.p : A->val32.p), A->rows};
 }
 
+// dst (n x knew) = the columns cols[] of src (n x kold); cols[j] < 0: zero
+template <class T>
+__global__ void k_take_cols(int64_t n, int kold, int knew, const int *__restrict__ cols, const T *__restrict__ src,
+                            T *__restrict__ dst) {
+  const int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (t >= n * knew) return;
+  const int64_t i = t / knew;
+  const int c = cols[t - i * knew];
+  dst[t] = c >= 0 ? src[i * kold + c] : T(0);
+}
+
+// dst column cols_out[j] (n x kout) = src column j (n x kin), for cols_out[j] >= 0
+__global__ void k_put_cols(int64_t n, int kin, int kout, const int *__restrict__ cols_out,
+                           const double *__restrict__ src, double *__restrict__ dst) {
+  const int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (t >= n * kin) return;
+  const int64_t i = t / kin;
+  const int c = cols_out[t - i * kin];
+  if (c >= 0) dst[i * kout + c] = src[t];
+}
+
 struct Multi {
   yafel_hip_ctx_s *ctx;
   yafel_hip_matrix_s *A;
@@ -1606,7 +1629,16 @@ struct Multi {
   }
 
   template <class T>
-  void run(const double *B, double *X, yafel_hip_solve_info *info) {
+  void run(const double *B0, double *X0, yafel_hip_solve_info *info) {
+    // B0, X0: the caller's n x k0 columns.  Once the batch has been compacted
+    // (below), B and X are owned n x k copies of the columns still in it, and
+    // map[j] is the caller's column of batch column j (-1: padding).
+    const int k0 = k;
+    const double *B = B0;
+    double *X = X0;
+    std::vector<int> map(k0);
+    for (int j = 0; j < k0; ++j) map[j] = j;
+    DBuf Bc, Xc, tmp, colsd;
     const size_t nk = (size_t)n * k;
     for (auto &v : vecs) v.alloc(nk * sizeof(T));
     T *r = vecs[0].as<T>(), *rhat = vecs[1].as<T>(), *p0 = vecs[2].as<T>(), *p1 = vecs[3].as<T>();
@@ -1687,6 +1719,88 @@ struct Multi {
       HIPCHK(hipMemcpyAsync(pin + 16384 + slot * si, st, 64, hipMemcpyDeviceToHost, s));
       HIPCHK(hipEventRecord(ev[si], s));
     };
+    auto final_info = [&](const MState &f, int j, yafel_hip_solve_info &out) {
+      out.status = f.status[j];
+      out.iterations = f.it[j];
+      out.refinements = f.refinements;
+      out.criterion = f.crit[j];
+      out.rel_residual = f.bb[j] > 0 ? std::sqrt(f.true_rr[j] / f.bb[j]) : 0.0;
+    };
+    // Compaction: the batch keeps paying for every column until its last
+    // right-hand side converges.  Once the active ones fit in three quarters
+    // of the batch (rounded up to a multiple of 4), the columns that left are
+    // finished (their info and x go to the caller: a right-hand side leaves
+    // only at a refinement, after its x was folded into X), and every vector
+    // that lives across iterations, X, B and the per-column state move to
+    // the first columns of a narrower batch.  Each column's arithmetic is
+    // unchanged; only the shared pause decisions see fewer columns.
+    // Returns true when the state already says done.
+    auto compact = [&]() -> bool {
+      MState fs;
+      HIPCHK(hipMemcpyAsync(&fs, st, sizeof(MState), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (fs.done) return true;
+      std::vector<int> cols;
+      for (int j = 0; j < k; ++j)
+        if (fs.active[j]) cols.push_back(j);
+      const int kn = std::max(4, ((int)cols.size() + 3) & ~3);
+      if (kn >= k) return false;
+      const int nkeep = (int)cols.size();
+      cols.resize(kn, -1);
+      std::vector<int> out(k, -1), nmap(kn, -1);
+      for (int j = 0; j < k; ++j)
+        if (map[j] >= 0 && !fs.active[j]) {
+          final_info(fs, j, info[map[j]]);
+          out[j] = map[j];
+        }
+      for (int jj = 0; jj < kn; ++jj) nmap[jj] = cols[jj] >= 0 ? map[cols[jj]] : -1;
+      if (colsd.bytes < (size_t)2 * KMAX * 4) colsd.alloc((size_t)2 * KMAX * 4);
+      int *cd = colsd.as<int>(), *od = cd + KMAX;
+      HIPCHK(hipMemcpyAsync(cd, cols.data(), kn * 4, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(od, out.data(), k * 4, hipMemcpyHostToDevice, s));
+      // the finished columns' x to the caller (in place while X is the caller's)
+      if (X != X0)
+        k_put_cols<<<ceil_div((int64_t)n * k, NT), NT, 0, s>>>(n, k, k0, od, X, X0);
+      if (tmp.bytes < (size_t)n * k * 8) tmp.alloc((size_t)n * k * 8);
+      for (T *buf : {r, rhat, p0, p1, v, xl, z}) {
+        k_take_cols<T><<<ceil_div((int64_t)n * kn, NT), NT, 0, s>>>(n, k, kn, cd, buf, tmp.as<T>());
+        HIPCHK(hipMemcpyAsync(buf, tmp.p, (size_t)n * kn * sizeof(T), hipMemcpyDeviceToDevice, s));
+      }
+      DBuf nx, nb;
+      nx.alloc((size_t)n * kn * 8);
+      nb.alloc((size_t)n * kn * 8);
+      k_take_cols<double><<<ceil_div((int64_t)n * kn, NT), NT, 0, s>>>(n, k, kn, cd, X, nx.as<double>());
+      k_take_cols<double><<<ceil_div((int64_t)n * kn, NT), NT, 0, s>>>(n, k, kn, cd, B, nb.as<double>());
+      HIPCHK(hipGetLastError());
+      MState ns = fs;
+      ns.k = kn;
+      ns.nact = nkeep;
+      for (int jj = 0; jj < kn; ++jj) {
+        const int c = cols[jj];
+        if (c >= 0) {
+          ns.rho[jj] = fs.rho[c]; ns.rho_prev[jj] = fs.rho_prev[c]; ns.rho0[jj] = fs.rho0[c];
+          ns.alpha[jj] = fs.alpha[c]; ns.omega[jj] = fs.omega[c]; ns.beta[jj] = fs.beta[c];
+          ns.rr[jj] = fs.rr[c]; ns.rr_ref[jj] = fs.rr_ref[c]; ns.bb[jj] = fs.bb[c];
+          ns.true_rr[jj] = fs.true_rr[c]; ns.crit[jj] = fs.crit[c];
+          ns.active[jj] = fs.active[c]; ns.status[jj] = fs.status[c]; ns.want[jj] = fs.want[c];
+          ns.it[jj] = fs.it[c];
+        } else {
+          ns.rho[jj] = ns.rho_prev[jj] = ns.rho0[jj] = ns.alpha[jj] = ns.omega[jj] = ns.beta[jj] = 0;
+          ns.rr[jj] = ns.rr_ref[jj] = ns.bb[jj] = ns.true_rr[jj] = ns.crit[jj] = 0;
+          ns.active[jj] = 0; ns.status[jj] = YAFEL_HIP_STATUS_CONVERGED; ns.want[jj] = 0; ns.it[jj] = 0;
+        }
+      }
+      std::memcpy(ctx->pinned, &ns, sizeof(MState));
+      HIPCHK(hipMemcpyAsync(st, ctx->pinned, sizeof(MState), hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));
+      Xc.swap(nx);
+      Bc.swap(nb);
+      X = Xc.as<double>();
+      B = Bc.as<double>();
+      map.swap(nmap);
+      k = kn;
+      return false;
+    };
     chunk(0);
     chunk(1);
     int polled = 0;
@@ -1694,8 +1808,17 @@ struct Multi {
     const auto t0 = std::chrono::steady_clock::now();
     while (true) {
       HIPCHK(hipEventSynchronize(ev[polled & 1]));
-      const int *hs_done = (const int *)(pin + 16384 + slot * (polled & 1));
-      if (hs_done[0]) break;
+      const int *hs = (const int *)(pin + 16384 + slot * (polled & 1));
+      if (hs[0]) break;
+      const int nact = hs[8];  // MState::nact
+      if (nact > 0 && k >= 8 && std::max(4, (nact + 3) & ~3) <= (3 * k) / 4) {
+        HIPCHK(hipStreamSynchronize(s));  // the other chunk in flight has run as well
+        if (compact()) break;
+        chunk(0);
+        chunk(1);
+        polled = 0;
+        continue;
+      }
       chunk(polled & 1);
       ++polled;
       if (++guard > 2 * o.max_iter + 1000) fail(YAFEL_HIP_EDEVICE, "solve_multi: no progress");
@@ -1712,13 +1835,16 @@ struct Multi {
     MState fs;
     HIPCHK(hipMemcpyAsync(&fs, st, sizeof(MState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    for (int j = 0; j < k; ++j) {
-      info[j].status = fs.status[j];
-      info[j].iterations = fs.it[j];
-      info[j].refinements = fs.refinements;
-      info[j].criterion = fs.crit[j];
-      info[j].rel_residual = fs.bb[j] > 0 ? std::sqrt(fs.true_rr[j] / fs.bb[j]) : 0.0;
+    for (int j = 0; j < k; ++j)
+      if (map[j] >= 0) final_info(fs, j, info[map[j]]);
+    if (X != X0) {
+      if (colsd.bytes < (size_t)2 * KMAX * 4) colsd.alloc((size_t)2 * KMAX * 4);
+      HIPCHK(hipMemcpyAsync(colsd.p, map.data(), k * 4, hipMemcpyHostToDevice, s));
+      k_put_cols<<<ceil_div((int64_t)n * k, NT), NT, 0, s>>>(n, k, k0, colsd.as<int>(), X, X0);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(s));
     }
+    k = k0;
   }
 };
 
