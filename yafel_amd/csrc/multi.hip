@@ -63,7 +63,8 @@ template <class T> struct CsrView {
 
 // Per-lane block reduction of NP sums into part[(j*grid + blk)*64 + lane];
 // k_mfinish folds them (fixed order) in a one-workgroup kernel launched next
-// on the stream and calls op.last_lane per RHS, then op.last_all once.
+// on the stream and calls op.last_lane per RHS, then op.last_all once with
+// the counts of active and pause-wanting right-hand sides.
 template <int NP, class Op>
 __device__ __forceinline__ void finish(Op &op, double (&acc)[NP > 0 ? NP : 1]) {
   if constexpr (NP > 0) {
@@ -132,7 +133,15 @@ __global__ __launch_bounds__(NTM) void k_mfinish(Op op0, int nblk) {
     op.last_lane(lane, v);
   }
   __syncthreads();
-  if (threadIdx.x == 0) op.last_all();
+  if (w == 0) {
+    // the batch's active and pause-wanting columns counted by one wave, a
+    // column per lane, for the batch-wide decisions of last_all
+    const int kk = op.st->k;
+    const bool a = lane < kk && op.st->active[lane] != 0;
+    const bool wt = a && op.st->want[lane] != 0;
+    const int na = (int)__popcll(__ballot(a)), nw = (int)__popcll(__ballot(wt));
+    if (lane == 0) op.last_all(na, nw);
+  }
 }
 
 // Y-producing kernel: a wavefront takes RB rows of a slice (32 for FP64, whose
@@ -708,7 +717,7 @@ struct MBase {
   unsigned int *counter;
   int k;
   __device__ bool running() const { return !st->done && !st->paused; }
-  __device__ void last_all() const {}
+  __device__ void last_all(int, int) const {}
   // product ops: no column scale unless the op supplies one
   __device__ const void *cscale() const { return nullptr; }
   // the four right-hand sides 4g .. 4g+3 of one lane (FP32 products): their
@@ -771,20 +780,12 @@ __device__ void request_pause_or_advance(MState *st, int j, double rho_new, doub
 }
 
 // after an iteration: pause for the shared residual replacement once half of
-// the active right-hand sides want one
-__device__ void pause_when_half_want(MState *st) {
-  int na = 0, nw = 0;
-  for (int j = 0; j < st->k; ++j)
-    if (st->active[j]) {
-      ++na;
-      nw += st->want[j];
-    }
+// the na active right-hand sides (nw of them wanting one) want one
+__device__ void pause_when_half_want(MState *st, int na, int nw) {
   if (nw > 0 && 2 * nw >= na) st->paused = 1;
 }
 
-__device__ void all_inactive_done(MState *st) {
-  int na = 0;
-  for (int j = 0; j < st->k; ++j) na += st->active[j] != 0;
+__device__ void all_inactive_done(MState *st, int na) {
   st->nact = na;
   if (!na) st->done = 1;
 }
@@ -869,9 +870,9 @@ struct MV : MBase {
       st->active[j] = 0;
     }
   }
-  __device__ void last_all() const {
+  __device__ void last_all(int na, int nw) const {
     st->first = 0;
-    all_inactive_done(st);
+    all_inactive_done(st, na);
   }
 };
 
@@ -953,7 +954,7 @@ struct MT : MBase {
       st->active[j] = 0;
     }
   }
-  __device__ void last_all() const { all_inactive_done(st); }
+  __device__ void last_all(int na, int nw) const { all_inactive_done(st, na); }
 };
 
 // x += alpha D^-1 p + omega D^-1 s, r = s - omega t; rhat.r, r.r
@@ -1014,10 +1015,10 @@ struct MX : MBase {
     if (!st->active[j]) return;
     request_pause_or_advance(st, j, sm[0], sm[1], false);
   }
-  __device__ void last_all() const {
+  __device__ void last_all(int na, int nw) const {
     st->iter += 1;
-    pause_when_half_want(st);
-    all_inactive_done(st);
+    pause_when_half_want(st, na, nw);
+    all_inactive_done(st, na);
   }
 };
 
@@ -1113,10 +1114,10 @@ struct MCgA : MBase {
       st->active[j] = 0;
     }
   }
-  __device__ void last_all() const {
+  __device__ void last_all(int na, int nw) const {
     st->first = 0;
     st->pidx = cur ^ 1;
-    all_inactive_done(st);
+    all_inactive_done(st, na);
   }
 };
 
@@ -1183,10 +1184,10 @@ struct MCgB : MBase {
     if (!st->active[j]) return;
     request_pause_or_advance(st, j, sm[0], sm[1], true);
   }
-  __device__ void last_all() const {
+  __device__ void last_all(int na, int nw) const {
     st->iter += 1;
-    pause_when_half_want(st);
-    all_inactive_done(st);
+    pause_when_half_want(st, na, nw);
+    all_inactive_done(st, na);
   }
 };
 
@@ -1273,7 +1274,7 @@ struct MRes : MBase {
     st->rr[j] = s[1];
     st->rr_ref[j] = s[1];
   }
-  __device__ void last_all() const {
+  __device__ void last_all(int na, int nw) const {
     if (MODE == 0) {
       st->first = 1;
       st->paused = 0;
@@ -1282,7 +1283,7 @@ struct MRes : MBase {
       st->refinements += 1;
       st->paused = 0;
     }
-    all_inactive_done(st);
+    all_inactive_done(st, na);
   }
 };
 
@@ -1298,6 +1299,22 @@ struct MFold : MBase {
     const int64_t o = i * k + j;
     X[o] += (double)xl[o];
     xl[o] = TL(0);
+  }
+  // k_melem4: every column, active or not, as elem does
+  static constexpr bool V4 = sizeof(TL) == 4;
+  struct In {
+    double2 x01, x23;
+    float4 l;
+  };
+  __device__ In load4(int64_t, int64_t o) const {
+    const double2 *x2 = reinterpret_cast<const double2 *>(X + o);
+    return In{x2[0], x2[1], ld4((const float *)xl, o)};
+  }
+  __device__ void apply4(int64_t, int64_t o, const In &a, double (&)[1][4]) const {
+    double2 *x2 = reinterpret_cast<double2 *>(X + o);
+    x2[0] = make_double2(a.x01.x + (double)a.l.x, a.x01.y + (double)a.l.y);
+    x2[1] = make_double2(a.x23.x + (double)a.l.z, a.x23.y + (double)a.l.w);
+    *reinterpret_cast<float4 *>((float *)xl + o) = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   __device__ void last_lane(int, const double *) const {}
 };
