@@ -451,10 +451,20 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds(StagedCsr A, Op op0) {
 // X -- and at a group's second pass, of the next group's entries and list --
 // are already in flight into registers, and are stored to LDS after the next
 // barrier.  Same rows, sums and order as k_mspmm_lds.
-// Two waves per SIMD: the LDS of two workgroups per CU is what the staged
-// lists were sized for, and the scheduler may then keep more loads in flight.
-template <class Op>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mspmm_pipe(StagedCsr A, Op op0) {
+// KI staging loads per thread and unit (lists of up to 32 KI rows).  With
+// KI = 6 and one 16-byte register per staged quad the kernel fits three waves
+// per SIMD (at most 168 registers), four when the op keeps no partial sums
+// (the plain product), else two.  Residency is what bounds these kernels:
+// each unit waits on its staging loads and on LDS round trips, and more
+// resident workgroups overlap those waits.
+template <class Op, int KI>
+constexpr int pipe_waves() {
+  return KI <= 6 && sizeof(typename Op::Raw) <= 16 ? (Op::NP == 0 ? 4 : 3) : 2;
+}
+
+template <class Op, int KI>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(pipe_waves<Op, KI>(), pipe_waves<Op, KI>())))
+void k_mspmm_pipe(StagedCsr A, Op op0) {
   extern __shared__ float4 xs4[];  // A.dcap rows of KC / 4
   __shared__ __align__(16) float ev[kStageEnt];
   __shared__ __align__(16) unsigned short el[kStageEnt];
@@ -484,8 +494,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   // to LDS: a conditional load, or arithmetic on its result, makes the
   // compiler wait for it at once instead of during the rows.
   using Raw = typename Op::Raw;
-  int pc[kStageIter];
-  Raw px[kStageIter];
+  int pc[KI];
+  Raw px[KI];
   float pv[SE];
   unsigned short pl[SE];
   int pp = 0, prow = 0, d0 = 0, d = 0, ne = 0;
@@ -505,7 +515,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       pl[m] = __builtin_nontemporal_load(&A.lidx[e]);
     }
 #pragma unroll
-    for (int m = 0; m < kStageIter; ++m) {
+    for (int m = 0; m < KI; ++m) {
       const int i = i0 + m * (NT / 8);
       pc[m] = A.dcol[d > 0 ? d0 + (i < d ? i : 0) : 0];
     }
@@ -514,7 +524,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     const int jq = h * (KC / 4) + q;
     const int jl = 4 * jq < k ? jq : h * (KC / 4);
 #pragma unroll
-    for (int m = 0; m < kStageIter; ++m) px[m] = op.load4(pc[m], jl);
+    for (int m = 0; m < KI; ++m) px[m] = op.load4(pc[m], jl);
   };
   load_meta(g);
   load_x(0);
@@ -549,7 +559,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       const bool on = 4 * jq < k;
       op.bind4s(on ? jq : h * (KC / 4));
 #pragma unroll
-      for (int m = 0; m < kStageIter; ++m)
+      for (int m = 0; m < KI; ++m)
         if (i0 + m * (NT / 8) < d && on) xs4[(i0 + m * (NT / 8)) * (KC / 4) + q] = op.make4(px[m]);
     }
     __syncthreads();
@@ -1538,17 +1548,31 @@ bool staged_ok(yafel_hip_matrix_s *A) {
   return A->mx_dmax > 0;
 }
 
+// grid: at most this many workgroups; the kernel's residency per CU caps it
+// further.  Returns the grid launched (the partials' block count).
 template <class Op>
-void launch_mspmm_lds(yafel_hip_matrix_s *A, const Op &op, int grid, hipStream_t s) {
+int launch_mspmm_lds(yafel_hip_matrix_s *A, const Op &op, int grid, hipStream_t s) {
   matrix_refresh_staged(A);
   const float *val = Op::kScaledVals ? matrix_staged_scaled(A) : A->mx_val32.as<float>();
   const StagedCsr v{A->mx_ptr.as<int>(), val, A->mx_lidx.as<unsigned short>(), A->mx_dptr.as<int>(),
                     A->mx_dcol.as<int>(), A->mx_grow.as<int>(), A->mx_ngr, std::min(A->mx_dmax, kStageRows)};
-  if (A->mx_dmax <= kStageRows && A->mx_emax <= kStageEnt)
-    k_mspmm_pipe<Op><<<grid, NT, (size_t)v.dcap * KC * sizeof(float), s>>>(v, op);
-  else
-    k_mspmm_lds<Op><<<grid, NT, (size_t)v.dcap * KC * sizeof(float), s>>>(v, op);
+  const size_t lds = (size_t)v.dcap * KC * sizeof(float);
+  auto cap = [&](int waves) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)grid, A->mx_ngr, (int64_t)waves * A->ctx->num_cus}));
+  };
+  int g;
+  if (A->mx_dmax <= kStageRows / 2 && A->mx_emax <= kStageEnt) {
+    g = cap(pipe_waves<Op, kStageIter / 2>());
+    k_mspmm_pipe<Op, kStageIter / 2><<<g, NT, lds, s>>>(v, op);
+  } else if (A->mx_dmax <= kStageRows && A->mx_emax <= kStageEnt) {
+    g = cap(pipe_waves<Op, kStageIter>());
+    k_mspmm_pipe<Op, kStageIter><<<g, NT, lds, s>>>(v, op);
+  } else {
+    g = cap(2);
+    k_mspmm_lds<Op><<<g, NT, lds, s>>>(v, op);
+  }
   HIPCHK(hipGetLastError());
+  return g;
 }
 
 template <class T>
@@ -1609,10 +1633,9 @@ struct Multi {
     if constexpr (sizeof(T) == 4) {
       if (staged && v4) {
         // one group of 64 rows and one pass of 32 right-hand sides per
-        // workgroup at a time; the LDS holds two workgroups per CU, so a
-        // larger grid would only add partials for the fold to read
-        g = (int)std::max<int64_t>(1, std::min<int64_t>(A->mx_ngr, std::min<int64_t>(grid, 2 * (int64_t)ctx->num_cus)));
-        launch_mspmm_lds(A, op, g, s);
+        // workgroup at a time, as many workgroups as are resident at once:
+        // a larger grid would only add partials for the fold to read
+        g = launch_mspmm_lds(A, op, grid, s);
       } else if (v4) {
         k_mspmm4<O><<<g, NT, 0, s>>>(cview<T>(A), op);
       } else {
@@ -1873,7 +1896,7 @@ bool spmm_lds_launch(yafel_hip_matrix_s *A, int k, const float *X, float *Y, hip
   if (k < 1 || k > KMAX || k % 4 != 0 || ((uintptr_t)X % 16) != 0 || ((uintptr_t)Y % 16) != 0 || !staged_ok(A))
     return false;
   MPlain op{X, Y, k, nullptr};
-  launch_mspmm_lds(A, op, (int)std::max<int64_t>(1, std::min<int64_t>(A->mx_ngr, (int64_t)A->ctx->num_cus * 2)), s);
+  launch_mspmm_lds(A, op, A->ctx->num_cus * 8, s);
   return true;
 }
 }  // namespace yh
