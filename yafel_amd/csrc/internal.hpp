@@ -183,6 +183,7 @@ struct yafel_hip_matrix_s {
   yh::DBuf asm_dmap, asm_mpos, asm_mbeg, asm_mstage;
   int64_t asm_nshared = -1;     // shared entries (-1: map not built)
   yh::DBuf mx_grow;             // row of each staged position (64 per group), -1 when empty
+  yh::DBuf mx_dl;               // uint16: each position's own column in its group's list (0: empty position)
   int64_t mx_ngr = 0;           // staged groups
   yh::DBuf mx_val32s;           // staged values times their column's D^-1
   int64_t mx_sgen = -1, mx_dgen = -1;  // the val32 and diagonal generations mx_val32s holds

@@ -34,6 +34,7 @@ constexpr int KMAX = 64;
 // the row's sum so that the loads overlap it
 struct Pre4 {
   float4 a, b;
+  bool own;  // a is the row's own staged value (own4), b unused
 };
 
 struct MState {
@@ -272,6 +273,7 @@ struct StagedCsr {
   const unsigned short *lidx;
   const int *dptr, *dcol;
   const int *grow;  // row of each position, -1 when empty
+  const unsigned short *dl;  // each position's own column in its group's list (0 when empty)
   int64_t ngr;      // groups
   int dcap;         // rows of X the launch's LDS holds
 };
@@ -470,6 +472,7 @@ void k_mspmm_pipe(StagedCsr A, Op op0) {
   __shared__ __align__(16) unsigned short el[kStageEnt];
   __shared__ int ep[65];
   __shared__ int grw[64];  // row of each position of the group
+  __shared__ int gdl[64];  // its own column's place in the group's list
   constexpr int NP = Op::NP, SE = kStageEnt / NT;
   Op op = op0;
   if (op.skip()) return;
@@ -498,7 +501,7 @@ void k_mspmm_pipe(StagedCsr A, Op op0) {
   Raw px[KI];
   float pv[SE];
   unsigned short pl[SE];
-  int pp = 0, prow = 0, d0 = 0, d = 0, ne = 0;
+  int pp = 0, prow = 0, pdl = 0, d0 = 0, d = 0, ne = 0;
   auto load_meta = [&](int64_t gg) {  // the group's list, entries and rows
     d0 = A.dptr[gg];
     d = A.dptr[gg + 1] - d0;
@@ -507,6 +510,7 @@ void k_mspmm_pipe(StagedCsr A, Op op0) {
     ne = A.ptr[r0 + 64] - e0;
     pp = A.ptr[r0 + (tid < 64 ? tid : 64)] - e0;  // used by threads 0..64
     prow = A.grow[r0 + (tid & 63)];                // used by threads 0..63
+    if constexpr (Op::kOwnStaged) pdl = A.dl[r0 + (tid & 63)];
 #pragma unroll
     for (int m = 0; m < SE; ++m) {
       const int i = tid + m * NT;
@@ -547,6 +551,8 @@ void k_mspmm_pipe(StagedCsr A, Op op0) {
     if (h == 0) {
       if (tid < 65) ep[tid] = pp;
       if (tid < 64) grw[tid] = prow;
+      if constexpr (Op::kOwnStaged)
+        if (tid < 64) gdl[tid] = pdl;
 #pragma unroll
       for (int m = 0; m < SE; ++m) {
         const bool in = tid + m * NT < ne;
@@ -574,7 +580,12 @@ void k_mspmm_pipe(StagedCsr A, Op op0) {
     for (int m = 0; m < 2; ++m) {
       rls[m] = staged_set(w, m) * 8 + rho;
       rws[m] = grw[rls[m]];
-      prs[m] = op.pre4(rws[m] >= 0 ? rws[m] : 0, 4 * (on ? jq : j0 / 4));
+      if constexpr (Op::kOwnStaged) {
+        // the row's own column is staged: its operand from LDS, no load
+        prs[m] = Op::own4(xs4[gdl[rls[m]] * (KC / 4) + q]);
+      } else {
+        prs[m] = op.pre4(rws[m] >= 0 ? rws[m] : 0, 4 * (on ? jq : j0 / 4));
+      }
     }
     // the next unit's loads, in flight during this unit's rows
     const int64_t gn = g + gridDim.x;
@@ -722,6 +733,9 @@ struct MBase {
   static constexpr bool V4 = false;           // has load4 / apply4 for k_melem4
   static constexpr int ME4 = 4;               // k_melem4 row sets per step
   static constexpr bool kScaledVals = false;  // staged products read A D^-1
+  // the row epilogue's operand is the row's own staged value (own4): the
+  // pipelined product reads it from LDS instead of pre4's global loads
+  static constexpr bool kOwnStaged = false;
   MState *st;
   double *part;
   unsigned int *counter;
@@ -861,7 +875,7 @@ struct MV : MBase {
     acc[0] += (double)rhat[i * k + j] * (double)y;
   }
   __device__ float4 x4(int c, int g) const { return ld4((const float *)p, (int64_t)c * k + 4 * g); }
-  __device__ Pre4 pre4(int64_t i, int j0) const { return Pre4{ld4((const float *)rhat, i * k + j0), {}}; }
+  __device__ Pre4 pre4(int64_t i, int j0) const { return Pre4{ld4((const float *)rhat, i * k + j0), {}, false}; }
   __device__ void row4p(int64_t i, int j0, const float4 &y, const Pre4 &pr, double (&acc)[NP][4]) const {
     store4(v, i * k + j0, y, a4c);
 #pragma unroll
@@ -941,7 +955,10 @@ struct MT : MBase {
     acc[1] += (double)y * (double)s[i * k + j];
   }
   __device__ float4 x4(int c, int g) const { return ld4((const float *)s, (int64_t)c * k + 4 * g); }
-  __device__ Pre4 pre4(int64_t i, int j0) const { return Pre4{ld4((const float *)s, i * k + j0), {}}; }
+  __device__ Pre4 pre4(int64_t i, int j0) const { return Pre4{ld4((const float *)s, i * k + j0), {}, false}; }
+  // the staged rows are rows of s, so the own row's staged quad is s itself
+  static constexpr bool kOwnStaged = true;
+  __device__ static Pre4 own4(const float4 &staged) { return Pre4{staged, {}, true}; }
   __device__ void row4p(int64_t i, int j0, const float4 &y, const Pre4 &pr, double (&acc)[NP][4]) const {
     store4(t, i * k + j0, y, a4c);
 #pragma unroll
@@ -1100,11 +1117,15 @@ struct MCgA : MBase {
   __device__ float4 stage4(int c, int g) const { return make4(load4(c, g)); }
   __device__ Pre4 pre4(int64_t i, int j0) const {
     const int64_t o = i * k + j0;
-    return Pre4{ld4((const float *)pold, o), ld4((const float *)z, o)};
+    return Pre4{ld4((const float *)pold, o), ld4((const float *)z, o), false};
   }
+  // the staged rows are p_new = z + beta p_old (make4 with this pass's betas,
+  // the betas row4p uses), so the own row's staged quad is p_new itself
+  static constexpr bool kOwnStaged = true;
+  __device__ static Pre4 own4(const float4 &staged) { return Pre4{staged, {}, true}; }
   __device__ void row4p(int64_t i, int j0, const float4 &y, const Pre4 &pr, double (&acc)[NP][4]) const {
     const int64_t o = i * k + j0;
-    const float4 pn = make_float4(fmaf(b4[0], pr.a.x, pr.b.x), fmaf(b4[1], pr.a.y, pr.b.y),
+    const float4 pn = pr.own ? pr.a : make_float4(fmaf(b4[0], pr.a.x, pr.b.x), fmaf(b4[1], pr.a.y, pr.b.y),
                                   fmaf(b4[2], pr.a.z, pr.b.z), fmaf(b4[3], pr.a.w, pr.b.w));
     store4(pnew, o, pn, a4c);
     store4(q, o, y, a4c);
@@ -1351,6 +1372,7 @@ struct MReport : MBase {
 struct MPlain {
   static constexpr int NP = 0;
   static constexpr bool kScaledVals = false;
+  static constexpr bool kOwnStaged = false;
   const float *X;
   float *Y;
   int k;
@@ -1462,7 +1484,7 @@ void matrix_build_staged(yafel_hip_matrix_s *A) {
   const int64_t npad = pp[npos];
   REQUIRE(npad < INT32_MAX, "staged products: too many entries");
   std::vector<int> src(std::max<int64_t>(npad, 1), -1), dptr(ngr + 1), dcol, tmp, ecol(std::max<int64_t>(npad, 1), 0);
-  std::vector<unsigned short> lidx(std::max<int64_t>(npad, 1), 0);
+  std::vector<unsigned short> lidx(std::max<int64_t>(npad, 1), 0), dl(std::max<int64_t>(npos, 1), 0);
   dcol.reserve(nnz / 3 + 1);
   int dmax = 0;
   int64_t emax = 0;
@@ -1478,6 +1500,9 @@ void matrix_build_staged(yafel_hip_matrix_s *A) {
         ecol[pp[p] + t] = rcol[ptr[r] + t];
         tmp.push_back(rcol[ptr[r] + t]);
       }
+      // every row's own column is staged (a product's row epilogue reads it
+      // from LDS), also where the row stores no diagonal entry
+      if (r < A->bcols) tmp.push_back(r);
     }
     std::sort(tmp.begin(), tmp.end());
     tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
@@ -1491,12 +1516,19 @@ void matrix_build_staged(yafel_hip_matrix_s *A) {
     }
     for (int64_t e = pp[p0]; e < pp[p1]; ++e)
       if (src[e] >= 0) lidx[e] = (unsigned short)(std::lower_bound(tmp.begin(), tmp.end(), ecol[e]) - tmp.begin());
+    for (int64_t p = p0; p < p1; ++p) {
+      const int r = grow[p];
+      if (r < 0 || r >= A->bcols) continue;
+      dl[p] = (unsigned short)(std::lower_bound(tmp.begin(), tmp.end(), r) - tmp.begin());
+    }
   }
   dptr[ngr] = (int)dcol.size();
   A->mx_ptr.alloc((npos + 1) * 4);
   HIPCHK(hipMemcpy(A->mx_ptr.p, pp.data(), (npos + 1) * 4, hipMemcpyHostToDevice));
   A->mx_grow.alloc(std::max<int64_t>(npos, 1) * 4);
   HIPCHK(hipMemcpy(A->mx_grow.p, grow.data(), std::max<int64_t>(npos, 1) * 4, hipMemcpyHostToDevice));
+  A->mx_dl.alloc(std::max<int64_t>(npos, 1) * 2);
+  HIPCHK(hipMemcpy(A->mx_dl.p, dl.data(), std::max<int64_t>(npos, 1) * 2, hipMemcpyHostToDevice));
   A->mx_src.alloc(src.size() * 4);
   A->mx_lidx.alloc(lidx.size() * 2);
   A->mx_dptr.alloc((ngr + 1) * 4);
@@ -1555,7 +1587,8 @@ int launch_mspmm_lds(yafel_hip_matrix_s *A, const Op &op, int grid, hipStream_t 
   matrix_refresh_staged(A);
   const float *val = Op::kScaledVals ? matrix_staged_scaled(A) : A->mx_val32.as<float>();
   const StagedCsr v{A->mx_ptr.as<int>(), val, A->mx_lidx.as<unsigned short>(), A->mx_dptr.as<int>(),
-                    A->mx_dcol.as<int>(), A->mx_grow.as<int>(), A->mx_ngr, std::min(A->mx_dmax, kStageRows)};
+                    A->mx_dcol.as<int>(), A->mx_grow.as<int>(), A->mx_dl.as<unsigned short>(), A->mx_ngr,
+                    std::min(A->mx_dmax, kStageRows)};
   const size_t lds = (size_t)v.dcap * KC * sizeof(float);
   auto cap = [&](int waves) {
     return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)grid, A->mx_ngr, (int64_t)waves * A->ctx->num_cus}));
