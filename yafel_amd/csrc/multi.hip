@@ -278,6 +278,24 @@ struct StagedCsr {
   int dcap;         // rows of X the launch's LDS holds
 };
 
+// The groups a workgroup of the staged products takes: the hardware deals
+// workgroups to the 8 XCDs round-robin (block b runs on XCD b % 8), so each
+// XCD takes a contiguous eighth of the groups and its blocks walk that range
+// side by side.  The groups in flight on one XCD are then neighbours in the
+// mesh, and the X rows that consecutive strips of groups share are still in
+// that XCD's L2 when the next strip stages them (taking every gridDim.x-th
+// group instead gave each XCD strips 8 apart, whose shared rows were fetched
+// past L2 once per strip).
+struct GroupWalk {
+  int64_t g, end, step;
+};
+__device__ __forceinline__ GroupWalk group_walk(int64_t ngr) {
+  const int G = gridDim.x, b = blockIdx.x;
+  if ((G & 7) != 0) return GroupWalk{b, ngr, G};
+  const int64_t chunk = (ngr + 7) / 8, lo = (int64_t)(b & 7) * chunk;
+  return GroupWalk{lo + (b >> 3), lo + chunk < ngr ? lo + chunk : ngr, G >> 3};
+}
+
 // the row set of sigma = 2w + m: sets w and 7 - w of wave w
 __device__ __forceinline__ int staged_set(int w, int m) { return m == 0 ? w : 7 - w; }
 
@@ -337,7 +355,8 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds(StagedCsr A, Op op0) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 7, rho = lane >> 3;
   const int k = op.k;
   const int nh = (k + KC - 1) / KC;
-  for (int64_t g = dev::xcd_block(); g < A.ngr; g += gridDim.x) {
+  const GroupWalk gw = group_walk(A.ngr);
+  for (int64_t g = gw.g; g < gw.end; g += gw.step) {
     const int d0 = A.dptr[g], d = A.dptr[g + 1] - d0;
     const int64_t r0 = g * 64;  // first position of the group
     const int e0 = A.ptr[r0], ne = A.ptr[r0 + 64] - e0;
@@ -486,8 +505,9 @@ void k_mspmm_pipe(StagedCsr A, Op op0) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 7, rho = lane >> 3, i0 = tid >> 3;
   const int k = op.k;
   const int nh = (k + KC - 1) / KC;
-  const int64_t ngr = A.ngr;
-  int64_t g = dev::xcd_block();
+  const GroupWalk gw = group_walk(A.ngr);
+  const int64_t ngr = gw.end;  // this workgroup's groups end here
+  int64_t g = gw.g;
   if (g >= ngr) {
     if constexpr (NP > 0) finish_staged<NP>(op, a0, a1);
     return;
@@ -588,7 +608,7 @@ void k_mspmm_pipe(StagedCsr A, Op op0) {
       }
     }
     // the next unit's loads, in flight during this unit's rows
-    const int64_t gn = g + gridDim.x;
+    const int64_t gn = g + gw.step;
     const bool second = h + 1 < nh;
     if (second) load_x(h + 1);
     else if (gn < ngr) load_meta(gn);
