@@ -791,6 +791,21 @@ struct MBase {
       if (a[3]) dst[o + 3] = v.w;
     }
   }
+  // The products' n x k outputs, written once and read by the next kernel
+  // (hundreds of MB: never from L2): non-temporal, so that they do not evict
+  // the staged rows of X that the next strip of row groups re-reads from L2.
+  template <class T>
+  __device__ static void store4nt(T *dst, int64_t o, const float4 &v, const bool (&a)[4]) {
+    if constexpr (sizeof(T) == 4) {
+      typedef float v4f __attribute__((ext_vector_type(4)));
+      if (a[0] && a[1] && a[2] && a[3]) {
+        const v4f w = {v.x, v.y, v.z, v.w};
+        __builtin_nontemporal_store(w, reinterpret_cast<v4f *>(dst + o));
+        return;
+      }
+    }
+    store4(dst, o, v, a);
+  }
 };
 
 __device__ __forceinline__ float4 ld4(const float *p, int64_t o) { return *reinterpret_cast<const float4 *>(p + o); }
@@ -897,7 +912,7 @@ struct MV : MBase {
   __device__ float4 x4(int c, int g) const { return ld4((const float *)p, (int64_t)c * k + 4 * g); }
   __device__ Pre4 pre4(int64_t i, int j0) const { return Pre4{ld4((const float *)rhat, i * k + j0), {}, false}; }
   __device__ void row4p(int64_t i, int j0, const float4 &y, const Pre4 &pr, double (&acc)[NP][4]) const {
-    store4(v, i * k + j0, y, a4c);
+    store4nt(v, i * k + j0, y, a4c);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
       if (a4c[c]) acc[0][c] += (double)comp(pr.a, c) * (double)comp(y, c);
@@ -980,7 +995,7 @@ struct MT : MBase {
   static constexpr bool kOwnStaged = true;
   __device__ static Pre4 own4(const float4 &staged) { return Pre4{staged, {}, true}; }
   __device__ void row4p(int64_t i, int j0, const float4 &y, const Pre4 &pr, double (&acc)[NP][4]) const {
-    store4(t, i * k + j0, y, a4c);
+    store4nt(t, i * k + j0, y, a4c);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
       if (a4c[c]) {
@@ -1147,8 +1162,8 @@ struct MCgA : MBase {
     const int64_t o = i * k + j0;
     const float4 pn = pr.own ? pr.a : make_float4(fmaf(b4[0], pr.a.x, pr.b.x), fmaf(b4[1], pr.a.y, pr.b.y),
                                   fmaf(b4[2], pr.a.z, pr.b.z), fmaf(b4[3], pr.a.w, pr.b.w));
-    store4(pnew, o, pn, a4c);
-    store4(q, o, y, a4c);
+    store4nt(pnew, o, pn, a4c);
+    store4nt(q, o, y, a4c);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
       if (a4c[c]) acc[0][c] += (double)comp(pn, c) * (double)comp(y, c);
@@ -1409,7 +1424,9 @@ struct MPlain {
   __device__ float4 stage4(int c, int g) const { return ld4(X, (int64_t)c * k + 4 * g); }
   __device__ Pre4 pre4(int64_t, int) const { return Pre4{}; }
   __device__ void row4p(int64_t i, int j0, const float4 &y, const Pre4 &, double (&)[1][4]) const {
-    *reinterpret_cast<float4 *>(Y + i * k + j0) = y;
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f v = {y.x, y.y, y.z, y.w};  // written once: non-temporal, as MBase::store4nt
+    __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(Y + i * k + j0));
   }
 };
 
