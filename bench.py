@@ -378,8 +378,9 @@ def serve_bench(args):
     from yafel_amd import hip
     ctx = hip.Context(0)
     rates = [float(r) for r in args.rates.split(",")]
+    graph_rates = tuple(float(r) for r in args.graph_rates.split(",") if r.strip())
     rec = served_stream(hip, ctx, args.sweep_requests, rates, args.fp64_sample, 6, args.slots, args.setup_threads,
-                        full_at_max=args.requests)
+                        full_at_max=args.requests, graph_rates=graph_rates)
     nr = rec["rate_sweep"][-1]["requests"]
     out = {
         "metric": "solves/s + p99 latency at 1e-8 rel residual vs plain FP64 CG; SpMV HBM GB/s",
@@ -837,6 +838,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=300, help="default line: requests per rate")
     ap.add_argument("--serve-rates", default="50,200,2000", help="default line: arrival rates")
     ap.add_argument("--serve-fp64-sample", type=int, default=40, help="default line: FP64 one-at-a-time sample")
+    ap.add_argument("--graph-rates", default="50,200",
+                    help="serve: rates also run with chunks replayed from HIP graphs ('' for none)")
     ap.add_argument("--slots", type=int, default=128)
     ap.add_argument("--fp64-sample", type=int, default=100)
     ap.add_argument("--setup-threads", type=int, default=0, help="server setup workers (0: library default)")

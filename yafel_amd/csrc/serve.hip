@@ -16,7 +16,7 @@
 // builds each one (DoF numbering, pattern, assembly, Dirichlet, FP32 copy,
 // diagonal) on its own setup stream, with stream-ordered allocations so that
 // it never waits on the solve stream.  The caller's thread (progress) joins
-// the slots the worker finished, keeps up to two chunks of batched iterations
+// the slots the worker finished, keeps up to four chunks of batched iterations
 // in flight on the solve stream, and retires slots whose solve finished.  A
 // server is driven from one caller thread at a time.
 #include <algorithm>
@@ -270,9 +270,12 @@ struct yafel_hip_server_s {
   int64_t drained_seq = 0;  // the last chunk drained (chunks drain in launch order)
   yafel_hip_server_stats stats{};
   // descriptor / status rings: each progress call uses at most two entries and
-  // at most two calls' work is in flight, so an entry is never rewritten while
-  // a launch that reads it can still run
-  static constexpr int RING = 8;
+  // at most kDepth calls' work is in flight, so an entry is never rewritten
+  // while a launch that reads it (or a copy from its staging area) can still
+  // run, nor a status area before the host has read it
+  static constexpr int kDepth = 4;  // chunks in flight with direct submission (graph replay: 2)
+  static constexpr int RING = 16;
+  static_assert(RING > 2 * (kDepth + 1), "server: descriptor ring too short for the chunks in flight");
   static constexpr size_t STAGE = 131072;  // pinned bytes per ring entry (table / statuses)
   DBuf desc[RING], status[RING];
   int ring = 0;
@@ -841,8 +844,12 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
     for (int64_t id : S->failed_ids) done.push_back(id);
     S->failed_ids.clear();
   }
-  // 2. drain the oldest chunk when two are in flight
-  while (S->inflight.size() >= 2) drain_one(S, done);
+  // 2. drain the oldest chunk when the pipeline is full: up to kDepth chunks
+  //    in flight (two with graph replay, whose status areas and events are
+  //    per state-version parity), so that the host builds and submits the
+  //    next chunk while the device still runs the ones before it
+  const size_t depth = S->graphs ? 2 : (size_t)yafel_hip_server_s::kDepth;
+  while (S->inflight.size() >= depth) drain_one(S, done);
   // 3. retire finished slots (their x is on the host): the solve buffers are
   //    freed on the solve stream now, the matrix once its last chunk drains
   std::vector<Slot *> keep;
