@@ -491,6 +491,14 @@ int yafel_hip_server_stats_get(yafel_hip_server s, yafel_hip_server_stats *st);
  * submit every launch directly.  Results are bit-identical either way.  No
  * reference counterpart (the reference has no request server). */
 int yafel_hip_server_set_graphs(yafel_hip_server s, int on);
+/* kernels per batched CG iteration with direct submission: 3 (the default:
+ * SpMV; alpha and r, z; beta, x and p) or 2 (the SpMV also folds the update
+ * partials, takes beta and the stopping test, applies x += alpha p and forms
+ * p = z + beta p in its gather; one launch fewer, but two vectors gathered,
+ * which pays only while every active matrix is small).  Graph replay always
+ * uses 3.  The iterates are bit-identical.  No reference counterpart (the
+ * reference has no request server). */
+int yafel_hip_server_set_cg_passes(yafel_hip_server s, int passes);
 
 /* ----------------------------------------------------- synthetic inputs */
 /* Structured unit-square/cube corner mesh (cells ordered as

@@ -18,10 +18,12 @@ def materialised(i, r):
     return workloads.c4_materialise(r)
 
 
-def run_all(H, ctx, reqs, budget=8 << 30, slots=16, stagger=False, graphs=False):
+def run_all(H, ctx, reqs, budget=8 << 30, slots=16, stagger=False, graphs=False, passes=None):
     srv = H.Server(ctx, budget, slots)
     if graphs:
         srv.set_graphs(True)
+    if passes:
+        srv.set_cg_passes(passes)
     ids, out = [], {}
     for i, m in enumerate(reqs):
         ids.append(srv.submit(m["dim"], m["p"], m["dpn"], m["nodes"], m["cells"], m["ph"], load=m["load"],
@@ -71,6 +73,27 @@ def test_graph_replay_matches_direct_submission(ctx):
         assert i1.status == 0 and i1.rel_residual < 1e-8
         assert np.array_equal(x0, x1)
         assert i0.iterations == i1.iterations
+
+
+def test_two_and_three_pass_iterations_are_identical(ctx):
+    """The batched CG iteration as two passes (the SpMV forms p = z + beta p
+    in its gather and applies x += alpha p) gives the same bits as three."""
+    from yafel_amd import hip as H
+    reqs = [materialised(i, r) for i, r in enumerate(MIX)]
+    two, st2 = run_all(H, ctx, reqs, stagger=True, passes=2)
+    three, st3 = run_all(H, ctx, reqs, stagger=True, passes=3)
+    assert st2["completed"] == st3["completed"] == len(reqs) and st2["failed"] == st3["failed"] == 0
+    for (x2, i2), (x3, i3) in zip(two, three):
+        assert i2.status == 0 and i2.rel_residual < 1e-8
+        assert np.array_equal(x2, x3)
+        assert i2.iterations == i3.iterations and i2.refinements == i3.refinements
+    assert st2["batched_launches"] < st3["batched_launches"]
+    with pytest.raises(H.YafelHipError):
+        srv = H.Server(ctx, 1 << 30, 4)
+        try:
+            srv.set_cg_passes(4)
+        finally:
+            srv.close()
 
 
 def test_served_result_matches_direct_solve(ctx, orc):

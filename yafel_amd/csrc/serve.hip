@@ -51,6 +51,7 @@ struct SlotDesc {
   const double *v64;
   const float *dinv;
   float *r, *z, *p, *q, *xl;
+  float *p1;  // the second p buffer of the two-pass iteration
   double *x;
   const double *b;
   KState *st;  // two versions: st[0], st[1]
@@ -70,11 +71,11 @@ __device__ __forceinline__ int find_slot(const SlotDesc *d, int ns, int b, bool 
   return lo;
 }
 
-enum { PH_INIT = 0, PH_Q = 1, PH_U = 2, PH_P = 3, PH_P0 = 4, PH_FOLD = 5, PH_REF = 6 };
+enum { PH_INIT = 0, PH_Q = 1, PH_U = 2, PH_P = 3, PH_P0 = 4, PH_FOLD = 5, PH_REF = 6, PH_QP = 7 };
 
 template <int PH>
 constexpr bool spmv_phase() {
-  return PH == PH_INIT || PH == PH_Q || PH == PH_REF;
+  return PH == PH_INIT || PH == PH_Q || PH == PH_REF || PH == PH_QP;
 }
 
 template <class O>
@@ -133,6 +134,21 @@ __device__ __forceinline__ void slot_phase(const SlotDesc &d, int blk, int par) 
     op.p = d.p;
     op.q = d.q;
     if (!FIN) run_spmv<float>(d, d.v32, op, blk);
+  } else if (PH == PH_QP) {
+    // two-pass iteration: U's partials folded, beta and the stopping test,
+    // x += alpha p, q = A (z + beta p) with p_new written into the other p
+    OpCgQP<float> op{};
+    vbind(op, d, par, upart(d), d.g_elem, d.part);
+    op.p0 = d.p;
+    op.p1 = d.p1;
+    op.q = d.q;
+    op.xv = d.xl;
+    op.z = d.z;
+    if (!FIN) {
+      const SellView<float> A{d.sptr, d.perm, d.col, d.v32, d.brows, d.ntot, d.col16, d.keep32};
+      if (d.fmt == YAFEL_HIP_BCSR3) spmv_qp_body<YAFEL_HIP_BCSR3, kSellUnroll, float>(A, op, blk, d.g_spmv, d.n);
+      else spmv_qp_body<YAFEL_HIP_CSR, kSellUnroll, float>(A, op, blk, d.g_spmv, d.n);
+    }
   } else if (PH == PH_U) {
     OpCgU<float, true> op{};
     vbind(op, d, par, d.part, d.g_spmv, upart(d));
@@ -307,6 +323,10 @@ struct yafel_hip_server_s {
   int64_t captures = 0, replays = 0;
   char *pin = nullptr;  // RING status areas, then RING table staging areas
   int K = 8;
+  // direct submission: CG iterations as two passes (QP, U) instead of three
+  // (yafel_hip_server_set_cg_passes): faster for batches of small matrices
+  // only, so three is the default
+  bool two_pass = false;
   int64_t ver = 0;  // state version parity of the next launch (every slot's)
   std::vector<Slot *> joining;
   hipStream_t copy_stream = nullptr;  // result copies to the host
@@ -425,7 +445,7 @@ void join_slot(yafel_hip_server_s *S, Slot *sl) {
   int64_t ge = ceil_div(n, (int64_t)NT * ve_of<float>());
   if (ge >= 8) ge = (ge + 7) / 8 * 8;
   const int g_elem = (int)std::max<int64_t>(1, std::min<int64_t>(GCAP, ge));
-  const size_t nbytes = 5 * (((size_t)n * 4 + 255) & ~(size_t)255) + (((size_t)n * 8 + 255) & ~(size_t)255) +
+  const size_t nbytes = 6 * (((size_t)n * 4 + 255) & ~(size_t)255) + (((size_t)n * 8 + 255) & ~(size_t)255) +
                         ((2 * sizeof(KState) + 255) & ~(size_t)255) + (size_t)4 * std::max(g_spmv, g_elem) * 8 + 256;
   HIPCHK(hipMallocAsync(&sl->buf, nbytes, S->ctx->stream));
   sl->buf_bytes = nbytes;
@@ -448,6 +468,7 @@ void join_slot(yafel_hip_server_s *S, Slot *sl) {
   d.r = carve<float>(p, n);
   d.z = carve<float>(p, n);
   d.p = carve<float>(p, n);
+  d.p1 = carve<float>(p, n);
   d.q = carve<float>(p, n);
   d.xl = carve<float>(p, n);
   d.x = carve<double>(p, n);
@@ -590,19 +611,30 @@ void launch_phase(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int block
     kb_finish<PH><<<ns, NTF, 0, S->ctx->stream>>>(desc, ns, par);
     HIPCHK(hipGetLastError());
   }
-  if (PH == PH_U || PH == PH_P || PH == PH_P0) ++S->ver;
+  if (PH == PH_U || PH == PH_P || PH == PH_P0 || PH == PH_QP) ++S->ver;
   S->stats.batched_launches++;
 }
 
-// One chunk for the slots of table `desc`: p = z + beta p after a slot's
-// initial or refinement residual, K iterations, the refinement, and every
-// slot's status into `status`.
-void chunk_launches(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int ts, int te, SlotStatus *status) {
-  launch_phase<PH_P0>(S, desc, ns, te);
-  for (int k = 0; k < S->K; ++k) {
-    launch_phase<PH_Q>(S, desc, ns, ts);
-    launch_phase<PH_U>(S, desc, ns, te);
-    launch_phase<PH_P>(S, desc, ns, te);
+// One chunk for the slots of table `desc`: K iterations, the refinement,
+// and every slot's status into `status`.  Three passes an iteration (Q, U,
+// P, after p = z + beta p for a slot's initial or refinement residual) or,
+// with two_pass, two (QP, U: the QP pass forms that p itself); the iterates
+// are bit-identical.  Graph replay uses the three-pass chunk, which
+// advances the state version by an odd count (its graphs are per parity).
+void chunk_launches(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int ts, int te, SlotStatus *status,
+                    bool two_pass) {
+  if (two_pass) {
+    for (int k = 0; k < S->K; ++k) {
+      launch_phase<PH_QP>(S, desc, ns, ts);
+      launch_phase<PH_U>(S, desc, ns, te);
+    }
+  } else {
+    launch_phase<PH_P0>(S, desc, ns, te);
+    for (int k = 0; k < S->K; ++k) {
+      launch_phase<PH_Q>(S, desc, ns, ts);
+      launch_phase<PH_U>(S, desc, ns, te);
+      launch_phase<PH_P>(S, desc, ns, te);
+    }
   }
   launch_phase<PH_FOLD>(S, desc, ns, te);
   launch_phase<PH_REF>(S, desc, ns, ts);
@@ -648,7 +680,7 @@ void capture_graphs(yafel_hip_server_s *S) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
     try {
-      chunk_launches(S, S->gdesc.as<SlotDesc>(), ns, ts, te, S->gstatus[p].as<SlotStatus>());
+      chunk_launches(S, S->gdesc.as<SlotDesc>(), ns, ts, te, S->gstatus[p].as<SlotStatus>(), false);
       HIPCHK(hipMemcpyAsync(S->gpin + p * SS::STAGE, S->gstatus[p].p, ns * sizeof(SlotStatus), hipMemcpyDeviceToHost,
                             st));
     } catch (...) {
@@ -899,7 +931,7 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
     } else {
       int ts, te;
       const int e = upload_table(S, S->active, &ts, &te);
-      chunk_launches(S, S->desc[e].as<SlotDesc>(), ns, ts, te, S->status[e].as<SlotStatus>());
+      chunk_launches(S, S->desc[e].as<SlotDesc>(), ns, ts, te, S->status[e].as<SlotStatus>(), S->two_pass);
       f.ev = S->evs[e];
       f.ring = e;
       f.pin = S->pin + e * yafel_hip_server_s::STAGE;
@@ -1035,6 +1067,15 @@ int yafel_hip_server_stats_get(yafel_hip_server S, yafel_hip_server_stats *st) {
   *st = S->stats;
   st->graph_captures = S->captures;
   st->graph_replays = S->replays;
+  return YAFEL_HIP_OK;
+  YH_TRY_END(S ? S->ctx : nullptr)
+}
+
+int yafel_hip_server_set_cg_passes(yafel_hip_server S, int passes) {
+  YH_TRY_BEGIN
+  REQUIRE(S, "server_set_cg_passes: null server");
+  REQUIRE(passes == 2 || passes == 3, "server_set_cg_passes: passes must be 2 or 3");
+  S->two_pass = passes == 2;
   return YAFEL_HIP_OK;
   YH_TRY_END(S ? S->ctx : nullptr)
 }

@@ -135,6 +135,7 @@ for _n, _a in [
     ("yafel_hip_server_times", [_P, _I64, _P]),
     ("yafel_hip_server_stats_get", [_P, C.POINTER(ServerStats)]),
     ("yafel_hip_server_set_graphs", [_P, _I]),
+    ("yafel_hip_server_set_cg_passes", [_P, _I]),
 ]:
     _sig(_n, _I, _a)
 _sig("yafel_hip_last_error", C.c_char_p, [_P])
@@ -697,6 +698,10 @@ class Server:
 
     def cancel(self, rid):
         check(lib.yafel_hip_server_cancel(self.h, rid), self.ctx.h)
+
+    def set_cg_passes(self, passes):
+        """Kernels per batched CG iteration with direct submission: 3 (default) or 2."""
+        check(lib.yafel_hip_server_set_cg_passes(self.h, int(passes)), self.ctx.h)
 
     def set_graphs(self, on):
         """Replay each chunk from a HIP graph while the slot set stays the same."""
