@@ -247,17 +247,19 @@ def req_class(m):
     return "3d_q1_elasticity" if m["dim"] == 3 else f"2d_q{m['p']}_poisson"
 
 
-def run_stream(hip, ctx, mats, arrivals, budget, slots, setup_threads=0, graphs=False, cg_passes=0):
+def run_stream(hip, ctx, mats, arrivals, budget, slots, setup_threads=0, graphs=False, cg_passes=0,
+               two_pass_limit=0):
     """Feed requests at their arrival times into a server; returns per-request
     latency (arrival -> found finished), makespan, the results, the server
     statistics and each request's (admission wait, setup, wait to join,
     solve) times in ms.  graphs: replay chunks from per-epoch HIP graphs;
-    cg_passes: kernels per batched CG iteration (0: the library's default)."""
+    cg_passes, two_pass_limit: kernels per batched CG iteration (0, 0: the
+    library's default)."""
     srv = hip.Server(ctx, budget, slots, setup_threads)
     if graphs:
         srv.set_graphs(True)
-    if cg_passes:
-        srv.set_cg_passes(cg_passes)
+    if cg_passes or two_pass_limit:
+        srv.set_cg_passes(cg_passes, two_pass_limit)
     n = len(mats)
     t0 = time.perf_counter()
     i, done_t, rid_of, times = 0, {}, {}, {}
@@ -325,7 +327,7 @@ def fp64_one_at_a_time(hip, ctx, mats):
 
 
 def served_stream(hip, ctx, nreq, rates, fp64_sample, n_alone, slots=128, setup_threads=0, full_at_max=None,
-                  graph_rates=(50.0, 200.0), cg_passes=0):
+                  graph_rates=(50.0, 200.0), cg_passes=0, two_pass_limit=0):
     """c4: a served stream of Poisson / elasticity solves with Poisson
     arrivals at each rate; the record of every rate, the alone-vs-mix check,
     the plain FP64 GPU path on a sample of the same requests, and the same
@@ -342,11 +344,12 @@ def served_stream(hip, ctx, nreq, rates, fp64_sample, n_alone, slots=128, setup_
         nr = full_at_max if (full_at_max and rate == max(rates)) else nreq
         arr = workloads.poisson_arrivals(nr, rate, seed=int(rate))
         lat, span, res, st, times = run_stream(hip, ctx, mats[:nr], arr, budget, slots, setup_threads,
-                                               cg_passes=cg_passes)
+                                               cg_passes=cg_passes, two_pass_limit=two_pass_limit)
         bad = sum(1 for x, inf in res.values() if inf.status != 0 or inf.rel_residual > TOL)
         rec = {"rate": rate, "requests": nr, "solves_per_s": nr / span, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                "p99_ms": float(np.percentile(lat, 99) * 1e3), "failed": bad, "hbm_peak_gb": st["hbm_peak"] / 2**30,
-               "batched_launches": st["batched_launches"], "by_class": latency_breakdown(mats[:nr], lat, times)}
+               "batched_launches": st["batched_launches"], "two_pass_chunks": st["two_pass_chunks"],
+               "three_pass_chunks": st["three_pass_chunks"], "by_class": latency_breakdown(mats[:nr], lat, times)}
         log(f"rate {rate}: { {k: v for k, v in rec.items() if k != 'by_class'} }")
         sweep.append(rec)
         if rate == max(rates):
@@ -367,7 +370,7 @@ def served_stream(hip, ctx, nreq, rates, fp64_sample, n_alone, slots=128, setup_
     diffs = []
     for j in range(min(n_alone, rec["requests"])):
         _, _, r1, _, _ = run_stream(hip, ctx, [mats[j]], np.zeros(1), budget, slots, setup_threads,
-                                    cg_passes=cg_passes)
+                                    cg_passes=cg_passes, two_pass_limit=two_pass_limit)
         diffs.append(float(np.abs(r1[0][0] - res[j][0]).max()))
     fp64_rate = fp64_one_at_a_time(hip, ctx, mats[:min(fp64_sample, rec["requests"])])
     return {"solves_per_s": rec["solves_per_s"], "p50_ms": rec["p50_ms"], "p99_ms": rec["p99_ms"],
@@ -385,7 +388,8 @@ def serve_bench(args):
     rates = [float(r) for r in args.rates.split(",")]
     graph_rates = tuple(float(r) for r in args.graph_rates.split(",") if r.strip())
     rec = served_stream(hip, ctx, args.sweep_requests, rates, args.fp64_sample, 6, args.slots, args.setup_threads,
-                        full_at_max=args.requests, graph_rates=graph_rates, cg_passes=args.server_cg_passes)
+                        full_at_max=args.requests, graph_rates=graph_rates, cg_passes=args.server_cg_passes,
+                        two_pass_limit=args.server_two_pass_limit)
     nr = rec["rate_sweep"][-1]["requests"]
     out = {
         "metric": "solves/s + p99 latency at 1e-8 rel residual vs plain FP64 CG; SpMV HBM GB/s",
@@ -846,7 +850,9 @@ def main():
     ap.add_argument("--graph-rates", default="50,200",
                     help="serve: rates also run with chunks replayed from HIP graphs ('' for none)")
     ap.add_argument("--server-cg-passes", type=int, default=0,
-                    help="serve: kernels per batched CG iteration, 2 or 3 (0: the library's default)")
+                    help="serve: kernels per batched CG iteration, 2 or 3 (0: two while the batch is small)")
+    ap.add_argument("--server-two-pass-limit", type=int, default=0,
+                    help="serve: stored scalars of the active slots up to which a chunk takes two passes (0: default)")
     ap.add_argument("--slots", type=int, default=128)
     ap.add_argument("--fp64-sample", type=int, default=100)
     ap.add_argument("--setup-threads", type=int, default=0, help="server setup workers (0: library default)")

@@ -451,6 +451,8 @@ typedef struct {
   double setup_ms;              /* host time the setup worker spent building requests */
   int64_t graph_captures;       /* slot-set epochs captured into graphs (graph replay on) */
   int64_t graph_replays;        /* chunks replayed from a graph */
+  int64_t two_pass_chunks;      /* directly submitted chunks of two-pass CG iterations */
+  int64_t three_pass_chunks;    /* ... and of three-pass iterations */
 } yafel_hip_server_stats;
 
 int yafel_hip_server_create(yafel_hip_ctx ctx, size_t hbm_budget, int max_slots, yafel_hip_server *out);
@@ -491,14 +493,16 @@ int yafel_hip_server_stats_get(yafel_hip_server s, yafel_hip_server_stats *st);
  * submit every launch directly.  Results are bit-identical either way.  No
  * reference counterpart (the reference has no request server). */
 int yafel_hip_server_set_graphs(yafel_hip_server s, int on);
-/* kernels per batched CG iteration with direct submission: 3 (the default:
- * SpMV; alpha and r, z; beta, x and p) or 2 (the SpMV also folds the update
- * partials, takes beta and the stopping test, applies x += alpha p and forms
- * p = z + beta p in its gather; one launch fewer, but two vectors gathered,
- * which pays only while every active matrix is small).  Graph replay always
- * uses 3.  The iterates are bit-identical.  No reference counterpart (the
- * reference has no request server). */
-int yafel_hip_server_set_cg_passes(yafel_hip_server s, int passes);
+/* kernels per batched CG iteration with direct submission: 3 (SpMV; alpha
+ * and r, z; beta, x and p), 2 (after a chunk's first iteration the SpMV also
+ * folds the update partials, takes beta and the stopping test, applies
+ * x += alpha p and forms p = z + beta p in its gather: a launch fewer, but
+ * two vectors gathered) or 0 (the default: a chunk takes two passes while its
+ * active slots store at most two_pass_limit scalars in all, three otherwise;
+ * two_pass_limit <= 0 keeps the default of 4 Mi).  Graph replay uses 3.  The
+ * iterates are bit-identical whatever the choice.  No reference counterpart
+ * (the reference has no request server). */
+int yafel_hip_server_set_cg_passes(yafel_hip_server s, int passes, int64_t two_pass_limit);
 
 /* ----------------------------------------------------- synthetic inputs */
 /* Structured unit-square/cube corner mesh (cells ordered as

@@ -18,12 +18,12 @@ def materialised(i, r):
     return workloads.c4_materialise(r)
 
 
-def run_all(H, ctx, reqs, budget=8 << 30, slots=16, stagger=False, graphs=False, passes=None):
+def run_all(H, ctx, reqs, budget=8 << 30, slots=16, stagger=False, graphs=False, passes=None, limit=0):
     srv = H.Server(ctx, budget, slots)
     if graphs:
         srv.set_graphs(True)
-    if passes:
-        srv.set_cg_passes(passes)
+    if passes is not None:
+        srv.set_cg_passes(passes, limit)
     ids, out = [], {}
     for i, m in enumerate(reqs):
         ids.append(srv.submit(m["dim"], m["p"], m["dpn"], m["nodes"], m["cells"], m["ph"], load=m["load"],
@@ -77,23 +77,32 @@ def test_graph_replay_matches_direct_submission(ctx):
 
 def test_two_and_three_pass_iterations_are_identical(ctx):
     """The batched CG iteration as two passes (the SpMV forms p = z + beta p
-    in its gather and applies x += alpha p) gives the same bits as three."""
+    in its gather and applies x += alpha p; a closing pass per chunk) gives
+    the same bits as three, also when the form changes from chunk to chunk
+    (a limit of 100 k stored scalars: the 3-D requests switch it as they join
+    and retire)."""
     from yafel_amd import hip as H
     reqs = [materialised(i, r) for i, r in enumerate(MIX)]
-    two, st2 = run_all(H, ctx, reqs, stagger=True, passes=2)
     three, st3 = run_all(H, ctx, reqs, stagger=True, passes=3)
-    assert st2["completed"] == st3["completed"] == len(reqs) and st2["failed"] == st3["failed"] == 0
-    for (x2, i2), (x3, i3) in zip(two, three):
-        assert i2.status == 0 and i2.rel_residual < 1e-8
-        assert np.array_equal(x2, x3)
-        assert i2.iterations == i3.iterations and i2.refinements == i3.refinements
-    assert st2["batched_launches"] < st3["batched_launches"]
-    with pytest.raises(H.YafelHipError):
-        srv = H.Server(ctx, 1 << 30, 4)
-        try:
+    assert st3["completed"] == len(reqs) and st3["failed"] == 0 and st3["two_pass_chunks"] == 0
+    for passes, limit in ((2, 0), (0, 100_000)):
+        other, st = run_all(H, ctx, reqs, stagger=True, passes=passes, limit=limit)
+        assert st["completed"] == len(reqs) and st["failed"] == 0
+        for (x2, i2), (x3, i3) in zip(other, three):
+            assert i2.status == 0 and i2.rel_residual < 1e-8
+            assert np.array_equal(x2, x3)
+            assert i2.iterations == i3.iterations and i2.refinements == i3.refinements
+        assert st["batched_launches"] < st3["batched_launches"]
+        if passes == 2:
+            assert st["three_pass_chunks"] == 0
+        else:
+            assert st["two_pass_chunks"] > 0 and st["three_pass_chunks"] > 0
+    srv = H.Server(ctx, 1 << 30, 4)
+    try:
+        with pytest.raises(H.YafelHipError):
             srv.set_cg_passes(4)
-        finally:
-            srv.close()
+    finally:
+        srv.close()
 
 
 def test_served_result_matches_direct_solve(ctx, orc):

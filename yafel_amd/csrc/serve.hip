@@ -71,7 +71,7 @@ __device__ __forceinline__ int find_slot(const SlotDesc *d, int ns, int b, bool 
   return lo;
 }
 
-enum { PH_INIT = 0, PH_Q = 1, PH_U = 2, PH_P = 3, PH_P0 = 4, PH_FOLD = 5, PH_REF = 6, PH_QP = 7 };
+enum { PH_INIT = 0, PH_Q = 1, PH_U = 2, PH_P = 3, PH_P0 = 4, PH_FOLD = 5, PH_REF = 6, PH_QP = 7, PH_PN = 8 };
 
 template <int PH>
 constexpr bool spmv_phase() {
@@ -107,6 +107,57 @@ __device__ __forceinline__ void run_spmv(const SlotDesc &d, const T *val, O &op,
   const SellView<T> A{d.sptr, d.perm, d.col, val, d.brows, d.ntot, d.col16, sizeof(T) == 4 ? d.keep32 : 0};
   if (d.fmt == YAFEL_HIP_BCSR3) spmv_fused_body<YAFEL_HIP_BCSR3, kSellUnroll, T, O>(A, op, blk, d.g_spmv);
   else spmv_fused_body<YAFEL_HIP_CSR, kSellUnroll, T, O>(A, op, blk, d.g_spmv);
+}
+
+// The closing pass of a two-pass chunk (versioned: state par in, par ^ 1
+// out): the three-pass P of the chunk's last iteration -- U's partials
+// folded, the stopping test and beta, x += alpha p and p = z + beta p -- with
+// the p that the QP passes left in either buffer read and the new p written
+// into the first, and pidx reset, so that the chunk ends in the state a
+// three-pass chunk leaves (p in d.p).  A slot that stopped or paused in a QP
+// pass (its U skipped) only has its p moved.  Its arithmetic is OpCgP's.
+// Unlike velem_body it reads the state before the vectors: which buffer holds
+// p is known only from the state.
+__device__ __forceinline__ void cg_close_body(const SlotDesc &d, int blk, int nblk, int par) {
+  __shared__ KState ss;
+  __shared__ int sfl;
+  constexpr int kX = 1, kP = 2, kCopy = 4;
+  const int t = threadIdx.x;
+  double f[2];
+  refold_load<2>(upart(d), d.g_elem, f);
+  const KState *in = d.st + par;
+  KState *out = d.st + (par ^ 1);
+  if (t < KSW) reinterpret_cast<long long *>(&ss)[t] = reinterpret_cast<const long long *>(in)[t];
+  __syncthreads();
+  const bool run = VOpBase::running(ss);
+  if (run) refold_reduce<2>(f, f);
+  __syncthreads();  // every wave has read ss before thread 0 changes it
+  if (t == 0) {
+    int fl = 0;
+    const int pi = ss.pidx;
+    if (run) {
+      after_iteration(&ss, f[0], f[1]);
+      fl |= kX;
+      if (VOpBase::running(ss)) fl |= kP;
+    }
+    if (!(fl & kP) && pi) fl |= kCopy;
+    ss.pidx = 0;
+    sfl = fl | (pi << 4);
+  }
+  __syncthreads();
+  if (blk == 0 && t < KSW) reinterpret_cast<long long *>(out)[t] = reinterpret_cast<const long long *>(&ss)[t];
+  const int fl = sfl;
+  if (!(fl & (kX | kCopy))) return;
+  const float *pold = (fl >> 4) ? d.p1 : d.p;
+  const float alpha = (float)ss.alpha, beta = (float)ss.beta;
+  const int64_t n = d.n, chunk = (n + nblk - 1) / nblk;
+  const int64_t i1 = (int64_t)blk * chunk + chunk < n ? (int64_t)blk * chunk + chunk : n;
+  for (int64_t i = (int64_t)blk * chunk + t; i < i1; i += NT) {
+    const float po = pold[i];
+    if (fl & kX) d.xl[i] = fma(alpha, po, d.xl[i]);
+    if (fl & kP) d.p[i] = fma(beta, po, d.z[i]);
+    else if (fl & kCopy) d.p[i] = po;
+  }
 }
 
 // Builds the phase's operation for slot d and either runs the phase body on
@@ -149,6 +200,8 @@ __device__ __forceinline__ void slot_phase(const SlotDesc &d, int blk, int par) 
       if (d.fmt == YAFEL_HIP_BCSR3) spmv_qp_body<YAFEL_HIP_BCSR3, kSellUnroll, float>(A, op, blk, d.g_spmv, d.n);
       else spmv_qp_body<YAFEL_HIP_CSR, kSellUnroll, float>(A, op, blk, d.g_spmv, d.n);
     }
+  } else if (PH == PH_PN) {
+    if (!FIN) cg_close_body(d, blk, nblk, par);
   } else if (PH == PH_U) {
     OpCgU<float, true> op{};
     vbind(op, d, par, d.part, d.g_spmv, upart(d));
@@ -321,12 +374,17 @@ struct yafel_hip_server_s {
   char *gpin = nullptr;        // two status areas
   hipEvent_t gev[2] = {nullptr, nullptr};
   int64_t captures = 0, replays = 0;
+  int64_t stats_two = 0, stats_three = 0;  // directly submitted chunks of each form
   char *pin = nullptr;  // RING status areas, then RING table staging areas
   int K = 8;
-  // direct submission: CG iterations as two passes (QP, U) instead of three
-  // (yafel_hip_server_set_cg_passes): faster for batches of small matrices
-  // only, so three is the default
-  bool two_pass = false;
+  // direct submission: CG chunks as two passes an iteration (QP, U) or three
+  // (yafel_hip_server_set_cg_passes).  Two cost a launch less but gather two
+  // vectors, which pays while the batch is small, so by default (passes 0) a
+  // chunk takes two passes while its active slots store at most
+  // two_pass_limit scalars in all.
+  int passes = 0;
+  int64_t two_pass_limit = kTwoPassLimit;
+  static constexpr int64_t kTwoPassLimit = (int64_t)1 << 22;
   int64_t ver = 0;  // state version parity of the next launch (every slot's)
   std::vector<Slot *> joining;
   hipStream_t copy_stream = nullptr;  // result copies to the host
@@ -611,23 +669,30 @@ void launch_phase(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int block
     kb_finish<PH><<<ns, NTF, 0, S->ctx->stream>>>(desc, ns, par);
     HIPCHK(hipGetLastError());
   }
-  if (PH == PH_U || PH == PH_P || PH == PH_P0 || PH == PH_QP) ++S->ver;
+  if (PH == PH_U || PH == PH_P || PH == PH_P0 || PH == PH_QP || PH == PH_PN) ++S->ver;
   S->stats.batched_launches++;
 }
 
-// One chunk for the slots of table `desc`: K iterations, the refinement,
-// and every slot's status into `status`.  Three passes an iteration (Q, U,
-// P, after p = z + beta p for a slot's initial or refinement residual) or,
-// with two_pass, two (QP, U: the QP pass forms that p itself); the iterates
-// are bit-identical.  Graph replay uses the three-pass chunk, which
-// advances the state version by an odd count (its graphs are per parity).
+// One chunk for the slots of table `desc`: p = z + beta p after a slot's
+// initial or refinement residual, K iterations, the refinement, and every
+// slot's status into `status`.  Three passes an iteration (Q, U, P) or, with
+// two_pass, two after the first (Q, U, then QP, U: the QP pass applies the
+// previous iteration's P in its gather) and a closing pass (PN) that applies
+// the last one's P and moves p back into d.p.  Both forms leave the same
+// state, advance the state version by an odd count (graph replay keeps one
+// graph per parity) and give bit-identical iterates, so the form can change
+// from chunk to chunk.
 void chunk_launches(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int ts, int te, SlotStatus *status,
                     bool two_pass) {
   if (two_pass) {
-    for (int k = 0; k < S->K; ++k) {
+    launch_phase<PH_P0>(S, desc, ns, te);
+    launch_phase<PH_Q>(S, desc, ns, ts);
+    launch_phase<PH_U>(S, desc, ns, te);
+    for (int k = 1; k < S->K; ++k) {
       launch_phase<PH_QP>(S, desc, ns, ts);
       launch_phase<PH_U>(S, desc, ns, te);
     }
+    launch_phase<PH_PN>(S, desc, ns, te);
   } else {
     launch_phase<PH_P0>(S, desc, ns, te);
     for (int k = 0; k < S->K; ++k) {
@@ -931,7 +996,14 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
     } else {
       int ts, te;
       const int e = upload_table(S, S->active, &ts, &te);
-      chunk_launches(S, S->desc[e].as<SlotDesc>(), ns, ts, te, S->status[e].as<SlotStatus>(), S->two_pass);
+      bool two = S->passes == 2;
+      if (S->passes == 0) {
+        int64_t scalars = 0;
+        for (Slot *sl : S->active) scalars += sl->A->ntot * (sl->A->format == YAFEL_HIP_BCSR3 ? 9 : 1);
+        two = scalars <= S->two_pass_limit;
+      }
+      chunk_launches(S, S->desc[e].as<SlotDesc>(), ns, ts, te, S->status[e].as<SlotStatus>(), two);
+      (two ? S->stats_two : S->stats_three)++;
       f.ev = S->evs[e];
       f.ring = e;
       f.pin = S->pin + e * yafel_hip_server_s::STAGE;
@@ -1067,15 +1139,18 @@ int yafel_hip_server_stats_get(yafel_hip_server S, yafel_hip_server_stats *st) {
   *st = S->stats;
   st->graph_captures = S->captures;
   st->graph_replays = S->replays;
+  st->two_pass_chunks = S->stats_two;
+  st->three_pass_chunks = S->stats_three;
   return YAFEL_HIP_OK;
   YH_TRY_END(S ? S->ctx : nullptr)
 }
 
-int yafel_hip_server_set_cg_passes(yafel_hip_server S, int passes) {
+int yafel_hip_server_set_cg_passes(yafel_hip_server S, int passes, int64_t two_pass_limit) {
   YH_TRY_BEGIN
   REQUIRE(S, "server_set_cg_passes: null server");
-  REQUIRE(passes == 2 || passes == 3, "server_set_cg_passes: passes must be 2 or 3");
-  S->two_pass = passes == 2;
+  REQUIRE(passes == 0 || passes == 2 || passes == 3, "server_set_cg_passes: passes must be 0, 2 or 3");
+  S->passes = passes;
+  S->two_pass_limit = two_pass_limit > 0 ? two_pass_limit : yafel_hip_server_s::kTwoPassLimit;
   return YAFEL_HIP_OK;
   YH_TRY_END(S ? S->ctx : nullptr)
 }

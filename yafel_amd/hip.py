@@ -66,7 +66,7 @@ class ServerStats(C.Structure):
                 ("cancelled", _I64), ("queued", _I64), ("active", _I64), ("chunks", _I64),
                 ("batched_launches", _I64), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t),
                 ("hbm_budget", C.c_size_t), ("setup_ms", C.c_double), ("graph_captures", _I64),
-                ("graph_replays", _I64)]
+                ("graph_replays", _I64), ("two_pass_chunks", _I64), ("three_pass_chunks", _I64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -135,7 +135,7 @@ for _n, _a in [
     ("yafel_hip_server_times", [_P, _I64, _P]),
     ("yafel_hip_server_stats_get", [_P, C.POINTER(ServerStats)]),
     ("yafel_hip_server_set_graphs", [_P, _I]),
-    ("yafel_hip_server_set_cg_passes", [_P, _I]),
+    ("yafel_hip_server_set_cg_passes", [_P, _I, _I64]),
 ]:
     _sig(_n, _I, _a)
 _sig("yafel_hip_last_error", C.c_char_p, [_P])
@@ -699,9 +699,10 @@ class Server:
     def cancel(self, rid):
         check(lib.yafel_hip_server_cancel(self.h, rid), self.ctx.h)
 
-    def set_cg_passes(self, passes):
-        """Kernels per batched CG iteration with direct submission: 3 (default) or 2."""
-        check(lib.yafel_hip_server_set_cg_passes(self.h, int(passes)), self.ctx.h)
+    def set_cg_passes(self, passes, two_pass_limit=0):
+        """Kernels per batched CG iteration with direct submission: 2, 3, or 0 (the
+        default: two while the active slots store at most two_pass_limit scalars)."""
+        check(lib.yafel_hip_server_set_cg_passes(self.h, int(passes), int(two_pass_limit)), self.ctx.h)
 
     def set_graphs(self, on):
         """Replay each chunk from a HIP graph while the slot set stays the same."""
