@@ -161,6 +161,37 @@ def test_spmm_sorted_rows_matches_columnwise_spmv(H, ctx, k):
         assert np.abs(Y32[:, j] - yj).max() <= 1e-5 * np.abs(yj).max()
 
 
+@pytest.mark.parametrize("pattern", ["q2", "scattered"])
+@pytest.mark.parametrize("k", [1, 5, 32, 33, 64, 70])
+def test_spmm_fp64_staged_is_the_spmv_bit_for_bit(H, ctx, pattern, k):
+    """The FP64 product stages rows of X in LDS per row group and sums each
+    row with one FMA per entry in column order, as the FP64 SpMV does, so
+    every column of Y equals the SpMV of that column of X bit for bit: on a
+    Q2 pattern (every group staged) and on a random pattern whose groups
+    touch more distinct columns than the LDS holds (those read X directly)."""
+    rng = np.random.default_rng(7 + k)
+    if pattern == "q2":
+        m = H.Mesh.structured(ctx, 2, 2, 1, 20)
+        A = m.pattern(H.CSR, H.FP64)
+        m.assemble(H.physics(H.POISSON, 2, coef=np.linspace(0.5, 2.0, 400)), A, None, H.TANGENT)
+    else:
+        n = 3000
+        rows = np.repeat(np.arange(n), 10)
+        cols = rng.integers(0, n, rows.size)
+        cols[: 10 * 640] = (rows[: 10 * 640] + rng.integers(-20, 21, 10 * 640)) % n
+        A = H.Matrix.from_triplets(ctx, rows, cols, rng.standard_normal(rows.size))
+    n = A.info().rows
+    X = rng.standard_normal((n, k))
+    Yd = ctx.alloc(n * k)
+    A.spmm(ctx.upload(X.ravel()), Yd, k)
+    Y = Yd.download().reshape(n, k)
+    xd, yd = ctx.alloc(n), ctx.alloc(n)
+    for j in range(k):
+        xd.upload(np.ascontiguousarray(X[:, j]))
+        A.spmv(xd, yd)
+        assert np.array_equal(Y[:, j], yd.download()), j
+
+
 def test_dot_matches_oracle(H, ctx, orc):
     rng = np.random.default_rng(9)
     a, b = rng.standard_normal(100003), rng.standard_normal(100003)

@@ -841,6 +841,7 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
       HIPCHK(hipGetLastError());
     }
     A->has32 = false;
+    A->v64_gen++;
     A->diag_ok = false;
   }
   if (wr) {
@@ -888,6 +889,7 @@ int yafel_hip_dirichlet_apply(yafel_hip_matrix A, const uint8_t *mask_dev, const
                                                            A->val64.as<double>(), mask_dev, gv, rhs_dev);
   HIPCHK(hipGetLastError());
   A->has32 = false;
+  A->v64_gen++;
   A->diag_ok = false;
   if (g_dev) HIPCHK(hipStreamSynchronize(s));
   return YAFEL_HIP_OK;

@@ -188,6 +188,10 @@ struct yafel_hip_matrix_s {
   yh::DBuf mx_val32s;           // staged values times their column's D^-1
   int64_t mx_sgen = -1, mx_dgen = -1;  // the val32 and diagonal generations mx_val32s holds
   int64_t diag_gen = 0;         // bumped whenever the diagonal is rebuilt
+  // the FP64 staged products' values (CSR order, as mx_val32 / mx_val32s)
+  yh::DBuf mx_val64, mx_val64s;
+  int64_t v64_gen = 0;          // bumped whenever val64 changes
+  int64_t mx_gen64 = -1, mx_sgen64 = -1, mx_dgen64 = -1;  // generations mx_val64 / mx_val64s hold
   yh::DBuf val64, val32;        // CSR: ntot; BCSR3: 9 arrays of ntot (v[e * ntot + k], e = 3i+j)
   yh::DBuf diag64, dinv64, dinv32;
   bool has64 = false, has32 = false, diag_ok = false;
@@ -236,6 +240,7 @@ void spmm_launch(yafel_hip_matrix_s *A, int precision, int k, const void *X, voi
 // FP32 CSR Y = A X with the slices' X rows staged in LDS (k <= 64); false when
 // the matrix does not qualify and the caller must use another kernel.
 bool spmm_lds_launch(yafel_hip_matrix_s *A, int k, const float *X, float *Y, hipStream_t s);
+bool spmm_lds64_launch(yafel_hip_matrix_s *A, int k, const double *X, double *Y, hipStream_t s);
 void matrix_build_fp32(yafel_hip_matrix_s *A);
 void matrix_build_diag(yafel_hip_matrix_s *A);
 // Converts CSR-ordered device arrays (int32 ptr[brows+1], col[nblk], FP64

@@ -659,6 +659,167 @@ void k_mspmm_pipe(StagedCsr A, Op op0) {
   if constexpr (NP > 0) finish_staged<NP>(op, a0, a1);
 }
 
+// FP64 products with rows of X staged in LDS: the row groups, lists and
+// padded CSR-order entries of the FP32 staged products, with FP64 values (the
+// CSR-order copy of the FP64 matrix, or of A D^-1 for ops whose product
+// scales the columns).  A pass stages 32 right-hand sides of the group's
+// distinct columns, 256 bytes per row, through the op's per-right-hand-side
+// gather (op.x, the same transform the sliced FP64 product applies to a
+// gathered value).  Lane (rho = lane >> 4, q = lane & 15) takes right-hand
+// sides 2q and 2q + 1 of the pass for position rho of a set of four
+// consecutive positions, so the 16 lanes of a position read one whole staged
+// row per entry: each 16-byte LDS read is free of bank conflicts.  Wave w
+// takes sets w, 7 - w, 8 + w and 15 - w (positions are longest first).  A row
+// is summed entry by entry in column order with one FMA per entry, as the
+// sliced FP64 product sums it, so the products are the same bit for bit; the
+// row epilogue is the op's per-right-hand-side row().
+constexpr int KC64 = 32;         // right-hand sides per pass
+constexpr int kStageRows64 = 192;   // staged rows: 48 KiB
+constexpr int kStageEnt64 = 1536;   // staged entries: 15 KiB (values and positions)
+
+struct StagedCsr64 {
+  const int *ptr;
+  const double *val;
+  const unsigned short *lidx;
+  const int *dptr, *dcol;
+  const int *grow;
+  int64_t ngr;
+  int dcap, ecap;  // rows of X and entries the launch's LDS holds
+};
+
+__device__ __forceinline__ int staged_set64(int w, int m) {
+  return m == 0 ? w : m == 1 ? 7 - w : m == 2 ? 8 + w : 15 - w;
+}
+
+// Per-RHS partials: lane (rho, q) holds right-hand sides 32h + 2q + c in
+// acc[h][c]; the four positions of a wave are folded by shuffles (fixed
+// order), then the waves through LDS (the staged rows' space, free by now).
+template <int NP, class Op>
+__device__ __forceinline__ void finish_staged64(Op &op, double (&acc)[2][2][NP], double *lds) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane & 15;
+  const int blk = dev::xcd_block();
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NP; ++j)
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        double s = acc[h][c][j];
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        if (lane < 16) lds[(j * (NT / 64) + w) * 64 + 32 * h + 2 * q + c] = s;
+      }
+  __syncthreads();
+  if (w == 0) {
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      double s = 0;
+#pragma unroll
+      for (int ww = 0; ww < NT / 64; ++ww) s += lds[(j * (NT / 64) + ww) * 64 + lane];
+      op.part[((size_t)j * gridDim.x + blk) * 64 + lane] = s;
+    }
+  }
+}
+
+template <class Op>
+__global__ __launch_bounds__(NT) void k_mspmm_lds64(StagedCsr64 A, Op op0) {
+  // A.dcap rows of KC64 / 2 double2, then A.ecap values, then A.ecap positions
+  extern __shared__ double2 xs2[];
+  __shared__ int ep[65];
+  double *ev = reinterpret_cast<double *>(xs2 + (size_t)A.dcap * (KC64 / 2));
+  unsigned short *el = reinterpret_cast<unsigned short *>(ev + A.ecap);
+  constexpr int NP = Op::NP, NPA = NP > 0 ? NP : 1;
+  Op op = op0;
+  if (op.skip()) return;
+  op.begin();
+  double acc[2][2][NPA];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int j = 0; j < NPA; ++j) acc[h][c][j] = 0;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 15, rho = lane >> 4;
+  const int k = op.k;
+  const int nh = (k + KC64 - 1) / KC64;
+  const GroupWalk gw = group_walk(A.ngr);
+  for (int64_t g = gw.g; g < gw.end; g += gw.step) {
+    const int d0 = A.dptr[g], d = A.dptr[g + 1] - d0;
+    const int64_t r0 = g * 64;
+    const int e0 = A.ptr[r0], ne = A.ptr[r0 + 64] - e0;
+    const bool staged = d <= A.dcap && ne <= A.ecap;
+    __syncthreads();  // every wave is done with the previous group's LDS
+    if (tid < 65) ep[tid] = A.ptr[r0 + tid] - e0;
+    if (staged)
+      for (int i = tid; i < ne; i += NT) {
+        ev[i] = __builtin_nontemporal_load(&A.val[e0 + i]);
+        el[i] = __builtin_nontemporal_load(&A.lidx[e0 + i]);
+      }
+#pragma unroll 1
+    for (int h = 0; h < nh; ++h) {
+      const int jb = h * KC64 + 2 * q;  // this lane's first right-hand side of the pass
+      Op oc[2] = {op, op};
+#pragma unroll
+      for (int c = 0; c < 2; ++c) oc[c].bind1(jb + c < k ? jb + c : 0);
+      if (staged) {
+        if (h > 0) __syncthreads();  // every wave is done with the previous pass's rows
+        // thread (tid >> 4, q) stages pair q of rows tid >> 4, + 16, ...
+#pragma unroll 4
+        for (int i = tid >> 4; i < d; i += NT / 16) {
+          const int c = A.dcol[d0 + i];
+          xs2[i * (KC64 / 2) + q] = make_double2(jb < k ? oc[0].x(c, jb) : 0.0, jb + 1 < k ? oc[1].x(c, jb + 1) : 0.0);
+        }
+      }
+      __syncthreads();  // the group's entries (first pass) and this pass's rows are in LDS
+#pragma unroll 1
+      for (int m = 0; m < 4; ++m) {
+        const int p = staged_set64(w, m) * 4 + rho;
+        const int64_t r = A.grow[r0 + p];
+        const int b = ep[p], n4 = r >= 0 ? (ep[p + 1] - b) >> 2 : 0;
+        double y0 = 0, y1 = 0;
+        if (staged) {
+          const double2 *ev2 = reinterpret_cast<const double2 *>(ev + b);
+          const uint2 *el4 = reinterpret_cast<const uint2 *>(el + b);
+          for (int t = 0; t < n4; ++t) {
+            const double2 va = ev2[2 * t], vb = ev2[2 * t + 1];
+            const uint2 l4 = el4[t];
+            const double vv[4] = {va.x, va.y, vb.x, vb.y};
+            const int li[4] = {(int)(l4.x & 0xffffu), (int)(l4.x >> 16), (int)(l4.y & 0xffffu), (int)(l4.y >> 16)};
+            double2 xq[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) xq[e] = xs2[li[e] * (KC64 / 2) + q];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              y0 = fma(vv[e], xq[e].x, y0);
+              y1 = fma(vv[e], xq[e].y, y1);
+            }
+          }
+        } else {
+          // a group too large to stage: its entries and X read directly
+          for (int t = 0; t < 4 * n4; ++t) {
+            const int64_t e = e0 + b + t;
+            const double v = A.val[e];
+            const int c = A.dcol[d0 + A.lidx[e]];
+            y0 = fma(v, jb < k ? oc[0].x(c, jb) : 0.0, y0);
+            y1 = fma(v, jb + 1 < k ? oc[1].x(c, jb + 1) : 0.0, y1);
+          }
+        }
+        // constant indexes keep the sums in registers (h > 1, k > 64: plain
+        // products only, which keep none)
+        if (r >= 0 && h == 0) {
+          if (jb < k) oc[0].row(r, jb, y0, acc[0][0]);
+          if (jb + 1 < k) oc[1].row(r, jb + 1, y1, acc[0][1]);
+        } else if (r >= 0) {
+          if (jb < k) oc[0].row(r, jb, y0, acc[1][0]);
+          if (jb + 1 < k) oc[1].row(r, jb + 1, y1, acc[1][1]);
+        }
+      }
+    }
+  }
+  if constexpr (NP > 0) finish_staged64<NP>(op, acc, reinterpret_cast<double *>(xs2));
+}
+
 template <class Op>
 __global__ __launch_bounds__(NT) void k_melem(int64_t n, Op op0) {
   Op op = op0;
@@ -1430,23 +1591,42 @@ struct MPlain {
   }
 };
 
+// Y = A X in FP64 for the plain product through the staged kernel.
+struct MPlain64 {
+  static constexpr int NP = 0;
+  static constexpr bool kScaledVals = false;
+  const double *X;
+  double *Y;
+  int k;
+  double *part;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() {}
+  __device__ void bind1(int) {}
+  __device__ double x(int c, int j) const { return X[(int64_t)c * k + j]; }
+  __device__ void row(int64_t i, int j, double y, double *) const {
+    __builtin_nontemporal_store(y, Y + i * k + j);  // written once, as MBase::store4nt
+  }
+};
+
 // The CSR-order copy of the staged products (pattern part, once per
 // pattern): each entry's sliced position, the distinct columns of each group
 // of 64 consecutive rows and each entry's position in its group's list.
 // mx_dmax = 0 when a list is too long for 16-bit positions (never staged).
-__global__ void k_take(int64_t n, const int *__restrict__ src, const float *__restrict__ v, float *__restrict__ out) {
+template <class T>
+__global__ void k_take(int64_t n, const int *__restrict__ src, const T *__restrict__ v, T *__restrict__ out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) out[i] = src[i] >= 0 ? v[src[i]] : 0.f;
+  if (i < n) out[i] = src[i] >= 0 ? v[src[i]] : T(0);
 }
 
 // the same, each value times its column's D^-1 (the product the FP32
 // sliced kernels form as they load a value and its column's scale)
+template <class T>
 __global__ void k_take_scaled(int64_t n, const int *__restrict__ src, const int *__restrict__ col,
-                              const float *__restrict__ v, const float *__restrict__ dinv, float *__restrict__ out) {
+                              const T *__restrict__ v, const T *__restrict__ dinv, T *__restrict__ out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) {
     const int e = src[i];
-    out[i] = e >= 0 ? v[e] * dinv[col[e]] : 0.f;
+    out[i] = e >= 0 ? v[e] * dinv[col[e]] : T(0);
   }
 }
 
@@ -1585,7 +1765,7 @@ void matrix_refresh_staged(yafel_hip_matrix_s *A) {
   const int64_t nnz = (int64_t)(A->mx_src.bytes / 4);
   if (A->mx_val32.bytes < (size_t)std::max<int64_t>(nnz, 1) * 4) A->mx_val32.alloc(std::max<int64_t>(nnz, 1) * 4);
   if (nnz) {
-    k_take<<<ceil_div(nnz, 256), 256, 0, A->ctx->stream>>>(nnz, A->mx_src.as<int>(), A->val32.as<float>(),
+    k_take<float><<<ceil_div(nnz, 256), 256, 0, A->ctx->stream>>>(nnz, A->mx_src.as<int>(), A->val32.as<float>(),
                                                           A->mx_val32.as<float>());
     HIPCHK(hipGetLastError());
   }
@@ -1600,7 +1780,7 @@ const float *matrix_staged_scaled(yafel_hip_matrix_s *A) {
   const int64_t nnz = (int64_t)(A->mx_src.bytes / 4);
   if (A->mx_val32s.bytes < (size_t)std::max<int64_t>(nnz, 1) * 4) A->mx_val32s.alloc(std::max<int64_t>(nnz, 1) * 4);
   if (nnz) {
-    k_take_scaled<<<ceil_div(nnz, 256), 256, 0, A->ctx->stream>>>(nnz, A->mx_src.as<int>(), A->col.as<int>(),
+    k_take_scaled<float><<<ceil_div(nnz, 256), 256, 0, A->ctx->stream>>>(nnz, A->mx_src.as<int>(), A->col.as<int>(),
                                                                  A->val32.as<float>(), A->dinv32.as<float>(),
                                                                  A->mx_val32s.as<float>());
     HIPCHK(hipGetLastError());
@@ -1608,6 +1788,34 @@ const float *matrix_staged_scaled(yafel_hip_matrix_s *A) {
   A->mx_sgen = A->v32_gen;
   A->mx_dgen = A->diag_gen;
   return A->mx_val32s.as<float>();
+}
+
+// The FP64 staged products' values: the CSR-order copy of the FP64 matrix,
+// or of A D^-1 (scaled), refreshed when the values or the diagonal changed.
+const double *matrix_staged_val64(yafel_hip_matrix_s *A, bool scaled) {
+  const int64_t nnz = (int64_t)(A->mx_src.bytes / 4);
+  DBuf &dst = scaled ? A->mx_val64s : A->mx_val64;
+  if (scaled ? (A->mx_sgen64 == A->v64_gen && A->mx_dgen64 == A->diag_gen) : A->mx_gen64 == A->v64_gen)
+    return dst.as<double>();
+  REQUIRE(!scaled || (A->diag_ok && A->dinv64.p), "staged products: no diagonal");
+  if (dst.bytes < (size_t)std::max<int64_t>(nnz, 1) * 8) dst.alloc(std::max<int64_t>(nnz, 1) * 8);
+  if (nnz) {
+    if (scaled)
+      k_take_scaled<double><<<ceil_div(nnz, 256), 256, 0, A->ctx->stream>>>(
+          nnz, A->mx_src.as<int>(), A->col.as<int>(), A->val64.as<double>(), A->dinv64.as<double>(),
+          dst.as<double>());
+    else
+      k_take<double><<<ceil_div(nnz, 256), 256, 0, A->ctx->stream>>>(nnz, A->mx_src.as<int>(),
+                                                                      A->val64.as<double>(), dst.as<double>());
+    HIPCHK(hipGetLastError());
+  }
+  if (scaled) {
+    A->mx_sgen64 = A->v64_gen;
+    A->mx_dgen64 = A->diag_gen;
+  } else {
+    A->mx_gen64 = A->v64_gen;
+  }
+  return dst.as<double>();
 }
 
 // True when the staged products run on A (CSR, lists with 16-bit positions).
@@ -1641,6 +1849,25 @@ int launch_mspmm_lds(yafel_hip_matrix_s *A, const Op &op, int grid, hipStream_t 
     g = cap(2);
     k_mspmm_lds<Op><<<g, NT, lds, s>>>(v, op);
   }
+  HIPCHK(hipGetLastError());
+  return g;
+}
+
+// FP64 staged product; same grid rule as launch_mspmm_lds (residency per CU
+// from the launch's LDS).
+template <class Op>
+int launch_mspmm_lds64(yafel_hip_matrix_s *A, const Op &op, int grid, hipStream_t s) {
+  const double *val = matrix_staged_val64(A, Op::kScaledVals);
+  // at least 32 staged rows: the partials' fold reuses their space
+  const int dcap = std::max(32, std::min(A->mx_dmax, kStageRows64));
+  const int ecap = (int)std::min<int64_t>((A->mx_emax + 3) & ~(int64_t)3, kStageEnt64);
+  const StagedCsr64 v{A->mx_ptr.as<int>(), val, A->mx_lidx.as<unsigned short>(), A->mx_dptr.as<int>(),
+                      A->mx_dcol.as<int>(), A->mx_grow.as<int>(), A->mx_ngr, dcap, ecap};
+  const size_t lds = (size_t)dcap * KC64 * 8 + (size_t)ecap * 10;
+  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / (lds + 512)));
+  const int g = (int)std::max<int64_t>(
+      1, std::min<int64_t>({(int64_t)grid, A->mx_ngr, (int64_t)per_cu * A->ctx->num_cus}));
+  k_mspmm_lds64<Op><<<g, NT, lds, s>>>(v, op);
   HIPCHK(hipGetLastError());
   return g;
 }
@@ -1686,6 +1913,7 @@ struct Multi {
   DBuf vecs[9];
   DBuf partbuf, stbuf;
   bool staged = false;  // FP32 products stage X in LDS
+  bool staged64 = false;  // FP64 products stage X in LDS
 
   template <class O> O base(O op, int c) const {
     op.st = st;
@@ -1711,6 +1939,8 @@ struct Multi {
       } else {
         k_mspmm<T, O><<<g, NT, 0, s>>>(cview<T>(A), op);
       }
+    } else if (staged64) {
+      g = launch_mspmm_lds64(A, op, grid, s);
     } else {
       k_mspmm<T, O><<<g, NT, 0, s>>>(cview<T>(A), op);
     }
@@ -1969,6 +2199,13 @@ bool spmm_lds_launch(yafel_hip_matrix_s *A, int k, const float *X, float *Y, hip
   launch_mspmm_lds(A, op, A->ctx->num_cus * 8, s);
   return true;
 }
+
+bool spmm_lds64_launch(yafel_hip_matrix_s *A, int k, const double *X, double *Y, hipStream_t s) {
+  if (k < 1 || !staged_ok(A)) return false;
+  MPlain64 op{X, Y, k, nullptr};  // the kernel walks passes of 32 right-hand sides up to k
+  launch_mspmm_lds64(A, op, A->ctx->num_cus * 8, s);
+  return true;
+}
 }  // namespace yh
 
 extern "C" int yafel_hip_solve_multi(yafel_hip_matrix A, const yafel_hip_solve_opts *opts, int k,
@@ -1992,6 +2229,7 @@ extern "C" int yafel_hip_solve_multi(yafel_hip_matrix A, const yafel_hip_solve_o
   const bool mixed = opts->precision == YAFEL_HIP_MIXED;
   if (mixed && !A->has32) matrix_build_fp32(A);
   const bool staged = mixed && staged_ok(A);  // the FP32 products stage X in LDS
+  const bool staged64 = staged_ok(A);          // and the FP64 ones
   HIPCHK(hipEventRecord(ctx->t0, ctx->stream));
   // batches of up to 64 right-hand sides, each a contiguous column block;
   // the batch is copied into a packed n x kb buffer and back
@@ -2005,6 +2243,7 @@ extern "C" int yafel_hip_solve_multi(yafel_hip_matrix A, const yafel_hip_solve_o
     M.k = kb;
     M.mixed = mixed;
     M.staged = staged;
+    M.staged64 = staged64;
     M.cgfam = opts->method == YAFEL_HIP_CG || opts->method == YAFEL_HIP_PCG_JACOBI;
     M.prec = opts->method == YAFEL_HIP_PCG_JACOBI || opts->method == YAFEL_HIP_BICGSTAB_JACOBI;
     if (!M.prec && !M.cgfam) fail(YAFEL_HIP_EINVAL, "solve_multi: BiCGSTAB runs with right Jacobi preconditioning only");

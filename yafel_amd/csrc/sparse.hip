@@ -456,6 +456,7 @@ void matrix_make_sliced(yafel_hip_matrix_s *A, DBuf &ptr, DBuf &col, DBuf &val64
   val64.release();
   A->has64 = true;
   A->has32 = false;
+  A->v64_gen++;
   A->diag_ok = false;
 }
 
@@ -494,6 +495,7 @@ void spmm_launch(yafel_hip_matrix_s *A, int precision, int k, const void *X, voi
                                               (const float *)X, (float *)Y);
   } else {
     REQUIRE(A->has64, "spmm: matrix has no FP64 copy");
+    if (spmm_lds64_launch(A, k, (const double *)X, (double *)Y, s)) return;
     if (A->format == YAFEL_HIP_CSR)
       k_csr_spmm<double><<<gcsr, 256, 0, s>>>(A->rows, k, sp, pm, c, A->val64.as<double>(), (const double *)X,
                                               (double *)Y);
