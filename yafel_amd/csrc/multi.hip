@@ -668,12 +668,14 @@ void k_mspmm_pipe(StagedCsr A, Op op0) {
 // gathered value).  Lane (rho = lane >> 4, q = lane & 15) takes right-hand
 // sides 2q and 2q + 1 of the pass for position rho of a set of four
 // consecutive positions, so the 16 lanes of a position read one whole staged
-// row per entry: each 16-byte LDS read is free of bank conflicts.  Wave w
-// takes sets w, 7 - w, 8 + w and 15 - w (positions are longest first).  A row
+// row per entry: each 16-byte LDS read is free of bank conflicts.  Eight
+// waves share a group (the LDS, not the registers, caps the workgroups per
+// CU); wave w takes sets w and 15 - w (positions are longest first).  A row
 // is summed entry by entry in column order with one FMA per entry, as the
 // sliced FP64 product sums it, so the products are the same bit for bit; the
 // row epilogue is the op's per-right-hand-side row().
 constexpr int KC64 = 32;         // right-hand sides per pass
+constexpr int NT64 = 512;        // threads per workgroup: 8 waves share a group's LDS
 constexpr int kStageRows64 = 192;   // staged rows: 48 KiB
 constexpr int kStageEnt64 = 1536;   // staged entries: 15 KiB (values and positions)
 
@@ -685,15 +687,14 @@ struct StagedCsr64 {
   const int *grow;
   int64_t ngr;
   int dcap, ecap;  // rows of X and entries the launch's LDS holds
+  int pair;        // ops with row2: a lane's two right-hand sides stored as one 16-byte pair
 };
 
-__device__ __forceinline__ int staged_set64(int w, int m) {
-  return m == 0 ? w : m == 1 ? 7 - w : m == 2 ? 8 + w : 15 - w;
-}
+__device__ __forceinline__ int staged_set64(int w, int m) { return m == 0 ? w : 15 - w; }
 
 // Per-RHS partials: lane (rho, q) holds right-hand sides 32h + 2q + c in
 // acc[h][c]; the four positions of a wave are folded by shuffles (fixed
-// order), then the waves through LDS (the staged rows' space, free by now).
+// order), then the 8 waves through LDS (the staged rows' space, free by now).
 template <int NP, class Op>
 __device__ __forceinline__ void finish_staged64(Op &op, double (&acc)[2][2][NP], double *lds) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = lane & 15;
@@ -708,7 +709,7 @@ __device__ __forceinline__ void finish_staged64(Op &op, double (&acc)[2][2][NP],
         double s = acc[h][c][j];
         s += __shfl_xor(s, 16, 64);
         s += __shfl_xor(s, 32, 64);
-        if (lane < 16) lds[(j * (NT / 64) + w) * 64 + 32 * h + 2 * q + c] = s;
+        if (lane < 16) lds[(j * (NT64 / 64) + w) * 64 + 32 * h + 2 * q + c] = s;
       }
   __syncthreads();
   if (w == 0) {
@@ -716,14 +717,22 @@ __device__ __forceinline__ void finish_staged64(Op &op, double (&acc)[2][2][NP],
     for (int j = 0; j < NP; ++j) {
       double s = 0;
 #pragma unroll
-      for (int ww = 0; ww < NT / 64; ++ww) s += lds[(j * (NT / 64) + ww) * 64 + lane];
+      for (int ww = 0; ww < NT64 / 64; ++ww) s += lds[(j * (NT64 / 64) + ww) * 64 + lane];
       op.part[((size_t)j * gridDim.x + blk) * 64 + lane] = s;
     }
   }
 }
 
+// Six waves per SIMD (at most 80 registers) for ops with up to two sums:
+// three workgroups per CU, as many as the LDS of c5-sized groups holds.
 template <class Op>
-__global__ __launch_bounds__(NT) void k_mspmm_lds64(StagedCsr64 A, Op op0) {
+constexpr int lds64_waves() {
+  return Op::NP <= 2 ? 6 : 4;
+}
+
+template <class Op>
+__global__ __launch_bounds__(NT64) __attribute__((amdgpu_waves_per_eu(lds64_waves<Op>(), lds64_waves<Op>())))
+void k_mspmm_lds64(StagedCsr64 A, Op op0) {
   // A.dcap rows of KC64 / 2 double2, then A.ecap values, then A.ecap positions
   extern __shared__ double2 xs2[];
   __shared__ int ep[65];
@@ -752,7 +761,7 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds64(StagedCsr64 A, Op op0) {
     __syncthreads();  // every wave is done with the previous group's LDS
     if (tid < 65) ep[tid] = A.ptr[r0 + tid] - e0;
     if (staged)
-      for (int i = tid; i < ne; i += NT) {
+      for (int i = tid; i < ne; i += NT64) {
         ev[i] = __builtin_nontemporal_load(&A.val[e0 + i]);
         el[i] = __builtin_nontemporal_load(&A.lidx[e0 + i]);
       }
@@ -764,16 +773,16 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds64(StagedCsr64 A, Op op0) {
       for (int c = 0; c < 2; ++c) oc[c].bind1(jb + c < k ? jb + c : 0);
       if (staged) {
         if (h > 0) __syncthreads();  // every wave is done with the previous pass's rows
-        // thread (tid >> 4, q) stages pair q of rows tid >> 4, + 16, ...
+        // thread (tid >> 4, q) stages pair q of rows tid >> 4, + 32, ...
 #pragma unroll 4
-        for (int i = tid >> 4; i < d; i += NT / 16) {
+        for (int i = tid >> 4; i < d; i += NT64 / 16) {
           const int c = A.dcol[d0 + i];
           xs2[i * (KC64 / 2) + q] = make_double2(jb < k ? oc[0].x(c, jb) : 0.0, jb + 1 < k ? oc[1].x(c, jb + 1) : 0.0);
         }
       }
       __syncthreads();  // the group's entries (first pass) and this pass's rows are in LDS
 #pragma unroll 1
-      for (int m = 0; m < 4; ++m) {
+      for (int m = 0; m < 2; ++m) {
         const int p = staged_set64(w, m) * 4 + rho;
         const int64_t r = A.grow[r0 + p];
         const int b = ep[p], n4 = r >= 0 ? (ep[p + 1] - b) >> 2 : 0;
@@ -807,10 +816,18 @@ __global__ __launch_bounds__(NT) void k_mspmm_lds64(StagedCsr64 A, Op op0) {
         }
         // constant indexes keep the sums in registers (h > 1, k > 64: plain
         // products only, which keep none)
-        if (r >= 0 && h == 0) {
+        bool done = false;
+        if constexpr (Op::kRow2) {
+          if (r >= 0 && A.pair && jb + 1 < k) {
+            if (h == 0) oc[0].row2(r, jb, y0, y1, oc[1], acc[0][0], acc[0][1]);
+            else oc[0].row2(r, jb, y0, y1, oc[1], acc[1][0], acc[1][1]);
+            done = true;
+          }
+        }
+        if (r >= 0 && !done && h == 0) {
           if (jb < k) oc[0].row(r, jb, y0, acc[0][0]);
           if (jb + 1 < k) oc[1].row(r, jb + 1, y1, acc[0][1]);
-        } else if (r >= 0) {
+        } else if (r >= 0 && !done) {
           if (jb < k) oc[0].row(r, jb, y0, acc[1][0]);
           if (jb + 1 < k) oc[1].row(r, jb + 1, y1, acc[1][1]);
         }
@@ -941,6 +958,22 @@ struct MBase {
   // while the FP64 products it was tried on came out 15-40 % slower.
   bool act1 = true;
   __device__ void bind1(int j) { act1 = st->active[j] != 0; }
+  // FP64 staged products: the row epilogue of right-hand sides j, j + 1 as
+  // one 16-byte pair (row2; o1 is the op bound to j + 1), where the op has one
+  static constexpr bool kRow2 = false;
+  template <class T>
+  __device__ static void store2nt(T *dst, int64_t o, T y0, T y1, bool a0, bool a1) {
+    if constexpr (sizeof(T) == 8) {
+      typedef double v2d __attribute__((ext_vector_type(2)));
+      if (a0 && a1) {
+        const v2d w = {y0, y1};
+        __builtin_nontemporal_store(w, reinterpret_cast<v2d *>(dst + o));
+        return;
+      }
+    }
+    if (a0) __builtin_nontemporal_store(y0, dst + o);
+    if (a1) __builtin_nontemporal_store(y1, dst + o + 1);
+  }
   template <class T>
   __device__ static void store4(T *dst, int64_t o, const float4 &v, const bool (&a)[4]) {
     if (a[0] && a[1] && a[2] && a[3]) {
@@ -1070,6 +1103,15 @@ struct MV : MBase {
     v[i * k + j] = y;
     acc[0] += (double)rhat[i * k + j] * (double)y;
   }
+  static constexpr bool kRow2 = sizeof(T) == 8;
+  __host__ bool pair_ok() const { return ((uintptr_t)v | (uintptr_t)rhat) % 16 == 0; }
+  __device__ void row2(int64_t i, int j, T y0, T y1, const MV &o1, double *acc0, double *acc1) const {
+    const int64_t o = i * k + j;
+    store2nt(v, o, y0, y1, act1, o1.act1);
+    const double2 rh = *reinterpret_cast<const double2 *>(rhat + o);
+    if (act1) acc0[0] += rh.x * (double)y0;
+    if (o1.act1) acc1[0] += rh.y * (double)y1;
+  }
   __device__ float4 x4(int c, int g) const { return ld4((const float *)p, (int64_t)c * k + 4 * g); }
   __device__ Pre4 pre4(int64_t i, int j0) const { return Pre4{ld4((const float *)rhat, i * k + j0), {}, false}; }
   __device__ void row4p(int64_t i, int j0, const float4 &y, const Pre4 &pr, double (&acc)[NP][4]) const {
@@ -1149,6 +1191,21 @@ struct MT : MBase {
     t[i * k + j] = y;
     acc[0] += (double)y * (double)y;
     acc[1] += (double)y * (double)s[i * k + j];
+  }
+  static constexpr bool kRow2 = sizeof(T) == 8;
+  __host__ bool pair_ok() const { return ((uintptr_t)t | (uintptr_t)s) % 16 == 0; }
+  __device__ void row2(int64_t i, int j, T y0, T y1, const MT &o1, double *acc0, double *acc1) const {
+    const int64_t o = i * k + j;
+    store2nt(t, o, y0, y1, act1, o1.act1);
+    const double2 sv = *reinterpret_cast<const double2 *>(s + o);
+    if (act1) {
+      acc0[0] += (double)y0 * (double)y0;
+      acc0[1] += (double)y0 * sv.x;
+    }
+    if (o1.act1) {
+      acc1[0] += (double)y1 * (double)y1;
+      acc1[1] += (double)y1 * sv.y;
+    }
   }
   __device__ float4 x4(int c, int g) const { return ld4((const float *)s, (int64_t)c * k + 4 * g); }
   __device__ Pre4 pre4(int64_t i, int j0) const { return Pre4{ld4((const float *)s, i * k + j0), {}, false}; }
@@ -1606,6 +1663,11 @@ struct MPlain64 {
   __device__ void row(int64_t i, int j, double y, double *) const {
     __builtin_nontemporal_store(y, Y + i * k + j);  // written once, as MBase::store4nt
   }
+  static constexpr bool kRow2 = true;
+  __host__ bool pair_ok() const { return (uintptr_t)Y % 16 == 0; }
+  __device__ void row2(int64_t i, int j, double y0, double y1, const MPlain64 &, double *, double *) const {
+    MBase::store2nt(Y, i * k + j, y0, y1, true, true);
+  }
 };
 
 // The CSR-order copy of the staged products (pattern part, once per
@@ -1858,16 +1920,18 @@ int launch_mspmm_lds(yafel_hip_matrix_s *A, const Op &op, int grid, hipStream_t 
 template <class Op>
 int launch_mspmm_lds64(yafel_hip_matrix_s *A, const Op &op, int grid, hipStream_t s) {
   const double *val = matrix_staged_val64(A, Op::kScaledVals);
-  // at least 32 staged rows: the partials' fold reuses their space
-  const int dcap = std::max(32, std::min(A->mx_dmax, kStageRows64));
+  // at least 64 staged rows: the partials' fold reuses their space
+  const int dcap = std::max(64, std::min(A->mx_dmax, kStageRows64));
   const int ecap = (int)std::min<int64_t>((A->mx_emax + 3) & ~(int64_t)3, kStageEnt64);
+  int pair = 0;  // k even and 16-byte aligned outputs: a lane's two right-hand sides are adjacent and aligned
+  if constexpr (Op::kRow2) pair = op.k % 2 == 0 && op.pair_ok();
   const StagedCsr64 v{A->mx_ptr.as<int>(), val, A->mx_lidx.as<unsigned short>(), A->mx_dptr.as<int>(),
-                      A->mx_dcol.as<int>(), A->mx_grow.as<int>(), A->mx_ngr, dcap, ecap};
+                      A->mx_dcol.as<int>(), A->mx_grow.as<int>(), A->mx_ngr, dcap, ecap, pair};
   const size_t lds = (size_t)dcap * KC64 * 8 + (size_t)ecap * 10;
   const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / (lds + 512)));
   const int g = (int)std::max<int64_t>(
       1, std::min<int64_t>({(int64_t)grid, A->mx_ngr, (int64_t)per_cu * A->ctx->num_cus}));
-  k_mspmm_lds64<Op><<<g, NT, lds, s>>>(v, op);
+  k_mspmm_lds64<Op><<<g, NT64, lds, s>>>(v, op);
   HIPCHK(hipGetLastError());
   return g;
 }
