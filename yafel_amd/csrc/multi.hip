@@ -736,6 +736,7 @@ void k_mspmm_lds64(StagedCsr64 A, Op op0) {
   // A.dcap rows of KC64 / 2 double2, then A.ecap values, then A.ecap positions
   extern __shared__ double2 xs2[];
   __shared__ int ep[65];
+  __shared__ int grw[64];  // row of each position of the group
   double *ev = reinterpret_cast<double *>(xs2 + (size_t)A.dcap * (KC64 / 2));
   unsigned short *el = reinterpret_cast<unsigned short *>(ev + A.ecap);
   constexpr int NP = Op::NP, NPA = NP > 0 ? NP : 1;
@@ -749,7 +750,7 @@ void k_mspmm_lds64(StagedCsr64 A, Op op0) {
     for (int c = 0; c < 2; ++c)
 #pragma unroll
       for (int j = 0; j < NPA; ++j) acc[h][c][j] = 0;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 15, rho = lane >> 4;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 15, rho = lane >> 4, i0 = tid >> 4;
   const int k = op.k;
   const int nh = (k + KC64 - 1) / KC64;
   const GroupWalk gw = group_walk(A.ngr);
@@ -758,8 +759,12 @@ void k_mspmm_lds64(StagedCsr64 A, Op op0) {
     const int64_t r0 = g * 64;
     const int e0 = A.ptr[r0], ne = A.ptr[r0 + 64] - e0;
     const bool staged = d <= A.dcap && ne <= A.ecap;
+    // the group's entry pointers and rows, loaded before the barrier
+    const int pp = A.ptr[r0 + (tid < 64 ? tid : 64)] - e0;  // threads 0..64
+    const int prow = A.grow[r0 + (tid & 63)];               // threads 0..63
     __syncthreads();  // every wave is done with the previous group's LDS
-    if (tid < 65) ep[tid] = A.ptr[r0 + tid] - e0;
+    if (tid < 65) ep[tid] = pp;
+    if (tid < 64) grw[tid] = prow;
     if (staged)
       for (int i = tid; i < ne; i += NT64) {
         ev[i] = __builtin_nontemporal_load(&A.val[e0 + i]);
@@ -773,9 +778,9 @@ void k_mspmm_lds64(StagedCsr64 A, Op op0) {
       for (int c = 0; c < 2; ++c) oc[c].bind1(jb + c < k ? jb + c : 0);
       if (staged) {
         if (h > 0) __syncthreads();  // every wave is done with the previous pass's rows
-        // thread (tid >> 4, q) stages pair q of rows tid >> 4, + 32, ...
+        // thread (i0, q) stages pair q of rows i0, i0 + 32, ...
 #pragma unroll 4
-        for (int i = tid >> 4; i < d; i += NT64 / 16) {
+        for (int i = i0; i < d; i += NT64 / 16) {
           const int c = A.dcol[d0 + i];
           xs2[i * (KC64 / 2) + q] = make_double2(jb < k ? oc[0].x(c, jb) : 0.0, jb + 1 < k ? oc[1].x(c, jb + 1) : 0.0);
         }
@@ -784,7 +789,7 @@ void k_mspmm_lds64(StagedCsr64 A, Op op0) {
 #pragma unroll 1
       for (int m = 0; m < 2; ++m) {
         const int p = staged_set64(w, m) * 4 + rho;
-        const int64_t r = A.grow[r0 + p];
+        const int64_t r = grw[p];
         const int b = ep[p], n4 = r >= 0 ? (ep[p + 1] - b) >> 2 : 0;
         double y0 = 0, y1 = 0;
         if (staged) {
