@@ -119,14 +119,21 @@ def test_spmm_matches_columnwise_spmv(H, ctx, golden):
 
 @pytest.mark.parametrize("k", [3, 16, 64])
 def test_spmm_fp32_scattered_columns(H, ctx, k):
-    """FP32 products on a pattern whose slices touch more distinct columns
-    than the staged product holds in LDS (random columns: about 600 per
-    slice), so those slices gather directly, beside slices that stage."""
+    """FP32 products on a pattern where a few row groups touch more distinct
+    columns than the staged product holds in LDS (random columns: about 600
+    per group), so those groups gather directly, beside groups that stage."""
     rng = np.random.default_rng(31 + k)
-    n = 3000
+    n = 6000
     rows = np.repeat(np.arange(n), 10)
     cols = rng.integers(0, n, rows.size)
-    cols[: 10 * 640] = (rows[: 10 * 640] + rng.integers(-20, 21, 10 * 640)) % n   # banded leading slices
+    # banded leading rows; the last 600 rows take half their columns among
+    # themselves (so that their groups fill up) and half anywhere: about 9 of
+    # the 160 groups touch 550 columns or more
+    nb = 10 * 5400
+    cols[:nb] = (rows[:nb] + rng.integers(-20, 21, nb)) % n
+    tail = cols[nb:]
+    half = rng.random(tail.size) < 0.5
+    tail[half] = 5400 + rng.integers(0, 600, half.sum())
     vals = rng.standard_normal(rows.size)
     A = H.Matrix.from_triplets(ctx, rows, cols, vals, precisions=H.FP64 | H.FP32)
     A.sync_fp32()
@@ -164,11 +171,11 @@ def test_spmm_sorted_rows_matches_columnwise_spmv(H, ctx, k):
 @pytest.mark.parametrize("pattern", ["q2", "scattered"])
 @pytest.mark.parametrize("k", [1, 5, 32, 33, 64, 70])
 def test_spmm_fp64_staged_is_the_spmv_bit_for_bit(H, ctx, pattern, k):
-    """The FP64 product stages rows of X in LDS per row group and sums each
-    row with one FMA per entry in column order, as the FP64 SpMV does, so
-    every column of Y equals the SpMV of that column of X bit for bit: on a
-    Q2 pattern (every group staged) and on a random pattern whose groups
-    touch more distinct columns than the LDS holds (those read X directly)."""
+    """The FP64 product sums each row with one FMA per entry in column
+    order, as the FP64 SpMV does, so every column of Y equals the SpMV of
+    that column of X bit for bit: on a Q2 pattern (rows of X staged in LDS per
+    row group) and on a random pattern whose groups touch more distinct
+    columns than the LDS holds (the sliced product, which gathers X)."""
     rng = np.random.default_rng(7 + k)
     if pattern == "q2":
         m = H.Mesh.structured(ctx, 2, 2, 1, 20)

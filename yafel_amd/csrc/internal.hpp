@@ -174,6 +174,7 @@ struct yafel_hip_matrix_s {
   yh::DBuf mx_val32;            // FP32 values in CSR order, zero padding (refreshed from val32)
   int mx_dmax = -1;             // longest list (-1: not built; 0: not usable)
   int64_t mx_emax = 0;          // most (padded) entries in a group
+  int64_t mx_nbig = 0;          // groups over the FP32 staged products' LDS budget (read X directly)
   int64_t v32_gen = 0;          // bumped whenever val32 is rebuilt
   int64_t mx_gen = -1;          // the val32 generation mx_val32 holds
   // direct-write map of the FP32 matrix-core assembly (built on first use):

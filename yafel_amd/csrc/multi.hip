@@ -673,7 +673,8 @@ void k_mspmm_pipe(StagedCsr A, Op op0) {
 // CU); wave w takes sets w and 15 - w (positions are longest first).  A row
 // is summed entry by entry in column order with one FMA per entry, as the
 // sliced FP64 product sums it, so the products are the same bit for bit; the
-// row epilogue is the op's per-right-hand-side row().
+// row epilogue is the op's per-right-hand-side row().  Every group must fit
+// the launch's LDS (staged64_ok).
 constexpr int KC64 = 32;         // right-hand sides per pass
 constexpr int NT64 = 512;        // threads per workgroup: 8 waves share a group's LDS
 constexpr int kStageRows64 = 192;   // staged rows: 48 KiB
@@ -757,33 +758,29 @@ void k_mspmm_lds64(StagedCsr64 A, Op op0) {
   for (int64_t g = gw.g; g < gw.end; g += gw.step) {
     const int d0 = A.dptr[g], d = A.dptr[g + 1] - d0;
     const int64_t r0 = g * 64;
-    const int e0 = A.ptr[r0], ne = A.ptr[r0 + 64] - e0;
-    const bool staged = d <= A.dcap && ne <= A.ecap;
+    const int e0 = A.ptr[r0], ne = A.ptr[r0 + 64] - e0;  // every group fits: d <= dcap, ne <= ecap
     // the group's entry pointers and rows, loaded before the barrier
     const int pp = A.ptr[r0 + (tid < 64 ? tid : 64)] - e0;  // threads 0..64
     const int prow = A.grow[r0 + (tid & 63)];               // threads 0..63
     __syncthreads();  // every wave is done with the previous group's LDS
     if (tid < 65) ep[tid] = pp;
     if (tid < 64) grw[tid] = prow;
-    if (staged)
-      for (int i = tid; i < ne; i += NT64) {
-        ev[i] = __builtin_nontemporal_load(&A.val[e0 + i]);
-        el[i] = __builtin_nontemporal_load(&A.lidx[e0 + i]);
-      }
+    for (int i = tid; i < ne; i += NT64) {
+      ev[i] = __builtin_nontemporal_load(&A.val[e0 + i]);
+      el[i] = __builtin_nontemporal_load(&A.lidx[e0 + i]);
+    }
 #pragma unroll 1
     for (int h = 0; h < nh; ++h) {
       const int jb = h * KC64 + 2 * q;  // this lane's first right-hand side of the pass
       Op oc[2] = {op, op};
 #pragma unroll
       for (int c = 0; c < 2; ++c) oc[c].bind1(jb + c < k ? jb + c : 0);
-      if (staged) {
-        if (h > 0) __syncthreads();  // every wave is done with the previous pass's rows
-        // thread (i0, q) stages pair q of rows i0, i0 + 32, ...
+      if (h > 0) __syncthreads();  // every wave is done with the previous pass's rows
+      // thread (i0, q) stages pair q of rows i0, i0 + 32, ...
 #pragma unroll 4
-        for (int i = i0; i < d; i += NT64 / 16) {
-          const int c = A.dcol[d0 + i];
-          xs2[i * (KC64 / 2) + q] = make_double2(jb < k ? oc[0].x(c, jb) : 0.0, jb + 1 < k ? oc[1].x(c, jb + 1) : 0.0);
-        }
+      for (int i = i0; i < d; i += NT64 / 16) {
+        const int c = A.dcol[d0 + i];
+        xs2[i * (KC64 / 2) + q] = make_double2(jb < k ? oc[0].x(c, jb) : 0.0, jb + 1 < k ? oc[1].x(c, jb + 1) : 0.0);
       }
       __syncthreads();  // the group's entries (first pass) and this pass's rows are in LDS
 #pragma unroll 1
@@ -792,31 +789,20 @@ void k_mspmm_lds64(StagedCsr64 A, Op op0) {
         const int64_t r = grw[p];
         const int b = ep[p], n4 = r >= 0 ? (ep[p + 1] - b) >> 2 : 0;
         double y0 = 0, y1 = 0;
-        if (staged) {
-          const double2 *ev2 = reinterpret_cast<const double2 *>(ev + b);
-          const uint2 *el4 = reinterpret_cast<const uint2 *>(el + b);
-          for (int t = 0; t < n4; ++t) {
-            const double2 va = ev2[2 * t], vb = ev2[2 * t + 1];
-            const uint2 l4 = el4[t];
-            const double vv[4] = {va.x, va.y, vb.x, vb.y};
-            const int li[4] = {(int)(l4.x & 0xffffu), (int)(l4.x >> 16), (int)(l4.y & 0xffffu), (int)(l4.y >> 16)};
-            double2 xq[4];
+        const double2 *ev2 = reinterpret_cast<const double2 *>(ev + b);
+        const uint2 *el4 = reinterpret_cast<const uint2 *>(el + b);
+        for (int t = 0; t < n4; ++t) {
+          const double2 va = ev2[2 * t], vb = ev2[2 * t + 1];
+          const uint2 l4 = el4[t];
+          const double vv[4] = {va.x, va.y, vb.x, vb.y};
+          const int li[4] = {(int)(l4.x & 0xffffu), (int)(l4.x >> 16), (int)(l4.y & 0xffffu), (int)(l4.y >> 16)};
+          double2 xq[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) xq[e] = xs2[li[e] * (KC64 / 2) + q];
+          for (int e = 0; e < 4; ++e) xq[e] = xs2[li[e] * (KC64 / 2) + q];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              y0 = fma(vv[e], xq[e].x, y0);
-              y1 = fma(vv[e], xq[e].y, y1);
-            }
-          }
-        } else {
-          // a group too large to stage: its entries and X read directly
-          for (int t = 0; t < 4 * n4; ++t) {
-            const int64_t e = e0 + b + t;
-            const double v = A.val[e];
-            const int c = A.dcol[d0 + A.lidx[e]];
-            y0 = fma(v, jb < k ? oc[0].x(c, jb) : 0.0, y0);
-            y1 = fma(v, jb + 1 < k ? oc[1].x(c, jb + 1) : 0.0, y1);
+          for (int e = 0; e < 4; ++e) {
+            y0 = fma(vv[e], xq[e].x, y0);
+            y1 = fma(vv[e], xq[e].y, y1);
           }
         }
         // constant indexes keep the sums in registers (h > 1, k > 64: plain
@@ -1771,7 +1757,7 @@ void matrix_build_staged(yafel_hip_matrix_s *A) {
   std::vector<unsigned short> lidx(std::max<int64_t>(npad, 1), 0), dl(std::max<int64_t>(npos, 1), 0);
   dcol.reserve(nnz / 3 + 1);
   int dmax = 0;
-  int64_t emax = 0;
+  int64_t emax = 0, nbig = 0;
   bool fits = true;
   for (int64_t g = 0; g < ngr; ++g) {
     const int64_t p0 = 64 * g, p1 = p0 + 64;
@@ -1794,6 +1780,7 @@ void matrix_build_staged(yafel_hip_matrix_s *A) {
     dcol.insert(dcol.end(), tmp.begin(), tmp.end());
     dmax = std::max(dmax, (int)tmp.size());
     emax = std::max<int64_t>(emax, pp[p1] - pp[p0]);
+    if ((int64_t)tmp.size() > kStageRows || pp[p1] - pp[p0] > kStageEnt) ++nbig;
     if (tmp.size() > 65535) {
       fits = false;
       continue;
@@ -1824,6 +1811,7 @@ void matrix_build_staged(yafel_hip_matrix_s *A) {
   A->mx_ngr = ngr;
   A->mx_dmax = fits ? dmax : 0;
   A->mx_emax = emax;
+  A->mx_nbig = nbig;
 }
 
 // The staged products' FP32 values, refreshed when the FP32 copy changed.
@@ -1892,6 +1880,20 @@ bool staged_ok(yafel_hip_matrix_s *A) {
   return A->mx_dmax > 0;
 }
 
+// True when the FP32 staged products run on A: at most one group in eight
+// is over their LDS budget.  Those groups read X directly, one dependent
+// gather per entry; when most are (3-D elasticity in CSR: about 5,000 entries
+// per group of 64 rows) the sliced products with 16-byte gathers are faster.
+bool staged32_ok(yafel_hip_matrix_s *A) { return staged_ok(A) && A->mx_nbig * 8 <= A->mx_ngr; }
+
+// True when the FP64 staged products run on A: every group's list and
+// entries fit their LDS budget.  Otherwise (3-D elasticity in CSR: about
+// 5,000 entries per group of 64 rows) the sliced FP64 product, which gathers
+// X directly, is the faster one.
+bool staged64_ok(yafel_hip_matrix_s *A) {
+  return staged_ok(A) && A->mx_dmax <= kStageRows64 && A->mx_emax <= kStageEnt64;
+}
+
 // grid: at most this many workgroups; the kernel's residency per CU caps it
 // further.  Returns the grid launched (the partials' block count).
 template <class Op>
@@ -1928,6 +1930,7 @@ int launch_mspmm_lds64(yafel_hip_matrix_s *A, const Op &op, int grid, hipStream_
   // at least 64 staged rows: the partials' fold reuses their space
   const int dcap = std::max(64, std::min(A->mx_dmax, kStageRows64));
   const int ecap = (int)std::min<int64_t>((A->mx_emax + 3) & ~(int64_t)3, kStageEnt64);
+  REQUIRE(A->mx_dmax <= dcap && A->mx_emax <= ecap, "staged FP64 product: a row group exceeds the LDS budget");
   int pair = 0;  // k even and 16-byte aligned outputs: a lane's two right-hand sides are adjacent and aligned
   if constexpr (Op::kRow2) pair = op.k % 2 == 0 && op.pair_ok();
   const StagedCsr64 v{A->mx_ptr.as<int>(), val, A->mx_lidx.as<unsigned short>(), A->mx_dptr.as<int>(),
@@ -2262,7 +2265,7 @@ struct Multi {
 namespace yh {
 bool spmm_lds_launch(yafel_hip_matrix_s *A, int k, const float *X, float *Y, hipStream_t s) {
   // float4 staging reads need 16-byte aligned rows of X
-  if (k < 1 || k > KMAX || k % 4 != 0 || ((uintptr_t)X % 16) != 0 || ((uintptr_t)Y % 16) != 0 || !staged_ok(A))
+  if (k < 1 || k > KMAX || k % 4 != 0 || ((uintptr_t)X % 16) != 0 || ((uintptr_t)Y % 16) != 0 || !staged32_ok(A))
     return false;
   MPlain op{X, Y, k, nullptr};
   launch_mspmm_lds(A, op, A->ctx->num_cus * 8, s);
@@ -2270,7 +2273,7 @@ bool spmm_lds_launch(yafel_hip_matrix_s *A, int k, const float *X, float *Y, hip
 }
 
 bool spmm_lds64_launch(yafel_hip_matrix_s *A, int k, const double *X, double *Y, hipStream_t s) {
-  if (k < 1 || !staged_ok(A)) return false;
+  if (k < 1 || !staged64_ok(A)) return false;
   MPlain64 op{X, Y, k, nullptr};  // the kernel walks passes of 32 right-hand sides up to k
   launch_mspmm_lds64(A, op, A->ctx->num_cus * 8, s);
   return true;
@@ -2297,8 +2300,8 @@ extern "C" int yafel_hip_solve_multi(yafel_hip_matrix A, const yafel_hip_solve_o
   if (!A->diag_ok) matrix_build_diag(A);
   const bool mixed = opts->precision == YAFEL_HIP_MIXED;
   if (mixed && !A->has32) matrix_build_fp32(A);
-  const bool staged = mixed && staged_ok(A);  // the FP32 products stage X in LDS
-  const bool staged64 = staged_ok(A);          // and the FP64 ones
+  const bool staged = mixed && staged32_ok(A);  // the FP32 products stage X in LDS
+  const bool staged64 = staged64_ok(A);        // and the FP64 ones
   HIPCHK(hipEventRecord(ctx->t0, ctx->stream));
   // batches of up to 64 right-hand sides, each a contiguous column block;
   // the batch is copied into a packed n x kb buffer and back
