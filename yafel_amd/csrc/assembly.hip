@@ -41,7 +41,10 @@ struct ElemShape {
   static constexpr int N1 = P + 1;
   static constexpr int NEN = DIM == 2 ? N1 * N1 : N1 * N1 * N1;
   static constexpr int NQP = NEN;
-  static constexpr int LDS_PER = NQP * NEN * DIM * 8 + NQP * 8 + NEN * DIM * 8;
+  // per element: gradients and jxw, each padded by one double (GP, WP) so
+  // that threads of consecutive elements read different LDS banks
+  static constexpr int GP = NQP * NEN * DIM + 1, WP = NQP + 1;
+  static constexpr int LDS_PER = GP * 8 + WP * 8 + NEN * DIM * 8;
   static constexpr int EPB0 = NT / NEN;
   static constexpr int EPB1 = 60000 / LDS_PER;
   static constexpr int EPB = (EPB0 < EPB1 ? EPB0 : EPB1) > 0 ? (EPB0 < EPB1 ? EPB0 : EPB1) : 1;
@@ -126,11 +129,14 @@ __device__ __forceinline__ void tangent_block(const Phys &ph, double ke, const d
 // gradients into LDS; then the tangent blocks and the residual rows.
 // SYM (Poisson, elasticity, whose tangents the reference fills for B >= A and
 // mirrors): only the blocks B >= A are formed, a thread per (element, block),
-// and staged block by block in the packed upper triangle,
-//   Ke[(e * NB + idx(A, B)) * DPN^2 + a * DPN + b],  NB = NEN (NEN + 1) / 2,
-// so consecutive threads write consecutive blocks; the gather reads block
-// (B, A) as the transpose.  Otherwise every block is formed, a thread per
-// (element, row block A), into the row-major element matrix.
+// consecutive threads taking consecutive elements of one block, and staged
+// component-major in the packed upper triangle,
+//   Ke[((idx(A, B) * DPN + a) * DPN + b) * nel + e],  NB = NEN (NEN + 1) / 2
+// blocks, so that both these stores and the gather's loads (a wavefront takes
+// neighbouring nodes, whose entries come from neighbouring elements) are
+// contiguous; the gather reads block (B, A) as the transpose.  Otherwise
+// every block is formed, a thread per (element, row block A), into the
+// row-major element matrix.
 template <int DIM, int P, int DPN, bool SYM>
 __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__restrict__ el,
                                                 const double *__restrict__ X,
@@ -141,9 +147,10 @@ __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__re
                                                 double *__restrict__ Re, int want_k, int want_r) {
   using S = ElemShape<DIM, P>;
   constexpr int NEN = S::NEN, NQP = S::NQP, EPB = S::EPB, NLD = NEN * DPN;
-  __shared__ double sG[EPB][NQP][NEN][DIM];
-  __shared__ double sW[EPB][NQP];
+  __shared__ double sGp[EPB][S::GP];  // [q][A][d] per element, padded
+  __shared__ double sW[EPB][S::WP];
   __shared__ double sX[EPB][NEN][DIM];
+  auto sG = [&](int le) { return reinterpret_cast<double(*)[NEN][DIM]>(&sGp[le][0]); };
   const int64_t e0 = (int64_t)blockIdx.x * EPB;
   for (int i = threadIdx.x; i < EPB * NEN; i += NT) {
     const int le = i / NEN, a = i - (i / NEN) * NEN;
@@ -180,7 +187,7 @@ __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__re
         double s = 0;
 #pragma unroll
         for (int k = 0; k < DIM; ++k) s += Ji[k * DIM + d] * dn[k];
-        sG[le][q][A][d] = s;
+        sG(le)[q][A][d] = s;
       }
     }
   }
@@ -188,7 +195,7 @@ __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__re
   if (want_k && SYM) {
     constexpr int NB = NEN * (NEN + 1) / 2;
     for (int i = threadIdx.x; i < EPB * NB; i += NT) {
-      const int le = i / NB, bi = i - le * NB;
+      const int bi = i / EPB, le = i - bi * EPB;
       const int64_t e = e0 + le;
       if (e >= nel) continue;
       int A = 0, rem = bi;
@@ -197,12 +204,11 @@ __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__re
         ++A;
       }
       double k[DPN][DPN];
-      tangent_block<DIM, DPN, NEN, NQP>(ph, coef ? coef[e] : 1.0, sG[le], sW[le], tabN, A, A + rem, k);
-      double *Kb = Ke + (e * NB + bi) * (DPN * DPN);
+      tangent_block<DIM, DPN, NEN, NQP>(ph, coef ? coef[e] : 1.0, sG(le), sW[le], tabN, A, A + rem, k);
 #pragma unroll
       for (int a = 0; a < DPN; ++a)
 #pragma unroll
-        for (int b = 0; b < DPN; ++b) Kb[a * DPN + b] = k[a][b];
+        for (int b = 0; b < DPN; ++b) Ke[(((int64_t)bi * DPN + a) * DPN + b) * nel + e] = k[a][b];
     }
   }
   for (int i = threadIdx.x; i < EPB * NEN; i += NT) {
@@ -214,7 +220,7 @@ __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__re
       double *Krow = Ke + e * (int64_t)NLD * NLD;
       for (int B = 0; B < NEN; ++B) {
         double k[DPN][DPN];
-        tangent_block<DIM, DPN, NEN, NQP>(ph, ke, sG[le], sW[le], tabN, A, B, k);
+        tangent_block<DIM, DPN, NEN, NQP>(ph, ke, sG(le), sW[le], tabN, A, B, k);
 #pragma unroll
         for (int a = 0; a < DPN; ++a)
 #pragma unroll
@@ -234,7 +240,7 @@ __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__re
 #pragma unroll
           for (int a = 0; a < DPN; ++a) r[a] += ph.body[a] * NA * jxw;
         } else {
-          const double *gA = sG[le][q][A];
+          const double *gA = sG(le)[q][A];
 #pragma unroll
           for (int Ac = 0; Ac < DIM; ++Ac) r[0] -= (NA * ph.Q + ph.Uh[Ac] * gA[Ac]) * jxw;
         }
@@ -446,6 +452,73 @@ __global__ __launch_bounds__(NT) void k_gather(int64_t nruns, int nen, const int
 #pragma unroll
       for (int jj = 0; jj < DPN; ++jj)
         val[dev::sell_pos(sptr, iperm, (int64_t)A * DPN + i, t * DPN + jj)] = acc[i][jj];
+  }
+}
+
+// The symmetric (component-major, upper-triangle) staging gathered in the
+// order of the sliced storage: SG wavefronts per slice, lane l taking stored
+// row s * 64 + l at every SG-th slot (an entry is a chain of dependent loads
+// -- pointer, contribution, values -- so the slots are spread over waves),
+// so every store of a slot is 64 consecutive values, and neighbouring lanes
+// (neighbouring nodes) read the same block of neighbouring elements.  Each entry sums its contributions in element
+// order, as k_gather does; padding slots are left as they are (zero).
+// Rows: block rows (BCSR3) or scalar rows (CSR, dof ci of node r / DPN, whose
+// slot t holds dof t % DPN of the node entry t / DPN).
+constexpr int kGatherSG = 16;  // wavefronts per slice
+
+template <int DPN, int FMT>
+__global__ __launch_bounds__(NT) void k_gather_sym(int64_t nsl, int64_t rows, int64_t nel, int nen,
+                                                   const int32_t *__restrict__ cptr,
+                                                   const uint32_t *__restrict__ contrib,
+                                                   const double *__restrict__ Ke,
+                                                   const int32_t *__restrict__ node_ptr,
+                                                   const int32_t *__restrict__ sptr,
+                                                   const int32_t *__restrict__ perm, int64_t ntot,
+                                                   double *__restrict__ val) {
+  constexpr int NV = FMT == YAFEL_HIP_BCSR3 ? DPN : 1;  // values per entry: NV x NV
+  const int64_t wid = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  const int64_t s = wid / kGatherSG;
+  if (s >= nsl) return;
+  const int l = threadIdx.x & 63;
+  const int64_t i = s * 64 + l;
+  const int64_t r = i < rows ? (perm ? (int64_t)perm[i] : i) : 0;
+  const int64_t A = FMT == YAFEL_HIP_BCSR3 ? r : r / DPN;
+  const int ci = FMT == YAFEL_HIP_BCSR3 ? 0 : (int)(r - A * DPN);
+  const int n0 = node_ptr[A], nlen = i < rows ? node_ptr[A + 1] - n0 : 0;
+  const int p0 = sptr[s], ns = sptr[s + 1] - p0;
+  const int64_t nen2 = (int64_t)nen * nen;
+  for (int t = (int)(wid - s * kGatherSG); t < ns; t += kGatherSG) {
+    const int tn = FMT == YAFEL_HIP_BCSR3 ? t : t / DPN, cj = FMT == YAFEL_HIP_BCSR3 ? 0 : t - (t / DPN) * DPN;
+    if (tn >= nlen) continue;
+    const int64_t j = n0 + tn;
+    double acc[NV][NV];
+#pragma unroll
+    for (int a = 0; a < NV; ++a)
+#pragma unroll
+      for (int b = 0; b < NV; ++b) acc[a][b] = 0;
+    const int c0 = cptr[j], c1 = cptr[j + 1];
+    for (int c = c0; c < c1; ++c) {
+      const int64_t tc = contrib[c];
+      const int64_t e = tc / nen2;
+      const int rr = (int)(tc - e * nen2);
+      const int a = rr / nen, b = rr - (rr / nen) * nen;
+      const int lo = a <= b ? a : b, hi = a <= b ? b : a;
+      const int64_t bi = (int64_t)lo * nen - (int64_t)lo * (lo - 1) / 2 + (hi - lo);
+      const double *Kb = Ke + bi * (DPN * DPN) * nel + e;  // component m at Kb[m * nel]
+      if (FMT == YAFEL_HIP_BCSR3) {
+#pragma unroll
+        for (int ii = 0; ii < NV; ++ii)
+#pragma unroll
+          for (int jj = 0; jj < NV; ++jj) acc[ii][jj] += Kb[(int64_t)(a <= b ? ii * DPN + jj : jj * DPN + ii) * nel];
+      } else {
+        acc[0][0] += Kb[(int64_t)(a <= b ? ci * DPN + cj : cj * DPN + ci) * nel];
+      }
+    }
+    const int64_t k = (int64_t)(p0 + t) * 64 + l;
+#pragma unroll
+    for (int ii = 0; ii < NV; ++ii)
+#pragma unroll
+      for (int jj = 0; jj < NV; ++jj) val[(int64_t)(ii * NV + jj) * ntot + k] = acc[ii][jj];
   }
 }
 
@@ -828,15 +901,23 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
                                                    pl->contrib.as<uint32_t>(), K_, pl->run_row.as<int32_t>(),     \
                                                    pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val)
       const double *k64 = m->stageK.as<double>();
+      const int gs = ceil_div(A->nsl * kGatherSG, NT / 64);
+      const int32_t *pm = A->perm.p ? A->perm.as<int32_t>() : nullptr;
+      const int64_t srows = A->format == YAFEL_HIP_BCSR3 ? A->brows : A->rows;
+#define GATHER_SYM(DPN_, FMT_)                                                                                   \
+  k_gather_sym<DPN_, FMT_><<<gs, NT, 0, s>>>(A->nsl, srows, m->nel, m->nen, pl->cptr.as<int32_t>(),               \
+                                             pl->contrib.as<uint32_t>(), k64, pl->node_ptr.as<int32_t>(), sp, pm, \
+                                             A->ntot, val)
       if (f32) GATHER(1, YAFEL_HIP_CSR, float, false, m->stageK32.as<float>());
-      else if (A->format == YAFEL_HIP_BCSR3 && sym) GATHER(3, YAFEL_HIP_BCSR3, double, true, k64);
+      else if (A->format == YAFEL_HIP_BCSR3 && sym) GATHER_SYM(3, YAFEL_HIP_BCSR3);
       else if (A->format == YAFEL_HIP_BCSR3) GATHER(3, YAFEL_HIP_BCSR3, double, false, k64);
-      else if (m->dpn == 1 && sym) GATHER(1, YAFEL_HIP_CSR, double, true, k64);
+      else if (m->dpn == 1 && sym) GATHER_SYM(1, YAFEL_HIP_CSR);
       else if (m->dpn == 1) GATHER(1, YAFEL_HIP_CSR, double, false, k64);
-      else if (m->dpn == 2 && sym) GATHER(2, YAFEL_HIP_CSR, double, true, k64);
+      else if (m->dpn == 2 && sym) GATHER_SYM(2, YAFEL_HIP_CSR);
       else if (m->dpn == 2) GATHER(2, YAFEL_HIP_CSR, double, false, k64);
-      else if (sym) GATHER(3, YAFEL_HIP_CSR, double, true, k64);
+      else if (sym) GATHER_SYM(3, YAFEL_HIP_CSR);
       else GATHER(3, YAFEL_HIP_CSR, double, false, k64);
+#undef GATHER_SYM
 #undef GATHER
       HIPCHK(hipGetLastError());
     }
