@@ -327,11 +327,11 @@ def fp64_one_at_a_time(hip, ctx, mats):
 
 
 def served_stream(hip, ctx, nreq, rates, fp64_sample, n_alone, slots=128, setup_threads=0, full_at_max=None,
-                  graph_rates=(50.0, 200.0), cg_passes=0, two_pass_limit=0):
+                  graph_rates=(50.0, 200.0, 2000.0), cg_passes=0, two_pass_limit=0):
     """c4: a served stream of Poisson / elasticity solves with Poisson
     arrivals at each rate; the record of every rate, the alone-vs-mix check,
     the plain FP64 GPU path on a sample of the same requests, and the same
-    streams at graph_rates with the chunks replayed from per-epoch graphs."""
+    streams at graph_rates with the chunks replayed from HIP graphs."""
     from yafel_amd import workloads
     budget = 200 << 30
     nmax = max(nreq, full_at_max or 0)
@@ -847,7 +847,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=300, help="default line: requests per rate")
     ap.add_argument("--serve-rates", default="50,200,2000", help="default line: arrival rates")
     ap.add_argument("--serve-fp64-sample", type=int, default=40, help="default line: FP64 one-at-a-time sample")
-    ap.add_argument("--graph-rates", default="50,200",
+    ap.add_argument("--graph-rates", default="50,200,2000",
                     help="serve: rates also run with chunks replayed from HIP graphs ('' for none)")
     ap.add_argument("--server-cg-passes", type=int, default=0,
                     help="serve: kernels per batched CG iteration, 2 or 3 (0: two while the batch is small)")

@@ -59,20 +59,25 @@ def test_alone_and_in_the_mix_are_identical(ctx):
 
 
 def test_graph_replay_matches_direct_submission(ctx):
-    """Chunks replayed from per-epoch graphs (captured again at every join and
-    retirement) give the same bits as directly submitted launches, over
-    epochs that last several chunks."""
+    """Chunks replayed from graphs whose kernels read the slot table and its
+    totals from device memory (uploaded when a request joins or retires; a
+    new capture only when the totals outgrow the captured grids) give the
+    same bits as directly submitted launches, with up to four chunks in
+    flight, in the three-pass form and with the form chosen per chunk."""
     from yafel_amd import hip as H
     reqs = [materialised(i, r) for i, r in enumerate(MIX)]
     direct, st0 = run_all(H, ctx, reqs, stagger=True)
-    replay, st1 = run_all(H, ctx, reqs, stagger=True, graphs=True)
     assert st0["graph_captures"] == 0 and st0["graph_replays"] == 0
-    assert st1["graph_captures"] >= 2 and st1["graph_replays"] > st1["graph_captures"]
-    assert st1["completed"] == len(reqs) and st1["failed"] == 0
-    for (x0, i0), (x1, i1) in zip(direct, replay):
-        assert i1.status == 0 and i1.rel_residual < 1e-8
-        assert np.array_equal(x0, x1)
-        assert i0.iterations == i1.iterations
+    for passes, limit in ((3, 0), (0, 100_000)):
+        replay, st1 = run_all(H, ctx, reqs, stagger=True, graphs=True, passes=passes, limit=limit)
+        assert st1["graph_captures"] >= 1 and st1["graph_replays"] > 2 * st1["graph_captures"]
+        assert st1["completed"] == len(reqs) and st1["failed"] == 0
+        if passes == 0:
+            assert st1["two_pass_chunks"] > 0 and st1["three_pass_chunks"] > 0
+        for (x0, i0), (x1, i1) in zip(direct, replay):
+            assert i1.status == 0 and i1.rel_residual < 1e-8
+            assert np.array_equal(x0, x1)
+            assert i0.iterations == i1.iterations
 
 
 def test_two_and_three_pass_iterations_are_identical(ctx):

@@ -232,9 +232,21 @@ __device__ __forceinline__ void slot_phase(const SlotDesc &d, int blk, int par) 
   }
 }
 
+// Replayed graphs take the table's slot count and block totals from here
+// (uploaded with the table), so that one capture serves every slot set whose
+// totals fit its grid; blocks past the totals return at once.
+struct TableHdr {
+  int ns, ts, te, pad;
+};
+
 template <int PH>
-__global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slots, int ns, int par) {
+__global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slots, int ns, int par,
+                                               const TableHdr *__restrict__ hdr) {
   constexpr bool SP = spmv_phase<PH>();
+  if (hdr) {
+    ns = hdr->ns;
+    if ((int)blockIdx.x >= (SP ? hdr->ts : hdr->te)) return;
+  }
   const int s = find_slot(slots, ns, blockIdx.x, SP);
   const SlotDesc d = slots[s];
   slot_phase<PH, false>(d, blockIdx.x - (SP ? d.off_spmv : d.off_elem), par);
@@ -242,7 +254,9 @@ __global__ __launch_bounds__(NT) void kb_phase(const SlotDesc *__restrict__ slot
 
 // one workgroup per slot folds that slot's partials
 template <int PH>
-__global__ __launch_bounds__(NTF) void kb_finish(const SlotDesc *__restrict__ slots, int ns, int par) {
+__global__ __launch_bounds__(NTF) void kb_finish(const SlotDesc *__restrict__ slots, int ns, int par,
+                                                 const TableHdr *__restrict__ hdr) {
+  if (hdr && (int)blockIdx.x >= hdr->ns) return;
   const SlotDesc d = slots[blockIdx.x];
   slot_phase<PH, true>(d, 0, par);
 }
@@ -271,9 +285,9 @@ struct SlotStatus {
   double crit;
 };
 
-__global__ void kb_status(const SlotDesc *slots, int ns, int par, SlotStatus *out) {
+__global__ void kb_status(const SlotDesc *slots, int ns, int par, SlotStatus *out, const TableHdr *hdr) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns) return;
+  if (j >= (hdr ? hdr->ns : ns)) return;
   const KState *st = slots[j].st + par;
   out[j].done = st->done;
   out[j].status = st->status;
@@ -342,7 +356,7 @@ struct yafel_hip_server_s {
   // at most kDepth calls' work is in flight, so an entry is never rewritten
   // while a launch that reads it (or a copy from its staging area) can still
   // run, nor a status area before the host has read it
-  static constexpr int kDepth = 4;  // chunks in flight with direct submission (graph replay: 2)
+  static constexpr int kDepth = 4;  // chunks in flight
   static constexpr int RING = 16;
   static_assert(RING > 2 * (kDepth + 1), "server: descriptor ring too short for the chunks in flight");
   static constexpr size_t STAGE = 131072;  // pinned bytes per ring entry (table / statuses)
@@ -354,27 +368,37 @@ struct yafel_hip_server_s {
     int ring;
     std::vector<Slot *> slots;
     char *pin;
+    int gr;  // graph replay: the status slot it uses (-1: direct)
   };
   std::deque<Inflight> inflight;
   hipEvent_t evs[RING];
-  // Graph replay (off unless yafel_hip_server_set_graphs turns it on): while
-  // the active slot set stays the same, a chunk's launches are replayed from
-  // an executable graph, one per state-version parity (a chunk advances the
-  // version by an odd count, so consecutive chunks alternate), captured at the
-  // epoch's first chunk with its own descriptor table, status buffer and
-  // pinned status area.  A join or a retirement ends the epoch.
+  // Graph replay (off unless yafel_hip_server_set_graphs turns it on): a
+  // chunk's launches are replayed from an executable graph whose kernels read
+  // the slot table and its totals from gdesc / ghdr.  The table is uploaded on
+  // the stream when the slot set changes (a join or a retirement ends the
+  // epoch), so that needs no new capture; the grids are captured with caps on
+  // the table's totals, and a slot set over a cap (or far below it) is
+  // captured again.  One graph per chunk form (three or two passes) and
+  // status slot r: slot r's chunks start at state-version parity r & 1 (a
+  // chunk advances the version by an odd count), and each slot has its own
+  // status buffer, pinned status area and event, busy while its chunk is in
+  // flight.
   bool graphs = false;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  static constexpr int GR = 4;
+  static_assert(GR >= kDepth, "server: a status slot per chunk in flight");
+  hipGraphExec_t gexec[2][GR] = {};
+  bool gbusy[GR] = {};
+  int gcap_s = 0, gcap_e = 0;  // the captured grids (block totals)
   int64_t epoch = 0;           // bumped at every join and retirement
-  int64_t gepoch = -1;         // the epoch the graphs were captured in
-  // graphs of ended epochs with the last chunk launched while they were
-  // current: destroyed once that chunk has drained
+  int64_t gepoch = -1;         // the epoch whose table gdesc / ghdr hold
+  // graphs replaced by a capture with other caps, with the last chunk launched
+  // while they were current: destroyed once that chunk has drained
   std::vector<std::pair<int64_t, hipGraphExec_t>> gold;
-  DBuf gdesc, gstatus[2];
-  char *gpin = nullptr;        // two status areas
-  hipEvent_t gev[2] = {nullptr, nullptr};
+  DBuf gdesc, ghdr, gstatus[GR];
+  char *gpin = nullptr;        // GR status areas
+  hipEvent_t gev[GR] = {};
   int64_t captures = 0, replays = 0;
-  int64_t stats_two = 0, stats_three = 0;  // directly submitted chunks of each form
+  int64_t stats_two = 0, stats_three = 0;  // chunks of each form
   char *pin = nullptr;  // RING status areas, then RING table staging areas
   int K = 8;
   // direct submission: CG chunks as two passes an iteration (QP, U) or three
@@ -660,13 +684,13 @@ int upload_table(yafel_hip_server_s *S, const std::vector<Slot *> &slots, int *t
 // followed by their per-slot folds, the versioned passes advance the state
 // version of every slot in the table.
 template <int PH>
-void launch_phase(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int blocks) {
+void launch_phase(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int blocks, const TableHdr *hdr = nullptr) {
   if (ns == 0 || blocks == 0) return;
   const int par = (int)(S->ver & 1);
-  kb_phase<PH><<<blocks, NT, 0, S->ctx->stream>>>(desc, ns, par);
+  kb_phase<PH><<<blocks, NT, 0, S->ctx->stream>>>(desc, ns, par, hdr);
   HIPCHK(hipGetLastError());
   if (PH == PH_INIT || PH == PH_REF) {
-    kb_finish<PH><<<ns, NTF, 0, S->ctx->stream>>>(desc, ns, par);
+    kb_finish<PH><<<ns, NTF, 0, S->ctx->stream>>>(desc, ns, par, hdr);
     HIPCHK(hipGetLastError());
   }
   if (PH == PH_U || PH == PH_P || PH == PH_P0 || PH == PH_QP || PH == PH_PN) ++S->ver;
@@ -681,40 +705,43 @@ void launch_phase(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int block
 // the last one's P and moves p back into d.p.  Both forms leave the same
 // state, advance the state version by an odd count (graph replay keeps one
 // graph per parity) and give bit-identical iterates, so the form can change
-// from chunk to chunk.
+// from chunk to chunk.  With hdr (graph capture) ns, ts and te are the caps
+// of the grids and the kernels read the table's own counts from hdr.
 void chunk_launches(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int ts, int te, SlotStatus *status,
-                    bool two_pass) {
+                    bool two_pass, const TableHdr *hdr = nullptr) {
   if (two_pass) {
-    launch_phase<PH_P0>(S, desc, ns, te);
-    launch_phase<PH_Q>(S, desc, ns, ts);
-    launch_phase<PH_U>(S, desc, ns, te);
+    launch_phase<PH_P0>(S, desc, ns, te, hdr);
+    launch_phase<PH_Q>(S, desc, ns, ts, hdr);
+    launch_phase<PH_U>(S, desc, ns, te, hdr);
     for (int k = 1; k < S->K; ++k) {
-      launch_phase<PH_QP>(S, desc, ns, ts);
-      launch_phase<PH_U>(S, desc, ns, te);
+      launch_phase<PH_QP>(S, desc, ns, ts, hdr);
+      launch_phase<PH_U>(S, desc, ns, te, hdr);
     }
-    launch_phase<PH_PN>(S, desc, ns, te);
+    launch_phase<PH_PN>(S, desc, ns, te, hdr);
   } else {
-    launch_phase<PH_P0>(S, desc, ns, te);
+    launch_phase<PH_P0>(S, desc, ns, te, hdr);
     for (int k = 0; k < S->K; ++k) {
-      launch_phase<PH_Q>(S, desc, ns, ts);
-      launch_phase<PH_U>(S, desc, ns, te);
-      launch_phase<PH_P>(S, desc, ns, te);
+      launch_phase<PH_Q>(S, desc, ns, ts, hdr);
+      launch_phase<PH_U>(S, desc, ns, te, hdr);
+      launch_phase<PH_P>(S, desc, ns, te, hdr);
     }
   }
-  launch_phase<PH_FOLD>(S, desc, ns, te);
-  launch_phase<PH_REF>(S, desc, ns, ts);
-  kb_status<<<ceil_div(ns, 64), 64, 0, S->ctx->stream>>>(desc, ns, (int)(S->ver & 1), status);
+  launch_phase<PH_FOLD>(S, desc, ns, te, hdr);
+  launch_phase<PH_REF>(S, desc, ns, ts, hdr);
+  kb_status<<<ceil_div(ns, 64), 64, 0, S->ctx->stream>>>(desc, ns, (int)(S->ver & 1), status, hdr);
   HIPCHK(hipGetLastError());
 }
 
 // Retire the current graphs: a chunk replayed from them may still run, so
 // they are destroyed after the last chunk launched so far has drained.
 void drop_graphs(yafel_hip_server_s *S) {
-  for (auto &g : S->gexec)
-    if (g) {
-      S->gold.emplace_back(S->chunk_seq, g);
-      g = nullptr;
-    }
+  for (auto &row : S->gexec)
+    for (auto &g : row)
+      if (g) {
+        S->gold.emplace_back(S->chunk_seq, g);
+        g = nullptr;
+      }
+  S->gcap_s = S->gcap_e = 0;
   S->gepoch = -1;
 }
 
@@ -727,41 +754,66 @@ void reap_graphs(yafel_hip_server_s *S, bool all) {
   S->gold.swap(keep);
 }
 
-// Capture the chunk of the current active set twice, starting at each
-// version parity; the launches only record (nothing runs, no counter moves).
-void capture_graphs(yafel_hip_server_s *S) {
-  drop_graphs(S);
+// a grid cap: the next power of two, at least 256 blocks
+int grid_cap(int blocks) {
+  int c = 256;
+  while (c < blocks) c <<= 1;
+  return c;
+}
+
+// The active slot set's table and totals into gdesc / ghdr (stream-ordered,
+// after every chunk launched before), and new caps when the totals outgrow
+// the captured grids or use less than an eighth of them.
+void graph_table(yafel_hip_server_s *S) {
   using SS = yafel_hip_server_s;
   if (S->gdesc.bytes < SS::STAGE) S->gdesc.alloc(SS::STAGE);
-  for (auto &b : S->gstatus)
-    if (b.bytes < SS::STAGE) b.alloc(SS::STAGE);
+  if (S->ghdr.bytes < 256) S->ghdr.alloc(256);
   int ts, te;
-  upload_table(S, S->active, &ts, &te, &S->gdesc);
-  const int ns = (int)S->active.size();
+  const int e = upload_table(S, S->active, &ts, &te, &S->gdesc);
+  // the header from the end of the same staging entry (the table fills at
+  // most max_slots descriptors of it)
+  TableHdr *h = reinterpret_cast<TableHdr *>(S->pin + (SS::RING + e + 1) * SS::STAGE - sizeof(TableHdr));
+  *h = TableHdr{(int)S->active.size(), ts, te, 0};
+  HIPCHK(hipMemcpyAsync(S->ghdr.p, h, sizeof(TableHdr), hipMemcpyHostToDevice, S->ctx->stream));
+  const bool fits = ts <= S->gcap_s && te <= S->gcap_e;
+  const bool slack = S->gcap_s > 8 * grid_cap(ts) || S->gcap_e > 8 * grid_cap(te);
+  if (!fits || slack) {
+    drop_graphs(S);
+    S->gcap_s = grid_cap(ts);
+    S->gcap_e = grid_cap(te);
+  }
+  S->gepoch = S->epoch;
+}
+
+// Capture the chunk of form `form` (1: two passes) for status slot r,
+// starting at version parity r & 1; the launches only record (nothing runs,
+// no counter moves).
+void capture_graph(yafel_hip_server_s *S, int form, int r) {
+  using SS = yafel_hip_server_s;
+  if (S->gstatus[r].bytes < SS::STAGE) S->gstatus[r].alloc(SS::STAGE);
   const int64_t ver = S->ver, launches = S->stats.batched_launches;
   hipStream_t st = S->ctx->stream;
-  for (int p = 0; p < 2; ++p) {
-    S->ver = p;
-    hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-    try {
-      chunk_launches(S, S->gdesc.as<SlotDesc>(), ns, ts, te, S->gstatus[p].as<SlotStatus>(), false);
-      HIPCHK(hipMemcpyAsync(S->gpin + p * SS::STAGE, S->gstatus[p].p, ns * sizeof(SlotStatus), hipMemcpyDeviceToHost,
-                            st));
-    } catch (...) {
-      (void)hipStreamEndCapture(st, &g);
-      if (g) (void)hipGraphDestroy(g);
-      S->ver = ver;
-      throw;
-    }
-    HIPCHK(hipStreamEndCapture(st, &g));
-    const hipError_t rc = hipGraphInstantiate(&S->gexec[p], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    HIPCHK(rc);
+  S->ver = r & 1;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+  try {
+    chunk_launches(S, S->gdesc.as<SlotDesc>(), S->max_slots, S->gcap_s, S->gcap_e, S->gstatus[r].as<SlotStatus>(),
+                   form == 1, S->ghdr.as<TableHdr>());
+    HIPCHK(hipMemcpyAsync(S->gpin + r * SS::STAGE, S->gstatus[r].p, S->max_slots * sizeof(SlotStatus),
+                          hipMemcpyDeviceToHost, st));
+  } catch (...) {
+    (void)hipStreamEndCapture(st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    S->ver = ver;
+    S->stats.batched_launches = launches;
+    throw;
   }
   S->ver = ver;
   S->stats.batched_launches = launches;
-  S->gepoch = S->epoch;
+  HIPCHK(hipStreamEndCapture(st, &g));
+  const hipError_t rc = hipGraphInstantiate(&S->gexec[form][r], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPCHK(rc);
   S->captures++;
 }
 
@@ -770,6 +822,7 @@ void drain_one(yafel_hip_server_s *S, std::vector<int64_t> &done_ids) {
   S->inflight.pop_front();
   HIPCHK(hipEventSynchronize(f.ev));
   S->drained_seq = f.seq;
+  if (f.gr >= 0) S->gbusy[f.gr] = false;
   const SlotStatus *st = (const SlotStatus *)f.pin;
   for (size_t i = 0; i < f.slots.size(); ++i) {
     Slot *sl = f.slots[i];
@@ -808,7 +861,10 @@ int yafel_hip_server_create_ex(yafel_hip_ctx ctx, size_t hbm_budget, int max_slo
   YH_TRY_BEGIN
   REQUIRE(ctx && out && max_slots > 0 && hbm_budget > 0, "server_create: bad argument");
   REQUIRE(setup_threads >= 0 && setup_threads <= 16, "server_create: setup_threads must be 0..16");
-  REQUIRE((size_t)max_slots * sizeof(SlotDesc) <= yafel_hip_server_s::STAGE, "server_create: max_slots too large");
+  // a table of max_slots descriptors and, at the end of its staging area, the
+  // graph replay header
+  REQUIRE((size_t)max_slots * sizeof(SlotDesc) + sizeof(TableHdr) <= yafel_hip_server_s::STAGE,
+          "server_create: max_slots too large");
   HIPCHK(hipSetDevice(ctx->device));
   if (setup_threads == 0) setup_threads = 3;
   auto *S = new yafel_hip_server_s();
@@ -831,7 +887,7 @@ int yafel_hip_server_create_ex(yafel_hip_ctx ctx, size_t hbm_budget, int max_slo
   for (auto &e : S->evs) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipHostMalloc((void **)&S->pin, 2 * yafel_hip_server_s::RING * yafel_hip_server_s::STAGE,
                        hipHostMallocDefault));
-  HIPCHK(hipHostMalloc((void **)&S->gpin, 2 * yafel_hip_server_s::STAGE, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&S->gpin, yafel_hip_server_s::GR * yafel_hip_server_s::STAGE, hipHostMallocDefault));
   for (auto &e : S->gev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking));
   for (auto *c : S->sctx) S->workers.emplace_back(worker_main, S, c);
@@ -942,11 +998,9 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
     S->failed_ids.clear();
   }
   // 2. drain the oldest chunk when the pipeline is full: up to kDepth chunks
-  //    in flight (two with graph replay, whose status areas and events are
-  //    per state-version parity), so that the host builds and submits the
-  //    next chunk while the device still runs the ones before it
-  const size_t depth = S->graphs ? 2 : (size_t)yafel_hip_server_s::kDepth;
-  while (S->inflight.size() >= depth) drain_one(S, done);
+  //    in flight, so that the host builds and submits the next chunk while
+  //    the device still runs the ones before it
+  while (S->inflight.size() >= (size_t)yafel_hip_server_s::kDepth) drain_one(S, done);
   // 3. retire finished slots (their x is on the host): the solve buffers are
   //    freed on the solve stream now, the matrix once its last chunk drains
   std::vector<Slot *> keep;
@@ -981,31 +1035,37 @@ int yafel_hip_server_progress(yafel_hip_server S, int64_t *done_ids, int max_ids
   if (!S->active.empty()) {
     const int ns = (int)S->active.size();
     yafel_hip_server_s::Inflight f;
+    bool two = S->passes == 2;
+    if (S->passes == 0) {
+      int64_t scalars = 0;
+      for (Slot *sl : S->active) scalars += sl->A->ntot * (sl->A->format == YAFEL_HIP_BCSR3 ? 9 : 1);
+      two = scalars <= S->two_pass_limit;
+    }
+    (two ? S->stats_two : S->stats_three)++;
     if (S->graphs) {
-      if (S->gepoch != S->epoch) capture_graphs(S);
-      // the two chunks in flight have opposite parities, so each graph's
-      // status area and event are free again once the older one drained
+      if (S->gepoch != S->epoch) graph_table(S);
+      // a free status slot of the chunk's starting parity
       const int p = (int)(S->ver & 1);
-      HIPCHK(hipGraphLaunch(S->gexec[p], S->ctx->stream));
+      while (S->gbusy[p] && S->gbusy[p + 2]) drain_one(S, done);
+      const int r = S->gbusy[p] ? p + 2 : p;
+      const int form = two ? 1 : 0;
+      if (!S->gexec[form][r]) capture_graph(S, form, r);
+      HIPCHK(hipGraphLaunch(S->gexec[form][r], S->ctx->stream));
+      S->gbusy[r] = true;
       S->ver += 2 * S->K + 1;
-      S->stats.batched_launches += 3 * S->K + 3;
+      S->stats.batched_launches += two ? 2 * S->K + 4 : 3 * S->K + 3;
       S->replays++;
-      f.ev = S->gev[p];
+      f.ev = S->gev[r];
       f.ring = -1;
-      f.pin = S->gpin + p * yafel_hip_server_s::STAGE;
+      f.gr = r;
+      f.pin = S->gpin + r * yafel_hip_server_s::STAGE;
     } else {
       int ts, te;
       const int e = upload_table(S, S->active, &ts, &te);
-      bool two = S->passes == 2;
-      if (S->passes == 0) {
-        int64_t scalars = 0;
-        for (Slot *sl : S->active) scalars += sl->A->ntot * (sl->A->format == YAFEL_HIP_BCSR3 ? 9 : 1);
-        two = scalars <= S->two_pass_limit;
-      }
       chunk_launches(S, S->desc[e].as<SlotDesc>(), ns, ts, te, S->status[e].as<SlotStatus>(), two);
-      (two ? S->stats_two : S->stats_three)++;
       f.ev = S->evs[e];
       f.ring = e;
+      f.gr = -1;
       f.pin = S->pin + e * yafel_hip_server_s::STAGE;
       HIPCHK(hipMemcpyAsync(f.pin, S->status[e].p, ns * sizeof(SlotStatus), hipMemcpyDeviceToHost, S->ctx->stream));
     }
