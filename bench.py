@@ -252,7 +252,7 @@ def run_stream(hip, ctx, mats, arrivals, budget, slots, setup_threads=0, graphs=
     """Feed requests at their arrival times into a server; returns per-request
     latency (arrival -> found finished), makespan, the results, the server
     statistics and each request's (admission wait, setup, wait to join,
-    solve) times in ms.  graphs: replay chunks from per-epoch HIP graphs;
+    solve) times in ms.  graphs: replay chunks from HIP graphs;
     cg_passes, two_pass_limit: kernels per batched CG iteration (0, 0: the
     library's default)."""
     srv = hip.Server(ctx, budget, slots, setup_threads)

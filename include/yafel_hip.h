@@ -449,7 +449,7 @@ typedef struct {
   int64_t chunks, batched_launches;
   size_t hbm_in_use, hbm_peak, hbm_budget;
   double setup_ms;              /* host time the setup worker spent building requests */
-  int64_t graph_captures;       /* slot-set epochs captured into graphs (graph replay on) */
+  int64_t graph_captures;       /* chunk graphs captured (graph replay on) */
   int64_t graph_replays;        /* chunks replayed from a graph */
   int64_t two_pass_chunks;      /* directly submitted chunks of two-pass CG iterations */
   int64_t three_pass_chunks;    /* ... and of three-pass iterations */
@@ -487,11 +487,13 @@ int yafel_hip_server_times(yafel_hip_server s, int64_t id, double *ms);
  * yafel_hip_server_result releases it */
 int yafel_hip_server_cancel(yafel_hip_server s, int64_t id);
 int yafel_hip_server_stats_get(yafel_hip_server s, yafel_hip_server_stats *st);
-/* on != 0: replay each chunk from a HIP graph while the active slot set stays
- * the same (captured at the set's first chunk, one graph per state-version
- * parity; a join or a retirement starts a new capture); 0 (the default):
- * submit every launch directly.  Results are bit-identical either way.  No
- * reference counterpart (the reference has no request server). */
+/* on != 0: replay each chunk from a HIP graph whose kernels read the slot
+ * table and its totals from device memory (uploaded when a request joins or
+ * retires; a graph is captured again only when the totals outgrow the grids
+ * it was captured with), one graph per chunk form and in-flight status slot;
+ * 0 (the default): submit every launch directly.  Results are bit-identical
+ * either way.  No reference counterpart (the reference has no request
+ * server). */
 int yafel_hip_server_set_graphs(yafel_hip_server s, int on);
 /* kernels per batched CG iteration with direct submission: 3 (SpMV; alpha
  * and r, z; beta, x and p), 2 (after a chunk's first iteration the SpMV also
@@ -499,7 +501,7 @@ int yafel_hip_server_set_graphs(yafel_hip_server s, int on);
  * x += alpha p and forms p = z + beta p in its gather: a launch fewer, but
  * two vectors gathered) or 0 (the default: a chunk takes two passes while its
  * active slots store at most two_pass_limit scalars in all, three otherwise;
- * two_pass_limit <= 0 keeps the default of 4 Mi).  Graph replay uses 3.  The
+ * two_pass_limit <= 0 keeps the default of 4 Mi), also with graph replay.  The
  * iterates are bit-identical whatever the choice.  No reference counterpart
  * (the reference has no request server). */
 int yafel_hip_server_set_cg_passes(yafel_hip_server s, int passes, int64_t two_pass_limit);

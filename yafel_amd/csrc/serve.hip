@@ -703,8 +703,8 @@ void launch_phase(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int block
 // two_pass, two after the first (Q, U, then QP, U: the QP pass applies the
 // previous iteration's P in its gather) and a closing pass (PN) that applies
 // the last one's P and moves p back into d.p.  Both forms leave the same
-// state, advance the state version by an odd count (graph replay keeps one
-// graph per parity) and give bit-identical iterates, so the form can change
+// state, advance the state version by an odd count (graph replay keeps
+// graphs per starting parity) and give bit-identical iterates, so the form can change
 // from chunk to chunk.  With hdr (graph capture) ns, ts and te are the caps
 // of the grids and the kernels read the table's own counts from hdr.
 void chunk_launches(yafel_hip_server_s *S, const SlotDesc *desc, int ns, int ts, int te, SlotStatus *status,

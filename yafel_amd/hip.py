@@ -705,7 +705,7 @@ class Server:
         check(lib.yafel_hip_server_set_cg_passes(self.h, int(passes), int(two_pass_limit)), self.ctx.h)
 
     def set_graphs(self, on):
-        """Replay each chunk from a HIP graph while the slot set stays the same."""
+        """Replay each chunk from a HIP graph (the slot table read from device memory)."""
         check(lib.yafel_hip_server_set_graphs(self.h, 1 if on else 0), self.ctx.h)
 
     def stats(self):
