@@ -41,14 +41,33 @@ struct ElemShape {
   static constexpr int N1 = P + 1;
   static constexpr int NEN = DIM == 2 ? N1 * N1 : N1 * N1 * N1;
   static constexpr int NQP = NEN;
-  // per element: gradients and jxw, each padded by one double (GP, WP) so
-  // that threads of consecutive elements read different LDS banks
-  static constexpr int GP = NQP * NEN * DIM + 1, WP = NQP + 1;
-  static constexpr int LDS_PER = GP * 8 + WP * 8 + NEN * DIM * 8;
+  static constexpr int LDS_PER = (NQP * NEN * DIM + 1) * 8 + (NQP + 1) * 8 + NEN * DIM * 8;
   static constexpr int EPB0 = NT / NEN;
   static constexpr int EPB1 = 60000 / LDS_PER;
   static constexpr int EPB = (EPB0 < EPB1 ? EPB0 : EPB1) > 0 ? (EPB0 < EPB1 ? EPB0 : EPB1) : 1;
+  // symmetric staging component-major across elements when a workgroup holds
+  // enough elements for its stores to form long runs (Q1 hexes and quads,
+  // Q2 quads); element-major otherwise (a Q4 element's 325 blocks are
+  // contiguous there, and a workgroup holds 5 elements)
+  static constexpr bool SOA = EPB >= 16;
+  // per element: gradients and jxw, with component-major staging each padded
+  // by one double so that threads of consecutive elements read different
+  // LDS banks
+  static constexpr int GP = NQP * NEN * DIM + (SOA ? 1 : 0), WP = NQP + (SOA ? 1 : 0);
 };
+
+// whether the symmetric staging of a (dim, p) mesh is component-major
+inline bool sym_soa(int dim, int p) {
+  if (dim == 2) {
+    switch (p) {
+      case 1: return ElemShape<2, 1>::SOA;
+      case 2: return ElemShape<2, 2>::SOA;
+      case 3: return ElemShape<2, 3>::SOA;
+      default: return ElemShape<2, 4>::SOA;
+    }
+  }
+  return p == 1 ? ElemShape<3, 1>::SOA : ElemShape<3, 2>::SOA;
+}
 
 template <int DIM>
 __device__ __forceinline__ void jac_inv(const double *J, double &det, double *Ji) {
@@ -129,12 +148,16 @@ __device__ __forceinline__ void tangent_block(const Phys &ph, double ke, const d
 // gradients into LDS; then the tangent blocks and the residual rows.
 // SYM (Poisson, elasticity, whose tangents the reference fills for B >= A and
 // mirrors): only the blocks B >= A are formed, a thread per (element, block),
-// consecutive threads taking consecutive elements of one block, and staged
-// component-major in the packed upper triangle,
-//   Ke[((idx(A, B) * DPN + a) * DPN + b) * nel + e],  NB = NEN (NEN + 1) / 2
-// blocks, so that both these stores and the gather's loads (a wavefront takes
+// and staged in the packed upper triangle, NB = NEN (NEN + 1) / 2 blocks,
+// either component-major (S::SOA), consecutive threads taking consecutive
+// elements of one block,
+//   Ke[((idx(A, B) * DPN + a) * DPN + b) * nel + e],
+// so that both these stores and the gather's loads (a wavefront takes
 // neighbouring nodes, whose entries come from neighbouring elements) are
-// contiguous; the gather reads block (B, A) as the transpose.  Otherwise
+// contiguous, or element-major, consecutive threads taking consecutive
+// blocks of one element,
+//   Ke[(e * NB + idx(A, B)) * DPN^2 + a * DPN + b];
+// the gather reads block (B, A) as the transpose.  Otherwise
 // every block is formed, a thread per (element, row block A), into the
 // row-major element matrix.
 template <int DIM, int P, int DPN, bool SYM>
@@ -195,7 +218,7 @@ __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__re
   if (want_k && SYM) {
     constexpr int NB = NEN * (NEN + 1) / 2;
     for (int i = threadIdx.x; i < EPB * NB; i += NT) {
-      const int bi = i / EPB, le = i - bi * EPB;
+      const int bi = S::SOA ? i / EPB : i - (i / NB) * NB, le = S::SOA ? i - bi * EPB : i / NB;
       const int64_t e = e0 + le;
       if (e >= nel) continue;
       int A = 0, rem = bi;
@@ -208,7 +231,9 @@ __global__ __launch_bounds__(NT) void k_element(int64_t nel, const int32_t *__re
 #pragma unroll
       for (int a = 0; a < DPN; ++a)
 #pragma unroll
-        for (int b = 0; b < DPN; ++b) Ke[(((int64_t)bi * DPN + a) * DPN + b) * nel + e] = k[a][b];
+        for (int b = 0; b < DPN; ++b)
+          Ke[S::SOA ? (((int64_t)bi * DPN + a) * DPN + b) * nel + e : (e * NB + bi) * (DPN * DPN) + a * DPN + b] =
+              k[a][b];
     }
   }
   for (int i = threadIdx.x; i < EPB * NEN; i += NT) {
@@ -901,6 +926,7 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
                                                    pl->contrib.as<uint32_t>(), K_, pl->run_row.as<int32_t>(),     \
                                                    pl->node_ptr.as<int32_t>(), sp, ip, A->ntot, val)
       const double *k64 = m->stageK.as<double>();
+      bool sym_aos = false;  // element-major symmetric staging: the thread-per-node-pair gather
       const int gs = ceil_div(A->nsl * kGatherSG, NT / 64);
       const int32_t *pm = A->perm.p ? A->perm.as<int32_t>() : nullptr;
       const int64_t srows = A->format == YAFEL_HIP_BCSR3 ? A->brows : A->rows;
@@ -908,7 +934,12 @@ int yafel_hip_assemble(yafel_hip_mesh m, const yafel_hip_physics *phys, int requ
   k_gather_sym<DPN_, FMT_><<<gs, NT, 0, s>>>(A->nsl, srows, m->nel, m->nen, pl->cptr.as<int32_t>(),               \
                                              pl->contrib.as<uint32_t>(), k64, pl->node_ptr.as<int32_t>(), sp, pm, \
                                              A->ntot, val)
+      if (sym && !sym_soa(m->dim, m->p)) sym_aos = true;
       if (f32) GATHER(1, YAFEL_HIP_CSR, float, false, m->stageK32.as<float>());
+      else if (A->format == YAFEL_HIP_BCSR3 && sym && sym_aos) GATHER(3, YAFEL_HIP_BCSR3, double, true, k64);
+      else if (m->dpn == 1 && sym && sym_aos) GATHER(1, YAFEL_HIP_CSR, double, true, k64);
+      else if (m->dpn == 2 && sym && sym_aos) GATHER(2, YAFEL_HIP_CSR, double, true, k64);
+      else if (sym && sym_aos) GATHER(3, YAFEL_HIP_CSR, double, true, k64);
       else if (A->format == YAFEL_HIP_BCSR3 && sym) GATHER_SYM(3, YAFEL_HIP_BCSR3);
       else if (A->format == YAFEL_HIP_BCSR3) GATHER(3, YAFEL_HIP_BCSR3, double, false, k64);
       else if (m->dpn == 1 && sym) GATHER_SYM(1, YAFEL_HIP_CSR);
