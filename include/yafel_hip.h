@@ -143,7 +143,9 @@ int yafel_hip_spmv(yafel_hip_matrix A, int precision, const void *x_dev, void *y
 /* Vector<T> operator*(A, x) (old_handmade_linalg/operators.hpp:117-131): host in, host out. */
 int yafel_hip_spmv_host(yafel_hip_matrix A, const double *x, double *y);
 /* Y = A X for k right-hand sides stored row-major (n x k, the k values of a
- * row contiguous).  No reference counterpart: multi-RHS form of csr_spmv. */
+ * row contiguous).  In FP64 each column of Y equals yafel_hip_spmv of that
+ * column of X bit for bit (rows summed in column order, one FMA per entry).
+ * No reference counterpart: multi-RHS form of csr_spmv. */
 int yafel_hip_spmm(yafel_hip_matrix A, int precision, int k, const void *X_dev, void *Y_dev);
 
 /* ------------------------------------------------------------ vector ops */
