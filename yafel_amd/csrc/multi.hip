@@ -995,7 +995,6 @@ struct MBase {
 
 __device__ __forceinline__ float4 ld4(const float *p, int64_t o) { return *reinterpret_cast<const float4 *>(p + o); }
 __device__ __forceinline__ float comp(const float4 &v, int c) { return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w; }
-__device__ __forceinline__ float4 scale4(const float4 &v, float d) { return make_float4(v.x * d, v.y * d, v.z * d, v.w * d); }
 
 
 __device__ void request_pause_or_advance(MState *st, int j, double rho_new, double rr, bool cgfam) {
