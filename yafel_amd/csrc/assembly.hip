@@ -113,20 +113,18 @@ __device__ __forceinline__ void tangent_block(const Phys &ph, double ke, const d
       for (int d = 0; d < DIM; ++d) dt += gA[d] * gB[d];
       k[0][0] -= ke * dt * jxw;
     } else if (ph.kind == YAFEL_HIP_PHYS_ELASTICITY) {
-      // apps/BasicLinearElasticity/main.cpp:70-110
+      // apps/BasicLinearElasticity/main.cpp:70-110: sum_jl C_ijkl dN_A,j
+      // dN_B,l with C_ijkl = lam d_ij d_kl + mu (d_ik d_jl + d_il d_jk),
+      // contracted in closed form (the tensor's zero terms are not formed)
       const double lam = ke * ph.lambda, mu = ke * ph.mu;
+      double dab = 0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) dab += gA[d] * gB[d];
 #pragma unroll
       for (int ii = 0; ii < DPN; ++ii)
 #pragma unroll
         for (int kk = 0; kk < DPN; ++kk) {
-          double s = 0;
-#pragma unroll
-          for (int j = 0; j < DIM; ++j)
-#pragma unroll
-            for (int l = 0; l < DIM; ++l) {
-              const double C = lam * (ii == j) * (kk == l) + mu * ((ii == kk) * (j == l) + (ii == l) * (j == kk));
-              s += C * (gA[j] * gB[l]);
-            }
+          const double s = lam * (gA[ii] * gB[kk]) + mu * (gA[kk] * gB[ii] + (ii == kk ? dab : 0.0));
           k[ii][kk] -= s * jxw;
         }
     } else {
