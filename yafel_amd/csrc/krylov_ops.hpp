@@ -516,14 +516,25 @@ struct OpReport : OpBase {
 // order; the result is the same in every thread of every workgroup.  Split in
 // two so that the loads (refold_load, this thread's share) can be issued
 // before other waits: refold_reduce combines the shares across the workgroup.
+// Up to RM of the thread's partials per sum are loaded before the first add
+// (a round trip per RM partials instead of one per few); the adds keep the
+// order, and the zeros added past nblk change nothing (v never becomes -0).
+constexpr int RM = 8;
 template <int NPIN>
 __device__ __forceinline__ void refold_load(const double *__restrict__ part, int nblk, double (&v)[NPIN]) {
 #pragma unroll
   for (int j = 0; j < NPIN; ++j) v[j] = 0;
-#pragma unroll 4
-  for (int i = threadIdx.x; i < nblk; i += NT)
+  for (int i0 = threadIdx.x; i0 < nblk; i0 += RM * NT) {
+    double t[NPIN][RM];
 #pragma unroll
-    for (int j = 0; j < NPIN; ++j) v[j] += part[(size_t)j * nblk + i];
+    for (int m = 0; m < RM; ++m)
+#pragma unroll
+      for (int j = 0; j < NPIN; ++j) t[j][m] = i0 + m * NT < nblk ? part[(size_t)j * nblk + i0 + m * NT] : 0.0;
+#pragma unroll
+    for (int m = 0; m < RM; ++m)
+#pragma unroll
+      for (int j = 0; j < NPIN; ++j) v[j] += t[j][m];
+  }
 }
 
 template <int NPIN>
