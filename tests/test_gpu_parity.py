@@ -168,6 +168,33 @@ def test_spmm_sorted_rows_matches_columnwise_spmv(H, ctx, k):
         assert np.abs(Y32[:, j] - yj).max() <= 1e-5 * np.abs(yj).max()
 
 
+def test_fp32_copy_keeps_the_row_sums_of_dense_rows(H, ctx):
+    """On dense rows (Q4 Poisson: about 34 stored scalars a row) each row's
+    rounding errors are folded into its FP32 diagonal, so the FP32 product
+    with the ones vector (the FP32 row summed in FP64, rounded once at the
+    store) is within half an FP32 ulp of the diagonal plus half an ulp of the
+    result from the FP64 row sum.  Rounding the entries alone leaves the sum
+    of 25-81 entry errors, over that bound in some rows (4 % here)."""
+    m = H.Mesh.structured(ctx, 2, 4, 1, 8)
+    A = m.pattern(H.CSR, H.FP64 | H.FP32)
+    m.assemble(H.physics(H.POISSON, 2, coef=np.linspace(0.5, 2.0, 64)), A, None, H.TANGENT)
+    A.sync_fp32()
+    n = A.info().rows
+    ptr, col, val = A.arrays()
+    assert len(val) >= 32 * n                           # dense rows: the fold applies
+    diag = np.array([val[ptr[i]:ptr[i + 1]][col[ptr[i]:ptr[i + 1]] == i][0] for i in range(n)])
+    ones = np.ones(n)
+    y64 = A.spmv_host(ones)
+    yd = ctx.alloc(n, np.float32)
+    A.spmv(ctx.upload(ones.astype(np.float32)), yd, precision=H.FP32)
+    err = np.abs(yd.download().astype(np.float64) - y64)
+    bound = (np.abs(diag) + np.abs(y64)) * 2.0 ** -24 * 1.001 + 1e-15 * np.abs(diag)
+    assert np.all(err <= bound)
+    # the entries alone, rounded and summed: over the bound in some rows
+    s32 = np.add.reduceat(val.astype(np.float32).astype(np.float64), ptr[:-1])
+    assert np.mean(np.abs(s32.astype(np.float32).astype(np.float64) - y64) > bound) > 0.01
+
+
 @pytest.mark.parametrize("pattern", ["q2", "scattered"])
 @pytest.mark.parametrize("k", [1, 5, 32, 33, 64, 70])
 def test_spmm_fp64_staged_is_the_spmv_bit_for_bit(H, ctx, pattern, k):
