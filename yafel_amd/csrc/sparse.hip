@@ -203,23 +203,34 @@ __global__ void k_diag_bcsr(int64_t brows, int64_t ntot, const int *__restrict__
 // elements, 3x3 blocks), where large entries of both signs cancel.  B = 3:
 // BCSR3 blocks; B = 1: CSR.
 template <int B>
-__global__ void k_fp32_rowsum(int64_t brows, int64_t ntot, const int *__restrict__ ptr,
-                              const int *__restrict__ sptr, const int *__restrict__ iperm,
+__global__ void k_fp32_rowsum(int64_t rows, int64_t nsl, int64_t ntot, const int *__restrict__ ptr,
+                              const int *__restrict__ sptr, const int *__restrict__ perm,
                               const int *__restrict__ col, const double *__restrict__ val64,
                               float *__restrict__ val32) {
-  const int64_t br = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (br >= brows) return;
+  // the conversion and the fold in one pass: a thread per stored row (slice
+  // s, lane l) converts every slot of its lane, padding included, summing
+  // its row's rounding errors in slot order
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t s = i >> 6;
+  if (s >= nsl) return;
+  const int64_t r = i < rows ? (perm ? (int64_t)perm[i] : i) : -1;
+  const int len = r >= 0 ? ptr[r + 1] - ptr[r] : 0;
+  const int p0 = sptr[s], ns = sptr[s + 1] - p0;
   double d[B * B];
 #pragma unroll
   for (int e = 0; e < B * B; ++e) d[e] = 0;
   int64_t kd = -1;
-  const int len = ptr[br + 1] - ptr[br];
-  for (int t = 0; t < len; ++t) {
-    const int64_t k = dev::sell_pos(sptr, iperm, br, t);
-    if (col[k] == br) kd = k;
-    // a64 - a32 is exact: a32 is a64 rounded
+  for (int t = 0; t < ns; ++t) {
+    const int64_t k = (int64_t)(p0 + t) * 64 + (i & 63);
+    const bool live = t < len;
+    if (live && col[k] == r) kd = k;
 #pragma unroll
-    for (int e = 0; e < B * B; ++e) d[e] += val64[(int64_t)e * ntot + k] - (double)val32[(int64_t)e * ntot + k];
+    for (int e = 0; e < B * B; ++e) {
+      const double a = val64[(int64_t)e * ntot + k];
+      const float a32 = (float)a;
+      val32[(int64_t)e * ntot + k] = a32;
+      if (live) d[e] += a - (double)a32;  // exact: a32 is a rounded
+    }
   }
   if (kd < 0) return;
 #pragma unroll
@@ -543,23 +554,22 @@ void matrix_build_fp32(yafel_hip_matrix_s *A) {
   const int64_t n = A->ntot * (A->format == YAFEL_HIP_CSR ? 1 : 9);
   if (!A->val32.p || A->val32.bytes < (size_t)std::max<int64_t>(n, 1) * 4) A->val32.alloc((size_t)std::max<int64_t>(n, 1) * 4);
   if (n > 0) {
-    k_to_fp32<<<grid_for(A->ctx, n, 256), 256, 0, A->ctx->stream>>>(A->val64.as<double>(), A->val32.as<float>(), n);
-    HIPCHK(hipGetLastError());
     // dense rows only (at least 32 stored scalars a row on average): on Q4
     // Poisson (36) the mixed solve took 3887 inner iterations with the fold
     // against 4408 without, on 3-D elasticity in BCSR3 (78) 862 against 868;
     // on Q1 and Q2 rows (9-16) it added 2-3 % to the served solves' times
     const bool bcsr = A->format == YAFEL_HIP_BCSR3;
     const bool fold = (double)A->nblk * (bcsr ? 9 : 1) >= 32.0 * (double)A->rows;
-    const int *ip = A->iperm.p ? A->iperm.as<int>() : nullptr;
+    const int *pm = A->perm.p ? A->perm.as<int>() : nullptr;
+    const int g = (int)ceil_div(A->nsl * 64, 256);
     if (fold && bcsr)
-      k_fp32_rowsum<3><<<ceil_div(A->brows, 256), 256, 0, A->ctx->stream>>>(
-          A->brows, A->ntot, A->ptr.as<int>(), A->sptr.as<int>(), ip, A->col.as<int>(), A->val64.as<double>(),
-          A->val32.as<float>());
+      k_fp32_rowsum<3><<<g, 256, 0, A->ctx->stream>>>(A->brows, A->nsl, A->ntot, A->ptr.as<int>(), A->sptr.as<int>(),
+                                                     pm, A->col.as<int>(), A->val64.as<double>(), A->val32.as<float>());
     else if (fold)
-      k_fp32_rowsum<1><<<ceil_div(A->brows, 256), 256, 0, A->ctx->stream>>>(
-          A->brows, A->ntot, A->ptr.as<int>(), A->sptr.as<int>(), ip, A->col.as<int>(), A->val64.as<double>(),
-          A->val32.as<float>());
+      k_fp32_rowsum<1><<<g, 256, 0, A->ctx->stream>>>(A->brows, A->nsl, A->ntot, A->ptr.as<int>(), A->sptr.as<int>(),
+                                                     pm, A->col.as<int>(), A->val64.as<double>(), A->val32.as<float>());
+    else
+      k_to_fp32<<<grid_for(A->ctx, n, 256), 256, 0, A->ctx->stream>>>(A->val64.as<double>(), A->val32.as<float>(), n);
     HIPCHK(hipGetLastError());
   }
   A->has32 = true;
