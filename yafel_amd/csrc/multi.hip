@@ -66,10 +66,10 @@ template <class T> struct CsrView {
 // k_mfinish folds them (fixed order) in a one-workgroup kernel launched next
 // on the stream and calls op.last_lane per RHS, then op.last_all once with
 // the counts of active and pause-wanting right-hand sides.
-template <int NP, class Op>
+template <int NP, int NB = NT, class Op>
 __device__ __forceinline__ void finish(Op &op, double (&acc)[NP > 0 ? NP : 1]) {
   if constexpr (NP > 0) {
-    __shared__ double lds[NP][NT / 64][64];
+    __shared__ double lds[NP][NB / 64][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int blk = dev::xcd_block();
 #pragma unroll
@@ -80,7 +80,7 @@ __device__ __forceinline__ void finish(Op &op, double (&acc)[NP > 0 ? NP : 1]) {
       for (int j = 0; j < NP; ++j) {
         double s = 0;
 #pragma unroll
-        for (int q = 0; q < NT / 64; ++q) s += lds[j][q][lane];
+        for (int q = 0; q < NB / 64; ++q) s += lds[j][q][lane];
         op.part[((size_t)j * gridDim.x + blk) * 64 + lane] = s;
       }
     }
@@ -828,8 +828,13 @@ void k_mspmm_lds64(StagedCsr64 A, Op op0) {
   if constexpr (NP > 0) finish_staged64<NP>(op, acc, reinterpret_cast<double *>(xs2));
 }
 
+// Vector passes with a lane per right-hand side: workgroups of 1024 threads,
+// two per CU, so that the one-workgroup fold after the pass reads a quarter
+// of the partials that 256-thread workgroups at eight per CU would leave.
+constexpr int NTE = 1024;  // threads of k_melem
+
 template <class Op>
-__global__ __launch_bounds__(NT) void k_melem(int64_t n, Op op0) {
+__global__ __launch_bounds__(NTE) void k_melem(int64_t n, Op op0) {
   Op op = op0;
   if (op.skip()) return;
   op.begin();
@@ -842,9 +847,9 @@ __global__ __launch_bounds__(NT) void k_melem(int64_t n, Op op0) {
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const int64_t r0 = (int64_t)dev::xcd_block() * chunk;
   const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
-  for (int64_t r = r0 + (threadIdx.x >> 6); r < r1; r += NT / 64)
+  for (int64_t r = r0 + (threadIdx.x >> 6); r < r1; r += NTE / 64)
     if (lane < k) op.elem(r, lane, acc);
-  finish<NP>(op, acc);
+  finish<NP, NTE>(op, acc);
 }
 
 // FP32 vector passes with k a multiple of 4.  Lane (rs = lane >> 4,
@@ -2030,7 +2035,10 @@ struct Multi {
       g = (int)((units + per - 1) / per);
       if constexpr (O::V4) k_melem4<O><<<g, NT4, 0, s>>>(n, per, op);
     } else {
-      k_melem<O><<<g, NT, 0, s>>>(n, op);
+      // at most two workgroups per CU, each with at least 64 rows
+      const int64_t cap = std::max<int64_t>(1, std::min<int64_t>(grid, 2 * (int64_t)ctx->num_cus));
+      g = (int)std::max<int64_t>(1, std::min<int64_t>(cap, (n + 63) / 64));
+      k_melem<O><<<g, NTE, 0, s>>>(n, op);
     }
     HIPCHK(hipGetLastError());
     if constexpr (O::NP > 0) {
